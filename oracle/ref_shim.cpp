@@ -1,0 +1,237 @@
+// C shim over the REFERENCE implementation, compiled in place from the
+// reference's own headers (include paths point into the reference tree; no
+// reference source is copied here).  Built by oracle/Makefile into
+// oracle/_ref/libspams_ref.so.
+//
+// TEST INFRASTRUCTURE ONLY: used by tests (as the checker), by the golden
+// fixture generator and by bench.py's CPU-baseline leg.  Never linked into the
+// engine.
+//
+// Entry points wrap the spams.h facade exactly as a C++ user would call it:
+//   _lassoD     cpp_library/spams.h:107-143
+//   _omp        cpp_library/spams.h:250-293
+//   _fistaFlat  cpp_library/spams.h:460-608
+// plus one per-signal-lambda LARS loop (ref_lasso_pervec_*) that mirrors the
+// OpenMP loop of lasso2 (decomp/decomp.h:1574-1646) with the constraint taken
+// per signal, calling the reference coreLARS2 (decomp/decomp.h:1706-1915).
+#include "spams.h"
+
+#include <cstring>
+#include <omp.h>
+
+namespace {
+
+template <typename T>
+struct SpOut {
+  SpMatrix<T>* sp;
+  Matrix<T>* path;
+};
+
+template <typename T>
+void* lasso_impl(const T* X, int m, int n, const T* D, int p, int L, T lambda1,
+                 T lambda2, int mode, int pos, int ols, int cholesky,
+                 int num_threads, int max_length_path, int return_path) {
+  Matrix<T> Xm(const_cast<T*>(X), m, n);
+  Matrix<T> Dm(const_cast<T*>(D), m, p);
+  Matrix<T>* path = nullptr;
+  SpMatrix<T>* sp = nullptr;
+  try {
+    sp = _lassoD<T>(&Xm, &Dm, &path, return_path != 0, L, lambda1, lambda2,
+                    static_cast<constraint_type>(mode), pos != 0, ols != 0,
+                    num_threads, max_length_path, false, cholesky != 0);
+  } catch (const char* msg) {
+    fprintf(stderr, "reference lasso: %s\n", msg);
+    return nullptr;
+  }
+  return new SpOut<T>{sp, path};
+}
+
+template <typename T>
+void* omp_impl(const T* X, int m, int n, const T* D, int p, const int* L,
+               int nL, const T* eps, int nEps, const T* lambda, int nLambda,
+               int num_threads, int return_path) {
+  Matrix<T> Xm(const_cast<T*>(X), m, n);
+  Matrix<T> Dm(const_cast<T*>(D), m, p);
+  // The facade reads the vectors' sizes to choose scalar vs per-signal mode.
+  Vector<int> vL(const_cast<int*>(L), nL > 0 ? nL : 1);
+  Vector<T> vE(const_cast<T*>(eps), nEps > 0 ? nEps : 1);
+  Vector<T> vLa(const_cast<T*>(lambda), nLambda > 0 ? nLambda : 1);
+  Matrix<T>* path = nullptr;
+  SpMatrix<T>* sp = nullptr;
+  try {
+    sp = _omp<T>(&Xm, &Dm, &path, return_path != 0, nL > 0, &vL, nEps > 0,
+                 &vE, nLambda > 0, &vLa, num_threads);
+  } catch (const char* msg) {
+    fprintf(stderr, "reference omp: %s\n", msg);
+    return nullptr;
+  }
+  return new SpOut<T>{sp, path};
+}
+
+template <typename T>
+void* lasso_pervec_impl(const T* X, int m, int n, const T* D, int p, int L,
+                        const T* lambdas, T lambda2, int mode, int pos,
+                        int num_threads, int max_length_path) {
+  // Same argument normalisation as _lassoD (cpp_library/spams.h:117-131).
+  if (L < 0) L = p;
+  if (max_length_path < 0) max_length_path = 4 * L;
+  if (L > m) L = m;
+  if (L > p) L = p;
+  Matrix<T> Xm(const_cast<T*>(X), m, n);
+  Matrix<T> Dm(const_cast<T*>(D), m, p);
+  // decomp/decomp.h:1566-1569
+  ProdMatrix<T> G(Dm, n > 10 && p < 50000);
+  ProdMatrix<T> DtX(Dm, Xm, false);
+  G.addDiag(MAX(lambda2, T(1e-10)));
+  Matrix<T> vM(L, n);
+  Matrix<INTM> rM(L, n);
+  const int NT = init_omp(num_threads);
+  Vector<T> norms;
+  Xm.norm_2sq_cols(norms);
+#pragma omp parallel num_threads(NT)
+  {
+    Vector<T> DtR(p), u(p);
+    u.setZeros();
+    Matrix<T> Gs(L, L), invGs(L, L), Ga(p, L), work(p, 3);
+    work.setZeros();
+#pragma omp for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      T normX = norms[i];
+      Vector<INTM> ind;
+      rM.refCol(i, ind);
+      Vector<T> coeffs;
+      vM.refCol(i, coeffs);
+      DtX.copyCol(i, DtR);
+      coreLARS2(DtR, G, Gs, Ga, invGs, u, coeffs, ind, work, normX,
+                static_cast<constraint_type>(mode), lambdas[i], pos != 0,
+                static_cast<T*>(nullptr), max_length_path);
+    }
+  }
+  SpMatrix<T>* sp = new SpMatrix<T>();
+  sp->convert(vM, rM, p);
+  return new SpOut<T>{sp, nullptr};
+}
+
+template <typename T>
+int sp_nnz(void* h) {
+  return static_cast<int>(static_cast<SpOut<T>*>(h)->sp->nzmax());
+}
+
+template <typename T>
+void sp_copy(void* h, int* pB, int* r, T* v, T* path) {
+  SpOut<T>* o = static_cast<SpOut<T>*>(h);
+  SpMatrix<T>* sp = o->sp;
+  const int n = sp->n();
+  if (sp->pB() == nullptr) {  // cleared output (L <= 0): no columns stored
+    pB[0] = 0;
+    return;
+  }
+  for (int j = 0; j <= n; ++j) pB[j] = static_cast<int>(sp->pB()[j]);
+  const int nnz = static_cast<int>(sp->pB()[n]);
+  for (int k = 0; k < nnz; ++k) {
+    r[k] = static_cast<int>(sp->r()[k]);
+    v[k] = sp->v()[k];
+  }
+  if (path && o->path)
+    std::memcpy(path, o->path->rawX(),
+                sizeof(T) * o->path->m() * o->path->n());
+}
+
+template <typename T>
+void sp_free(void* h) {
+  SpOut<T>* o = static_cast<SpOut<T>*>(h);
+  delete o->sp;
+  delete o->path;
+  delete o;
+}
+
+template <typename T>
+int fista_impl(const T* X, int m, int n, const T* D, int p, const T* W0, T* W,
+               T* optim_info, int num_threads, int max_it, T L0,
+               int fixed_step, T gamma, T lambda1, T lambda2, T tol, int it0,
+               int max_iter_backtracking, int compute_gram, const char* regul,
+               const char* loss, int pos, int intercept, int ista) {
+  Matrix<T> Xm(const_cast<T*>(X), m, n);
+  Matrix<T> Dm(const_cast<T*>(D), m, p);
+  Matrix<T> A0(const_cast<T*>(W0), p, n);
+  Matrix<T> A(W, p, n);
+  Vector<int> groups;  // empty: not given
+  Vector<T> inner;     // unused
+  char name_regul[64], name_loss[64], log_name[4] = "";
+  std::strncpy(name_regul, regul, 63);
+  name_regul[63] = 0;
+  std::strncpy(name_loss, loss, 63);
+  name_loss[63] = 0;
+  Matrix<T>* info = nullptr;
+  try {
+    // Python defaults for the arguments the decomposition path does not use
+    // (swig/python/spams.py:260-288).
+    info = _fistaFlat<T>(&Xm, &Dm, &A0, &A, &groups, num_threads, max_it, L0,
+                         fixed_step != 0, gamma, lambda1, T(1.0), lambda2,
+                         T(0.0), T(1.0), T(0.0), T(1.0), tol, it0,
+                         max_iter_backtracking, compute_gram != 0, false,
+                         false, intercept != 0, false, name_regul, name_loss,
+                         false, pos != 0, true, false, ista != 0, false,
+                         log_name, false, &inner, 1, true, false, 0);
+  } catch (const char* msg) {
+    fprintf(stderr, "reference fistaFlat: %s\n", msg);
+    return -1;
+  }
+  std::memcpy(optim_info, info->rawX(), sizeof(T) * 4 * n);
+  delete info;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+#define SPX_REF_DEFS(SUF, T)                                                   \
+  void* ref_lasso_##SUF(const T* X, int m, int n, const T* D, int p, int L,    \
+                        T lambda1, T lambda2, int mode, int pos, int ols,      \
+                        int cholesky, int num_threads, int max_length_path,    \
+                        int return_path) {                                     \
+    return lasso_impl<T>(X, m, n, D, p, L, lambda1, lambda2, mode, pos, ols,   \
+                         cholesky, num_threads, max_length_path, return_path); \
+  }                                                                            \
+  void* ref_omp_##SUF(const T* X, int m, int n, const T* D, int p,             \
+                      const int* L, int nL, const T* eps, int nEps,            \
+                      const T* lambda, int nLambda, int num_threads,           \
+                      int return_path) {                                       \
+    return omp_impl<T>(X, m, n, D, p, L, nL, eps, nEps, lambda, nLambda,       \
+                       num_threads, return_path);                              \
+  }                                                                            \
+  void* ref_lasso_pervec_##SUF(const T* X, int m, int n, const T* D, int p,    \
+                               int L, const T* lambdas, T lambda2, int mode,   \
+                               int pos, int num_threads,                       \
+                               int max_length_path) {                          \
+    return lasso_pervec_impl<T>(X, m, n, D, p, L, lambdas, lambda2, mode, pos, \
+                                num_threads, max_length_path);                 \
+  }                                                                            \
+  int ref_sp_nnz_##SUF(void* h) { return sp_nnz<T>(h); }                       \
+  int ref_sp_n_##SUF(void* h) {                                                \
+    return static_cast<int>(static_cast<SpOut<T>*>(h)->sp->n());               \
+  }                                                                            \
+  void ref_sp_copy_##SUF(void* h, int* pB, int* r, T* v, T* path) {            \
+    sp_copy<T>(h, pB, r, v, path);                                             \
+  }                                                                            \
+  void ref_sp_free_##SUF(void* h) { sp_free<T>(h); }                           \
+  int ref_fista_flat_##SUF(const T* X, int m, int n, const T* D, int p,        \
+                           const T* W0, T* W, T* optim_info, int num_threads,  \
+                           int max_it, T L0, int fixed_step, T gamma,          \
+                           T lambda1, T lambda2, T tol, int it0,               \
+                           int max_iter_backtracking, int compute_gram,        \
+                           const char* regul, const char* loss, int pos,       \
+                           int intercept, int ista) {                          \
+    return fista_impl<T>(X, m, n, D, p, W0, W, optim_info, num_threads,        \
+                         max_it, L0, fixed_step, gamma, lambda1, lambda2, tol, \
+                         it0, max_iter_backtracking, compute_gram, regul,      \
+                         loss, pos, intercept, ista);                          \
+  }
+
+SPX_REF_DEFS(f32, float)
+SPX_REF_DEFS(f64, double)
+
+int ref_num_procs(void) { return omp_get_num_procs(); }
+
+}  // extern "C"
