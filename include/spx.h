@@ -1,0 +1,250 @@
+/* spx -- MI355X-native sparse coding engine: C ABI.
+ *
+ * This is the drop-in boundary of the engine.  Every entry point takes plain
+ * pointers and sizes (column-major, `int` dimensions, CSC with `int` indices),
+ * returns an int status (0 on success) and leaves a message for
+ * spx_last_error() on failure.  The C++ header spams_amd.h re-creates the
+ * reference's spams.h templates (_lassoD, _omp, _fistaFlat) on top of it.
+ *
+ * Reference interfaces replaced (paths relative to the SPAMS tree):
+ *   spx_lasso_*       _lassoD      cpp_library/spams.h:107-143
+ *                     lasso2       decomp/decomp.h:1563-1646 (+coreLARS2 :1706)
+ *   spx_omp_*         _omp         cpp_library/spams.h:250-293
+ *                     omp          decomp/decomp.h:322-404 (+coreORMP :538)
+ *   spx_fista_flat_*  _fistaFlat   cpp_library/spams.h:460-608
+ *                     FISTA::solver prox/fista.h:3618-3752
+ *   spx_csc           SpMatrix<T>  linalg/linalg.h:775-934 (pB/pE/r/v CSC)
+ *
+ * Two families of entry points:
+ *   - host entry points (spx_lasso_f32, ...) take host buffers and return a
+ *     host CSC matrix allocated by the library (free with spx_csc_free), like
+ *     the reference's heap-allocated SpMatrix;
+ *   - device entry points (suffix _dev) take device pointers that are already
+ *     resident in HBM and run asynchronously on the context's stream.
+ */
+#ifndef SPX_H
+#define SPX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SPX_OK 0
+#define SPX_ERR_ARG 1      /* invalid argument (dimension mismatch, ...) */
+#define SPX_ERR_HIP 2      /* HIP runtime failure */
+#define SPX_ERR_NOMEM 3    /* device allocation / admission budget exceeded */
+#define SPX_ERR_UNSUP 4    /* option combination not supported */
+
+/* constraint_type (decomp/decomp.h:95) */
+#define SPX_L1COEFFS 0
+#define SPX_L2ERROR 1
+#define SPX_PENALTY 2
+
+/* arithmetic of the D'X / Gram / FISTA products (f32 entry points) */
+#define SPX_PREC_FP32 0    /* FP32 MFMA (v_mfma_f32_16x16x4_f32)           */
+#define SPX_PREC_BF16X3 1  /* BF16 hi/lo split, 3 MFMAs, FP32 accumulation  */
+
+/* solver launch form */
+#define SPX_FORM_BATCHED 0 /* persistent kernel + device work queue (default) */
+#define SPX_FORM_PLAIN 1   /* one workgroup per signal, chunked grid launches  */
+
+typedef struct spx_ctx spx_ctx;
+typedef struct spx_dict spx_dict;
+
+/* CSC matrix, rows sorted ascending within each column (the output format of
+ * SpMatrix<T>::convert, linalg/linalg.h:4840-4862).  pB has n+1 entries;
+ * pE of the reference is pB+1.  `v` is float* or double* by entry point. */
+typedef struct {
+  int m;          /* rows (= p, atoms)   */
+  int n;          /* columns (= signals) */
+  long long nnz;
+  int* pB;
+  int* r;
+  void* v;
+} spx_csc;
+
+/* Options of spx_lasso_*.  Field meaning follows _lassoD's arguments
+ * (cpp_library/spams.h:107-112); -1 selects the same defaults. */
+typedef struct {
+  int L;               /* max active set, -1 -> p (then clamped as _lassoD) */
+  int mode;            /* SPX_L1COEFFS / SPX_L2ERROR / SPX_PENALTY           */
+  int pos;             /* positivity constraint                              */
+  int max_length_path; /* homotopy step cap, -1 -> 4*L (before clamping)     */
+  int precision;       /* SPX_PREC_* (f32 entry points only)                 */
+  int form;            /* SPX_FORM_*                                         */
+  int waves_per_signal;/* 0 = autotuned; else 1, 2, 4 or 16                  */
+} spx_lasso_opts;
+
+/* Options of spx_omp_* (the _omp facade's vector arguments are passed
+ * separately: NULL/0 = not given, size 1 = scalar, size n = per signal). */
+typedef struct {
+  int precision;
+  int form;
+  int waves_per_signal;
+} spx_omp_opts;
+
+/* Options of spx_fista_flat_*.  Field meaning follows _fistaFlat
+ * (cpp_library/spams.h:460-499) restricted to the square loss and the vector
+ * regularisers "l1" and "elastic-net". */
+typedef struct {
+  int max_it;
+  double L0;
+  int fixed_step;
+  double gamma;
+  double lambda1;
+  double lambda2;
+  double tol;
+  int it0;
+  int max_iter_backtracking;
+  int regul;           /* 0 = "l1", 1 = "elastic-net" */
+  int pos;
+  int ista;
+  int precision;       /* SPX_PREC_* for the two products per iteration */
+} spx_fista_opts;
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a context on HIP device `device` (the calling thread's current
+ * device is left unchanged).  The context owns a stream and a pool of device
+ * scratch buffers. */
+int spx_ctx_create(spx_ctx** out, int device);
+void spx_ctx_destroy(spx_ctx* ctx);
+/* Replace the context's stream by a caller-owned hipStream_t (NULL = own). */
+int spx_ctx_set_stream(spx_ctx* ctx, void* hip_stream);
+int spx_ctx_synchronize(spx_ctx* ctx);
+/* Thread-local message of the last failing call on this thread. */
+const char* spx_last_error(void);
+/* Library version string. */
+const char* spx_version(void);
+
+/* ------------------------------------------------------------ dictionaries */
+
+/* A resident dictionary: D (m x p, column-major) and its Gram matrix
+ * G = D'D (computed once on the device with the chosen precision) kept in
+ * HBM.  `D` is a host pointer when on_device == 0, else a device pointer
+ * (copied).  Replaces the materialised ProdMatrix G (linalg/linalg.h:
+ * 5090-5142, decomp/decomp.h:1566 and :342). */
+int spx_dict_create_f32(spx_ctx* ctx, const float* D, int m, int p,
+                        int on_device, int precision, spx_dict** out);
+int spx_dict_create_f64(spx_ctx* ctx, const double* D, int m, int p,
+                        int on_device, spx_dict** out);
+void spx_dict_destroy(spx_dict* dict);
+int spx_dict_dims(const spx_dict* dict, int* m, int* p);
+/* Device bytes held by the dictionary (D, D', G). */
+size_t spx_dict_bytes(const spx_dict* dict);
+
+/* ------------------------------------------------------- host entry points */
+
+/* LARS-Lasso: replaces _lassoD (cpp_library/spams.h:107-143) with
+ * cholesky=false (lasso2 + coreLARS2).  `lambda` holds n_lambda constraints:
+ * 1 (the reference's scalar) or n (one per signal).  lambda2 adds
+ * max(lambda2, 1e-10) to the Gram diagonal (decomp/decomp.h:1569). */
+int spx_lasso_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D,
+                  int p, const float* lambda, int n_lambda, float lambda2,
+                  const spx_lasso_opts* opts, spx_csc* out);
+int spx_lasso_f64(spx_ctx* ctx, const double* X, int m, int n,
+                  const double* D, int p, const double* lambda, int n_lambda,
+                  double lambda2, const spx_lasso_opts* opts, spx_csc* out);
+
+/* ORMP: replaces _omp (cpp_library/spams.h:250-293).  L/eps/lambda follow the
+ * facade: pointer NULL or size 0 = not given, size 1 = scalar, size n = per
+ * signal.  At least one must be given. */
+int spx_omp_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D,
+                int p, const int* L, int nL, const float* eps, int n_eps,
+                const float* lambda, int n_lambda, const spx_omp_opts* opts,
+                spx_csc* out);
+int spx_omp_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D,
+                int p, const int* L, int nL, const double* eps, int n_eps,
+                const double* lambda, int n_lambda, const spx_omp_opts* opts,
+                spx_csc* out);
+
+/* FISTA-flat, square loss + l1 / elastic-net: replaces _fistaFlat
+ * (cpp_library/spams.h:460-608).  W (p x n) is written; optim_info is 4 x n
+ * (loss, best dual, relative gap, iterations; prox/fista.h:2994-3006). */
+int spx_fista_flat_f32(spx_ctx* ctx, const float* X, int m, int n,
+                       const float* D, int p, const float* W0, float* W,
+                       const spx_fista_opts* opts, float* optim_info);
+int spx_fista_flat_f64(spx_ctx* ctx, const double* X, int m, int n,
+                       const double* D, int p, const double* W0, double* W,
+                       const spx_fista_opts* opts, double* optim_info);
+
+void spx_csc_free(spx_csc* csc);
+
+/* ----------------------------------------------------- device entry points */
+
+/* Per-signal solver outputs on the device, the L x n (value, index) lists of
+ * the reference (vM/rM in lasso2/omp, decomp/decomp.h:1578-1583, :337-339):
+ * column i holds signal i's active set, index -1 terminated, in active-set
+ * order.  `count` (n ints) receives each signal's list length. */
+typedef struct {
+  void* v;     /* T[L x n] */
+  int* r;      /* int[L x n] */
+  int* count;  /* int[n] */
+  int L;       /* leading dimension of v/r (see spx_lasso_L / spx_omp_L) */
+  int* stats;  /* optional int[2 x n]: homotopy steps, atom admissions */
+} spx_lists_dev;
+
+/* Effective L after the facade's clamping (cpp_library/spams.h:117-131 and
+ * decomp/decomp.h:333-335). */
+int spx_lasso_L(int m, int p, int L, int mode, double lambda1, int pos,
+                double lambda2);
+int spx_omp_L(int m, int p, const int* L, int nL, int n);
+
+/* Device LARS over a resident dictionary.  X: device m x n (ldx >= m);
+ * lambda: device array of n_lambda (1 or n).  Asynchronous on ctx's stream. */
+int spx_lasso_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* X,
+                      int ldx, int n, const float* lambda, int n_lambda,
+                      float lambda2, const spx_lasso_opts* opts,
+                      spx_lists_dev* out);
+int spx_lasso_dev_f64(spx_ctx* ctx, const spx_dict* dict, const double* X,
+                      int ldx, int n, const double* lambda, int n_lambda,
+                      double lambda2, const spx_lasso_opts* opts,
+                      spx_lists_dev* out);
+
+/* Device ORMP over a resident dictionary.  L/eps/lambda are device arrays
+ * (NULL = not given; size 1 or n). */
+int spx_omp_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* X,
+                    int ldx, int n, const int* L, int nL, const float* eps,
+                    int n_eps, const float* lambda, int n_lambda,
+                    const spx_omp_opts* opts, spx_lists_dev* out);
+int spx_omp_dev_f64(spx_ctx* ctx, const spx_dict* dict, const double* X,
+                    int ldx, int n, const int* L, int nL, const double* eps,
+                    int n_eps, const double* lambda, int n_lambda,
+                    const spx_omp_opts* opts, spx_lists_dev* out);
+
+/* Device FISTA-flat over a resident dictionary: X m x n, W0/W p x n
+ * (device, leading dimension p), optim_info 4 x n (device). */
+int spx_fista_flat_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* X,
+                           int n, const float* W0, float* W,
+                           const spx_fista_opts* opts, float* optim_info);
+int spx_fista_flat_dev_f64(spx_ctx* ctx, const spx_dict* dict,
+                           const double* X, int n, const double* W0, double* W,
+                           const spx_fista_opts* opts, double* optim_info);
+
+/* Device CSC assembly from solver lists (SpMatrix::convert + per-column sort,
+ * linalg/linalg.h:4840-4862): pB (n+1), r and v (capacity >= total count)
+ * are device arrays; *nnz receives the total (host value, synchronous). */
+int spx_lists_to_csc_dev_f32(spx_ctx* ctx, const spx_lists_dev* lists, int n,
+                             int* pB, int* r, float* v, long long* nnz);
+int spx_lists_to_csc_dev_f64(spx_ctx* ctx, const spx_lists_dev* lists, int n,
+                             int* pB, int* r, double* v, long long* nnz);
+
+/* Products on a resident dictionary (exposed for tests and for callers that
+ * need them): DtX = D'X (p x n, leading dimension ldc), norms[i] = ||x_i||^2. */
+int spx_dtx_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* X,
+                    int ldx, int n, float* DtX, int ldc, int precision);
+int spx_dtx_dev_f64(spx_ctx* ctx, const spx_dict* dict, const double* X,
+                    int ldx, int n, double* DtX, int ldc);
+/* Copies the resident Gram matrix (p x p, leading dimension p) to `G`
+ * (device pointer). */
+int spx_dict_gram_f32(spx_ctx* ctx, const spx_dict* dict, float* G);
+int spx_dict_gram_f64(spx_ctx* ctx, const spx_dict* dict, double* G);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPX_H */

@@ -1,0 +1,760 @@
+// C ABI of the engine (declared in include/spx.h).
+//
+// Host entry points follow the reference facade (cpp_library/spams.h):
+// argument normalisation identical to _lassoD / _omp / _fistaFlat, then
+// upload, device products (D'X, ||x||^2) on MFMA, the solver kernel, device
+// CSC assembly and download.  No CPU arithmetic path exists: every result is
+// produced by the HIP kernels of this library.
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+
+#include "spx_common.h"
+#include "spx_internal.h"
+
+namespace spx {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+namespace {
+
+// Makes ctx's device current for the lifetime of the guard.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = true;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev)
+      (void)hipSetDevice(prev);
+  }
+};
+
+__global__ void zero_ints_kernel(int* p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <typename T>
+__global__ void transpose_kernel(const T* __restrict__ A, int m, int n,
+                                 T* __restrict__ B) {
+  // B (n x m) = A' for A m x n, both column-major
+  __shared__ T tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int i = bx + tx, j = by + k;
+    tile[k][tx] = (i < m && j < n) ? A[(size_t)i + (size_t)j * m] : T(0);
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int j = by + tx, i = bx + k;
+    if (i < m && j < n) B[(size_t)j + (size_t)i * n] = tile[tx][k];
+  }
+}
+
+}  // namespace
+
+// Clears n ints with a kernel (capture-safe replacement for a memset node).
+int zero_ints(hipStream_t st, int* p, int n) {
+  if (n <= 0) return SPX_OK;
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+int tuned_waves(Ctx* ctx, int solver, int p, int elem_bytes) {
+  const long key = (long)solver * 1000000000L + (long)elem_bytes * 100000000L + p;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (auto& kv : ctx->tuned)
+      if (kv.first == key) return kv.second;
+  }
+  // default before tuning: one wavefront per signal while a wavefront holds
+  // at most 16 chunks of 16 bytes per lane (p <= 4096 in FP32)
+  const int per_wave = 64 * 16 * (16 / elem_bytes);
+  if (p <= per_wave) return 1;
+  if (p <= 4 * per_wave) return 4;
+  return 16;
+}
+
+template <typename T>
+static int dict_create(spx_ctx* ctx, const T* D, int m, int p, int on_device,
+                       int precision, spx_dict** out) {
+  if (!ctx || !out || !D || m <= 0 || p <= 0) {
+    set_error("dict_create: invalid arguments (m=%d, p=%d)", m, p);
+    return SPX_ERR_ARG;
+  }
+  DeviceGuard g(ctx->device);
+  spx_dict* d = new spx_dict();
+  d->ctx = ctx;
+  d->m = m;
+  d->p = p;
+  d->ldg = pad_ld(p);
+  d->elem = sizeof(T);
+  d->precision = precision;
+  hipStream_t st = ctx->stream;
+  const size_t nD = (size_t)m * p;
+  if (hipMalloc(&d->D, nD * sizeof(T)) != hipSuccess ||
+      hipMalloc(&d->Dt, nD * sizeof(T)) != hipSuccess ||
+      hipMalloc(&d->G, (size_t)d->ldg * p * sizeof(T)) != hipSuccess) {
+    spx_dict_destroy(d);
+    set_error("dict_create: device allocation failed (p=%d)", p);
+    return SPX_ERR_NOMEM;
+  }
+  SPX_HIP(hipMemcpyAsync(d->D, D, nD * sizeof(T),
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  dim3 tg((m + 31) / 32, (p + 31) / 32);
+  hipLaunchKernelGGL(transpose_kernel<T>, tg, dim3(256), 0, st,
+                     static_cast<const T*>(d->D), m, p, static_cast<T*>(d->Dt));
+  SPX_HIP(hipGetLastError());
+  // zero the padding rows once, then G = D'D on MFMA (Matrix::XtX)
+  SPX_HIP(hipMemsetAsync(d->G, 0, (size_t)d->ldg * p * sizeof(T), st));
+  int rc = gemm_tn<T>(st, p, p, m, static_cast<const T*>(d->D), m,
+                      static_cast<const T*>(d->D), m, static_cast<T*>(d->G),
+                      d->ldg, T(1), T(0), nullptr, 0, precision);
+  if (rc) {
+    spx_dict_destroy(d);
+    return rc;
+  }
+  SPX_HIP(hipStreamSynchronize(st));
+  *out = d;
+  return SPX_OK;
+}
+
+// D'X and ||x||^2 for n signals into context scratch; returns pointers.
+template <typename T>
+static int products(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
+                    int precision, T** DtX, T** norms) {
+  void* b1 = nullptr;
+  void* b2 = nullptr;
+  int rc = ctx->scratch_get(3, sizeof(T) * (size_t)d->ldg * (n > 0 ? n : 1), &b1);
+  if (rc) return rc;
+  rc = ctx->scratch_get(4, sizeof(T) * (size_t)(n > 0 ? n : 1), &b2);
+  if (rc) return rc;
+  *DtX = static_cast<T*>(b1);
+  *norms = static_cast<T*>(b2);
+  rc = gemm_tn<T>(ctx->stream, d->p, n, d->m, static_cast<const T*>(d->D), d->m,
+                  X, ldx, *DtX, d->ldg, T(1), T(0), nullptr, 0, precision);
+  if (rc) return rc;
+  return col_sqnorms<T>(ctx->stream, d->m, n, X, ldx, *norms);
+}
+
+template <typename T>
+static int lasso_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx,
+                     int n, const T* lambda, int nlam, T lambda2,
+                     const spx_lasso_opts* o, spx_lists_dev* out) {
+  if (!ctx || !d || !o || !out || n < 0 || (n > 0 && (!X || !lambda)) ||
+      ldx < d->m || !(nlam == 1 || nlam == n)) {
+    set_error("lasso_dev: invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  if ((int)sizeof(T) != d->elem) {
+    set_error("lasso_dev: dictionary precision does not match the entry point");
+    return SPX_ERR_ARG;
+  }
+  if (o->mode < 0 || o->mode > 2) {
+    set_error("lasso: unsupported mode %d", o->mode);
+    return SPX_ERR_UNSUP;
+  }
+  if (n == 0) return SPX_OK;
+  DeviceGuard g(ctx->device);
+  T* DtX = nullptr;
+  T* norms = nullptr;
+  int rc = products<T>(ctx, d, X, ldx, n, o->precision, &DtX, &norms);
+  if (rc) return rc;
+  LarsArgs<T> a{};
+  a.G = static_cast<const T*>(d->G);
+  a.ldg = d->ldg;
+  a.diag = static_cast<T>(lambda2 > T(1e-10) ? double(lambda2) : 1e-10);
+  a.DtX = DtX;
+  a.ldd = d->ldg;
+  a.norms = norms;
+  a.lambda = lambda;
+  a.nlam = nlam;
+  a.p = d->p;
+  a.n = n;
+  a.L = out->L;
+  a.mode = o->mode;
+  a.pos = o->pos;
+  int Lp = o->L < 0 ? d->p : o->L;
+  a.length_path = o->max_length_path < 0 ? 4 * Lp : o->max_length_path;
+  a.outV = static_cast<T*>(out->v);
+  a.outR = out->r;
+  a.outCount = out->count;
+  a.stats = out->stats;
+  if (a.L <= 0) {
+    set_error("lasso_dev: lists have L=%d", a.L);
+    return SPX_ERR_ARG;
+  }
+  return lars_solve<T>(ctx, a, o->form, o->waves_per_signal);
+}
+
+template <typename T>
+static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
+                   const int* L, int nL, const T* eps, int nEps,
+                   const T* lambda, int nLambda, const spx_omp_opts* o,
+                   spx_lists_dev* out) {
+  if (!ctx || !d || !out || n < 0 || ldx < d->m) {
+    set_error("omp_dev: invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  if ((int)sizeof(T) != d->elem) {
+    set_error("omp_dev: dictionary precision does not match the entry point");
+    return SPX_ERR_ARG;
+  }
+  if (!L) nL = 0;
+  if (!eps) nEps = 0;
+  if (!lambda) nLambda = 0;
+  if (nL == 0 && nEps == 0 && nLambda == 0) {
+    // cpp_library/spams.h:267-268
+    set_error("omp : You should either provide L, eps or lambda");
+    return SPX_ERR_ARG;
+  }
+  if ((nL > 1 && nL != n) || (nEps > 1 && nEps != n) || (nLambda > 1 && nLambda != n)) {
+    set_error("omp: per-signal vectors must have n entries");
+    return SPX_ERR_ARG;
+  }
+  if (n == 0 || out->L <= 0) return SPX_OK;
+  DeviceGuard g(ctx->device);
+  spx_omp_opts def{SPX_PREC_FP32, SPX_FORM_BATCHED, 0};
+  if (!o) o = &def;
+  T* DtX = nullptr;
+  T* norms = nullptr;
+  int rc = products<T>(ctx, d, X, ldx, n, o->precision, &DtX, &norms);
+  if (rc) return rc;
+  OmpArgs<T> a{};
+  a.G = static_cast<const T*>(d->G);
+  a.ldg = d->ldg;
+  a.DtX = DtX;
+  a.ldd = d->ldg;
+  a.norms = norms;
+  a.Ls = L;
+  a.nL = nL;
+  a.eps = eps;
+  a.nEps = nEps;
+  a.lambda = lambda;
+  a.nLambda = nLambda;
+  a.scalarL = d->m < d->p ? d->m : d->p;
+  a.p = d->p;
+  a.n = n;
+  a.L = out->L;
+  a.outV = static_cast<T*>(out->v);
+  a.outR = out->r;
+  a.outCount = out->count;
+  return omp_solve<T>(ctx, a, o->form, o->waves_per_signal);
+}
+
+// Device CSC of the lists, then download into a host spx_csc.
+template <typename T>
+static int lists_to_host_csc(spx_ctx* ctx, const spx_lists_dev& l, int p, int n,
+                             spx_csc* out) {
+  hipStream_t st = ctx->stream;
+  int* dpB = nullptr;
+  void* b = nullptr;
+  int rc = ctx->scratch_get(5, sizeof(int) * (size_t)(n + 1), &b);
+  if (rc) return rc;
+  dpB = static_cast<int*>(b);
+  long long nnz = 0;
+  rc = csc_count_scan<T>(ctx, l.count, n, dpB, &nnz);
+  if (rc) return rc;
+  void* b2 = nullptr;
+  rc = ctx->scratch_get(6, (sizeof(int) + sizeof(T)) * (size_t)(nnz > 0 ? nnz : 1), &b2);
+  if (rc) return rc;
+  int* dr = static_cast<int*>(b2);
+  T* dv = reinterpret_cast<T*>(static_cast<char*>(b2) + sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  dv = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(dv) + 7) & ~uintptr_t(7));
+  rc = csc_fill<T>(ctx, static_cast<const T*>(l.v), l.r, l.L, n, dpB, dr, dv);
+  if (rc) return rc;
+  out->m = p;
+  out->n = n;
+  out->nnz = nnz;
+  out->pB = static_cast<int*>(malloc(sizeof(int) * (size_t)(n + 1)));
+  out->r = static_cast<int*>(malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1)));
+  out->v = malloc(sizeof(T) * (size_t)(nnz > 0 ? nnz : 1));
+  if (!out->pB || !out->r || !out->v) {
+    set_error("out of host memory");
+    return SPX_ERR_NOMEM;
+  }
+  SPX_HIP(hipMemcpyAsync(out->pB, dpB, sizeof(int) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
+  if (nnz > 0) {
+    SPX_HIP(hipMemcpyAsync(out->r, dr, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+    SPX_HIP(hipMemcpyAsync(out->v, dv, sizeof(T) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+  }
+  SPX_HIP(hipStreamSynchronize(st));
+  return SPX_OK;
+}
+
+// RAII device buffer for the host entry points.
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes) {
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+      p = nullptr;
+      set_error("device allocation of %zu bytes failed", bytes);
+      return SPX_ERR_NOMEM;
+    }
+    return SPX_OK;
+  }
+};
+
+template <typename T>
+static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
+                      const T* lambda, int nlam, T lambda2,
+                      const spx_lasso_opts* opts, spx_csc* out) {
+  if (!ctx || !out || !D || !lambda || m <= 0 || p <= 0 || n < 0 || (n > 0 && !X)) {
+    set_error("lasso : invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  if (!(nlam == 1 || nlam == n)) {
+    set_error("lasso : lambda must have 1 or n entries");
+    return SPX_ERR_ARG;
+  }
+  memset(out, 0, sizeof(*out));
+  spx_lasso_opts o{-1, SPX_PENALTY, 0, -1, SPX_PREC_FP32, SPX_FORM_BATCHED, 0};
+  if (opts) o = *opts;
+  const int Le = spx_lasso_L(m, p, o.L, o.mode, double(lambda[0]), o.pos, double(lambda2));
+  if (o.max_length_path < 0) o.max_length_path = 4 * (o.L < 0 ? p : o.L);
+  o.L = Le;
+  DeviceGuard g(ctx->device);
+  if (Le <= 0 || n == 0) {
+    // decomp/decomp.h:1587: L <= 0 -> empty result
+    out->m = p;
+    out->n = n;
+    out->nnz = 0;
+    out->pB = static_cast<int*>(calloc((size_t)n + 1, sizeof(int)));
+    out->r = static_cast<int*>(malloc(sizeof(int)));
+    out->v = malloc(sizeof(T));
+    return SPX_OK;
+  }
+  spx_dict* d = nullptr;
+  int rc = dict_create<T>(ctx, D, m, p, 0, o.precision, &d);
+  if (rc) return rc;
+  DevBuf bX, bL, bV, bR, bC;
+  if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bL.alloc(sizeof(T) * nlam)) ||
+      (rc = bV.alloc(sizeof(T) * (size_t)Le * n)) ||
+      (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) || (rc = bC.alloc(sizeof(int) * (size_t)n))) {
+    spx_dict_destroy(d);
+    return rc;
+  }
+  hipStream_t st = ctx->stream;
+  if (hipMemcpyAsync(bX.p, X, sizeof(T) * (size_t)m * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(bL.p, lambda, sizeof(T) * nlam, hipMemcpyHostToDevice, st) != hipSuccess) {
+    spx_dict_destroy(d);
+    set_error("lasso: upload failed");
+    return SPX_ERR_HIP;
+  }
+  spx_lists_dev l{bV.p, static_cast<int*>(bR.p), static_cast<int*>(bC.p), Le, nullptr};
+  rc = lasso_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
+                    static_cast<const T*>(bL.p), nlam, lambda2, &o, &l);
+  if (!rc) rc = lists_to_host_csc<T>(ctx, l, p, n, out);
+  spx_dict_destroy(d);
+  return rc;
+}
+
+template <typename T>
+static int omp_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
+                    const int* L, int nL, const T* eps, int nEps, const T* lambda,
+                    int nLambda, const spx_omp_opts* opts, spx_csc* out) {
+  if (!ctx || !out || !D || m <= 0 || p <= 0 || n < 0 || (n > 0 && !X)) {
+    set_error("omp : invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  memset(out, 0, sizeof(*out));
+  if (!L) nL = 0;
+  if (!eps) nEps = 0;
+  if (!lambda) nLambda = 0;
+  if (nL == 0 && nEps == 0 && nLambda == 0) {
+    set_error("omp : You should either provide L, eps or lambda");
+    return SPX_ERR_ARG;
+  }
+  if ((nL > 1 && nL != n) || (nEps > 1 && nEps != n) || (nLambda > 1 && nLambda != n)) {
+    set_error("omp : per-signal vectors must have n entries");
+    return SPX_ERR_ARG;
+  }
+  const int Le = spx_omp_L(m, p, L, nL, n);
+  DeviceGuard g(ctx->device);
+  if (Le <= 0 || n == 0) {
+    out->m = p;
+    out->n = n;
+    out->pB = static_cast<int*>(calloc((size_t)n + 1, sizeof(int)));
+    out->r = static_cast<int*>(malloc(sizeof(int)));
+    out->v = malloc(sizeof(T));
+    return SPX_OK;
+  }
+  spx_omp_opts o{SPX_PREC_FP32, SPX_FORM_BATCHED, 0};
+  if (opts) o = *opts;
+  spx_dict* d = nullptr;
+  int rc = dict_create<T>(ctx, D, m, p, 0, o.precision, &d);
+  if (rc) return rc;
+  DevBuf bX, bLs, bE, bLa, bV, bR, bC;
+  if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bLs.alloc(sizeof(int) * (nL ? nL : 1))) ||
+      (rc = bE.alloc(sizeof(T) * (nEps ? nEps : 1))) || (rc = bLa.alloc(sizeof(T) * (nLambda ? nLambda : 1))) ||
+      (rc = bV.alloc(sizeof(T) * (size_t)Le * n)) || (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) ||
+      (rc = bC.alloc(sizeof(int) * (size_t)n))) {
+    spx_dict_destroy(d);
+    return rc;
+  }
+  hipStream_t st = ctx->stream;
+  bool ok = hipMemcpyAsync(bX.p, X, sizeof(T) * (size_t)m * n, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (nL) ok = ok && hipMemcpyAsync(bLs.p, L, sizeof(int) * nL, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (nEps) ok = ok && hipMemcpyAsync(bE.p, eps, sizeof(T) * nEps, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (nLambda) ok = ok && hipMemcpyAsync(bLa.p, lambda, sizeof(T) * nLambda, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (!ok) {
+    spx_dict_destroy(d);
+    set_error("omp: upload failed");
+    return SPX_ERR_HIP;
+  }
+  spx_lists_dev l{bV.p, static_cast<int*>(bR.p), static_cast<int*>(bC.p), Le, nullptr};
+  rc = omp_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
+                  nL ? static_cast<const int*>(bLs.p) : nullptr, nL,
+                  nEps ? static_cast<const T*>(bE.p) : nullptr, nEps,
+                  nLambda ? static_cast<const T*>(bLa.p) : nullptr, nLambda, &o, &l);
+  if (!rc) rc = lists_to_host_csc<T>(ctx, l, p, n, out);
+  spx_dict_destroy(d);
+  return rc;
+}
+
+template <typename T>
+static int fista_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
+                      const T* W0, T* W, const spx_fista_opts* o, T* info) {
+  if (!ctx || !o || !D || !W0 || !W || !info || m <= 0 || p <= 0 || n < 0 || (n > 0 && !X)) {
+    set_error("fistaFlat : Argument sizes are not consistent");
+    return SPX_ERR_ARG;
+  }
+  if (o->regul != 0 && o->regul != 1) {
+    set_error("fistaFlat: Unknown regularization.");
+    return SPX_ERR_ARG;
+  }
+  if (n == 0) return SPX_OK;
+  DeviceGuard g(ctx->device);
+  spx_dict* d = nullptr;
+  int rc = dict_create<T>(ctx, D, m, p, 0, o->precision, &d);
+  if (rc) return rc;
+  DevBuf bX, bW0, bW, bI;
+  if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bW0.alloc(sizeof(T) * (size_t)p * n)) ||
+      (rc = bW.alloc(sizeof(T) * (size_t)p * n)) || (rc = bI.alloc(sizeof(T) * 4 * (size_t)n))) {
+    spx_dict_destroy(d);
+    return rc;
+  }
+  hipStream_t st = ctx->stream;
+  if (hipMemcpyAsync(bX.p, X, sizeof(T) * (size_t)m * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(bW0.p, W0, sizeof(T) * (size_t)p * n, hipMemcpyHostToDevice, st) != hipSuccess) {
+    spx_dict_destroy(d);
+    set_error("fistaFlat: upload failed");
+    return SPX_ERR_HIP;
+  }
+  rc = fista_run<T>(ctx, d, static_cast<const T*>(bX.p), n, static_cast<const T*>(bW0.p),
+                    static_cast<T*>(bW.p), o, static_cast<T*>(bI.p));
+  if (!rc) {
+    if (hipMemcpyAsync(W, bW.p, sizeof(T) * (size_t)p * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(info, bI.p, sizeof(T) * 4 * (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      set_error("fistaFlat: download failed");
+      rc = SPX_ERR_HIP;
+    }
+  }
+  spx_dict_destroy(d);
+  return rc;
+}
+
+}  // namespace spx
+
+// ------------------------------------------------------------------ ctx
+
+int spx_ctx::scratch_get(int id, size_t bytes, void** out) {
+  std::lock_guard<std::mutex> lk(mu);
+  if ((size_t)id >= slots.size()) slots.resize(id + 1);
+  Slot& s = slots[id];
+  if (s.bytes < bytes) {
+    if (s.ptr) {
+      if (hipStreamSynchronize(stream) != hipSuccess || hipFree(s.ptr) != hipSuccess) {
+        spx::set_error("scratch: release failed");
+        return SPX_ERR_HIP;
+      }
+      s.ptr = nullptr;
+      s.bytes = 0;
+    }
+    const size_t want = bytes + bytes / 8;
+    if (hipMalloc(&s.ptr, want) != hipSuccess) {
+      s.ptr = nullptr;
+      spx::set_error("scratch: device allocation of %zu bytes failed", want);
+      return SPX_ERR_NOMEM;
+    }
+    s.bytes = want;
+  }
+  *out = s.ptr;
+  return SPX_OK;
+}
+
+spx_ctx::~spx_ctx() {
+  for (auto& s : slots)
+    if (s.ptr) (void)hipFree(s.ptr);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+const char* spx_last_error(void) { return spx::g_err; }
+const char* spx_version(void) { return "spx 0.1 (gfx950)"; }
+
+int spx_ctx_create(spx_ctx** out, int device) {
+  if (!out) return SPX_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    spx::set_error("spx_ctx_create: no HIP device %d", device);
+    return SPX_ERR_HIP;
+  }
+  spx::DeviceGuard g(device);
+  spx_ctx* c = new spx_ctx();
+  c->device = device;
+  if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    spx::set_error("spx_ctx_create: device query / stream creation failed");
+    return SPX_ERR_HIP;
+  }
+  c->own_stream = true;
+  *out = c;
+  return SPX_OK;
+}
+
+void spx_ctx_destroy(spx_ctx* ctx) {
+  if (!ctx) return;
+  spx::DeviceGuard g(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+int spx_ctx_set_stream(spx_ctx* ctx, void* s) {
+  if (!ctx) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (s) {
+    ctx->stream = static_cast<hipStream_t>(s);
+    ctx->own_stream = false;
+  } else {
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SPX_ERR_HIP;
+    ctx->own_stream = true;
+  }
+  return SPX_OK;
+}
+
+int spx_ctx_synchronize(spx_ctx* ctx) {
+  if (!ctx) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    spx::set_error("spx_ctx_synchronize: %s", hipGetErrorString(hipGetLastError()));
+    return SPX_ERR_HIP;
+  }
+  return SPX_OK;
+}
+
+int spx_dict_create_f32(spx_ctx* ctx, const float* D, int m, int p, int on_device,
+                        int precision, spx_dict** out) {
+  return spx::dict_create<float>(ctx, D, m, p, on_device, precision, out);
+}
+int spx_dict_create_f64(spx_ctx* ctx, const double* D, int m, int p, int on_device,
+                        spx_dict** out) {
+  return spx::dict_create<double>(ctx, D, m, p, on_device, SPX_PREC_FP32, out);
+}
+
+void spx_dict_destroy(spx_dict* d) {
+  if (!d) return;
+  spx::DeviceGuard g(d->ctx ? d->ctx->device : 0);
+  if (d->ctx && d->ctx->stream) (void)hipStreamSynchronize(d->ctx->stream);
+  if (d->D) (void)hipFree(d->D);
+  if (d->Dt) (void)hipFree(d->Dt);
+  if (d->G) (void)hipFree(d->G);
+  delete d;
+}
+
+int spx_dict_dims(const spx_dict* d, int* m, int* p) {
+  if (!d) return SPX_ERR_ARG;
+  if (m) *m = d->m;
+  if (p) *p = d->p;
+  return SPX_OK;
+}
+
+size_t spx_dict_bytes(const spx_dict* d) {
+  if (!d) return 0;
+  return (size_t)d->elem * ((size_t)2 * d->m * d->p + (size_t)d->ldg * d->p);
+}
+
+int spx_lasso_L(int m, int p, int L, int mode, double lambda1, int pos, double lambda2) {
+  // cpp_library/spams.h:120-131
+  if (L < 0) L = p;
+  if (L > m && !(mode == SPX_PENALTY && lambda1 == 0 && !pos && lambda2 > 0)) L = m;
+  if (L > p) L = p;
+  return L;
+}
+
+int spx_omp_L(int m, int p, const int* L, int nL, int n) {
+  // cpp_library/spams.h:269-271, decomp/decomp.h:326-335
+  int Lv;
+  if (!L || nL == 0) Lv = m < p ? m : p;
+  else if (nL == 1) Lv = L[0];
+  else {
+    Lv = L[0];
+    for (int i = 1; i < n && i < nL; ++i) Lv = L[i] > Lv ? L[i] : Lv;
+  }
+  if (Lv <= 0) return 0;
+  if (Lv > p) Lv = p;
+  if (Lv > m) Lv = m;
+  return Lv;
+}
+
+int spx_lasso_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
+                  const float* lambda, int n_lambda, float lambda2,
+                  const spx_lasso_opts* opts, spx_csc* out) {
+  return spx::lasso_host<float>(ctx, X, m, n, D, p, lambda, n_lambda, lambda2, opts, out);
+}
+int spx_lasso_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, int p,
+                  const double* lambda, int n_lambda, double lambda2,
+                  const spx_lasso_opts* opts, spx_csc* out) {
+  return spx::lasso_host<double>(ctx, X, m, n, D, p, lambda, n_lambda, lambda2, opts, out);
+}
+
+int spx_omp_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
+                const int* L, int nL, const float* eps, int n_eps, const float* lambda,
+                int n_lambda, const spx_omp_opts* opts, spx_csc* out) {
+  return spx::omp_host<float>(ctx, X, m, n, D, p, L, nL, eps, n_eps, lambda, n_lambda, opts, out);
+}
+int spx_omp_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, int p,
+                const int* L, int nL, const double* eps, int n_eps, const double* lambda,
+                int n_lambda, const spx_omp_opts* opts, spx_csc* out) {
+  return spx::omp_host<double>(ctx, X, m, n, D, p, L, nL, eps, n_eps, lambda, n_lambda, opts, out);
+}
+
+int spx_fista_flat_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
+                       const float* W0, float* W, const spx_fista_opts* opts,
+                       float* optim_info) {
+  return spx::fista_host<float>(ctx, X, m, n, D, p, W0, W, opts, optim_info);
+}
+int spx_fista_flat_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, int p,
+                       const double* W0, double* W, const spx_fista_opts* opts,
+                       double* optim_info) {
+  return spx::fista_host<double>(ctx, X, m, n, D, p, W0, W, opts, optim_info);
+}
+
+void spx_csc_free(spx_csc* c) {
+  if (!c) return;
+  free(c->pB);
+  free(c->r);
+  free(c->v);
+  c->pB = nullptr;
+  c->r = nullptr;
+  c->v = nullptr;
+}
+
+int spx_lasso_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, int n,
+                      const float* lambda, int n_lambda, float lambda2,
+                      const spx_lasso_opts* o, spx_lists_dev* out) {
+  return spx::lasso_dev<float>(ctx, d, X, ldx, n, lambda, n_lambda, lambda2, o, out);
+}
+int spx_lasso_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx, int n,
+                      const double* lambda, int n_lambda, double lambda2,
+                      const spx_lasso_opts* o, spx_lists_dev* out) {
+  return spx::lasso_dev<double>(ctx, d, X, ldx, n, lambda, n_lambda, lambda2, o, out);
+}
+
+int spx_omp_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, int n,
+                    const int* L, int nL, const float* eps, int n_eps, const float* lambda,
+                    int n_lambda, const spx_omp_opts* o, spx_lists_dev* out) {
+  return spx::omp_dev<float>(ctx, d, X, ldx, n, L, nL, eps, n_eps, lambda, n_lambda, o, out);
+}
+int spx_omp_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx, int n,
+                    const int* L, int nL, const double* eps, int n_eps, const double* lambda,
+                    int n_lambda, const spx_omp_opts* o, spx_lists_dev* out) {
+  return spx::omp_dev<double>(ctx, d, X, ldx, n, L, nL, eps, n_eps, lambda, n_lambda, o, out);
+}
+
+int spx_fista_flat_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int n,
+                           const float* W0, float* W, const spx_fista_opts* o, float* info) {
+  if (!ctx || !d || !o || d->elem != 4) {
+    spx::set_error("fista_dev: invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  spx::DeviceGuard g(ctx->device);
+  return spx::fista_run<float>(ctx, d, X, n, W0, W, o, info);
+}
+int spx_fista_flat_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int n,
+                           const double* W0, double* W, const spx_fista_opts* o, double* info) {
+  if (!ctx || !d || !o || d->elem != 8) {
+    spx::set_error("fista_dev: invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  spx::DeviceGuard g(ctx->device);
+  return spx::fista_run<double>(ctx, d, X, n, W0, W, o, info);
+}
+
+int spx_lists_to_csc_dev_f32(spx_ctx* ctx, const spx_lists_dev* l, int n, int* pB, int* r,
+                             float* v, long long* nnz) {
+  if (!ctx || !l || !pB || !nnz) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  int rc = spx::csc_count_scan<float>(ctx, l->count, n, pB, nnz);
+  if (rc) return rc;
+  return spx::csc_fill<float>(ctx, static_cast<const float*>(l->v), l->r, l->L, n, pB, r, v);
+}
+int spx_lists_to_csc_dev_f64(spx_ctx* ctx, const spx_lists_dev* l, int n, int* pB, int* r,
+                             double* v, long long* nnz) {
+  if (!ctx || !l || !pB || !nnz) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  int rc = spx::csc_count_scan<double>(ctx, l->count, n, pB, nnz);
+  if (rc) return rc;
+  return spx::csc_fill<double>(ctx, static_cast<const double*>(l->v), l->r, l->L, n, pB, r, v);
+}
+
+int spx_dtx_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, int n,
+                    float* DtX, int ldc, int precision) {
+  if (!ctx || !d || d->elem != 4 || ldc < d->p || ldx < d->m) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  return spx::gemm_tn<float>(ctx->stream, d->p, n, d->m, static_cast<const float*>(d->D),
+                             d->m, X, ldx, DtX, ldc, 1.f, 0.f, nullptr, 0, precision);
+}
+int spx_dtx_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx, int n,
+                    double* DtX, int ldc) {
+  if (!ctx || !d || d->elem != 8 || ldc < d->p || ldx < d->m) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  return spx::gemm_tn<double>(ctx->stream, d->p, n, d->m, static_cast<const double*>(d->D),
+                              d->m, X, ldx, DtX, ldc, 1.0, 0.0, nullptr, 0, 0);
+}
+
+int spx_dict_gram_f32(spx_ctx* ctx, const spx_dict* d, float* G) {
+  if (!ctx || !d || d->elem != 4 || !G) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  if (hipMemcpy2DAsync(G, sizeof(float) * d->p, d->G, sizeof(float) * d->ldg,
+                       sizeof(float) * d->p, d->p, hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess) {
+    spx::set_error("dict_gram: copy failed");
+    return SPX_ERR_HIP;
+  }
+  return SPX_OK;
+}
+int spx_dict_gram_f64(spx_ctx* ctx, const spx_dict* d, double* G) {
+  if (!ctx || !d || d->elem != 8 || !G) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  if (hipMemcpy2DAsync(G, sizeof(double) * d->p, d->G, sizeof(double) * d->ldg,
+                       sizeof(double) * d->p, d->p, hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess) {
+    spx::set_error("dict_gram: copy failed");
+    return SPX_ERR_HIP;
+  }
+  return SPX_OK;
+}
+
+}  // extern "C"

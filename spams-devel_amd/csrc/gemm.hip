@@ -1,0 +1,300 @@
+// MFMA GEMMs of the engine: C = alpha * A' * B + beta * E.
+//
+// A is K x M and B is K x N, both column-major with K contiguous, so every
+// product the decomposition path needs is of this one form:
+//   D'X        (A = D,  m x p;  B = X,  m x n)      ProdMatrix DtX
+//   G = D'D    (A = B = D)                           Matrix::XtX (ssyrk)
+//   D*Y - X    (A = D', p x m; B = Y, p x n; E = X)  SqLoss::grad residual
+//   D'R        (A = D;  B = R)                       SqLoss::grad
+// (linalg/linalg.h:1751-1756, :1967-1972, prox/fista.h:390-402).
+//
+// Kernels:
+//   gemm_tn<float>   v_mfma_f32_16x16x4_f32   (FP32 in, FP32 accumulate)
+//   gemm_tn<double>  v_mfma_f64_16x16x4_f64   (FP64 in, FP64 accumulate)
+//   gemm_tn_bf16x3   v_mfma_f32_16x16x32_bf16 x3 on a hi/lo BF16 split of
+//                    both operands (a*b ~ ah*bh + ah*bl + al*bh), FP32
+//                    accumulate.
+// Block tile 128 x 128, four wavefronts in 2 x 2, each owning a 64 x 64
+// sub-tile (4 x 4 MFMA 16x16 blocks); K staged through LDS.
+#include "spx_common.h"
+#include "spx_internal.h"
+
+namespace spx {
+
+namespace {
+
+constexpr int BM = 128, BN = 128;
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<float> {
+  typedef f32x4 type;
+  static __device__ __forceinline__ f32x4 mfma(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Acc<double> {
+  typedef f64x4 type;
+  static __device__ __forceinline__ f64x4 mfma(double a, double b, f64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+// Writes one 16 x 16 accumulator block.  MFMA 16x16 output layout: lane l,
+// register r holds element (row 4*(l/16) + r, column l%16).
+template <typename T, typename V>
+__device__ __forceinline__ void store_block(const V& acc, int i0, int j0,
+                                            int M, int N, T* C, int ldc,
+                                            T alpha, T beta, const T* E,
+                                            int lde) {
+  const int lane = threadIdx.x & 63;
+  const int j = j0 + (lane & 15);
+  if (j >= N) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + 4 * (lane >> 4) + r;
+    if (i < M) {
+      T v = alpha * static_cast<T>(acc[r]);
+      if (E) v += beta * E[(size_t)i + (size_t)j * lde];
+      C[(size_t)i + (size_t)j * ldc] = v;
+    }
+  }
+}
+
+// ----------------------------------------------------------- FP32 / FP64
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_tn_kernel(
+    int M, int N, int K, const T* __restrict__ A, int lda,
+    const T* __restrict__ B, int ldb, T* __restrict__ C, int ldc, T alpha,
+    T beta, const T* __restrict__ E, int lde) {
+  constexpr int BK = 16;
+  constexpr int LDS_LD = BM + 4;  // [k][i] layout, padded
+  __shared__ T As[BK * LDS_LD];
+  __shared__ T Bs[BK * LDS_LD];
+  typedef typename Acc<T>::type V;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
+  const int wm = (wid & 1) * 64, wn = (wid >> 1) * 64;
+  const int bm = blockIdx.x * BM, bn = blockIdx.y * BN;
+
+  V acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
+
+  // loader mapping: thread t loads 8 consecutive k of row (t >> 1)
+  const int lr = tid >> 1, lk = (tid & 1) * 8;
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    {
+      const int i = bm + lr;
+      const T* src = A + (size_t)i * lda + k0 + lk;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + lk + e;
+        As[(lk + e) * LDS_LD + lr] = (i < M && k < K) ? src[e] : T(0);
+      }
+      const int j = bn + lr;
+      const T* srcb = B + (size_t)j * ldb + k0 + lk;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + lk + e;
+        Bs[(lk + e) * LDS_LD + lr] = (j < N && k < K) ? srcb[e] : T(0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      T a[4], b[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        a[x] = As[kr * LDS_LD + wm + 16 * x + (lane & 15)];
+        b[x] = Bs[kr * LDS_LD + wn + 16 * x + (lane & 15)];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = Acc<T>::mfma(a[x], b[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+      store_block<T>(acc[x][y], bm + wm + 16 * x, bn + wn + 16 * y, M, N, C,
+                     ldc, alpha, beta, E, lde);
+}
+
+// ------------------------------------------------------------- BF16 x 3
+
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+__global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
+    int M, int N, int K, const float* __restrict__ A, int lda,
+    const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
+    float alpha, float beta, const float* __restrict__ E, int lde) {
+  constexpr int BK = 32;
+  constexpr int LDS_LD = BK + 8;  // [i][k] layout (k contiguous), padded
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM * LDS_LD], Al[BM * LDS_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * LDS_LD], Bl[BN * LDS_LD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
+  const int wm = (wid & 1) * 64, wn = (wid >> 1) * 64;
+  const int bm = blockIdx.x * BM, bn = blockIdx.y * BN;
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
+
+  // loader: thread t loads 16 consecutive k of row (t >> 1)
+  const int lr = tid >> 1, lk = (tid & 1) * 16;
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    {
+      const int i = bm + lr;
+      const float* src = A + (size_t)i * lda + k0 + lk;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int k = k0 + lk + e;
+        float x = (i < M && k < K) ? src[e] : 0.f;
+        __bf16 h, l;
+        split_bf16(x, h, l);
+        Ah[lr * LDS_LD + lk + e] = h;
+        Al[lr * LDS_LD + lk + e] = l;
+      }
+      const int j = bn + lr;
+      const float* srcb = B + (size_t)j * ldb + k0 + lk;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int k = k0 + lk + e;
+        float x = (j < N && k < K) ? srcb[e] : 0.f;
+        __bf16 h, l;
+        split_bf16(x, h, l);
+        Bh[lr * LDS_LD + lk + e] = h;
+        Bl[lr * LDS_LD + lk + e] = l;
+      }
+    }
+    __syncthreads();
+    // 16x16x32 operand layout: lane l holds row (l % 16), k = 8*(l/16)+0..7
+    const int ko = 8 * (lane >> 4);
+    bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int ra = (wm + 16 * x + (lane & 15)) * LDS_LD + ko;
+      const int rb = (wn + 16 * x + (lane & 15)) * LDS_LD + ko;
+      ah[x] = *reinterpret_cast<const bf16x8*>(&Ah[ra]);
+      al[x] = *reinterpret_cast<const bf16x8*>(&Al[ra]);
+      bh[x] = *reinterpret_cast<const bf16x8*>(&Bh[rb]);
+      bl[x] = *reinterpret_cast<const bf16x8*>(&Bl[rb]);
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+      store_block<float>(acc[x][y], bm + wm + 16 * x, bn + wn + 16 * y, M, N,
+                         C, ldc, alpha, beta, E, lde);
+}
+
+// ----------------------------------------------------- column sq. norms
+
+template <typename T>
+__global__ __launch_bounds__(256) void col_sqnorm_kernel(
+    int m, int n, const T* __restrict__ X, int ldx, T* __restrict__ out) {
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (col >= n) return;
+  const T* x = X + (size_t)col * ldx;
+  T s = 0;
+  for (int i = lane; i < m; i += 64) s += x[i] * x[i];
+  s = wave_sum(s);
+  if (lane == 0) out[col] = s;
+}
+
+}  // namespace
+
+template <typename T>
+int gemm_tn(hipStream_t st, int M, int N, int K, const T* A, int lda,
+            const T* B, int ldb, T* C, int ldc, T alpha, T beta, const T* E,
+            int lde, int precision) {
+  if (M <= 0 || N <= 0) return SPX_OK;
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN);
+  if (grid.y > 65535) {
+    // split very wide products into column panels
+    const int panel = 65535 * BN;
+    for (int j0 = 0; j0 < N; j0 += panel) {
+      const int nb = (N - j0) < panel ? (N - j0) : panel;
+      int rc = gemm_tn<T>(st, M, nb, K, A, lda, B + (size_t)j0 * ldb, ldb,
+                          C + (size_t)j0 * ldc, ldc, alpha, beta,
+                          E ? E + (size_t)j0 * lde : nullptr, lde, precision);
+      if (rc) return rc;
+    }
+    return SPX_OK;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (precision == SPX_PREC_BF16X3) {
+      hipLaunchKernelGGL(gemm_tn_bf16x3_kernel, grid, dim3(256), 0, st, M, N,
+                         K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde);
+      SPX_HIP(hipGetLastError());
+      return SPX_OK;
+    }
+  }
+  hipLaunchKernelGGL(gemm_tn_kernel<T>, grid, dim3(256), 0, st, M, N, K, A,
+                     lda, B, ldb, C, ldc, alpha, beta, E, lde);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+template <typename T>
+int col_sqnorms(hipStream_t st, int m, int n, const T* X, int ldx, T* out) {
+  if (n <= 0) return SPX_OK;
+  hipLaunchKernelGGL(col_sqnorm_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st,
+                     m, n, X, ldx, out);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+template int gemm_tn<float>(hipStream_t, int, int, int, const float*, int,
+                            const float*, int, float*, int, float, float,
+                            const float*, int, int);
+template int gemm_tn<double>(hipStream_t, int, int, int, const double*, int,
+                             const double*, int, double*, int, double, double,
+                             const double*, int, int);
+template int col_sqnorms<float>(hipStream_t, int, int, const float*, int,
+                                float*);
+template int col_sqnorms<double>(hipStream_t, int, int, const double*, int,
+                                 double*);
+
+}  // namespace spx

@@ -1,0 +1,327 @@
+// ORMP ("omp") solver kernel: coreORMP semantics (decomp/decomp.h:538-640).
+//
+// One group of W wavefronts (one workgroup) per signal at a time.  Per-row
+// state (the residual correlations Rdn and the projected atom norms `norm`)
+// lives in registers of the owning thread; the selected-atom flags, RUn and
+// the selection order live in LDS; the inverse Cholesky factor Un (upper
+// triangular, L x L) lives in a per-workgroup HBM slot.
+//
+// The reference keeps Undn = Gs * Un (p x L) only to read row c of it when
+// atom c is selected (decomp/decomp.h:596).  That row equals
+// G(c, active) * Un, an O(j^2) product, so it is recomputed from the resident
+// Gram column instead of storing p x L per signal.  Each step streams the
+// j+1 selected Gram columns once to form Undn_j = G(:, active) * Un_j.
+#include <climits>
+
+#include "spx_common.h"
+#include "spx_internal.h"
+
+namespace spx {
+
+namespace {
+
+template <typename T>
+struct Vec16o;
+template <>
+struct Vec16o<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int N = 4;
+};
+template <>
+struct Vec16o<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int N = 2;
+};
+
+template <typename T, int W, int NQ>
+__global__ __launch_bounds__(64 * W) void ormp_kernel(OmpArgs<T> a) {
+  constexpr int GT = 64 * W;
+  constexpr int VEC = Vec16o<T>::N;
+  constexpr int PT = VEC * GT * NQ;
+  typedef typename Vec16o<T>::type V;
+  const T INF = dev_inf<T>();
+
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int L = a.L;
+  const int p = a.p;
+  const int ldg = a.ldg;
+  T* sRUn = reinterpret_cast<T*>(smem);  // L
+  T* sX = sRUn + L;                       // L  (new Un column / scratch)
+  T* sGc = sX + L;                        // L  (G(c, active))
+  T* sScal = sGc + L;                     // 4  (Rdn[c], norm[c])
+  int* sInd = reinterpret_cast<int*>(sScal + 4);  // L
+  unsigned char* sAct = reinterpret_cast<unsigned char*>(sInd + L);  // PT
+  KeyIdx<T>* sRed = reinterpret_cast<KeyIdx<T>*>(
+      (reinterpret_cast<uintptr_t>(sAct + PT) + 15) & ~uintptr_t(15));
+  int* sSig = reinterpret_cast<int*>(sRed + W);
+
+  const int tid = threadIdx.x;
+  T* Un = a.scratch + (size_t)blockIdx.x * L * L;  // column-major L x L
+
+  for (int round = 0;; ++round) {
+    int sig;
+    if (a.queue) {
+      __syncthreads();
+      if (tid == 0) *sSig = atomicAdd(a.queue, 1);
+      __syncthreads();
+      sig = *sSig;
+    } else {
+      sig = round == 0 ? (int)blockIdx.x : a.n;
+    }
+    if (sig >= a.n) break;
+
+    const T* dtx = a.DtX + (size_t)sig * a.ldd;
+    V Rdn[NQ], nrm[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int j0 = VEC * (tid + GT * q);
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        const int j = j0 + e;
+        Rdn[q][e] = j < p ? dtx[j] : T(0);
+        nrm[q][e] = T(1);
+        sAct[j] = j < p ? 0 : 1;
+      }
+    }
+    for (int j = tid; j < L; j += GT) sInd[j] = -1;
+    T normX = a.norms[sig];
+    const T eps = fabs(a.nEps == 0 ? T(0) : a.eps[a.nEps > 1 ? sig : 0]);
+    const T lambda = a.nLambda == 0 ? T(0) : a.lambda[a.nLambda > 1 ? sig : 0];
+    int Li = a.nL == 0 ? a.scalarL : a.Ls[a.nL > 1 ? sig : 0];
+    if (Li > L) Li = L;
+    __syncthreads();
+
+    int j = 0;
+    if (!((normX <= eps) || Li <= 0)) {
+      for (j = 0; j < Li; ++j) {
+        // ------------------------------------------ c = i?amax(scores)
+        T key = INF, pay = 0;
+        int idx = INT_MAX;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+            const int r = VEC * (tid + GT * q) + e;
+            if (r < p) {
+              // scores = Rdn before the first update, then Rdn^2 / norm with
+              // the selected atoms zeroed (decomp/decomp.h:550, :626-629)
+              T sc;
+              if (j == 0) sc = Rdn[q][e];
+              else sc = sAct[r] ? T(0) : (Rdn[q][e] * Rdn[q][e]) / nrm[q][e];
+              const T k = -fabs(sc);
+              if (key_better(k, r, key, idx)) {
+                key = k;
+                idx = r;
+              }
+            }
+          }
+        group_argmin<T, W>(key, idx, pay, sRed);
+        const int c = idx == INT_MAX ? 0 : idx;
+        // owner publishes Rdn[c], norm[c]
+        {
+          const int chunk = c / VEC;
+          if (chunk % GT == tid) {
+            const int qc = chunk / GT;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+              if (q == qc) {
+                sScal[0] = Rdn[q][c % VEC];
+                sScal[1] = nrm[q][c % VEC];
+              }
+          }
+        }
+        __syncthreads();
+        const T normc = sScal[1];
+        if (normc < T(1e-8)) {
+          if (tid == 0) sInd[j] = -1;
+          break;
+        }
+        const T invNorm = T(1) / sqrt(normc);
+        const T RU = sScal[0] * invNorm;
+        const T delta = RU * RU;
+        if (delta < 2 * lambda) break;
+        normX -= delta;
+        // G(c, active) for the Un column
+        const T* gcol = a.G + (size_t)c * ldg;
+        for (int k = tid; k < j; k += GT) sGc[k] = gcol[sInd[k]];
+        __syncthreads();
+        // Undn(c, k) = sum_{k' <= k} G(c, ind k') Un(k', k), k < j
+        for (int k = tid; k < j; k += GT) {
+          T acc = 0;
+          for (int kp = 0; kp <= k; ++kp) acc += sGc[kp] * Un[kp + (size_t)k * L];
+          sX[k] = acc;
+        }
+        __syncthreads();
+        // Un(:, j) = -invNorm * [Un(0:j,0:j) * Undn(c, 0:j); -1]
+        for (int r = tid; r < j; r += GT) {
+          T acc = 0;
+          for (int q = r; q < j; ++q) acc += Un[r + (size_t)q * L] * sX[q];
+          Un[r + (size_t)j * L] = -invNorm * acc;
+        }
+        if (tid == 0) {
+          Un[j + (size_t)j * L] = invNorm;
+          sRUn[j] = RU;
+          sInd[j] = c;
+          sAct[c] = 1;
+        }
+        __syncthreads();
+        if (j == Li - 1 || normX <= eps) {
+          ++j;
+          break;
+        }
+        // ----------------------- Undn_j = G(:, active) * Un(0:j+1, j)
+        V und[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) und[q] = V(0);
+        for (int k = 0; k <= j; ++k) {
+          const T uk = Un[k + (size_t)j * L];
+          const T* col = a.G + (size_t)sInd[k] * ldg;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const int j0 = VEC * (tid + GT * q);
+            if (j0 < ldg) {
+              const V g = *reinterpret_cast<const V*>(col + j0);
+              und[q] += uk * g;
+            }
+          }
+        }
+        // Rdn -= RU * Undn_j ; norm -= Undn_j^2
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          Rdn[q] -= RU * und[q];
+          nrm[q] -= und[q] * und[q];
+        }
+      }
+    }
+
+    // ------------------- coefficients: RUn := Un(0:j,0:j) * RUn (trmv)
+    __syncthreads();
+    for (int r = tid; r < j; r += GT) {
+      T acc = 0;
+      for (int q = r; q < j; ++q) acc += Un[r + (size_t)q * L] * sRUn[q];
+      sX[r] = acc;
+    }
+    __syncthreads();
+    T* ov = a.outV + (size_t)sig * L;
+    int* orr = a.outR + (size_t)sig * L;
+    for (int k = tid; k < L; k += GT) {
+      const int id = k < j ? sInd[k] : -1;
+      orr[k] = id;
+      ov[k] = k < j ? sX[k] : T(0);
+    }
+    if (tid == 0) {
+      int cnt = 0;
+      while (cnt < j && sInd[cnt] >= 0) ++cnt;
+      a.outCount[sig] = cnt;
+    }
+  }
+}
+
+template <typename T, int W, int NQ>
+size_t ormp_lds_bytes(int L) {
+  constexpr int GT = 64 * W;
+  constexpr int VEC = Vec16o<T>::N;
+  constexpr int PT = VEC * GT * NQ;
+  size_t b = sizeof(T) * (3 * (size_t)L + 4) + sizeof(int) * (size_t)L + PT;
+  b = (b + 15) & ~size_t(15);
+  b += sizeof(KeyIdx<T>) * W + 16;
+  return b;
+}
+
+template <typename T, int W, int NQ>
+int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
+  OmpArgs<T> a = a0;
+  const size_t lds = ormp_lds_bytes<T, W, NQ>(a.L);
+  auto kern = ormp_kernel<T, W, NQ>;
+  SPX_HIP(hipFuncSetAttribute((const void*)kern,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  const size_t slot = sizeof(T) * (size_t)a.L * a.L;
+  hipStream_t st = ctx->stream;
+  if (form == SPX_FORM_BATCHED) {
+    int per_cu = 0;
+    SPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds));
+    if (per_cu < 1) per_cu = 1;
+    long blocks = (long)per_cu * ctx->num_cus;
+    if (blocks > a.n) blocks = a.n;
+    if (blocks < 1) blocks = 1;
+    T* scratch = nullptr;
+    int* queue = nullptr;
+    int rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
+    if (rc) return rc;
+    rc = ctx->scratch_get(1, sizeof(int), reinterpret_cast<void**>(&queue));
+    if (rc) return rc;
+    rc = zero_ints(st, queue, 1);
+    if (rc) return rc;
+    a.scratch = scratch;
+    a.queue = queue;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, a);
+    SPX_HIP(hipGetLastError());
+    return SPX_OK;
+  }
+  long chunk = (long)(512ull * 1024 * 1024 / (slot ? slot : 1));
+  if (chunk > 65535) chunk = 65535;
+  if (chunk < 1) chunk = 1;
+  const long alloc_chunk = chunk < a0.n ? chunk : a0.n;
+  T* scratch = nullptr;
+  int rc = ctx->scratch_get(0, slot * (alloc_chunk > 0 ? alloc_chunk : 1),
+                            reinterpret_cast<void**>(&scratch));
+  if (rc) return rc;
+  for (long s0 = 0; s0 < a0.n; s0 += chunk) {
+    const long cnt = (a0.n - s0) < chunk ? (a0.n - s0) : chunk;
+    a = a0;
+    a.queue = nullptr;
+    a.scratch = scratch;
+    a.n = (int)cnt;
+    a.DtX = a0.DtX + (size_t)s0 * a0.ldd;
+    a.norms = a0.norms + s0;
+    if (a0.nL > 1) a.Ls = a0.Ls + s0;
+    if (a0.nEps > 1) a.eps = a0.eps + s0;
+    if (a0.nLambda > 1) a.lambda = a0.lambda + s0;
+    a.outV = a0.outV + (size_t)s0 * a0.L;
+    a.outR = a0.outR + (size_t)s0 * a0.L;
+    a.outCount = a0.outCount + s0;
+    hipLaunchKernelGGL(kern, dim3((unsigned)cnt), dim3(64 * W), lds, st, a);
+    SPX_HIP(hipGetLastError());
+  }
+  return SPX_OK;
+}
+
+template <typename T, int W>
+int ormp_dispatch_nq(Ctx* ctx, const OmpArgs<T>& a, int form) {
+  constexpr int VEC = Vec16o<T>::N;
+  const int per = VEC * 64 * W;
+  const int nq = (a.p + per - 1) / per;
+  if (nq <= 1) return ormp_run<T, W, 1>(ctx, a, form);
+  if (nq <= 2) return ormp_run<T, W, 2>(ctx, a, form);
+  if (nq <= 4) return ormp_run<T, W, 4>(ctx, a, form);
+  if (nq <= 8) return ormp_run<T, W, 8>(ctx, a, form);
+  if (nq <= 16) return ormp_run<T, W, 16>(ctx, a, form);
+  return -1;
+}
+
+}  // namespace
+
+template <typename T>
+int omp_solve(Ctx* ctx, const OmpArgs<T>& a, int form, int waves) {
+  if (waves <= 0) waves = form == SPX_FORM_PLAIN ? 4 : tuned_waves(ctx, 1, a.p, sizeof(T));
+  static const int widths[] = {1, 2, 4, 16};
+  for (int w : widths) {
+    if (w < waves) continue;
+    int rc;
+    switch (w) {
+      case 1: rc = ormp_dispatch_nq<T, 1>(ctx, a, form); break;
+      case 2: rc = ormp_dispatch_nq<T, 2>(ctx, a, form); break;
+      case 4: rc = ormp_dispatch_nq<T, 4>(ctx, a, form); break;
+      default: rc = ormp_dispatch_nq<T, 16>(ctx, a, form); break;
+    }
+    if (rc >= 0) return rc;
+  }
+  set_error("omp: p=%d exceeds the largest supported dictionary", a.p);
+  return SPX_ERR_UNSUP;
+}
+
+template int omp_solve<float>(Ctx*, const OmpArgs<float>&, int, int);
+template int omp_solve<double>(Ctx*, const OmpArgs<double>&, int, int);
+
+}  // namespace spx

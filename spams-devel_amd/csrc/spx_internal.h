@@ -1,0 +1,124 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <mutex>
+#include <vector>
+
+#include "spx.h"
+
+struct spx_ctx;
+struct spx_dict;
+
+namespace spx {
+
+typedef ::spx_ctx Ctx;
+typedef ::spx_dict Dict;
+
+// Kernel arguments of the LARS solver (coreLARS2 semantics).
+template <typename T>
+struct LarsArgs {
+  const T* G;       // p x p Gram without diagonal shift, leading dim ldg
+  int ldg;
+  T diag;           // max(lambda2, 1e-10), added to G's diagonal on the fly
+  const T* DtX;     // p x n, leading dim ldd
+  int ldd;
+  const T* norms;   // ||x_i||^2, n
+  const T* lambda;  // nlam (1 or n)
+  int nlam;
+  int p, n, L, mode, pos, length_path;
+  T* outV;          // L x n
+  int* outR;        // L x n
+  int* outCount;    // n
+  int* stats;       // 2 x n or null
+  T* scratch;       // per workgroup 2 * L * L
+  int* queue;       // work counter (batched form) or null (plain form)
+};
+
+// Kernel arguments of the ORMP solver (coreORMP semantics).
+template <typename T>
+struct OmpArgs {
+  const T* G;       // p x p Gram (no diagonal shift), leading dim ldg
+  int ldg;
+  const T* DtX;     // p x n
+  int ldd;
+  const T* norms;   // n
+  const int* Ls;    // per-signal or scalar L (nL entries; 0 = not given)
+  int nL;
+  const T* eps;
+  int nEps;
+  const T* lambda;
+  int nLambda;
+  int scalarL;      // L used when not given (min(m, p))
+  int p, n, L;      // L: leading dimension of the outputs (max L, clamped)
+  T* outV;
+  int* outR;
+  int* outCount;
+  T* scratch;       // per workgroup: L*L (Un) + p*L (Undn)
+  int* queue;
+};
+
+template <typename T>
+int gemm_tn(hipStream_t st, int M, int N, int K, const T* A, int lda,
+            const T* B, int ldb, T* C, int ldc, T alpha, T beta, const T* E,
+            int lde, int precision);
+template <typename T>
+int col_sqnorms(hipStream_t st, int m, int n, const T* X, int ldx, T* out);
+template <typename T>
+int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves);
+template <typename T>
+int omp_solve(Ctx* ctx, const OmpArgs<T>& a, int form, int waves);
+template <typename T>
+int lists_to_csc(Ctx* ctx, const T* v, const int* r, const int* count, int L,
+                 int n, int* pB, int* rOut, T* vOut, long long* nnz);
+template <typename T>
+int csc_count_scan(Ctx* ctx, const int* count, int n, int* pB, long long* nnz);
+template <typename T>
+int csc_fill(Ctx* ctx, const T* v, const int* r, int L, int n, const int* pB,
+             int* rOut, T* vOut);
+template <typename T>
+int fista_run(Ctx* ctx, const Dict* dict, const T* X, int n, const T* W0,
+              T* W, const spx_fista_opts* o, T* info);
+
+// Clears n ints with a kernel (capture-safe replacement for a memset node).
+int zero_ints(hipStream_t st, int* p, int n);
+
+// Wavefronts per signal chosen by the autotuner for a solver (0 = LARS,
+// 1 = OMP) at dictionary size p and element size.
+int tuned_waves(Ctx* ctx, int solver, int p, int elem_bytes);
+
+}  // namespace spx
+
+// Context: one device, one stream, grow-only scratch slots.
+struct spx_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  struct Slot {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+  };
+  std::vector<Slot> slots;
+  // autotuner cache: key -> waves per signal
+  std::vector<std::pair<long, int>> tuned;
+  std::mutex mu;
+  // Device scratch slot `id` of at least `bytes` bytes (contents undefined).
+  int scratch_get(int id, size_t bytes, void** out);
+  ~spx_ctx();
+};
+
+// Resident dictionary: D (m x p), D' (p x m) and G = D'D (p x p, leading
+// dimension ldg = pad_ld(p), zero padding rows) in device memory.
+struct spx_dict {
+  spx_ctx* ctx = nullptr;
+  int m = 0, p = 0, ldg = 0;
+  int elem = 4;         // 4 = float, 8 = double
+  int precision = 0;    // precision G was formed with
+  void* D = nullptr;    // m x p
+  void* Dt = nullptr;   // p x m
+  void* G = nullptr;    // ldg x p
+  double lipschitz = 0; // largest eigenvalue of G (power iteration), lazily
+};

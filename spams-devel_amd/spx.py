@@ -1,0 +1,446 @@
+"""Python host mirror of SPAMS's decomposition API over the spx C ABI.
+
+``lasso``, ``omp`` and ``fistaFlat`` keep the argument names, defaults,
+return types (scipy.sparse.csc_matrix / Fortran-ordered numpy arrays) and
+error behaviour (ValueError / RuntimeError) of the reference's
+``swig/python/spams.py`` (:109-142, :167-199, :260-293); the arithmetic runs
+in the HIP kernels of ``libspx.so`` (include/spx.h).  There is no CPU
+fallback: a missing library or a missing GPU raises.
+
+Device-resident use (inputs already in HBM, e.g. torch tensors) goes through
+``Context`` / ``Dictionary`` and the ``*_dev`` functions.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import scipy.sparse as ssp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspx.so")
+
+# constraint_type (decomp/decomp.h:95)
+L1COEFFS, L2ERROR, PENALTY = 0, 1, 2
+PREC = {"fp32": 0, "bf16x3": 1}
+FORM = {"batched": 0, "plain": 1}
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+
+class spx_csc(ctypes.Structure):
+    _fields_ = [("m", _I), ("n", _I), ("nnz", ctypes.c_longlong),
+                ("pB", ctypes.POINTER(_I)), ("r", ctypes.POINTER(_I)),
+                ("v", _P)]
+
+
+class spx_lasso_opts(ctypes.Structure):
+    _fields_ = [("L", _I), ("mode", _I), ("pos", _I), ("max_length_path", _I),
+                ("precision", _I), ("form", _I), ("waves_per_signal", _I)]
+
+
+class spx_omp_opts(ctypes.Structure):
+    _fields_ = [("precision", _I), ("form", _I), ("waves_per_signal", _I)]
+
+
+class spx_fista_opts(ctypes.Structure):
+    _fields_ = [("max_it", _I), ("L0", ctypes.c_double), ("fixed_step", _I),
+                ("gamma", ctypes.c_double), ("lambda1", ctypes.c_double),
+                ("lambda2", ctypes.c_double), ("tol", ctypes.c_double),
+                ("it0", _I), ("max_iter_backtracking", _I), ("regul", _I),
+                ("pos", _I), ("ista", _I), ("precision", _I)]
+
+
+class spx_lists_dev(ctypes.Structure):
+    _fields_ = [("v", _P), ("r", _P), ("count", _P), ("L", _I), ("stats", _P)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded engine library (raises when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "spx: %s is missing; build it with spams-devel_amd/build.py" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.spx_last_error.restype = ctypes.c_char_p
+        L.spx_version.restype = ctypes.c_char_p
+        L.spx_dict_bytes.restype = ctypes.c_size_t
+        L.spx_dict_bytes.argtypes = [_P]
+        L.spx_dict_destroy.argtypes = [_P]
+        L.spx_ctx_destroy.argtypes = [_P]
+        L.spx_csc_free.argtypes = [ctypes.POINTER(spx_csc)]
+        L.spx_lasso_L.argtypes = [_I, _I, _I, _I, ctypes.c_double, _I, ctypes.c_double]
+        L.spx_omp_L.argtypes = [_I, _I, _P, _I, _I]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().spx_last_error().decode(errors="replace")
+        if rc == 1:
+            raise ValueError(msg)
+        raise RuntimeError("spx error %d: %s" % (rc, msg))
+
+
+def _sfx(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return "f32"
+    if dt == np.float64:
+        return "f64"
+    raise ValueError("spx: only float32 and float64 are supported")
+
+
+def _ct(dtype):
+    return ctypes.c_float if np.dtype(dtype) == np.float32 else ctypes.c_double
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """An engine context on one HIP device (own stream, scratch pool)."""
+
+    def __init__(self, device: int = 0):
+        self.h = _P()
+        _check(lib().spx_ctx_create(ctypes.byref(self.h), device))
+        self.device = device
+
+    def set_stream(self, stream_handle: int | None) -> None:
+        lib().spx_ctx_set_stream.argtypes = [_P, _P]
+        _check(lib().spx_ctx_set_stream(self.h, stream_handle))
+
+    def synchronize(self) -> None:
+        lib().spx_ctx_synchronize.argtypes = [_P]
+        _check(lib().spx_ctx_synchronize(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            lib().spx_ctx_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Dictionary:
+    """A resident dictionary (D, D' and G = D'D in HBM).
+
+    ``D`` is a numpy array (m x p, any order) or a torch CUDA tensor holding
+    the column-major m x p matrix, i.e. shaped (p, m) and contiguous."""
+
+    def __init__(self, ctx: Context, D, precision: str = "fp32"):
+        self.ctx = ctx
+        self.h = _P()
+        if hasattr(D, "data_ptr"):
+            p, m = D.shape
+            dt = np.float32 if str(D.dtype) == "torch.float32" else np.float64
+            ptr, on_dev = ctypes.c_void_p(D.data_ptr()), 1
+            keep = D
+        else:
+            keep = np.asfortranarray(D)
+            m, p = keep.shape
+            dt = keep.dtype
+            ptr, on_dev = _ptr(keep), 0
+        self.m, self.p, self.dtype = m, p, np.dtype(dt)
+        if self.dtype == np.float32:
+            f = lib().spx_dict_create_f32
+            f.argtypes = [_P, _P, _I, _I, _I, _I, _P]
+            _check(f(ctx.h, ptr, m, p, on_dev, PREC[precision], ctypes.byref(self.h)))
+        else:
+            f = lib().spx_dict_create_f64
+            f.argtypes = [_P, _P, _I, _I, _I, _P]
+            _check(f(ctx.h, ptr, m, p, on_dev, ctypes.byref(self.h)))
+        del keep
+
+    @property
+    def nbytes(self) -> int:
+        return int(lib().spx_dict_bytes(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            lib().spx_dict_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _csc_to_scipy(c: spx_csc, dtype) -> ssp.csc_matrix:
+    n, nnz = c.n, c.nnz
+    pB = np.ctypeslib.as_array(c.pB, shape=(n + 1,)).copy()
+    if nnz > 0:
+        r = np.ctypeslib.as_array(c.r, shape=(nnz,)).copy()
+        vp = ctypes.cast(c.v, ctypes.POINTER(_ct(dtype)))
+        v = np.ctypeslib.as_array(vp, shape=(nnz,)).copy()
+    else:
+        r = np.zeros(0, np.int32)
+        v = np.zeros(0, dtype)
+    lib().spx_csc_free(ctypes.byref(c))
+    return ssp.csc_matrix((v, r, pB), shape=(c.m, n))
+
+
+# --------------------------------------------------------------- host API
+
+def lasso(X, D=None, Q=None, q=None, return_reg_path=False, L=-1,
+          lambda1=None, lambda2=0., mode=PENALTY, pos=False, ols=False,
+          numThreads=-1, max_length_path=-1, verbose=False, cholesky=False,
+          precision="fp32", form="batched", waves_per_signal=0, ctx=None):
+    """LARS-Lasso (spams.lasso, swig/python/spams.py:109-142).
+
+    ``lambda1`` is the reference's scalar constraint, or a length-n vector
+    (one constraint per signal).  ``cholesky=True`` solves the same problem
+    with the same homotopy (the engine keeps an explicit inverse of the active
+    Gram block).  ``numThreads`` is accepted and ignored (no CPU threads)."""
+    if Q is not None:
+        if q is None:
+            raise ValueError("lasso : q is needed when Q is given")
+        raise NotImplementedError("lasso: the (Q, q) form is not provided by spx")
+    if D is None:
+        raise ValueError("lasso : you must give D or Q and q")
+    if lambda1 is None:
+        raise ValueError("lasso : lambda1 must be defined")
+    if return_reg_path:
+        raise NotImplementedError("lasso: return_reg_path is not provided by spx")
+    if ols:
+        raise NotImplementedError("lasso: ols=True is not provided by spx")
+    dtype = np.asarray(X).dtype
+    sfx = _sfx(dtype)
+    X = np.asfortranarray(X, dtype=dtype)
+    D = np.asfortranarray(D, dtype=dtype)
+    m, n = X.shape
+    if D.shape[0] != m:
+        raise ValueError("lasso : incompatible matrix dimensions")
+    p = D.shape[1]
+    lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lambda1, dtype=dtype)))
+    if lam.size not in (1, n):
+        raise ValueError("lasso : lambda1 must be a scalar or have one entry per signal")
+    o = spx_lasso_opts(L, mode, int(pos), max_length_path, PREC[precision],
+                       FORM[form], waves_per_signal)
+    out = spx_csc()
+    ctx = ctx or default_context()
+    f = getattr(lib(), "spx_lasso_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _I, _ct(dtype),
+                  ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_csc)]
+    _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(lam), lam.size, lambda2,
+             ctypes.byref(o), ctypes.byref(out)))
+    return _csc_to_scipy(out, dtype)
+
+
+def omp(X, D, L=None, eps=None, lambda1=None, return_reg_path=False,
+        numThreads=-1, precision="fp32", form="batched", waves_per_signal=0,
+        ctx=None):
+    """Cholesky-based ORMP (spams.omp, swig/python/spams.py:167-199)."""
+    if return_reg_path:
+        raise NotImplementedError("omp: return_reg_path is not provided by spx")
+    dtype = np.asarray(X).dtype
+    sfx = _sfx(dtype)
+    X = np.asfortranarray(X, dtype=dtype)
+    D = np.asfortranarray(D, dtype=dtype)
+    m, n = X.shape
+    if D.shape[0] != m:
+        raise ValueError("omp : incompatible matrix dimensions")
+    p = D.shape[1]
+
+    def vec(a, dt):
+        if a is None:
+            return None, 0
+        a = np.ascontiguousarray(np.atleast_1d(np.asarray(a, dtype=dt)))
+        return a, a.size
+
+    Ls, nL = vec(L, np.int32)
+    E, nE = vec(eps, dtype)
+    La, nLa = vec(lambda1, dtype)
+    o = spx_omp_opts(PREC[precision], FORM[form], waves_per_signal)
+    out = spx_csc()
+    ctx = ctx or default_context()
+    f = getattr(lib(), "spx_omp_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _I, _P, _I, _P, _I,
+                  ctypes.POINTER(spx_omp_opts), ctypes.POINTER(spx_csc)]
+    _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(Ls), nL, _ptr(E), nE,
+             _ptr(La), nLa, ctypes.byref(o), ctypes.byref(out)))
+    return _csc_to_scipy(out, dtype)
+
+
+def fistaFlat(Y, X, W0, return_optim_info=False, numThreads=-1, max_it=1000,
+              L0=1.0, fixed_step=False, gamma=1.5, lambda1=1.0, delta=1.0,
+              lambda2=0., lambda3=0., a=1.0, b=0., c=1.0, tol=0.000001,
+              it0=100, max_iter_backtracking=1000, compute_gram=False,
+              lin_admm=False, admm=False, intercept=False, resetflow=False,
+              regul="", loss="", verbose=False, pos=False, clever=False,
+              log=False, ista=False, subgrad=False, logName="",
+              is_inner_weights=False, inner_weights=None, size_group=1,
+              groups=None, sqrt_step=True, transpose=False,
+              linesearch_mode=0, precision="fp32", ctx=None):
+    """FISTA-flat (spams.fistaFlat, swig/python/spams.py:260-293) for
+    loss='square' with regul='l1' or 'elastic-net'.  As in the reference, Y
+    holds the signals (m x n), X the dictionary (m x p) and W0 the start
+    (p x n)."""
+    if loss != "square":
+        raise ValueError("fistaFlat: spx provides the square loss only (got %r)" % loss)
+    regs = {"l1": 0, "elastic-net": 1}
+    if regul not in regs:
+        raise ValueError("fistaFlat: Unknown regularization %r (spx provides "
+                         "'l1' and 'elastic-net')" % regul)
+    if intercept or admm or lin_admm or subgrad or linesearch_mode != 0 or groups is not None:
+        raise NotImplementedError("fistaFlat: option not provided by spx")
+    dtype = np.asarray(Y).dtype
+    sfx = _sfx(dtype)
+    Yf = np.asfortranarray(Y, dtype=dtype)
+    Xf = np.asfortranarray(X, dtype=dtype)
+    W0f = np.asfortranarray(W0, dtype=dtype)
+    m, n = Yf.shape
+    p = Xf.shape[1]
+    if W0f.shape != (p, n) or Xf.shape[0] != m:
+        raise ValueError("fistaFlat : Argument sizes are not consistent")
+    W = np.zeros((p, n), dtype=dtype, order="F")
+    info = np.zeros((4, n), dtype=dtype, order="F")
+    o = spx_fista_opts(max_it, L0, int(fixed_step), gamma, lambda1, lambda2,
+                       tol, it0, max_iter_backtracking, regs[regul], int(pos),
+                       int(ista), PREC[precision])
+    ctx = ctx or default_context()
+    f = getattr(lib(), "spx_fista_flat_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _P,
+                  ctypes.POINTER(spx_fista_opts), _P]
+    _check(f(ctx.h, _ptr(Yf), m, n, _ptr(Xf), p, _ptr(W0f), _ptr(W),
+             ctypes.byref(o), _ptr(info)))
+    if return_optim_info:
+        return W, info
+    return W
+
+
+# ------------------------------------------------------------- device API
+
+def lasso_L(m, p, L=-1, mode=PENALTY, lambda1=1.0, pos=False, lambda2=0.0):
+    return lib().spx_lasso_L(m, p, L, mode, float(lambda1), int(pos), float(lambda2))
+
+
+def omp_L(m, p, L=None, n=1):
+    if L is None:
+        return lib().spx_omp_L(m, p, None, 0, n)
+    a = np.ascontiguousarray(np.atleast_1d(np.asarray(L, np.int32)))
+    return lib().spx_omp_L(m, p, _ptr(a), a.size, n)
+
+
+def _tptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def lasso_dev(ctx, d: Dictionary, X, lam, V, R, count, lambda2=0.0, L=-1,
+              mode=PENALTY, pos=False, max_length_path=-1, precision="fp32",
+              form="batched", waves_per_signal=0, stats=None):
+    """Device LARS: X torch (n, m) (column-major m x n), lam torch (1,) or
+    (n,), V/R torch (n, Lout) outputs, count torch (n,) int32."""
+    n = X.shape[0]
+    Lout = V.shape[1]
+    sfx = "f32" if d.dtype == np.float32 else "f64"
+    o = spx_lasso_opts(L, mode, int(pos), max_length_path, PREC[precision],
+                       FORM[form], waves_per_signal)
+    lists = spx_lists_dev(_tptr(V), _tptr(R), _tptr(count), Lout, _tptr(stats))
+    f = getattr(lib(), "spx_lasso_dev_" + sfx)
+    f.argtypes = [_P, _P, _P, _I, _I, _P, _I, _ct(d.dtype),
+                  ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_lists_dev)]
+    _check(f(ctx.h, d.h, _tptr(X), X.shape[1], n, _tptr(lam), lam.numel(),
+             lambda2, ctypes.byref(o), ctypes.byref(lists)))
+
+
+def omp_dev(ctx, d: Dictionary, X, V, R, count, L=None, eps=None, lambda1=None,
+            precision="fp32", form="batched", waves_per_signal=0):
+    """Device ORMP; L (int32), eps, lambda1: torch device tensors or None."""
+    n = X.shape[0]
+    Lout = V.shape[1]
+    sfx = "f32" if d.dtype == np.float32 else "f64"
+    o = spx_omp_opts(PREC[precision], FORM[form], waves_per_signal)
+    lists = spx_lists_dev(_tptr(V), _tptr(R), _tptr(count), Lout, None)
+    f = getattr(lib(), "spx_omp_dev_" + sfx)
+    f.argtypes = [_P, _P, _P, _I, _I, _P, _I, _P, _I, _P, _I,
+                  ctypes.POINTER(spx_omp_opts), ctypes.POINTER(spx_lists_dev)]
+    nn = lambda t: 0 if t is None else t.numel()
+    _check(f(ctx.h, d.h, _tptr(X), X.shape[1], n, _tptr(L), nn(L), _tptr(eps),
+             nn(eps), _tptr(lambda1), nn(lambda1), ctypes.byref(o),
+             ctypes.byref(lists)))
+
+
+def fista_flat_dev(ctx, d: Dictionary, X, W0, W, info, lambda1, regul="l1",
+                   lambda2=0.0, max_it=1000, L0=1.0, fixed_step=False,
+                   gamma=1.5, tol=1e-6, it0=100, max_iter_backtracking=1000,
+                   pos=False, ista=False, precision="fp32"):
+    """Device FISTA: X torch (n, m), W0/W torch (n, p), info torch (n, 4)."""
+    n = X.shape[0]
+    sfx = "f32" if d.dtype == np.float32 else "f64"
+    o = spx_fista_opts(max_it, L0, int(fixed_step), gamma, lambda1, lambda2,
+                       tol, it0, max_iter_backtracking,
+                       {"l1": 0, "elastic-net": 1}[regul], int(pos), int(ista),
+                       PREC[precision])
+    f = getattr(lib(), "spx_fista_flat_dev_" + sfx)
+    f.argtypes = [_P, _P, _P, _I, _P, _P, ctypes.POINTER(spx_fista_opts), _P]
+    _check(f(ctx.h, d.h, _tptr(X), n, _tptr(W0), _tptr(W), ctypes.byref(o),
+             _tptr(info)))
+
+
+def lists_to_csc_dev(ctx, V, R, count, pB, r, v):
+    """Device CSC assembly; returns nnz."""
+    n, Lout = V.shape
+    sfx = "f32" if str(V.dtype) == "torch.float32" else "f64"
+    lists = spx_lists_dev(_tptr(V), _tptr(R), _tptr(count), Lout, None)
+    nnz = ctypes.c_longlong()
+    f = getattr(lib(), "spx_lists_to_csc_dev_" + sfx)
+    f.argtypes = [_P, ctypes.POINTER(spx_lists_dev), _I, _P, _P, _P,
+                  ctypes.POINTER(ctypes.c_longlong)]
+    _check(f(ctx.h, ctypes.byref(lists), n, _tptr(pB), _tptr(r), _tptr(v),
+             ctypes.byref(nnz)))
+    return nnz.value
+
+
+def dtx_dev(ctx, d: Dictionary, X, out, precision="fp32"):
+    """out (n, ldc) torch = D'X (column-major p x n with leading dim ldc)."""
+    n = X.shape[0]
+    if d.dtype == np.float32:
+        f = lib().spx_dtx_dev_f32
+        f.argtypes = [_P, _P, _P, _I, _I, _P, _I, _I]
+        _check(f(ctx.h, d.h, _tptr(X), X.shape[1], n, _tptr(out), out.shape[1],
+                 PREC[precision]))
+    else:
+        f = lib().spx_dtx_dev_f64
+        f.argtypes = [_P, _P, _P, _I, _I, _P, _I]
+        _check(f(ctx.h, d.h, _tptr(X), X.shape[1], n, _tptr(out), out.shape[1]))
+
+
+def dict_gram(ctx, d: Dictionary, G):
+    """G torch (p, p) receives the resident Gram matrix (column-major)."""
+    sfx = "f32" if d.dtype == np.float32 else "f64"
+    f = getattr(lib(), "spx_dict_gram_" + sfx)
+    f.argtypes = [_P, _P, _P]
+    _check(f(ctx.h, d.h, _tptr(G)))
+
+
+def exported_symbols():
+    """Names declared in include/spx.h (for the load check)."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "spx.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(spx_[a-z0-9_]+)\s*\(", txt)))

@@ -1,0 +1,103 @@
+"""Writes the golden fixtures of the decomposition path.
+
+Runs the REFERENCE implementation compiled in place (oracle/_ref, built by
+oracle/Makefile from the reference's own headers) on small seeded inputs and
+stores inputs and outputs as .npz files next to this script.  Needs the
+reference tree at build time; the fixtures themselves are plain data.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "spams-devel_amd"))
+
+import oracle  # noqa: E402
+import workloads as W  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def csc_parts(a):
+    a = a.tocsc()
+    return a.indptr.astype(np.int32), a.indices.astype(np.int32), a.data
+
+
+def small(seed, m, p, n, k, noise, norm, dtype):
+    rng = np.random.default_rng(seed)
+    D = W.unit_dict(rng, m, p, dtype)
+    X = W.sparse_signals(rng, D, n, k, noise, norm, dtype)
+    return X, D
+
+
+def main():
+    oracle.build()
+    assert oracle.has_ref(), "reference build missing"
+    cases = []
+    for dt, tag in ((np.float32, "f32"), (np.float64, "f64")):
+        # LARS: shapes of the CPU case and a tiny one; all three modes
+        for (m, p, n, seed) in ((64, 256, 40, 11), (16, 32, 25, 12)):
+            X, D = small(seed, m, p, n, 5 if p > 32 else 3, 0.05, True, dt)
+            for mode, lam, pos, l2 in ((2, 0.15, 0, 0.0), (2, 0.05, 0, 0.0),
+                                       (2, 0.1, 1, 0.0), (2, 0.1, 0, 0.01),
+                                       (0, 0.8, 0, 0.0), (1, 0.3, 0, 0.0)):
+                A = oracle.ref_lasso(X, D, lam, lambda2=l2, mode=mode, pos=bool(pos))
+                pB, r, v = csc_parts(A)
+                name = f"lasso_{tag}_m{m}_p{p}_mode{mode}_lam{lam}_pos{pos}_l2{l2}"
+                cases.append((name, dict(kind="lasso", X=X, D=D, lam=np.asarray(lam, dt),
+                                         lambda2=np.asarray(l2, dt), mode=mode, pos=pos,
+                                         pB=pB, r=r, v=v)))
+        # per-signal lambda
+        X, D = small(13, 64, 256, 40, 5, 0.05, True, dt)
+        lam = np.exp(np.random.default_rng(14).uniform(np.log(0.05), np.log(0.5), 40)).astype(dt)
+        A = oracle.ref_lasso_pervec(X, D, lam)
+        pB, r, v = csc_parts(A)
+        cases.append((f"lasso_pervec_{tag}", dict(kind="lasso", X=X, D=D, lam=lam,
+                                                  lambda2=np.asarray(0.0, dt), mode=2, pos=0,
+                                                  pB=pB, r=r, v=v)))
+        # OMP
+        X, D = small(15, 64, 256, 40, np.arange(40) % 12 + 1, 0.01, False, dt)
+        for name, L, eps, lam in (("L8", 8, None, None), ("eps", None, 1e-3, None),
+                                  ("L32eps", 32, 1e-3, None), ("lam", None, None, 0.01),
+                                  ("vecL", (np.arange(40) % 7 + 1).astype(np.int32), None, None)):
+            A = oracle.ref_omp(X, D, L=L, eps=eps, lambda1=lam)
+            pB, r, v = csc_parts(A)
+            cases.append((f"omp_{tag}_{name}", dict(
+                kind="omp", X=X, D=D,
+                L=np.asarray(-1 if L is None else L, np.int32),
+                eps=np.asarray(np.nan if eps is None else eps, dt),
+                lam=np.asarray(np.nan if lam is None else lam, dt),
+                pB=pB, r=r, v=v)))
+        # FISTA
+        X, D = small(16, 32, 64, 12, 4, 0.02, False, dt)
+        W0 = np.zeros((64, 12), dt, order="F")
+        for regul, fixed in (("l1", 1), ("l1", 0), ("elastic-net", 1), ("elastic-net", 0)):
+            Wt, info = oracle.ref_fista_flat(X, D, W0, 0.05, regul=regul, lambda2=0.01,
+                                             max_it=200, L0=0.1, fixed_step=bool(fixed),
+                                             tol=1e-10, it0=10)
+            if fixed:
+                # fixed step at the Lipschitz constant
+                L0 = float(np.linalg.eigvalsh(D.astype(np.float64).T @ D)[-1])
+                Wt, info = oracle.ref_fista_flat(X, D, W0, 0.05, regul=regul, lambda2=0.01,
+                                                 max_it=200, L0=L0, fixed_step=True,
+                                                 tol=1e-10, it0=10)
+            else:
+                L0 = 0.1
+            cases.append((f"fista_{tag}_{regul}_fixed{fixed}", dict(
+                kind="fista", X=X, D=D, W0=W0, W=Wt, info=info, regul=regul,
+                lambda1=np.asarray(0.05, dt), lambda2=np.asarray(0.01, dt),
+                L0=np.asarray(L0, dt), fixed=fixed, max_it=200, it0=10,
+                tol=np.asarray(1e-10, dt))))
+    for name, d in cases:
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+    print(f"wrote {len(cases)} fixtures to {OUT}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,237 @@
+"""Pins the C restatement oracle (CPU only).
+
+Every oracle function is checked against
+  * the golden fixtures written by the reference compiled in place
+    (tests/golden, bit-exact),
+  * the in-place reference build itself when present (bit-exact, random
+    cases),
+  * closed forms (D = I), optimality conditions (lasso KKT, least-squares
+    orthogonality of OMP) and brute force on tiny problems,
+chosen so that a dropped term, a wrong sign or a transposed operand fails.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import csc_equal, golden_files, load_golden
+
+import workloads as W
+
+
+def dense(a):
+    return np.asarray(a.todense())
+
+
+# ------------------------------------------------------------ golden vectors
+
+@pytest.mark.parametrize("name", golden_files("lasso_"))
+def test_lasso_matches_golden(orc, name):
+    g = load_golden(name)
+    A = orc.lasso(g["X"], g["D"], g["lam"] if g["lam"].ndim else g["lam"].item(),
+                  lambda2=g["lambda2"].item(), mode=int(g["mode"]), pos=bool(g["pos"]))
+    import scipy.sparse as sp
+    ref = sp.csc_matrix((g["v"], g["r"], g["pB"]), shape=(g["D"].shape[1], g["X"].shape[1]))
+    assert csc_equal(A, ref)
+
+
+@pytest.mark.parametrize("name", golden_files("omp_"))
+def test_omp_matches_golden(orc, name):
+    g = load_golden(name)
+    L = None if int(np.atleast_1d(g["L"])[0]) < 0 else g["L"]
+    eps = None if np.isnan(g["eps"]) else g["eps"].item()
+    lam = None if np.isnan(g["lam"]) else g["lam"].item()
+    A = orc.omp(g["X"], g["D"], L=L, eps=eps, lambda1=lam)
+    import scipy.sparse as sp
+    ref = sp.csc_matrix((g["v"], g["r"], g["pB"]), shape=(g["D"].shape[1], g["X"].shape[1]))
+    assert csc_equal(A, ref)
+
+
+@pytest.mark.parametrize("name", golden_files("fista_"))
+def test_fista_matches_golden(orc, name):
+    g = load_golden(name)
+    Wt, info = orc.fista_flat(g["X"], g["D"], g["W0"], g["lambda1"].item(),
+                              regul=str(g["regul"]), lambda2=g["lambda2"].item(),
+                              max_it=int(g["max_it"]), L0=g["L0"].item(),
+                              fixed_step=bool(g["fixed"]), tol=g["tol"].item(),
+                              it0=int(g["it0"]))
+    assert np.array_equal(Wt, g["W"])
+    assert np.array_equal(info, g["info"], equal_nan=True)
+
+
+# ------------------------------------------------ reference build, random
+
+def _rand(seed, m, p, n, dt, k=4, noise=0.05, norm=True):
+    rng = np.random.default_rng(seed)
+    D = W.unit_dict(rng, m, p, dt)
+    X = W.sparse_signals(rng, D, n, min(k, p), noise, norm, dt)
+    return X, D
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_lasso_matches_reference_build(orc, dt, seed):
+    if not orc.has_ref():
+        pytest.skip("reference build absent")
+    m, p = [(20, 40), (32, 16), (48, 96)][seed]
+    X, D = _rand(100 + seed, m, p, 30, dt)
+    for mode, lam in ((2, 0.1), (0, 0.6), (1, 0.2)):
+        for pos in (False, True):
+            a = orc.lasso(X, D, lam, mode=mode, pos=pos)
+            b = orc.ref_lasso(X, D, lam, mode=mode, pos=pos)
+            assert csc_equal(a, b), (mode, pos)
+    lamv = np.linspace(0.02, 0.4, 30).astype(dt)
+    assert csc_equal(orc.lasso(X, D, lamv), orc.ref_lasso_pervec(X, D, lamv))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_omp_matches_reference_build(orc, dt):
+    if not orc.has_ref():
+        pytest.skip("reference build absent")
+    X, D = _rand(7, 24, 50, 30, dt, k=6, noise=0.01, norm=False)
+    for L, eps, lam in ((6, None, None), (None, 0.01, None), (10, 0.05, 0.001)):
+        assert csc_equal(orc.omp(X, D, L=L, eps=eps, lambda1=lam),
+                         orc.ref_omp(X, D, L=L, eps=eps, lambda1=lam))
+
+
+# ------------------------------------------------------------- closed forms
+
+def soft(x, t):
+    return np.sign(x) * np.maximum(np.abs(x) - t, 0.0)
+
+
+def test_lasso_identity_dictionary_is_soft_threshold(orc):
+    rng = np.random.default_rng(3)
+    p = 24
+    D = np.eye(p)
+    X = np.asfortranarray(rng.standard_normal((p, 15)))
+    for lam in (0.1, 0.5, 1.2):
+        A = dense(orc.lasso(X, D, lam))
+        # G = I + 1e-10 I (decomp/decomp.h:1569)
+        np.testing.assert_allclose(A, soft(X, lam) / (1 + 1e-10), rtol=0, atol=1e-12)
+        Apos = dense(orc.lasso(X, D, lam, pos=True))
+        np.testing.assert_allclose(Apos, np.maximum(X - lam, 0) / (1 + 1e-10), atol=1e-12)
+
+
+def test_omp_identity_dictionary_is_top_L(orc):
+    rng = np.random.default_rng(4)
+    p = 30
+    X = np.asfortranarray(rng.standard_normal((p, 20)))
+    A = dense(orc.omp(X, np.eye(p), L=5))
+    for j in range(20):
+        top = np.argsort(-np.abs(X[:, j]), kind="stable")[:5]
+        expect = np.zeros(p)
+        expect[top] = X[top, j]
+        np.testing.assert_allclose(A[:, j], expect, atol=1e-13)
+
+
+def test_fista_elastic_net_identity_closed_form(orc):
+    rng = np.random.default_rng(5)
+    p = 16
+    X = np.asfortranarray(rng.standard_normal((p, 6)))
+    W0 = np.zeros((p, 6), order="F")
+    lam, lam2 = 0.3, 0.2
+    Wt, _ = orc.fista_flat(X, np.eye(p), W0, lam, regul="elastic-net", lambda2=lam2,
+                           max_it=500, L0=1.0, fixed_step=True, tol=0.0, it0=1000)
+    np.testing.assert_allclose(Wt, soft(X, lam) / (1 + lam2), atol=1e-10)
+
+
+# ---------------------------------------------------- optimality conditions
+
+def test_lasso_kkt_conditions(orc):
+    X, D = _rand(21, 40, 120, 25, np.float64)
+    lam = 0.08
+    A = dense(orc.lasso(X, D, lam))
+    # the Gram carries +1e-10 on its diagonal (decomp/decomp.h:1569)
+    C = D.T @ (X - D @ A) - 1e-10 * A
+    assert np.all(np.abs(C) <= lam * (1 + 1e-9))
+    on = A != 0
+    np.testing.assert_allclose(C[on], lam * np.sign(A[on]), rtol=1e-9)
+
+
+def test_kkt_check_detects_a_planted_error(orc):
+    """A 1e-3 perturbation of one coefficient must break the KKT check."""
+    X, D = _rand(21, 40, 120, 25, np.float64)
+    lam = 0.08
+    A = dense(orc.lasso(X, D, lam))
+    i, j = np.argwhere(A != 0)[0]
+    A[i, j] += 1e-3
+    C = D.T @ (X - D @ A) - 1e-10 * A
+    on = A != 0
+    ok = np.all(np.abs(C) <= lam * (1 + 1e-9)) and np.allclose(
+        C[on], lam * np.sign(A[on]), rtol=1e-9)
+    assert not ok
+
+
+def test_omp_least_squares_on_support(orc):
+    X, D = _rand(22, 30, 80, 20, np.float64, k=6, noise=0.01, norm=False)
+    A = dense(orc.omp(X, D, L=8))
+    for j in range(X.shape[1]):
+        S = np.flatnonzero(A[:, j])
+        coef, *_ = np.linalg.lstsq(D[:, S], X[:, j], rcond=None)
+        np.testing.assert_allclose(A[S, j], coef, rtol=1e-9, atol=1e-12)
+        r = X[:, j] - D[:, S] @ coef
+        assert np.abs(D[:, S].T @ r).max() < 1e-10
+
+
+# ------------------------------------------------------------- brute force
+
+def test_ormp_matches_exhaustive_greedy(orc):
+    """ORMP = at each step add the atom whose least-squares refit on the
+    enlarged support leaves the smallest residual."""
+    X, D = _rand(23, 10, 12, 8, np.float64, k=3, noise=0.01, norm=False)
+    L = 4
+    A = dense(orc.omp(X, D, L=L))
+    for j in range(X.shape[1]):
+        x = X[:, j]
+        S = []
+        for _ in range(L):
+            best, bestr = None, np.inf
+            for a in range(D.shape[1]):
+                if a in S:
+                    continue
+                c, *_ = np.linalg.lstsq(D[:, S + [a]], x, rcond=None)
+                r = np.linalg.norm(x - D[:, S + [a]] @ c)
+                if r < bestr - 1e-12:
+                    best, bestr = a, r
+            S.append(best)
+        assert set(np.flatnonzero(A[:, j])) == set(S)
+
+
+def test_lasso_matches_sign_pattern_enumeration(orc):
+    """Tiny lasso solved exactly by enumerating supports and signs."""
+    X, D = _rand(24, 6, 5, 6, np.float64, k=2, noise=0.1, norm=True)
+    lam = 0.05
+    A = dense(orc.lasso(X, D, lam))
+    G = D.T @ D
+    for j in range(X.shape[1]):
+        x = X[:, j]
+        best, bestobj = np.zeros(5), 0.5 * x @ x
+        for k in range(1, 6):
+            for S in itertools.combinations(range(5), k):
+                S = list(S)
+                for sg in itertools.product((-1.0, 1.0), repeat=k):
+                    sg = np.array(sg)
+                    try:
+                        a = np.linalg.solve(G[np.ix_(S, S)], D[:, S].T @ x - lam * sg)
+                    except np.linalg.LinAlgError:
+                        continue
+                    if np.any(np.sign(a) != sg):
+                        continue
+                    full = np.zeros(5)
+                    full[S] = a
+                    obj = 0.5 * np.sum((x - D @ full) ** 2) + lam * np.abs(full).sum()
+                    if obj < bestobj - 1e-14:
+                        best, bestobj = full, obj
+        np.testing.assert_allclose(A[:, j], best, atol=1e-8)
+
+
+def test_fista_l1_converges_to_lars(orc):
+    X, D = _rand(25, 20, 40, 6, np.float64, k=3, noise=0.05, norm=True)
+    lam = 0.05
+    A = dense(orc.lasso(X, D, lam))
+    L0 = float(np.linalg.eigvalsh(D.T @ D)[-1])
+    Wt, info = orc.fista_flat(X, D, np.zeros((40, 6), order="F"), lam, regul="l1",
+                              max_it=20000, L0=L0, fixed_step=True, tol=1e-14, it0=100)
+    np.testing.assert_allclose(Wt, A, atol=1e-6)
+    assert np.all(info[2] < 1e-6)
