@@ -1,0 +1,331 @@
+// spams_amd.h -- spams.h-compatible C++ entry points over the spx C ABI.
+//
+// Drop-in for the decomposition part of SPAMS's C++ facade: the templates
+// below keep the reference signatures, argument normalisation, ownership and
+// error conventions, and run on the GPU through include/spx.h.
+//
+//   _lassoD     cpp_library/spams.h:107-143   (lasso2 / coreLARS2 homotopy)
+//   _omp        cpp_library/spams.h:250-293   (omp / coreORMP)
+//   _fistaFlat  cpp_library/spams.h:460-608   (square loss; l1, elastic-net)
+//   lasso2      decomp/decomp.h:126-135        omp  decomp/decomp.h:53-57
+//   cppLasso / cppOMP  cpp_library/cppspams.h:45-51, :106-113
+//
+// Types mirror the reference's layouts (linalg/linalg.h): Matrix<T> is
+// column-major with int dimensions and may wrap external memory; Vector<T>
+// likewise; SpMatrix<T> is CSC with pB/pE/r/v and rows sorted per column.
+// Errors are thrown as `const char*` like the reference facade (its dynamic
+// exception specifications are omitted: they are ill-formed since C++17).
+#ifndef SPAMS_AMD_H
+#define SPAMS_AMD_H
+
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "spx.h"
+
+// constraint_type (decomp/decomp.h:95)
+enum constraint_type { L1COEFFS, L2ERROR, PENALTY, SPARSITY, L2ERROR2, PENALTY2, FISTAMODE };
+
+template <typename T>
+class Vector {
+ public:
+  Vector() {}
+  explicit Vector(int n) { resize(n); }
+  Vector(T* X, int n) : _X(X), _n(n), _extern(true) {}
+  ~Vector() { clear(); }
+  Vector(const Vector&) = delete;
+  Vector& operator=(const Vector&) = delete;
+  void resize(int n) {
+    clear();
+    _n = n;
+    _X = n > 0 ? static_cast<T*>(std::calloc((size_t)n, sizeof(T))) : nullptr;
+    _extern = false;
+  }
+  void clear() {
+    if (!_extern) std::free(_X);
+    _X = nullptr;
+    _n = 0;
+  }
+  int n() const { return _n; }
+  T* rawX() const { return _X; }
+  T& operator[](int i) { return _X[i]; }
+  T operator[](int i) const { return _X[i]; }
+
+ private:
+  T* _X = nullptr;
+  int _n = 0;
+  bool _extern = false;
+};
+
+template <typename T>
+class Matrix {
+ public:
+  Matrix() {}
+  Matrix(int m, int n) { resize(m, n); }
+  Matrix(T* X, int m, int n) : _X(X), _m(m), _n(n), _extern(true) {}
+  ~Matrix() { clear(); }
+  Matrix(const Matrix&) = delete;
+  Matrix& operator=(const Matrix&) = delete;
+  void resize(int m, int n) {
+    clear();
+    _m = m;
+    _n = n;
+    const size_t sz = (size_t)m * n;
+    _X = sz ? static_cast<T*>(std::calloc(sz, sizeof(T))) : nullptr;
+    _extern = false;
+  }
+  void clear() {
+    if (!_extern) std::free(_X);
+    _X = nullptr;
+    _m = _n = 0;
+  }
+  void setZeros() {
+    if (_X) std::memset(_X, 0, sizeof(T) * (size_t)_m * _n);
+  }
+  int m() const { return _m; }
+  int n() const { return _n; }
+  T* rawX() const { return _X; }
+  T& operator()(int i, int j) { return _X[i + (size_t)j * _m]; }
+  T operator()(int i, int j) const { return _X[i + (size_t)j * _m]; }
+
+ private:
+  T* _X = nullptr;
+  int _m = 0, _n = 0;
+  bool _extern = false;
+};
+
+// AbstractMatrixB stand-in for _fistaFlat's dictionary argument: a dense
+// column-major matrix (the SpMatrix dictionary form is not provided).
+template <typename T>
+using AbstractMatrixB = Matrix<T>;
+
+template <typename T>
+class SpMatrix {
+ public:
+  SpMatrix() {}
+  ~SpMatrix() { clear(); }
+  SpMatrix(const SpMatrix&) = delete;
+  SpMatrix& operator=(const SpMatrix&) = delete;
+  void clear() {
+    std::free(_pB);
+    std::free(_r);
+    std::free(_v);
+    _pB = _r = nullptr;
+    _v = nullptr;
+    _m = _n = 0;
+    _nzmax = 0;
+  }
+  int m() const { return _m; }
+  int n() const { return _n; }
+  int nzmax() const { return (int)_nzmax; }
+  int* pB() const { return _pB; }
+  int* pE() const { return _pB ? _pB + 1 : nullptr; }
+  int* r() const { return _r; }
+  T* v() const { return _v; }
+  // takes ownership of a library CSC result
+  void adopt(spx_csc& c) {
+    clear();
+    _m = c.m;
+    _n = c.n;
+    _nzmax = c.nnz;
+    _pB = c.pB;
+    _r = c.r;
+    _v = static_cast<T*>(c.v);
+    c.pB = c.r = nullptr;
+    c.v = nullptr;
+  }
+
+ private:
+  int* _pB = nullptr;
+  int* _r = nullptr;
+  T* _v = nullptr;
+  int _m = 0, _n = 0;
+  long long _nzmax = 0;
+};
+
+namespace spams_amd_detail {
+
+inline spx_ctx* context() {
+  static spx_ctx* ctx = nullptr;
+  if (!ctx) {
+    if (spx_ctx_create(&ctx, 0) != SPX_OK) throw spx_last_error();
+  }
+  return ctx;
+}
+
+inline void check(int rc) {
+  if (rc != SPX_OK) throw spx_last_error();
+}
+
+inline int lasso_call(const float* X, int m, int n, const float* D, int p, const float* lam,
+                      int nlam, float l2, const spx_lasso_opts* o, spx_csc* out) {
+  return spx_lasso_f32(context(), X, m, n, D, p, lam, nlam, l2, o, out);
+}
+inline int lasso_call(const double* X, int m, int n, const double* D, int p,
+                      const double* lam, int nlam, double l2, const spx_lasso_opts* o,
+                      spx_csc* out) {
+  return spx_lasso_f64(context(), X, m, n, D, p, lam, nlam, l2, o, out);
+}
+inline int omp_call(const float* X, int m, int n, const float* D, int p, const int* L, int nL,
+                    const float* e, int ne, const float* la, int nla, spx_csc* out) {
+  return spx_omp_f32(context(), X, m, n, D, p, L, nL, e, ne, la, nla, nullptr, out);
+}
+inline int omp_call(const double* X, int m, int n, const double* D, int p, const int* L,
+                    int nL, const double* e, int ne, const double* la, int nla, spx_csc* out) {
+  return spx_omp_f64(context(), X, m, n, D, p, L, nL, e, ne, la, nla, nullptr, out);
+}
+inline int fista_call(const float* X, int m, int n, const float* D, int p, const float* W0,
+                      float* W, const spx_fista_opts* o, float* info) {
+  return spx_fista_flat_f32(context(), X, m, n, D, p, W0, W, o, info);
+}
+inline int fista_call(const double* X, int m, int n, const double* D, int p,
+                      const double* W0, double* W, const spx_fista_opts* o, double* info) {
+  return spx_fista_flat_f64(context(), X, m, n, D, p, W0, W, o, info);
+}
+
+}  // namespace spams_amd_detail
+
+// lasso2 (decomp/decomp.h:126-135): fills spalpha, L already normalised.
+template <typename T>
+void lasso2(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, int L,
+            const T constraint, const T lambda2 = 0, constraint_type mode = PENALTY,
+            const bool pos = false, const int numThreads = -1, Matrix<T>* path = nullptr,
+            const int length_path = -1) {
+  (void)numThreads;
+  if (path) throw "lasso2: the regularisation path output is not provided by spx";
+  spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, length_path, SPX_PREC_FP32, SPX_FORM_BATCHED, 0};
+  spx_csc out{};
+  spams_amd_detail::check(spams_amd_detail::lasso_call(X.rawX(), X.m(), X.n(), D.rawX(), D.n(),
+                                                       &constraint, 1, lambda2, &o, &out));
+  spalpha.adopt(out);
+}
+
+// _lassoD (cpp_library/spams.h:107-143).  cholevsky selects the Cholesky
+// variant in the reference; both variants solve the same problem and spx
+// runs its single inverse-Gram homotopy for either.
+template <typename T>
+SpMatrix<T>* _lassoD(Matrix<T>* X, Matrix<T>* D, Matrix<T>** path, bool return_reg_path, int L,
+                     const T constraint, const T lambda2, constraint_type mode, const bool pos,
+                     const bool ols, const int numThreads, int max_length_path,
+                     const bool verbose, bool cholevsky) {
+  (void)verbose;
+  (void)cholevsky;
+  if (X->m() != D->m()) throw "lasso : incompatible matrix dimensions";
+  if (return_reg_path) throw "lasso : return_reg_path is not provided by spx";
+  if (ols) throw "lasso : ols is not provided by spx";
+  if (L < 0) L = D->n();
+  if (max_length_path < 0) max_length_path = 4 * L;
+  *path = nullptr;
+  SpMatrix<T>* alpha = new SpMatrix<T>();
+  try {
+    lasso2(*X, *D, *alpha, L, constraint, lambda2, mode, pos, numThreads, (Matrix<T>*)nullptr,
+           max_length_path);
+  } catch (...) {
+    delete alpha;
+    throw;
+  }
+  return alpha;
+}
+
+// omp (decomp/decomp.h:53-57)
+template <typename T>
+void omp(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, const int* L,
+         const T* eps, const T* lambda, const bool vecL = false, const bool vecEps = false,
+         const bool Lambda = false, const int numThreads = -1, Matrix<T>* path = nullptr) {
+  (void)numThreads;
+  if (path) throw "omp: the regularisation path output is not provided by spx";
+  const int n = X.n();
+  spx_csc out{};
+  spams_amd_detail::check(spams_amd_detail::omp_call(
+      X.rawX(), X.m(), n, D.rawX(), D.n(), L, L ? (vecL ? n : 1) : 0, eps,
+      eps ? (vecEps ? n : 1) : 0, lambda, lambda ? (Lambda ? n : 1) : 0, &out));
+  spalpha.adopt(out);
+}
+
+// _omp (cpp_library/spams.h:250-293)
+template <typename T>
+SpMatrix<T>* _omp(Matrix<T>* X, Matrix<T>* D, Matrix<T>** path, bool return_reg_path,
+                  bool given_L, Vector<int>* L, bool given_eps, Vector<T>* eps,
+                  bool given_Lambda, Vector<T>* Lambda, const int numThreads) {
+  if (X->m() != D->m()) throw "omp : incompatible matrix dimensions";
+  if (!given_L && !given_eps && !given_Lambda)
+    throw "omp : You should either provide L, eps or lambda";
+  if (return_reg_path) throw "omp : return_reg_path is not provided by spx";
+  *path = nullptr;
+  SpMatrix<T>* alpha = new SpMatrix<T>();
+  try {
+    omp(*X, *D, *alpha, given_L ? L->rawX() : nullptr, given_eps ? eps->rawX() : nullptr,
+        given_Lambda ? Lambda->rawX() : nullptr, given_L && L->n() > 1,
+        given_eps && eps->n() > 1, given_Lambda && Lambda->n() > 1, numThreads,
+        (Matrix<T>*)nullptr);
+  } catch (...) {
+    delete alpha;
+    throw;
+  }
+  return alpha;
+}
+
+// _fistaFlat (cpp_library/spams.h:460-608), square loss with "l1" or
+// "elastic-net".  alpha (p x n) is written in place; returns optim_info 4 x n.
+template <typename T>
+Matrix<T>* _fistaFlat(Matrix<T>* X, AbstractMatrixB<T>* D, Matrix<T>* alpha0, Matrix<T>* alpha,
+                      Vector<int>* groups, int num_threads, int max_it, T L0, bool fixed_step,
+                      T gamma, T _lambda, T delta, T lambda2, T lambda3, T a, T b, T c, T tol,
+                      int it0, int max_iter_backtracking, bool compute_gram, bool lin_admm,
+                      bool admm, bool intercept, bool resetflow, char* name_regul,
+                      char* name_loss, bool verbose, bool pos, bool clever, bool log,
+                      bool ista, bool subgrad, char* logName, bool is_inner_weights,
+                      Vector<T>* inner_weights, int size_group, bool sqrt_step,
+                      bool transpose, int linesearch_mode) {
+  (void)num_threads; (void)delta; (void)lambda3; (void)a; (void)b; (void)c;
+  (void)compute_gram; (void)resetflow; (void)verbose; (void)clever; (void)log;
+  (void)logName; (void)inner_weights; (void)size_group; (void)sqrt_step;
+  const std::string loss = name_loss ? name_loss : "";
+  const std::string reg = name_regul ? name_regul : "";
+  if (loss != "square") throw "fistaFlat: Unknown loss";
+  int regul;
+  if (reg == "l1") regul = 0;
+  else if (reg == "elastic-net") regul = 1;
+  else throw "fistaFlat: Unknown regularization.";
+  if (lin_admm || admm || intercept || subgrad || transpose || is_inner_weights ||
+      linesearch_mode != 0 || (groups && groups->n() > 0))
+    throw "fistaFlat: option not provided by spx";
+  const int m = D->m(), p = D->n(), n = X->n();
+  if (alpha0->m() != p || alpha0->n() != n || X->m() != m)
+    throw "fistaFlat : Argument sizes are not consistent";
+  if (alpha->m() != p || alpha->n() != n) alpha->resize(p, n);
+  spx_fista_opts o{max_it, (double)L0, fixed_step ? 1 : 0, (double)gamma, (double)_lambda,
+                   (double)lambda2, (double)tol, it0, max_iter_backtracking, regul,
+                   pos ? 1 : 0, ista ? 1 : 0, SPX_PREC_FP32};
+  Matrix<T>* info = new Matrix<T>(4, n);
+  const int rc = spams_amd_detail::fista_call(X->rawX(), m, n, D->rawX(), p, alpha0->rawX(),
+                                              alpha->rawX(), &o, info->rawX());
+  if (rc != SPX_OK) {
+    delete info;
+    throw spx_last_error();
+  }
+  return info;
+}
+
+// cppLasso / cppOMP (cpp_library/cppspams.h:45-51, :106-113).
+template <typename T>
+SpMatrix<T>* cppLasso(Matrix<T>& X, Matrix<T>& D, Matrix<T>** path, bool return_reg_path,
+                      int L, const T constraint, const T lambda2 = 0.,
+                      constraint_type mode = PENALTY, const bool pos = false,
+                      const bool ols = false, const int numThreads = -1,
+                      int max_length_path = -1, const bool verbose = false,
+                      bool cholevsky = false) {
+  return _lassoD(&X, &D, path, return_reg_path, L, constraint, lambda2, mode, pos, ols,
+                 numThreads, max_length_path, verbose, cholevsky);
+}
+
+template <typename T>
+void cppOMP(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, const int* L,
+            const T* eps, const T* lambda, const bool vecL = false, const bool vecEps = false,
+            const bool vecLambda = false, const int numThreads = -1,
+            Matrix<T>* path = nullptr) {
+  omp(X, D, spalpha, L, eps, lambda, vecL, vecEps, vecLambda, numThreads, path);
+}
+
+#endif  // SPAMS_AMD_H

@@ -204,6 +204,26 @@ int spx_lasso_dev_f64(spx_ctx* ctx, const spx_dict* dict, const double* X,
                       double lambda2, const spx_lasso_opts* opts,
                       spx_lists_dev* out);
 
+/* Device LARS from precomputed correlations (the Gram-given form of the
+ * reference, lasso2(X, G, DtX, ...) decomp/decomp.h:1574-1646 and _lassoQq
+ * cpp_library/spams.h:145-188): G is the dictionary's resident Gram, DtX a
+ * device p x n matrix (leading dimension ldd >= p, rows p..ldd-1 ignored) and
+ * norms the device ||x_i||^2 (n). */
+int spx_lasso_dtx_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* DtX,
+                          int ldd, const float* norms, int n,
+                          const float* lambda, int n_lambda, float lambda2,
+                          const spx_lasso_opts* opts, spx_lists_dev* out);
+int spx_lasso_dtx_dev_f64(spx_ctx* ctx, const spx_dict* dict,
+                          const double* DtX, int ldd, const double* norms,
+                          int n, const double* lambda, int n_lambda,
+                          double lambda2, const spx_lasso_opts* opts,
+                          spx_lists_dev* out);
+/* ||x_i||^2 of the n columns of X (device, leading dimension ldx). */
+int spx_sqnorms_dev_f32(spx_ctx* ctx, const float* X, int m, int ldx, int n,
+                        float* norms);
+int spx_sqnorms_dev_f64(spx_ctx* ctx, const double* X, int m, int ldx, int n,
+                        double* norms);
+
 /* Device ORMP over a resident dictionary.  L/eps/lambda are device arrays
  * (NULL = not given; size 1 or n). */
 int spx_omp_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* X,

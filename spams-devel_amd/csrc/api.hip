@@ -99,6 +99,7 @@ static int dict_create(spx_ctx* ctx, const T* D, int m, int p, int on_device,
   DeviceGuard g(ctx->device);
   spx_dict* d = new spx_dict();
   d->ctx = ctx;
+  d->device = ctx->device;
   d->m = m;
   d->p = p;
   d->ldg = pad_ld(p);
@@ -152,6 +153,54 @@ static int products(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
 }
 
 template <typename T>
+static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
+                         const T* norms, int n, const T* lambda, int nlam, T lambda2,
+                         const spx_lasso_opts* o, spx_lists_dev* out) {
+  if (!ctx || !d || !o || !out || n < 0 || (n > 0 && (!DtX || !norms || !lambda)) ||
+      ldd < d->p || !(nlam == 1 || nlam == n)) {
+    set_error("lasso_dev: invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  if ((int)sizeof(T) != d->elem) {
+    set_error("lasso_dev: dictionary precision does not match the entry point");
+    return SPX_ERR_ARG;
+  }
+  if (o->mode < 0 || o->mode > 2) {
+    set_error("lasso: unsupported mode %d", o->mode);
+    return SPX_ERR_UNSUP;
+  }
+  if (out->L <= 0) {
+    set_error("lasso_dev: lists have L=%d", out->L);
+    return SPX_ERR_ARG;
+  }
+  if (n == 0) return SPX_OK;
+  DeviceGuard g(ctx->device);
+  LarsArgs<T> a{};
+  a.G = static_cast<const T*>(d->G);
+  a.ldg = d->ldg;
+  a.diag = static_cast<T>(lambda2 > T(1e-10) ? double(lambda2) : 1e-10);
+  a.DtX = DtX;
+  a.ldd = ldd;
+  a.norms = norms;
+  a.lambda = lambda;
+  a.nlam = nlam;
+  a.p = d->p;
+  a.n = n;
+  a.L = out->L;
+  a.mode = o->mode;
+  a.pos = o->pos;
+  const int Lp = o->L < 0 ? d->p : o->L;
+  a.length_path = o->max_length_path < 0 ? 4 * Lp : o->max_length_path;
+  // coreLARS2: length_path <= 1 means 4 * min(L, p) (decomp/decomp.h:1724)
+  if (a.length_path <= 1) a.length_path = 4 * (a.L < a.p ? a.L : a.p);
+  a.outV = static_cast<T*>(out->v);
+  a.outR = out->r;
+  a.outCount = out->count;
+  a.stats = out->stats;
+  return lars_solve<T>(ctx, a, o->form, o->waves_per_signal);
+}
+
+template <typename T>
 static int lasso_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx,
                      int n, const T* lambda, int nlam, T lambda2,
                      const spx_lasso_opts* o, spx_lists_dev* out) {
@@ -164,41 +213,13 @@ static int lasso_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx,
     set_error("lasso_dev: dictionary precision does not match the entry point");
     return SPX_ERR_ARG;
   }
-  if (o->mode < 0 || o->mode > 2) {
-    set_error("lasso: unsupported mode %d", o->mode);
-    return SPX_ERR_UNSUP;
-  }
   if (n == 0) return SPX_OK;
   DeviceGuard g(ctx->device);
   T* DtX = nullptr;
   T* norms = nullptr;
   int rc = products<T>(ctx, d, X, ldx, n, o->precision, &DtX, &norms);
   if (rc) return rc;
-  LarsArgs<T> a{};
-  a.G = static_cast<const T*>(d->G);
-  a.ldg = d->ldg;
-  a.diag = static_cast<T>(lambda2 > T(1e-10) ? double(lambda2) : 1e-10);
-  a.DtX = DtX;
-  a.ldd = d->ldg;
-  a.norms = norms;
-  a.lambda = lambda;
-  a.nlam = nlam;
-  a.p = d->p;
-  a.n = n;
-  a.L = out->L;
-  a.mode = o->mode;
-  a.pos = o->pos;
-  int Lp = o->L < 0 ? d->p : o->L;
-  a.length_path = o->max_length_path < 0 ? 4 * Lp : o->max_length_path;
-  a.outV = static_cast<T*>(out->v);
-  a.outR = out->r;
-  a.outCount = out->count;
-  a.stats = out->stats;
-  if (a.L <= 0) {
-    set_error("lasso_dev: lists have L=%d", a.L);
-    return SPX_ERR_ARG;
-  }
-  return lars_solve<T>(ctx, a, o->form, o->waves_per_signal);
+  return lasso_dtx_dev<T>(ctx, d, DtX, d->ldg, norms, n, lambda, nlam, lambda2, o, out);
 }
 
 template <typename T>
@@ -575,8 +596,10 @@ int spx_dict_create_f64(spx_ctx* ctx, const double* D, int m, int p, int on_devi
 
 void spx_dict_destroy(spx_dict* d) {
   if (!d) return;
-  spx::DeviceGuard g(d->ctx ? d->ctx->device : 0);
-  if (d->ctx && d->ctx->stream) (void)hipStreamSynchronize(d->ctx->stream);
+  // the creating context may already be gone: only the device is used, and
+  // every stream using the dictionary is drained before the buffers go
+  spx::DeviceGuard g(d->device);
+  (void)hipDeviceSynchronize();
   if (d->D) (void)hipFree(d->D);
   if (d->Dt) (void)hipFree(d->Dt);
   if (d->G) (void)hipFree(d->G);
@@ -670,6 +693,30 @@ int spx_lasso_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx,
                       const double* lambda, int n_lambda, double lambda2,
                       const spx_lasso_opts* o, spx_lists_dev* out) {
   return spx::lasso_dev<double>(ctx, d, X, ldx, n, lambda, n_lambda, lambda2, o, out);
+}
+
+int spx_lasso_dtx_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* DtX, int ldd,
+                          const float* norms, int n, const float* lambda, int n_lambda,
+                          float lambda2, const spx_lasso_opts* o, spx_lists_dev* out) {
+  return spx::lasso_dtx_dev<float>(ctx, d, DtX, ldd, norms, n, lambda, n_lambda, lambda2, o,
+                                   out);
+}
+int spx_lasso_dtx_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* DtX, int ldd,
+                          const double* norms, int n, const double* lambda, int n_lambda,
+                          double lambda2, const spx_lasso_opts* o, spx_lists_dev* out) {
+  return spx::lasso_dtx_dev<double>(ctx, d, DtX, ldd, norms, n, lambda, n_lambda, lambda2, o,
+                                    out);
+}
+
+int spx_sqnorms_dev_f32(spx_ctx* ctx, const float* X, int m, int ldx, int n, float* norms) {
+  if (!ctx || ldx < m || n < 0) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  return spx::col_sqnorms<float>(ctx->stream, m, n, X, ldx, norms);
+}
+int spx_sqnorms_dev_f64(spx_ctx* ctx, const double* X, int m, int ldx, int n, double* norms) {
+  if (!ctx || ldx < m || n < 0) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  return spx::col_sqnorms<double>(ctx->stream, m, n, X, ldx, norms);
 }
 
 int spx_omp_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, int n,

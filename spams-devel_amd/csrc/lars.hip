@@ -358,6 +358,9 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
           --s_active;
           newAtom = false;
           i = i - 2;
+          // an emptied active set has no direction: end the path (the
+          // reference would read ind[-1] here)
+          if (s_active == 0) break;
         } else {
           newAtom = true;
           currentInd = nextInd;

@@ -113,7 +113,8 @@ struct spx_ctx {
 // Resident dictionary: D (m x p), D' (p x m) and G = D'D (p x p, leading
 // dimension ldg = pad_ld(p), zero padding rows) in device memory.
 struct spx_dict {
-  spx_ctx* ctx = nullptr;
+  spx_ctx* ctx = nullptr;  // creating context (not used after creation)
+  int device = 0;
   int m = 0, p = 0, ldg = 0;
   int elem = 4;         // 4 = float, 8 = double
   int precision = 0;    // precision G was formed with

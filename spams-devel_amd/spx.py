@@ -368,6 +368,32 @@ def lasso_dev(ctx, d: Dictionary, X, lam, V, R, count, lambda2=0.0, L=-1,
              lambda2, ctypes.byref(o), ctypes.byref(lists)))
 
 
+def lasso_dtx_dev(ctx, d: Dictionary, DtX, norms, lam, V, R, count, lambda2=0.0,
+                  L=-1, mode=PENALTY, pos=False, max_length_path=-1,
+                  form="batched", waves_per_signal=0, stats=None):
+    """Device LARS from precomputed D'X (torch (n, ldd)) and ||x||^2 (n,)."""
+    n = DtX.shape[0]
+    Lout = V.shape[1]
+    sfx = "f32" if d.dtype == np.float32 else "f64"
+    o = spx_lasso_opts(L, mode, int(pos), max_length_path, 0, FORM[form],
+                       waves_per_signal)
+    lists = spx_lists_dev(_tptr(V), _tptr(R), _tptr(count), Lout, _tptr(stats))
+    f = getattr(lib(), "spx_lasso_dtx_dev_" + sfx)
+    f.argtypes = [_P, _P, _P, _I, _P, _I, _P, _I, _ct(d.dtype),
+                  ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_lists_dev)]
+    _check(f(ctx.h, d.h, _tptr(DtX), DtX.shape[1], _tptr(norms), n, _tptr(lam),
+             lam.numel(), lambda2, ctypes.byref(o), ctypes.byref(lists)))
+
+
+def sqnorms_dev(ctx, X, out):
+    """out (n,) = squared column norms of X torch (n, m)."""
+    n, m = X.shape
+    sfx = "f32" if str(X.dtype) == "torch.float32" else "f64"
+    f = getattr(lib(), "spx_sqnorms_dev_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _I, _P]
+    _check(f(ctx.h, _tptr(X), m, m, n, _tptr(out)))
+
+
 def omp_dev(ctx, d: Dictionary, X, V, R, count, L=None, eps=None, lambda1=None,
             precision="fp32", form="batched", waves_per_signal=0):
     """Device ORMP; L (int32), eps, lambda1: torch device tensors or None."""

@@ -1,0 +1,40 @@
+// Compiles the spams.h-compatible templates the way the reference's C++
+// harness (cpp_library/test_decomp.cpp) uses them.
+#include "spams_amd.h"
+
+void use_lasso() {
+  Matrix<double> X(100, 10), D(100, 200);
+  Matrix<double>* path;
+  SpMatrix<double>* spa = cppLasso(X, D, &path, false, 10, 0.15);
+  delete spa;
+  Matrix<float> Xf(64, 5), Df(64, 256);
+  SpMatrix<float>* s2 = _lassoD<float>(&Xf, &Df, reinterpret_cast<Matrix<float>**>(&path), false,
+                                       -1, 0.15f, 0.f, PENALTY, false, false, -1, -1, false, false);
+  delete s2;
+}
+
+void use_omp() {
+  Matrix<double> X(64, 10), D(64, 200);
+  const int L = 10;
+  double eps = 1.0, lambda = 0.;
+  SpMatrix<double> spa;
+  cppOMP(X, D, spa, &L, &eps, &lambda);
+  Vector<int> vL(1);
+  vL[0] = 5;
+  Vector<double> ve(1), vl(1);
+  Matrix<double>* path;
+  SpMatrix<double>* s = _omp(&X, &D, &path, false, true, &vL, false, &ve, false, &vl, -1);
+  delete s;
+}
+
+void use_fista() {
+  Matrix<double> Y(32, 4), D(32, 64), W0(64, 4), W(64, 4);
+  Vector<int> groups;
+  Vector<double> iw;
+  char reg[] = "l1", loss[] = "square", logn[] = "";
+  Matrix<double>* info = _fistaFlat(&Y, &D, &W0, &W, &groups, -1, 100, 0.1, false, 1.5, 0.05,
+                                    1.0, 0.0, 0.0, 1.0, 0.0, 1.0, 1e-6, 10, 1000, false, false,
+                                    false, false, false, reg, loss, false, false, false, false,
+                                    false, false, logn, false, &iw, 1, true, false, 0);
+  delete info;
+}
