@@ -41,14 +41,20 @@ def cpu_baseline(X, D, lam, seconds):
     kind = "reference" if oracle.has_ref() else "port"
     run = (lambda Xs, l: oracle.ref_lasso_pervec(Xs, D, l, num_threads=threads)) \
         if kind == "reference" else (lambda Xs, l: oracle.lasso(Xs, D, l, num_threads=threads))
-    n0 = 256
-    t = time.perf_counter()
-    run(np.asfortranarray(X[:, :n0]), lam[:n0])
-    dt = time.perf_counter() - t
-    ns = int(min(X.shape[1], max(n0, n0 * seconds / max(dt, 1e-6))))
-    t = time.perf_counter()
-    run(np.asfortranarray(X[:, :ns]), lam[:ns])
-    dt = time.perf_counter() - t
+    # calibrate on growing prefixes, then time a sample of about `seconds`
+    ns, dt = 256, 0.0
+    while True:
+        t = time.perf_counter()
+        run(np.asfortranarray(X[:, :ns]), lam[:ns])
+        dt = time.perf_counter() - t
+        if dt >= 0.25 * seconds or ns >= X.shape[1]:
+            break
+        ns = int(min(X.shape[1], ns * max(2.0, min(16.0, 0.5 * seconds / max(dt, 1e-3)))))
+    if dt < 0.5 * seconds and ns < X.shape[1]:
+        ns = int(min(X.shape[1], ns * seconds / dt))
+        t = time.perf_counter()
+        run(np.asfortranarray(X[:, :ns]), lam[:ns])
+        dt = time.perf_counter() - t
     return {"value": ns / dt, "unit": "signals/s", "cores": threads, "kind": kind,
             "sample": f"first {ns} signals of the same workload, {dt:.1f} s, "
                       f"per-signal-lambda lasso2/coreLARS2 loop, {threads} OpenMP threads"}

@@ -112,8 +112,10 @@ typedef struct {
  * scratch buffers. */
 int spx_ctx_create(spx_ctx** out, int device);
 void spx_ctx_destroy(spx_ctx* ctx);
-/* Replace the context's stream by a caller-owned hipStream_t (NULL = own). */
+/* Run the context's work on a caller-owned hipStream_t (NULL = the HIP null
+ * stream).  spx_ctx_own_stream() returns to a stream owned by the context. */
 int spx_ctx_set_stream(spx_ctx* ctx, void* hip_stream);
+int spx_ctx_own_stream(spx_ctx* ctx);
 int spx_ctx_synchronize(spx_ctx* ctx);
 /* Thread-local message of the last failing call on this thread. */
 const char* spx_last_error(void);

@@ -564,14 +564,25 @@ void spx_ctx_destroy(spx_ctx* ctx) {
 int spx_ctx_set_stream(spx_ctx* ctx, void* s) {
   if (!ctx) return SPX_ERR_ARG;
   spx::DeviceGuard g(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
-  if (s) {
-    ctx->stream = static_cast<hipStream_t>(s);
-    ctx->own_stream = false;
-  } else {
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SPX_ERR_HIP;
-    ctx->own_stream = true;
+  ctx->stream = static_cast<hipStream_t>(s);
+  ctx->own_stream = false;
+  return SPX_OK;
+}
+
+int spx_ctx_own_stream(spx_ctx* ctx) {
+  if (!ctx) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  if (ctx->own_stream) return SPX_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+    spx::set_error("spx_ctx_own_stream: stream creation failed");
+    return SPX_ERR_HIP;
   }
+  ctx->stream = st;
+  ctx->own_stream = true;
   return SPX_OK;
 }
 

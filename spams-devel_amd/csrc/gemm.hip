@@ -46,8 +46,10 @@ struct Acc<double> {
   }
 };
 
-// Writes one 16 x 16 accumulator block.  MFMA 16x16 output layout: lane l,
-// register r holds element (row 4*(l/16) + r, column l%16).
+// Writes one 16 x 16 accumulator block.  Output layouts (column l%16 for
+// lane l in both): v_mfma_f32_16x16x4_f32 / _16x16x32_bf16 keep 4 consecutive
+// rows per lane group, row 4*(l/16) + r; v_mfma_f64_16x16x4_f64 interleaves
+// the groups, row (l/16) + 4*r.
 template <typename T, typename V>
 __device__ __forceinline__ void store_block(const V& acc, int i0, int j0,
                                             int M, int N, T* C, int ldc,
@@ -58,7 +60,7 @@ __device__ __forceinline__ void store_block(const V& acc, int i0, int j0,
   if (j >= N) return;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int i = i0 + 4 * (lane >> 4) + r;
+    const int i = sizeof(T) == 8 ? i0 + (lane >> 4) + 4 * r : i0 + 4 * (lane >> 4) + r;
     if (i < M) {
       T v = alpha * static_cast<T>(acc[r]);
       if (E) v += beta * E[(size_t)i + (size_t)j * lde];

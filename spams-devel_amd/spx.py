@@ -64,6 +64,14 @@ def lib():
     """The loaded engine library (raises when it is not built)."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: torch bundles its own libamdhip64.so.7
+        # (same soname as ROCm's).  Loading torch first lets the engine bind to
+        # that runtime instead of a second copy, which would hide the GPU from
+        # torch.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 "spx: %s is missing; build it with spams-devel_amd/build.py" % LIB_PATH)
@@ -115,6 +123,7 @@ class Context:
         self.device = device
 
     def set_stream(self, stream_handle: int | None) -> None:
+        """Run on a caller's hipStream_t (0 / None = the HIP null stream)."""
         lib().spx_ctx_set_stream.argtypes = [_P, _P]
         _check(lib().spx_ctx_set_stream(self.h, stream_handle))
 
