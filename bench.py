@@ -60,6 +60,22 @@ def cpu_baseline(X, D, lam, seconds):
                       f"per-signal-lambda lasso2/coreLARS2 loop, {threads} OpenMP threads"}
 
 
+def measured_traffic(prefix):
+    """HBM bytes per dispatch of the kernel whose name starts with `prefix`,
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
+    workload (null when absent)."""
+    path = os.path.join(ROOT, "profiles", "headline_counters.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)["bytes_per_dispatch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for k, v in d.items():
+        if k.startswith(prefix):
+            return v
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +86,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 256 MiB read + 256 MiB write copy kernel first "
+                         "(calibrates HBM byte counters under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
@@ -114,6 +133,12 @@ def main():
         if ev:
             ev[2].record(stream)
 
+    if args.calib_copy:
+        a = torch.ones((64 << 20,), dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.add(a, 0.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     log("bench: warmup %d" % args.warmup)
     for _ in range(args.warmup):
         step("batched")
@@ -177,7 +202,8 @@ def main():
                      "achieved": solver_bytes / (solve_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": solver_bytes / (solve_ms * 1e-3) / HBM_PEAK,
-                     "traffic": None},
+                     "traffic": measured_traffic("lars_kernel"),
+                     "algorithmic_bytes": solver_bytes},
         "roofline_gemm": {"bound": "mfma", "kernel": "gemm_tn_kernel<float>",
                           "achieved": gemm_flops / (gemm_ms * 1e-3) / 1e12,
                           "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
