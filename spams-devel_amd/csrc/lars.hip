@@ -2,17 +2,28 @@
 //
 // One group of W wavefronts (one workgroup) solves one signal at a time:
 //   - the correlation vector DtR (p) and the active-atom flags live in LDS;
-//   - the correlation change A = G(:,active) * u is accumulated in registers
-//     (each thread owns NQ chunks of VEC = 16 bytes of rows), streaming the
-//     active Gram columns from the resident G (L2 / MALL);
+//   - the correlation change A = (G + diag I)(:, active) * u is accumulated
+//     in registers (each thread owns NQ chunks of VEC = 16 bytes of rows),
+//     streaming the active Gram columns from the resident G (L2 / MALL),
+//     several columns in flight per round;
 //   - the explicit inverse of the active Gram block, invGs (s x s, stored full
-//     and exactly symmetric), lives in a per-workgroup HBM slot, double
-//     buffered for the downdate;
+//     and exactly symmetric), is double buffered for the downdate;
 //   - arg-min / arg-max searches are deterministic (value, index) wavefront
 //     reductions with the reference's tie rules (first index for i?amax and
 //     the custom iamin; last index for the zero-crossing search).
-// Batched form: persistent grid that pulls signal indices from an atomic
-// counter.  Plain form: one workgroup per signal over a chunk of signals.
+//
+// The kernel is instantiated with an active-set capacity CAP:
+//   CAP > 0  the active-set arrays and invGs live in LDS (capacity CAP); a
+//            signal whose active set would outgrow CAP is appended to an
+//            overflow list and left to the full-capacity pass;
+//   CAP == 0 capacity L: active-set arrays in LDS, invGs in a per-workgroup
+//            HBM slot.
+// Both instantiations run the same arithmetic for a given group width, so a
+// signal's result does not depend on which pass solved it.
+//
+// Batched form: persistent grids pulling signal indices from atomic counters
+// (fast pass over 0..n-1, then the full pass over the overflow list, on the
+// same stream).  Plain form: one workgroup per signal, full capacity.
 #include <climits>
 
 #include "spx_common.h"
@@ -35,51 +46,90 @@ struct Vec16<double> {
   static constexpr int N = 2;
 };
 
-template <typename T, int W, int NQ>
+// LDS capacity of the fast pass (active-set size it solves without HBM).
+template <typename T>
+struct FastCap {
+  static constexpr int C = sizeof(T) == 4 ? 24 : 16;
+};
+
+template <typename T, int W, int NQ, int CAP>
+struct LarsLds {
+  static constexpr int GT = 64 * W;
+  static constexpr int VEC = Vec16<T>::N;
+  static constexpr int PT = VEC * GT * NQ;
+  // byte size for active-set capacity LC
+  static size_t bytes(int LC) {
+    size_t b = sizeof(T) * (PT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC + PT;
+    b = (b + 15) & ~size_t(15);
+    b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
+    b = (b + 15) & ~size_t(15);
+    if (CAP > 0) b += 2 * sizeof(T) * CAP * CAP + 16;
+    return b;
+  }
+};
+
+template <typename T, int W, int NQ, int CAP>
 __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   constexpr int GT = 64 * W;
   constexpr int VEC = Vec16<T>::N;
   constexpr int PT = VEC * GT * NQ;  // rows covered by the group
+  constexpr int UNR = NQ >= 4 ? 2 : 4;  // Gram columns in flight per round
   typedef typename Vec16<T>::type V;
   const T INF = dev_inf<T>();
 
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.L;
+  const int LC = CAP > 0 ? (CAP < L ? CAP : L) : L;  // active-set capacity
   const int p = a.p;
   const int ldg = a.ldg;
   T* sDtR = reinterpret_cast<T*>(smem);  // PT
-  T* sCoef = sDtR + PT;                   // L
-  T* sU = sCoef + L;                      // L
-  T* sW = sU + L;                         // L
-  T* sGs = sW + L;                        // L
-  T* sV = sGs + L;                        // L
-  int* sInd = reinterpret_cast<int*>(sV + L);  // L
-  int* sTmpI = sInd + L;                       // L
-  unsigned char* sAct = reinterpret_cast<unsigned char*>(sTmpI + L);  // PT
+  T* sCoef = sDtR + PT;                   // LC
+  T* sU = sCoef + LC;                     // LC
+  T* sW = sU + LC;                        // LC
+  T* sGs = sW + LC;                       // LC
+  T* sV = sGs + LC;                       // LC
+  int* sInd = reinterpret_cast<int*>(sV + LC);  // LC
+  int* sTmpI = sInd + LC;                       // LC
+  unsigned char* sAct = reinterpret_cast<unsigned char*>(sTmpI + LC);  // PT
   KeyIdx<T>* sRed = reinterpret_cast<KeyIdx<T>*>(
       (reinterpret_cast<uintptr_t>(sAct + PT) + 15) & ~uintptr_t(15));
   T* sRed3 = reinterpret_cast<T*>(sRed + W);  // 3W
   int* sSig = reinterpret_cast<int*>(sRed3 + 3 * W);
+  T* sM0 = reinterpret_cast<T*>(
+      (reinterpret_cast<uintptr_t>(sSig + 1) + 15) & ~uintptr_t(15));  // CAP x CAP
+  T* sM1 = sM0 + (CAP > 0 ? CAP * CAP : 0);
 
   const int tid = threadIdx.x;
-  T* Mbuf0 = a.scratch + (size_t)blockIdx.x * 2 * L * L;
-  T* Mbuf1 = Mbuf0 + (size_t)L * L;
+  T* const M0 = CAP > 0 ? sM0 : a.scratch + (size_t)blockIdx.x * 2 * L * L;
+  T* const M1 = CAP > 0 ? sM1 : M0 + (size_t)L * L;
+  const int ldm = CAP > 0 ? CAP : L;
   const int Lm = L < p ? L : p;
   const T diag = a.diag;
+  const int nwork = a.list ? *a.list_count : a.n;
 
+  // work distribution: chunks of `chunk` consecutive work items per atomic
+  // grab (one device-scope atomic per signal serialises on its address)
+  const int chunk = a.chunk > 0 ? a.chunk : 1;
+  int w = 0, wend = 0;
   for (int round = 0;; ++round) {
-    int sig;
     if (a.queue) {
-      __syncthreads();
-      if (tid == 0) *sSig = atomicAdd(a.queue, 1);
-      __syncthreads();
-      sig = *sSig;
+      if (w >= wend) {
+        __syncthreads();
+        if (tid == 0) *sSig = atomicAdd(a.queue, chunk);
+        __syncthreads();
+        w = *sSig;
+        wend = w + chunk;
+      }
     } else {
-      sig = round == 0 ? (int)blockIdx.x : a.n;
+      w = round == 0 ? (int)blockIdx.x : nwork;
+      wend = w + 1;
     }
-    if (sig >= a.n) break;
+    if (w >= nwork) break;
+    const int wcur = w++;
+    const int sig = a.list ? a.list[wcur] : (a.order ? a.order[wcur] : wcur);
 
     // ---------------------------------------------------------- set-up
+    __syncthreads();  // the previous signal's LDS state is no longer read
     const T* dtx = a.DtX + (size_t)sig * a.ldd;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -91,7 +141,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         sAct[j] = j < p ? 0 : 1;
       }
     }
-    for (int j = tid; j < L; j += GT) {
+    for (int j = tid; j < LC; j += GT) {
       sCoef[j] = 0;
       sInd[j] = -1;
     }
@@ -102,29 +152,23 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
     // first atom: i?amax of DtR, or the plain max when pos
     int currentInd;
     {
-      T key = INF;
+      T key = INF, pay = 0;
       int idx = INT_MAX;
-      T pay = 0;
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
           const int j = VEC * (tid + GT * q) + e;
-          if (j < p) {
-            const T d = sDtR[j];
-            const T k = a.pos ? -d : -fabs(d);
-            if (key_better(k, j, key, idx)) {
-              key = k;
-              idx = j;
-            }
-          }
+          const T d = sDtR[j];
+          const T k = j < p ? (a.pos ? -d : -fabs(d)) : INF;
+          take_if_better(key, idx, pay, k, j < p ? j : INT_MAX, T(0));
         }
       group_argmin<T, W>(key, idx, pay, sRed);
       currentInd = idx == INT_MAX ? 0 : idx;
     }
     int s_active = 0;
     int iter = 0, admissions = 0;
-    bool empty = false;
+    bool empty = false, overflow = false;
     {
       const T dc = sDtR[currentInd];
       if (a.mode == SPX_PENALTY && fabs(dc) < lam) empty = true;
@@ -134,11 +178,16 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
     if (!empty) {
       bool newAtom = true;
       T thrs = 0;
-      T* Mc = Mbuf0;
-      T* Mn = Mbuf1;
+      T* Mc = M0;
+      T* Mn = M1;
       for (int i = 0; i < Lm; ++i) {
         ++iter;
         if (newAtom) {
+          if (CAP > 0 && i >= LC) {
+            // the active set outgrows the fast pass: hand the signal over
+            overflow = true;
+            break;
+          }
           // ---------------------------------------------- admit an atom
           const int c = currentInd;
           ++admissions;
@@ -160,7 +209,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
             // v = invGs(i x i) * gs(0:i)
             for (int r = tid; r < i; r += GT) {
               T acc = 0;
-              for (int q = 0; q < i; ++q) acc += Mc[r + (size_t)q * L] * sGs[q];
+              for (int q = 0; q < i; ++q) acc += Mc[r + q * ldm] * sGs[q];
               sV[r] = acc;
             }
             __syncthreads();
@@ -176,13 +225,13 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
                 // symmetric rounding: element (r,q) and (q,r) get the
                 // identical update v[min] * (schur * v[max])
                 const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-                Mc[r + (size_t)q * L] += upd;
+                Mc[r + q * ldm] += upd;
               }
               const T off = -schur * vr;
-              Mc[r + (size_t)i * L] = off;
-              Mc[i + (size_t)r * L] = off;
+              Mc[r + i * ldm] = off;
+              Mc[i + r * ldm] = off;
             }
-            if (tid == 0) Mc[i + (size_t)i * L] = schur;
+            if (tid == 0) Mc[i + i * ldm] = schur;
             __syncthreads();
           }
         }
@@ -193,7 +242,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         __syncthreads();
         for (int r = tid; r < s; r += GT) {
           T acc = 0;
-          for (int q = 0; q < s; ++q) acc += Mc[r + (size_t)q * L] * sW[q];
+          for (int q = 0; q < s; ++q) acc += Mc[r + q * ldm] * sW[q];
           sU[r] = acc;
         }
         __syncthreads();
@@ -203,17 +252,14 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         int first_zero;
         T coeff1 = 0, coeff2 = 0;
         {
-          T key = INF;
+          T key = INF, pay = 0, z = 0;
           int idx = INT_MAX;
-          T pay = 0, z = 0;
           for (int j = tid; j < s; j += GT) {
             const T uj = sU[j];
             const T ratio = -sCoef[j] / uj;
-            // sequential scan keeps the LAST j on ties: encode index L - j
-            if (ratio > 0 && key_better(ratio, L - j, key, idx)) {
-              key = ratio;
-              idx = L - j;
-            }
+            // the sequential scan keeps the LAST j on ties: index L - j
+            take_if_better(key, idx, pay, ratio > 0 ? ratio : INF,
+                           ratio > 0 ? L - j : INT_MAX, T(0));
             const T dj = sDtR[sInd[j]];
             coeff1 += dj > 0 ? uj : -uj;
             coeff2 += dj * uj;
@@ -229,23 +275,37 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         V Areg[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) Areg[q] = V(0);
-        for (int k = 0; k < s; ++k) {
-          const T uk = sU[k];
-          const T* col = a.G + (size_t)sInd[k] * ldg;
+        // UNR Gram columns per round so their loads are in flight together;
+        // the sum over k keeps its sequential order
+        for (int k0 = 0; k0 < s; k0 += UNR) {
+          T uk[UNR];
+          const T* col[UNR];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const int j0 = VEC * (tid + GT * q);
-            if (j0 < ldg) {
-              const V g = *reinterpret_cast<const V*>(col + j0);
-              Areg[q] += uk * g;
-            }
+          for (int e = 0; e < UNR; ++e) {
+            const int k = k0 + e < s ? k0 + e : k0;
+            uk[e] = k0 + e < s ? sU[k] : T(0);
+            col[e] = a.G + (size_t)sInd[k] * ldg;
           }
+          V g[UNR][NQ];
+#pragma unroll
+          for (int e = 0; e < UNR; ++e)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              const int j0 = VEC * (tid + GT * q);
+              g[e][q] = j0 < ldg ? *reinterpret_cast<const V*>(col[e] + j0) : V(0);
+            }
+#pragma unroll
+          for (int e = 0; e < UNR; ++e)
+            if (k0 + e < s) {
+#pragma unroll
+              for (int q = 0; q < NQ; ++q) Areg[q] += uk[e] * g[e][q];
+            }
         }
         for (int k = 0; k < s; ++k) {
           const int r = sInd[k];
-          const int chunk = r / VEC;
-          if (chunk % GT == tid) {
-            const int q = chunk / GT;
+          const int rc = r / VEC;
+          if (rc % GT == tid) {
+            const int q = rc / GT;
             const T add = diag * sU[k];
 #pragma unroll
             for (int qq = 0; qq < NQ; ++qq)
@@ -261,23 +321,19 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
 #pragma unroll
           for (int e = 0; e < VEC; ++e) {
             const int j = VEC * (tid + GT * q) + e;
-            if (j < p && !sAct[j]) {
-              const T Aj = Areg[q][e];
-              const T d = sDtR[j];
-              T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
-              if (a.pos) c1 = INF;
-              const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
-              if (key_better(T(fabs(c1)), j, bestk, besti)) {
-                bestk = fabs(c1);
-                besti = j;
-                bestv = c1;
-              }
-              if (key_better(T(fabs(c2)), p + j, bestk, besti)) {
-                bestk = fabs(c2);
-                besti = p + j;
-                bestv = c2;
-              }
-            }
+            const bool cand = (j < p) & !sAct[j];
+            const T Aj = Areg[q][e];
+            const T d = sDtR[j];
+            T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
+            c1 = a.pos ? INF : c1;
+            const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
+            take_if_better(bestk, besti, bestv, cand ? T(fabs(c1)) : INF,
+                           cand ? j : INT_MAX, c1);
+            take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
+                           cand ? p + j : INT_MAX, c2);
+            // keep one row's division chain live at a time (register
+            // pressure sets the number of signals in flight per CU)
+            __builtin_amdgcn_sched_barrier(0);
           }
         group_argmin<T, W>(bestk, besti, bestv, sRed);
         T step = besti == INT_MAX ? INF : bestv;
@@ -319,10 +375,10 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
           // ------------------- drop atom first_zero (decomp/decomp.h:1875-1903)
           const int fz = first_zero;
           const int dropped = sInd[fz];
-          const T schur = Mc[fz + (size_t)fz * L];
+          const T schur = Mc[fz + fz * ldm];
           for (int r = tid; r < i; r += GT) {
             const int r2 = r + (r >= fz);
-            sV[r] = Mc[r2 + (size_t)fz * L];
+            sV[r] = Mc[r2 + fz * ldm];
           }
           __syncthreads();
           const T alpha = T(-1) / schur;
@@ -333,7 +389,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
               const int q2 = q + (q >= fz);
               const T uq = sV[q];
               const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
-              Mn[r + (size_t)q * L] = Mc[r2 + (size_t)q2 * L] + upd;
+              Mn[r + q * ldm] = Mc[r2 + q2 * ldm] + upd;
             }
           }
           for (int j = tid; j < i; j += GT) {
@@ -375,11 +431,21 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
 
     // ------------------------------------------------------- outputs
     __syncthreads();
+    if (overflow) {
+      if (tid == 0) {
+        const int pos = atomicAdd(a.ovf_count, 1);
+        a.ovf_list[pos] = sig;
+      }
+      continue;
+    }
     T* ov = a.outV + (size_t)sig * L;
     int* orr = a.outR + (size_t)sig * L;
-    for (int j = tid; j < L; j += GT) {
-      ov[j] = sCoef[j];
-      orr[j] = sInd[j];
+    // the live part of the list and its -1 terminator (entries past the
+    // terminator are left unspecified; consumers read `count` entries)
+    const int nout = s_active < L ? s_active + 1 : L;
+    for (int j = tid; j < nout; j += GT) {
+      ov[j] = j < s_active ? sCoef[j] : T(0);
+      orr[j] = j < s_active ? sInd[j] : -1;
     }
     if (tid == 0) {
       a.outCount[sig] = s_active;
@@ -391,91 +457,145 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   }
 }
 
-template <typename T, int W, int NQ>
-size_t lars_lds_bytes(int L) {
-  constexpr int GT = 64 * W;
-  constexpr int VEC = Vec16<T>::N;
-  constexpr int PT = VEC * GT * NQ;
-  size_t b = sizeof(T) * (PT + 5 * (size_t)L) + sizeof(int) * 2 * (size_t)L + PT;
-  b = (b + 15) & ~size_t(15);
-  b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
-  return b;
-}
-
-template <typename T, int W, int NQ>
-int lars_run(Ctx* ctx, const LarsArgs<T>& a0, int form) {
-  LarsArgs<T> a = a0;
-  const size_t lds = lars_lds_bytes<T, W, NQ>(a.L);
+// Launches one instantiation: persistent grid over a counter (batched) or
+// one workgroup per signal (plain; CAP must be 0).
+template <typename T, int W, int NQ, int CAP>
+int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
+                long max_blocks) {
+  const int LC = CAP > 0 ? (CAP < a.L ? CAP : a.L) : a.L;
+  const size_t lds = LarsLds<T, W, NQ, CAP>::bytes(LC);
   if (lds > 160 * 1024) {
     set_error("lasso: L=%d, p=%d need %zu bytes of LDS per signal", a.L, a.p, lds);
     return SPX_ERR_UNSUP;
   }
-  auto kern = lars_kernel<T, W, NQ>;
+  auto kern = lars_kernel<T, W, NQ, CAP>;
   SPX_HIP(hipFuncSetAttribute((const void*)kern,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds));
-  const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipStream_t st = ctx->stream;
-  if (form == SPX_FORM_BATCHED) {
+  LarsArgs<T> b = a;
+  if (persistent) {
     int per_cu = 0;
     SPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds));
     if (per_cu < 1) per_cu = 1;
     long blocks = (long)per_cu * ctx->num_cus;
-    if (blocks > a.n) blocks = a.n;
+    if (blocks > max_blocks) blocks = max_blocks;
     if (blocks < 1) blocks = 1;
-    T* scratch = nullptr;
-    int* queue = nullptr;
-    int rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
-    if (rc) return rc;
-    rc = ctx->scratch_get(1, sizeof(int), reinterpret_cast<void**>(&queue));
-    if (rc) return rc;
-    rc = zero_ints(st, queue, 1);
-    if (rc) return rc;
-    a.scratch = scratch;
-    a.queue = queue;
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, a);
+    if (CAP == 0) {
+      const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
+      T* scratch = nullptr;
+      int rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
+      if (rc) return rc;
+      b.scratch = scratch;
+    }
+    b.queue = counter;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, b);
     SPX_HIP(hipGetLastError());
     return SPX_OK;
   }
   // plain form: one workgroup per signal, chunks bounded by scratch size
+  const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
   long chunk = (long)(512ull * 1024 * 1024 / (slot ? slot : 1));
   if (chunk > 65535) chunk = 65535;
   if (chunk < 1) chunk = 1;
+  const long alloc_chunk = chunk < a.n ? chunk : a.n;
   T* scratch = nullptr;
-  const long alloc_chunk = chunk < a0.n ? chunk : a0.n;
   int rc = ctx->scratch_get(0, slot * (alloc_chunk > 0 ? alloc_chunk : 1),
                             reinterpret_cast<void**>(&scratch));
   if (rc) return rc;
-  for (long s0 = 0; s0 < a0.n; s0 += chunk) {
-    const long cnt = (a0.n - s0) < chunk ? (a0.n - s0) : chunk;
-    a = a0;
-    a.queue = nullptr;
-    a.scratch = scratch;
-    a.n = (int)cnt;
-    a.DtX = a0.DtX + (size_t)s0 * a0.ldd;
-    a.norms = a0.norms + s0;
-    if (a0.nlam > 1) a.lambda = a0.lambda + s0;
-    a.outV = a0.outV + (size_t)s0 * a0.L;
-    a.outR = a0.outR + (size_t)s0 * a0.L;
-    a.outCount = a0.outCount + s0;
-    if (a0.stats) a.stats = a0.stats + 2 * s0;
-    hipLaunchKernelGGL(kern, dim3((unsigned)cnt), dim3(64 * W), lds, st, a);
+  for (long s0 = 0; s0 < a.n; s0 += chunk) {
+    const long cnt = (a.n - s0) < chunk ? (a.n - s0) : chunk;
+    b = a;
+    b.queue = nullptr;
+    b.list = nullptr;
+    b.scratch = scratch;
+    b.n = (int)cnt;
+    b.DtX = a.DtX + (size_t)s0 * a.ldd;
+    b.norms = a.norms + s0;
+    if (a.nlam > 1) b.lambda = a.lambda + s0;
+    b.outV = a.outV + (size_t)s0 * a.L;
+    b.outR = a.outR + (size_t)s0 * a.L;
+    b.outCount = a.outCount + s0;
+    if (a.stats) b.stats = a.stats + 2 * s0;
+    hipLaunchKernelGGL(kern, dim3((unsigned)cnt), dim3(64 * W), lds, st, b);
     SPX_HIP(hipGetLastError());
   }
   return SPX_OK;
 }
 
 template <typename T, int W>
-int lars_dispatch_nq(Ctx* ctx, const LarsArgs<T>& a, int form) {
-  constexpr int VEC = Vec16<T>::N;
-  const int per = VEC * 64 * W;
-  const int nq = (a.p + per - 1) / per;
-  if (nq <= 1) return lars_run<T, W, 1>(ctx, a, form);
-  if (nq <= 2) return lars_run<T, W, 2>(ctx, a, form);
-  if (nq <= 4) return lars_run<T, W, 4>(ctx, a, form);
-  if (nq <= 8) return lars_run<T, W, 8>(ctx, a, form);
-  if (nq <= 16) return lars_run<T, W, 16>(ctx, a, form);
-  return -1;  // does not fit this group width
+int lars_nq(int p) {
+  const int per = Vec16<T>::N * 64 * W;
+  const int nq = (p + per - 1) / per;
+  return nq <= 1 ? 1 : nq <= 2 ? 2 : nq <= 4 ? 4 : nq <= 8 ? 8 : nq <= 16 ? 16 : -1;
+}
+
+// One pass of group width W: CAP = FastCap (fast) or 0 (full capacity).
+template <typename T, int W, bool FAST>
+int lars_pass(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter, long max_blocks) {
+  constexpr int C = FAST ? FastCap<T>::C : 0;
+  switch (lars_nq<T, W>(a.p)) {
+    case 1: return lars_launch<T, W, 1, C>(ctx, a, persistent, counter, max_blocks);
+    case 2: return lars_launch<T, W, 2, C>(ctx, a, persistent, counter, max_blocks);
+    case 4: return lars_launch<T, W, 4, C>(ctx, a, persistent, counter, max_blocks);
+    case 8: return lars_launch<T, W, 8, C>(ctx, a, persistent, counter, max_blocks);
+    case 16: return lars_launch<T, W, 16, C>(ctx, a, persistent, counter, max_blocks);
+    default: return -1;  // does not fit this group width
+  }
+}
+
+template <typename T>
+int lars_pass_w(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
+                int* counter, long max_blocks) {
+  switch (w) {
+    case 1: return fast ? lars_pass<T, 1, true>(ctx, a, persistent, counter, max_blocks)
+                        : lars_pass<T, 1, false>(ctx, a, persistent, counter, max_blocks);
+    case 2: return fast ? lars_pass<T, 2, true>(ctx, a, persistent, counter, max_blocks)
+                        : lars_pass<T, 2, false>(ctx, a, persistent, counter, max_blocks);
+    case 4: return fast ? lars_pass<T, 4, true>(ctx, a, persistent, counter, max_blocks)
+                        : lars_pass<T, 4, false>(ctx, a, persistent, counter, max_blocks);
+    default: return fast ? lars_pass<T, 16, true>(ctx, a, persistent, counter, max_blocks)
+                         : lars_pass<T, 16, false>(ctx, a, persistent, counter, max_blocks);
+  }
+}
+
+// Batched form: fast pass (capacity FastCap) over every signal in work order,
+// then the full-capacity pass over the signals that outgrew it, with at least
+// 4 wavefronts per signal (their active sets are the large ones).
+template <typename T>
+int lars_batched(Ctx* ctx, const LarsArgs<T>& a0, int w) {
+  int* cnt = nullptr;
+  int rc = ctx->scratch_get(1, 4 * sizeof(int), reinterpret_cast<void**>(&cnt));
+  if (rc) return rc;
+  int* ovf = nullptr;
+  rc = ctx->scratch_get(8, sizeof(int) * (size_t)(a0.n > 0 ? a0.n : 1),
+                        reinterpret_cast<void**>(&ovf));
+  if (rc) return rc;
+  rc = zero_ints(ctx->stream, cnt, 4);
+  if (rc) return rc;
+  LarsArgs<T> a = a0;
+  if (a.chunk <= 0) a.chunk = 1;
+  a.ovf_list = ovf;
+  a.ovf_count = cnt + 1;
+  a.list = nullptr;
+  a.list_count = nullptr;
+  if (a0.nlam > 1 && !a.order) {
+    // longest paths first: the homotopy is longest for the smallest lambda
+    int* order = nullptr;
+    rc = ctx->scratch_get(10, sizeof(int) * (size_t)a0.n, reinterpret_cast<void**>(&order));
+    if (rc) return rc;
+    rc = sort_order<T>(ctx, a0.lambda, a0.n, order);
+    if (rc) return rc;
+    a.order = order;
+  }
+  rc = lars_pass_w<T>(ctx, w, true, a, true, cnt, a0.n);
+  if (rc) return rc;
+  if (a0.L <= FastCap<T>::C) return SPX_OK;  // every active set fitted
+  a.list = ovf;
+  a.list_count = cnt + 1;
+  int w2 = w < 4 ? 4 : w;
+  rc = lars_pass_w<T>(ctx, w2, false, a, true, cnt + 2, a0.n);
+  if (rc < 0) rc = lars_pass_w<T>(ctx, 16, false, a, true, cnt + 2, a0.n);
+  return rc;
 }
 
 }  // namespace
@@ -490,11 +610,18 @@ int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves) {
   for (int w : widths) {
     if (w < waves) continue;
     int rc;
-    switch (w) {
-      case 1: rc = lars_dispatch_nq<T, 1>(ctx, a, form); break;
-      case 2: rc = lars_dispatch_nq<T, 2>(ctx, a, form); break;
-      case 4: rc = lars_dispatch_nq<T, 4>(ctx, a, form); break;
-      default: rc = lars_dispatch_nq<T, 16>(ctx, a, form); break;
+    if (form == SPX_FORM_BATCHED) {
+      int nq;
+      switch (w) {
+        case 1: nq = lars_nq<T, 1>(a.p); break;
+        case 2: nq = lars_nq<T, 2>(a.p); break;
+        case 4: nq = lars_nq<T, 4>(a.p); break;
+        default: nq = lars_nq<T, 16>(a.p); break;
+      }
+      if (nq < 0) continue;
+      rc = lars_batched<T>(ctx, a, w);
+    } else {
+      rc = lars_pass_w<T>(ctx, w, false, a, false, nullptr, 0);
     }
     if (rc >= 0) return rc;
   }

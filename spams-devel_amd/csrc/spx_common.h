@@ -34,36 +34,99 @@ inline int pad_ld(int p) { return (p + 15) & ~15; }
 
 constexpr int kWave = 64;
 
+// ---- cross-lane moves (all 64 lanes active) --------------------------------
+// DPP controls used below (gfx9 encoding): quad_perm [1,0,3,2] = 0xB1,
+// quad_perm [2,3,0,1] = 0x4E, row_ror:4 = 0x124, row_ror:8 = 0x128.  After
+// the four steps every lane holds the reduction of its row of 16 lanes; the
+// four row results are then combined from lanes 0, 16, 32 and 48.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_v(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_v(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i<CTRL>(static_cast<int>(b & 0xffffffffLL));
+  const int hi = dpp_i<CTRL>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ int lane_i(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+__device__ __forceinline__ float lane_v(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_v(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+template <typename T, int CTRL>
+__device__ __forceinline__ T sum_step(T v) {
+  return v + dpp_v<CTRL>(v);
+}
+
+// Wave-wide sum; every lane returns the same value and the association
+// order is fixed (deterministic).
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+  v = sum_step<T, 0xB1>(v);
+  v = sum_step<T, 0x4E>(v);
+  v = sum_step<T, 0x124>(v);
+  v = sum_step<T, 0x128>(v);
+  return (lane_v(v, 0) + lane_v(v, 16)) + (lane_v(v, 32) + lane_v(v, 48));
 }
 
-// (key, idx) arg-min with the lowest index winning ties; NaN keys never win
-// against a number.  Every lane returns the wave-wide winner.
+// (key, idx) ordering of the arg-min searches: smaller key wins, the lower
+// index wins ties, a NaN key loses to any number.  Branch-free.
 template <typename T>
 __device__ __forceinline__ bool key_better(T k1, int i1, T k2, int i2) {
-  // true when (k1, i1) beats (k2, i2)
-  if (k1 < k2) return true;
-  if (k1 == k2) return i1 < i2;
-  return (k2 != k2) && (k1 == k1);  // k2 is NaN, k1 is not
+  const bool lt = k1 < k2;
+  const bool tie = (k1 == k2) & (i1 < i2);
+  const bool nan2 = (k2 != k2) & (k1 == k1);
+  return lt | tie | nan2;
 }
 
 template <typename T>
+__device__ __forceinline__ void take_if_better(T& key, int& idx, T& pay, T k2,
+                                               int i2, T p2) {
+  const bool b = key_better(k2, i2, key, idx);
+  key = b ? k2 : key;
+  idx = b ? i2 : idx;
+  pay = b ? p2 : pay;
+}
+
+template <typename T, int CTRL>
+__device__ __forceinline__ void argmin_step(T& key, int& idx, T& pay) {
+  const T k2 = dpp_v<CTRL>(key);
+  const int i2 = dpp_i<CTRL>(idx);
+  const T p2 = dpp_v<CTRL>(pay);
+  take_if_better(key, idx, pay, k2, i2, p2);
+}
+
+// Wave-wide arg-min; every lane returns the winner.
+template <typename T>
 __device__ __forceinline__ void wave_argmin(T& key, int& idx, T& payload) {
+  argmin_step<T, 0xB1>(key, idx, payload);
+  argmin_step<T, 0x4E>(key, idx, payload);
+  argmin_step<T, 0x124>(key, idx, payload);
+  argmin_step<T, 0x128>(key, idx, payload);
+  T k = lane_v(key, 0), p = lane_v(payload, 0);
+  int i = lane_i(idx, 0);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    T k2 = __shfl_xor(key, o, kWave);
-    int i2 = __shfl_xor(idx, o, kWave);
-    T p2 = __shfl_xor(payload, o, kWave);
-    if (key_better(k2, i2, key, idx)) {
-      key = k2;
-      idx = i2;
-      payload = p2;
-    }
-  }
+  for (int r = 16; r < 64; r += 16)
+    take_if_better(k, i, p, lane_v(key, r), lane_i(idx, r), lane_v(payload, r));
+  key = k;
+  idx = i;
+  payload = p;
 }
 
 template <typename T>
@@ -88,8 +151,8 @@ __device__ __forceinline__ void group_argmin(T& key, int& idx, T& payload,
     KeyIdx<T> b = red[0];
 #pragma unroll
     for (int w = 1; w < W; ++w) {
-      KeyIdx<T> c = red[w];
-      if (key_better(c.key, c.idx, b.key, b.idx)) b = c;
+      const KeyIdx<T> c = red[w];
+      take_if_better(b.key, b.idx, b.payload, c.key, c.idx, c.payload);
     }
     key = b.key;
     idx = b.idx;

@@ -33,8 +33,18 @@ struct LarsArgs {
   int* outR;        // L x n
   int* outCount;    // n
   int* stats;       // 2 x n or null
-  T* scratch;       // per workgroup 2 * L * L
+  T* scratch;       // per workgroup 2 * L * L (full-capacity pass)
   int* queue;       // work counter (batched form) or null (plain form)
+  // Batched form, two passes: the fast pass (active set capped at a small
+  // LDS capacity) appends the signals that outgrow it to ovf_list /
+  // ovf_count; the full-capacity pass then solves exactly those, reading the
+  // list (list / list_count) instead of 0..n-1.
+  int* ovf_list;
+  int* ovf_count;
+  const int* list;
+  const int* list_count;
+  int chunk;        // signals taken per atomic grab of the work counter
+  const int* order; // work order (signal index per work item) or null
 };
 
 // Kernel arguments of the ORMP solver (coreORMP semantics).
@@ -58,6 +68,7 @@ struct OmpArgs {
   int* outCount;
   T* scratch;       // per workgroup: L*L (Un) + p*L (Undn)
   int* queue;
+  int chunk;        // signals taken per atomic grab of the work counter
 };
 
 template <typename T>
@@ -81,6 +92,10 @@ int csc_fill(Ctx* ctx, const T* v, const int* r, int L, int n, const int* pB,
 template <typename T>
 int fista_run(Ctx* ctx, const Dict* dict, const T* X, int n, const T* W0,
               T* W, const spx_fista_opts* o, T* info);
+
+// order[i] = indices 0..n-1 sorted by key ascending (stable); scratch slot 9.
+template <typename T>
+int sort_order(Ctx* ctx, const T* key, int n, int* order);
 
 // Clears n ints with a kernel (capture-safe replacement for a memset node).
 int zero_ints(hipStream_t st, int* p, int n);
