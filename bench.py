@@ -1,14 +1,18 @@
-"""Benchmark of the headline path: LARS-Lasso, m=256, p=1024, n=200000
-signals, lambda log-uniform in [0.05, 0.5] per signal, FP32, on one MI355X.
+"""Benchmarks of the sparse-coding engine on one MI355X.
 
-One step = one pass of the path over the whole batch with inputs resident in
-HBM: D'X on MFMA, ||x||^2, then the LARS solver kernel producing the
-per-signal (value, index) lists.  `value` is the batched (persistent
-work-queue) form's signals/s; the plain form (one workgroup per signal) is
-measured beside it.  Kernel durations come from HIP events recorded on the
-engine's stream; roofline figures use the algorithmic work of each kernel.
+Default (headline): LARS-Lasso, m=256, p=1024, n=200000 signals, lambda
+log-uniform in [0.05, 0.5] per signal, FP32.  One step = one pass of the path
+over the whole batch with inputs resident in HBM: D'X on MFMA, ||x||^2, then
+the LARS solver kernels producing the per-signal (value, index) lists.
+`value` is the batched form's signals/s (persistent work-queue kernels); the
+plain form (one workgroup per signal) is measured beside it, with the CPU
+reference, the error of each precision variant against the FP64 oracle, and
+p50/p99 latency of a served mixed-request load.  Kernel durations come from
+HIP events on the engine's stream; roofline figures use each kernel's
+algorithmic work.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --workload omp|fista|served     (other configurations)
 """
 from __future__ import annotations
 
@@ -27,114 +31,159 @@ sys.path.insert(0, os.path.join(ROOT, "spams-devel_amd"))
 METRIC = "signals/sec LARS-Lasso m256 p1024 served vs plain GPU; p50/p99 request latency"
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 FP32_MFMA_PEAK = 157.3e12  # FLOP/s, dense FP32 matrix
+BF16_MFMA_PEAK = 2.5e15    # FLOP/s, dense BF16 matrix
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(X, D, lam, seconds):
-    """The reference's OpenMP LARS (compiled in place, oracle/_ref) on a
-    bounded sample of the same workload; falls back to the C restatement."""
-    import oracle
-    threads = min(os.cpu_count() or 1, 16)
-    kind = "reference" if oracle.has_ref() else "port"
-    run = (lambda Xs, l: oracle.ref_lasso_pervec(Xs, D, l, num_threads=threads)) \
-        if kind == "reference" else (lambda Xs, l: oracle.lasso(Xs, D, l, num_threads=threads))
-    # calibrate on growing prefixes, then time a sample of about `seconds`
-    ns, dt = 256, 0.0
+def cpu_threads():
+    return min(os.cpu_count() or 1, 16)
+
+
+def timed_sample(run, n_total, seconds):
+    """Times `run(ns)` on a prefix sized to take about `seconds`."""
+    ns, dt = min(256, n_total), 0.0
     while True:
         t = time.perf_counter()
-        run(np.asfortranarray(X[:, :ns]), lam[:ns])
+        run(ns)
         dt = time.perf_counter() - t
-        if dt >= 0.25 * seconds or ns >= X.shape[1]:
+        if dt >= 0.25 * seconds or ns >= n_total:
             break
-        ns = int(min(X.shape[1], ns * max(2.0, min(16.0, 0.5 * seconds / max(dt, 1e-3)))))
-    if dt < 0.5 * seconds and ns < X.shape[1]:
-        ns = int(min(X.shape[1], ns * seconds / dt))
+        ns = int(min(n_total, ns * max(2.0, min(16.0, 0.5 * seconds / max(dt, 1e-3)))))
+    if dt < 0.5 * seconds and ns < n_total:
+        ns = int(min(n_total, ns * seconds / dt))
         t = time.perf_counter()
-        run(np.asfortranarray(X[:, :ns]), lam[:ns])
+        run(ns)
         dt = time.perf_counter() - t
-    return {"value": ns / dt, "unit": "signals/s", "cores": threads, "kind": kind,
-            "sample": f"first {ns} signals of the same workload, {dt:.1f} s, "
-                      f"per-signal-lambda lasso2/coreLARS2 loop, {threads} OpenMP threads"}
+    return ns, dt
 
 
 def measured_traffic(prefix):
-    """HBM bytes per dispatch of the kernel whose name starts with `prefix`,
-    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
-    workload (null when absent)."""
+    """HBM bytes per launch of the kernels whose names start with `prefix`,
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the
+    headline workload (null when absent)."""
     path = os.path.join(ROOT, "profiles", "headline_counters.json")
     try:
         with open(path) as f:
             d = json.load(f)["bytes_per_dispatch"]
     except (OSError, ValueError, KeyError):
         return None
+    tot, hit = 0.0, False
     for k, v in d.items():
         if k.startswith(prefix):
-            return v
-    return None
+            tot += v
+            hit = True
+    return tot if hit else None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=200000)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--waves", type=int, default=0)
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="run one 256 MiB read + 256 MiB write copy kernel first "
-                         "(calibrates HBM byte counters under a profiler)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
-              "(signals shard across GPUs with no exchange, see the design notes)"
-              % args.gpus, file=sys.stderr)
-        sys.exit(2)
+class Engine:
+    """Engine context bound to torch's current stream, plus helpers."""
 
-    import torch
+    def __init__(self):
+        import torch
 
+        import spx
+        self.torch = torch
+        self.spx = spx
+        self.dev = torch.device("cuda:0")
+        self.stream = torch.cuda.current_stream(self.dev)
+        self.ctx = spx.Context(0)
+        self.ctx.set_stream(self.stream.cuda_stream)
+
+    def to_dev(self, a):
+        return self.torch.from_numpy(np.ascontiguousarray(a)).to(self.dev)
+
+    def events(self, k):
+        return [self.torch.cuda.Event(enable_timing=True) for _ in range(k)]
+
+
+# ----------------------------------------------------------------- LARS
+
+def lasso_obj(X, D, A, lam):
+    R = X.astype(np.float64) - D.astype(np.float64) @ A
+    return 0.5 * np.sum(R * R, axis=0) + np.asarray(lam, np.float64) * np.abs(A).sum(axis=0)
+
+
+def precision_report(X, D, lam, ns=2000):
+    """Error of each precision variant against the FP64 oracle (the
+    restatement of the reference at T=double) on the first `ns` signals."""
+    import oracle
     import spx
-    import workloads as W
+    Xs = np.asfortranarray(X[:, :ns])
+    lams = lam[:ns]
+    ref = oracle.lasso(Xs.astype(np.float64), D.astype(np.float64), lams.astype(np.float64),
+                       num_threads=cpu_threads()).toarray()
+    fref = lasso_obj(Xs, D, ref, lams)
+    out = {"signals": ns, "oracle": "C restatement of coreLARS2 at FP64"}
+    for prec in ("fp32", "fp32+refit64", "bf16x3", "bf16x3+refit64"):
+        A = spx.lasso(Xs, D, lambda1=lams, precision=prec).toarray().astype(np.float64)
+        num = np.linalg.norm(A - ref, axis=0)
+        nr = np.linalg.norm(ref, axis=0)
+        rel = np.where(nr > 0, num / np.maximum(nr, 1e-30), num)
+        gap = (lasso_obj(Xs, D, A, lams) - fref) / np.maximum(np.abs(fref), 1e-30)
+        out[prec] = {"coef_rel_err_median": float(np.median(rel)),
+                     "coef_rel_err_p99": float(np.percentile(rel, 99)),
+                     "objective_rel_gap_max": float(np.max(np.abs(gap))),
+                     "support_agreement": float(np.mean(np.all((A != 0) == (ref != 0), axis=0)))}
+    return out
 
+
+def served_report(n_jobs=120, rate=150.0, max_n=20000, lanes=4):
+    import server
+    import workloads as W
+    dicts, trace = W.served_jobs(seed=1005, n_jobs=n_jobs, rate=rate, max_n=max_n)
+    jobs = [server.Job(t, di, kind, X, prm) for (t, di, kind, X, prm) in trace]
+    srv = server.SparseCodingServer(dicts, lanes=lanes, hbm_budget=32 << 30)
+    try:
+        srv.run_trace(jobs[:8], realtime=False)  # load the kernels once
+        fresh = [server.Job(j.arrival, j.dict_id, j.kind, j.X, j.param) for j in jobs]
+        done = srv.run_trace(fresh, realtime=True)
+        rep = server.latency_report(done)
+        rep["batches"] = srv.n_batches
+    finally:
+        srv.close()
+    rep.update({"offered_jobs_per_s": rate, "lanes": lanes,
+                "mix": "4 resident dictionaries (64,256) (128,512) (256,1024) (192,512); "
+                       "n log-U[1,20000]; 50/50 LARS lambda log-U[0.05,0.5] / OMP L U{4..32}"})
+    return rep
+
+
+def run_lars(args):
+    import workloads as W
+    E = Engine()
+    spx, torch = E.spx, E.torch
     log("bench: generating workload n=%d" % args.n)
     X, D, lam = W.lars_headline(n=args.n)
     m, n = X.shape
     p = D.shape[1]
-    dev = torch.device("cuda:0")
-    Xt = torch.from_numpy(X.T).to(dev)            # column-major m x n
-    lam_t = torch.from_numpy(lam).to(dev)
-    stream = torch.cuda.current_stream(dev)
-    ctx = spx.Context(0)
-    ctx.set_stream(stream.cuda_stream)
-    d = spx.Dictionary(ctx, D)
+    Xt = E.to_dev(X.T)                 # column-major m x n
+    lam_t = E.to_dev(lam)
+    d = spx.Dictionary(E.ctx, D)
     Lout = spx.lasso_L(m, p, -1, spx.PENALTY, float(lam.max()))
     ldg = (p + 15) // 16 * 16
-    DtX = torch.empty((n, ldg), dtype=torch.float32, device=dev)
-    norms = torch.empty((n,), dtype=torch.float32, device=dev)
-    V = torch.empty((n, Lout), dtype=torch.float32, device=dev)
-    R = torch.empty((n, Lout), dtype=torch.int32, device=dev)
-    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
-    stats = torch.empty((n, 2), dtype=torch.int32, device=dev)
+    DtX = torch.empty((n, ldg), dtype=torch.float32, device=E.dev)
+    norms = torch.empty((n,), dtype=torch.float32, device=E.dev)
+    V = torch.empty((n, Lout), dtype=torch.float32, device=E.dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=E.dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=E.dev)
+    stats = torch.empty((n, 2), dtype=torch.int32, device=E.dev)
 
     def step(form, ev=None):
         if ev:
-            ev[0].record(stream)
-        spx.dtx_dev(ctx, d, Xt, DtX)
-        spx.sqnorms_dev(ctx, Xt, norms)
+            ev[0].record(E.stream)
+        spx.dtx_dev(E.ctx, d, Xt, DtX)
+        spx.sqnorms_dev(E.ctx, Xt, norms)
         if ev:
-            ev[1].record(stream)
-        spx.lasso_dtx_dev(ctx, d, DtX, norms, lam_t, V, R, cnt, form=form,
+            ev[1].record(E.stream)
+        spx.lasso_dtx_dev(E.ctx, d, DtX, norms, lam_t, V, R, cnt, form=form,
                           waves_per_signal=args.waves, stats=stats)
         if ev:
-            ev[2].record(stream)
+            ev[2].record(E.stream)
 
     if args.calib_copy:
-        a = torch.ones((64 << 20,), dtype=torch.float32, device=dev)
+        a = torch.ones((64 << 20,), dtype=torch.float32, device=E.dev)
         b = torch.empty_like(a)
         torch.add(a, 0.0, out=b)
         torch.cuda.synchronize()
@@ -143,14 +192,13 @@ def main():
     for _ in range(args.warmup):
         step("batched")
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [E.events(3) for _ in range(args.steps)]
     log("bench: timing %d steps" % args.steps)
     t0 = time.perf_counter()
     for k in range(args.steps):
         step("batched", evs[k])
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = (t1 - t0) / args.steps
+    wall = (time.perf_counter() - t0) / args.steps
     gemm_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     solve_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
     value = n / wall
@@ -168,7 +216,7 @@ def main():
         log("bench: plain form")
         step("plain")
         torch.cuda.synchronize()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        ev = E.events(3)
         tp = time.perf_counter()
         step("plain", ev)
         torch.cuda.synchronize()
@@ -176,12 +224,33 @@ def main():
         plain = {"value": n / tp, "unit": "signals/s", "solver_ms": ev[1].elapsed_time(ev[2]),
                  "batched_over_plain": value / (n / tp)}
 
+    prec = None
+    if not args.no_precision:
+        log("bench: precision report")
+        prec = precision_report(X, D, lam)
+
+    served = None
+    if not args.no_served:
+        log("bench: served mix")
+        served = served_report()
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline")
-        cpu = cpu_baseline(X, D, lam, args.cpu_seconds)
+        import oracle
+        th = cpu_threads()
+        kind = "reference" if oracle.has_ref() else "port"
+        if kind == "reference":
+            run = lambda ns: oracle.ref_lasso_pervec(np.asfortranarray(X[:, :ns]), D, lam[:ns],
+                                                     num_threads=th)
+        else:
+            run = lambda ns: oracle.lasso(np.asfortranarray(X[:, :ns]), D, lam[:ns], num_threads=th)
+        ns, dt = timed_sample(run, n, args.cpu_seconds)
+        cpu = {"value": ns / dt, "unit": "signals/s", "cores": th, "kind": kind,
+               "sample": f"first {ns} signals of the same workload, {dt:.1f} s, per-signal-lambda "
+                         f"lasso2/coreLARS2 loop of the reference built in place, {th} OpenMP threads"}
 
-    out = {
+    return {
         "metric": METRIC,
         "value": value,
         "unit": "signals/s",
@@ -198,7 +267,7 @@ def main():
                                "5-sparse + 0.05 noise normalised, lambda log-U[0.05,0.5] "
                                "per signal, FP32 MFMA D'X" % n,
                    "m": m, "p": p, "n": n, "form": "batched"},
-        "roofline": {"bound": "hbm", "kernel": "lars_kernel",
+        "roofline": {"bound": "hbm", "kernel": "lars_kernel (fast + full-capacity passes)",
                      "achieved": solver_bytes / (solve_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": solver_bytes / (solve_ms * 1e-3) / HBM_PEAK,
@@ -210,11 +279,185 @@ def main():
                           "frac": gemm_flops / (gemm_ms * 1e-3) / FP32_MFMA_PEAK},
         "kernel_ms": {"dtx_and_norms": gemm_ms, "lars_solver": solve_ms},
         "solver_stats": {"mean_admissions": float(admissions.mean()),
+                         "max_admissions": int(admissions.max()),
                          "mean_steps": float(st[:, 0].mean()),
                          "mean_nnz": float(counts.mean())},
         "plain_gpu": plain,
+        "precision": prec,
+        "served": served,
+        "latency_ms": None if served is None else {"p50": served["p50_ms"], "p99": served["p99_ms"]},
         "cpu_baseline": cpu,
     }
+
+
+# ------------------------------------------------------------------ OMP
+
+def run_omp(args):
+    import workloads as W
+    E = Engine()
+    spx, torch = E.spx, E.torch
+    n = args.n if args.n != 200000 else 100000
+    log("bench: generating OMP workload n=%d" % n)
+    X, D, L, eps = W.omp_batch(n=n)
+    m, p = D.shape
+    Xt = E.to_dev(X.T)
+    d = spx.Dictionary(E.ctx, D)
+    Lout = spx.omp_L(m, p, L)
+    L_t = E.to_dev(np.array([L], np.int32))
+    e_t = E.to_dev(np.array([eps], np.float32))
+    V = torch.empty((n, Lout), dtype=torch.float32, device=E.dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=E.dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=E.dev)
+    ldg = (p + 15) // 16 * 16
+    DtX = torch.empty((n, ldg), dtype=torch.float32, device=E.dev)
+
+    def step(form, ev=None):
+        if ev:
+            ev[0].record(E.stream)
+        spx.omp_dev(E.ctx, d, Xt, V, R, cnt, L=L_t, eps=e_t, form=form,
+                    waves_per_signal=args.waves)
+        if ev:
+            ev[1].record(E.stream)
+
+    for _ in range(args.warmup):
+        step("batched")
+    torch.cuda.synchronize()
+    evs = [E.events(2) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step("batched", evs[k])
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    tot_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    ev = E.events(2)
+    ev[0].record(E.stream)
+    spx.dtx_dev(E.ctx, d, Xt, DtX)
+    ev[1].record(E.stream)
+    torch.cuda.synchronize()
+    gemm_ms = ev[0].elapsed_time(ev[1])
+    solve_ms = tot_ms - gemm_ms
+    counts = cnt.cpu().numpy().astype(np.int64)
+    solver_bytes = float(np.sum(4 * p * (1 + counts) + 8 * counts))
+    plain = None
+    if not args.no_plain:
+        step("plain")
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        step("plain")
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - tp
+        plain = {"value": n / tp, "batched_over_plain": (n / wall) / (n / tp)}
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        th = cpu_threads()
+        run = lambda ns: oracle.ref_omp(np.asfortranarray(X[:, :ns]), D, L=L, eps=eps,
+                                        num_threads=th)
+        ns, dt = timed_sample(run, n, args.cpu_seconds)
+        cpu = {"value": ns / dt, "unit": "signals/s", "cores": th, "kind": "reference",
+               "sample": f"first {ns} signals, {dt:.1f} s, reference _omp built in place"}
+    return {"metric": "signals/sec batch OMP m256 p1024 L32 eps1e-3", "value": n / wall,
+            "unit": "signals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "OMP D 256x1024, X 256x%d k~U{1..32} + 0.01 noise, L=32, "
+                                   "eps=1e-3" % n},
+            "roofline": {"bound": "hbm", "kernel": "ormp_kernel",
+                         "achieved": solver_bytes / (solve_ms * 1e-3) / 1e9,
+                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": solver_bytes / (solve_ms * 1e-3) / HBM_PEAK, "traffic": None},
+            "roofline_gemm": {"bound": "mfma",
+                              "achieved": 2.0 * m * p * n / (gemm_ms * 1e-3) / 1e12,
+                              "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s"},
+            "kernel_ms": {"dtx": gemm_ms, "ormp_solver": solve_ms},
+            "mean_nnz": float(counts.mean()), "plain_gpu": plain, "cpu_baseline": cpu}
+
+
+# ---------------------------------------------------------------- FISTA
+
+def run_fista(args):
+    import oracle
+    import workloads as W
+    E = Engine()
+    spx, torch = E.spx, E.torch
+    n = args.n if args.n != 200000 else 8192
+    X, D = W.fista_flat(n=n)
+    m, p = D.shape
+    Xt = E.to_dev(X.T)
+    out = {"metric": "FISTA-flat 200 iterations, m512 p2048", "unit": "signals/s", "n_gpus": 1,
+           "dtype": "f32", "data": "synthetic", "higher_is_better": True,
+           "config": {"workload": "FISTA-flat square loss, D 512x2048, X 512x%d 20-sparse + 0.02 "
+                                  "noise; l1 lambda=0.05 and elastic-net lambda2=0.01; 200 "
+                                  "iterations, fixed step 1/L from the device power iteration" % n},
+           "variants": {}}
+    ns = 32
+    Xs = np.asfortranarray(X[:, :ns])
+    for prec in ("fp32", "bf16x3"):
+        d = spx.Dictionary(E.ctx, D, precision=prec)
+        L0 = d.lipschitz(200)
+        for regul in ("l1", "elastic-net"):
+            W0 = torch.zeros((n, p), dtype=torch.float32, device=E.dev)
+            Wt = torch.zeros((n, p), dtype=torch.float32, device=E.dev)
+            info = torch.zeros((n, 4), dtype=torch.float32, device=E.dev)
+            kw = dict(regul=regul, lambda2=0.01, max_it=200, L0=L0, fixed_step=True,
+                      tol=0.0, it0=1000, precision=prec)
+            spx.fista_flat_dev(E.ctx, d, Xt, W0, Wt, info, 0.05, **kw)
+            torch.cuda.synchronize()
+            ev = E.events(2)
+            ev[0].record(E.stream)
+            spx.fista_flat_dev(E.ctx, d, Xt, W0, Wt, info, 0.05, **kw)
+            ev[1].record(E.stream)
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1])
+            flops = 200 * 4.0 * m * p * n
+            obj = info.cpu().numpy()[:ns, 0].astype(np.float64)
+            _, io = oracle.fista_flat(Xs.astype(np.float64), D.astype(np.float64),
+                                      np.zeros((p, ns)), 0.05, regul=regul, lambda2=0.01,
+                                      max_it=200, L0=L0, fixed_step=True, tol=0.0, it0=1000,
+                                      num_threads=cpu_threads())
+            gap = (obj - io[0]) / np.abs(io[0])
+            peak = FP32_MFMA_PEAK if prec == "fp32" else BF16_MFMA_PEAK / 3
+            out["variants"][f"{prec}/{regul}"] = {
+                "ms": ms, "signals_per_s": n / (ms * 1e-3),
+                "gemm_tflops_achieved": flops / (ms * 1e-3) / 1e12,
+                "mfma_peak_tflops": peak / 1e12, "frac": flops / (ms * 1e-3) / peak,
+                "objective_rel_gap_vs_fp64_max": float(np.max(np.abs(gap))),
+                "lipschitz": L0}
+        d.close()
+    out["value"] = out["variants"]["fp32/l1"]["signals_per_s"]
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=200000)
+    ap.add_argument("--workload", default="lars", choices=["lars", "omp", "fista", "served"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 256 MiB read + 256 MiB write copy kernel first "
+                         "(calibrates HBM byte counters under a profiler)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
+              "(signals shard across GPUs with no exchange, see the design notes)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "lars":
+        out = run_lars(args)
+    elif args.workload == "omp":
+        out = run_omp(args)
+    elif args.workload == "fista":
+        out = run_fista(args)
+    else:
+        out = {"metric": "served sparse coding p50/p99 latency", "served": served_report(n_jobs=300)}
     print(json.dumps(out), flush=True)
 
 

@@ -46,6 +46,12 @@ extern "C" {
 /* arithmetic of the D'X / Gram / FISTA products (f32 entry points) */
 #define SPX_PREC_FP32 0    /* FP32 MFMA (v_mfma_f32_16x16x4_f32)           */
 #define SPX_PREC_BF16X3 1  /* BF16 hi/lo split, 3 MFMAs, FP32 accumulation  */
+/* flag, combinable with the above (lasso, PENALTY mode): once the homotopy
+ * reaches lambda, refine the coefficients on the final active set to the
+ * FP64 solution of (G_AA + d I) a = (D'x)_A - lambda sign(a) by two steps of
+ * mixed-precision iterative refinement (FP64 residual from the resident Gram,
+ * the solver's own inverse as preconditioner). */
+#define SPX_PREC_REFIT64 2
 
 /* solver launch form */
 #define SPX_FORM_BATCHED 0 /* persistent kernel + device work queue (default) */
@@ -260,6 +266,11 @@ int spx_dtx_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* X,
                     int ldx, int n, float* DtX, int ldc, int precision);
 int spx_dtx_dev_f64(spx_ctx* ctx, const spx_dict* dict, const double* X,
                     int ldx, int n, double* DtX, int ldc);
+/* Largest eigenvalue of the resident Gram matrix (the Lipschitz constant of
+ * the square loss's gradient, the fixed FISTA step 1/L) by `iters` steps of
+ * the power iteration from the all-ones vector.  Synchronous. */
+int spx_dict_lipschitz(spx_ctx* ctx, const spx_dict* dict, int iters, double* L);
+
 /* Copies the resident Gram matrix (p x p, leading dimension p) to `G`
  * (device pointer). */
 int spx_dict_gram_f32(spx_ctx* ctx, const spx_dict* dict, float* G);

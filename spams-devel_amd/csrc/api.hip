@@ -64,7 +64,72 @@ __global__ void transpose_kernel(const T* __restrict__ A, int m, int n,
   }
 }
 
+// y = scale * G x for a p x p column-major G (leading dimension ld),
+// one thread per row.
+template <typename T>
+__global__ __launch_bounds__(256) void gram_gemv_kernel(int p, const T* __restrict__ G,
+                                                        int ld, const T* __restrict__ x,
+                                                        const double* __restrict__ nrm,
+                                                        T* __restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p) return;
+  double acc = 0;
+  for (int j = 0; j < p; ++j) acc += static_cast<double>(G[i + (size_t)j * ld]) * x[j];
+  const double sc = nrm ? 1.0 / *nrm : 1.0;
+  y[i] = static_cast<T>(acc * sc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void norm2_kernel(int p, const T* __restrict__ y,
+                                                     double* __restrict__ out) {
+  __shared__ double red[1024];
+  double s = 0;
+  for (int i = threadIdx.x; i < p; i += 1024) s += static_cast<double>(y[i]) * y[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sqrt(red[0]);
+}
+
+template <typename T>
+__global__ void fill_kernel(T* v, int n, T val) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) v[i] = val;
+}
+
 }  // namespace
+
+template <typename T>
+static int lipschitz(spx_ctx* ctx, const spx_dict* d, int iters, double* L) {
+  const int p = d->p;
+  void* b = nullptr;
+  int rc = ctx->scratch_get(12, 2 * sizeof(T) * (size_t)p + 64, &b);
+  if (rc) return rc;
+  T* x = static_cast<T*>(b);
+  T* y = x + p;
+  double* nrm = reinterpret_cast<double*>(
+      (reinterpret_cast<uintptr_t>(y + p) + 15) & ~uintptr_t(15));
+  hipStream_t st = ctx->stream;
+  const dim3 g((p + 255) / 256);
+  hipLaunchKernelGGL(fill_kernel<T>, g, dim3(256), 0, st, x, p, T(1));
+  hipLaunchKernelGGL(norm2_kernel<T>, dim3(1), dim3(1024), 0, st, p, x, nrm);
+  for (int it = 0; it < iters; ++it) {
+    // x <- G (x / ||x||), then ||x|| -> the eigenvalue estimate
+    hipLaunchKernelGGL(gram_gemv_kernel<T>, g, dim3(256), 0, st, p,
+                       static_cast<const T*>(d->G), d->ldg, x, nrm, y);
+    hipLaunchKernelGGL(norm2_kernel<T>, dim3(1), dim3(1024), 0, st, p, y, nrm);
+    T* t = x;
+    x = y;
+    y = t;
+  }
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipMemcpyAsync(L, nrm, sizeof(double), hipMemcpyDeviceToHost, st));
+  SPX_HIP(hipStreamSynchronize(st));
+  return SPX_OK;
+}
 
 // Clears n ints with a kernel (capture-safe replacement for a memset node).
 int zero_ints(hipStream_t st, int* p, int n) {
@@ -197,6 +262,7 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   a.outR = out->r;
   a.outCount = out->count;
   a.stats = out->stats;
+  a.refit = (o->precision & SPX_PREC_REFIT64) ? 1 : 0;
   return lars_solve<T>(ctx, a, o->form, o->waves_per_signal);
 }
 
@@ -217,7 +283,7 @@ static int lasso_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx,
   DeviceGuard g(ctx->device);
   T* DtX = nullptr;
   T* norms = nullptr;
-  int rc = products<T>(ctx, d, X, ldx, n, o->precision, &DtX, &norms);
+  int rc = products<T>(ctx, d, X, ldx, n, o->precision & SPX_PREC_BF16X3, &DtX, &norms);
   if (rc) return rc;
   return lasso_dtx_dev<T>(ctx, d, DtX, d->ldg, norms, n, lambda, nlam, lambda2, o, out);
 }
@@ -363,7 +429,7 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
     return SPX_OK;
   }
   spx_dict* d = nullptr;
-  int rc = dict_create<T>(ctx, D, m, p, 0, o.precision, &d);
+  int rc = dict_create<T>(ctx, D, m, p, 0, o.precision & SPX_PREC_BF16X3, &d);
   if (rc) return rc;
   DevBuf bX, bL, bV, bR, bC;
   if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bL.alloc(sizeof(T) * nlam)) ||
@@ -790,6 +856,13 @@ int spx_dtx_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx, i
   spx::DeviceGuard g(ctx->device);
   return spx::gemm_tn<double>(ctx->stream, d->p, n, d->m, static_cast<const double*>(d->D),
                               d->m, X, ldx, DtX, ldc, 1.0, 0.0, nullptr, 0, 0);
+}
+
+int spx_dict_lipschitz(spx_ctx* ctx, const spx_dict* d, int iters, double* L) {
+  if (!ctx || !d || !L || iters < 1) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  return d->elem == 4 ? spx::lipschitz<float>(ctx, d, iters, L)
+                      : spx::lipschitz<double>(ctx, d, iters, L);
 }
 
 int spx_dict_gram_f32(spx_ctx* ctx, const spx_dict* d, float* G) {

@@ -64,6 +64,8 @@ struct LarsLds {
     b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
     b = (b + 15) & ~size_t(15);
     if (CAP > 0) b += 2 * sizeof(T) * CAP * CAP + 16;
+    b = (b + 15) & ~size_t(15);
+    b += sizeof(double) * (size_t)(LC + (LC > 64 ? LC : 64)) + 16;  // FP64 vectors
     return b;
   }
 };
@@ -98,6 +100,9 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   T* sM0 = reinterpret_cast<T*>(
       (reinterpret_cast<uintptr_t>(sSig + 1) + 15) & ~uintptr_t(15));  // CAP x CAP
   T* sM1 = sM0 + (CAP > 0 ? CAP * CAP : 0);
+  double* sAlpha = reinterpret_cast<double*>(
+      (reinterpret_cast<uintptr_t>(sM1 + (CAP > 0 ? CAP * CAP : 0)) + 15) & ~uintptr_t(15));
+  double* sRes = sAlpha + LC;  // FP64 refit vectors; sRes has max(LC, 64)
 
   const int tid = threadIdx.x;
   T* const M0 = CAP > 0 ? sM0 : a.scratch + (size_t)blockIdx.x * 2 * L * L;
@@ -168,18 +173,18 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
     }
     int s_active = 0;
     int iter = 0, admissions = 0;
-    bool empty = false, overflow = false;
+    bool empty = false, overflow = false, reached = false;
     {
       const T dc = sDtR[currentInd];
       if (a.mode == SPX_PENALTY && fabs(dc) < lam) empty = true;
       if (a.mode == SPX_L2ERROR && normX < lam) empty = true;
     }
 
+    T* Mc = M0;
+    T* Mn = M1;
     if (!empty) {
       bool newAtom = true;
       T thrs = 0;
-      T* Mc = M0;
-      T* Mn = M1;
       for (int i = 0; i < Lm; ++i) {
         ++iter;
         if (newAtom) {
@@ -216,7 +221,24 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
             T dv = 0, z1 = 0, z2 = 0;
             for (int r = tid; r < i; r += GT) dv += sV[r] * sGs[r];
             group_sum3<T, W>(dv, z1, z2, sRed3);
-            const T schur = T(1) / (sGs[i] - dv);
+            T schur = T(1) / (sGs[i] - dv);
+            if (sizeof(T) == 4 && !(sGs[i] - dv > T(1e-3) * sGs[i])) {
+              // low pivot (below 1e-3 of the diagonal, or not positive):
+              // recompute v = invGs gs and the pivot with FP64 accumulation
+              __syncthreads();
+              double pd = 0;
+              for (int r = tid; r < i; r += GT) {
+                double acc = 0;
+                for (int q = 0; q < i; ++q)
+                  acc += static_cast<double>(Mc[r + q * ldm]) * static_cast<double>(sGs[q]);
+                sV[r] = static_cast<T>(acc);
+                pd += acc * static_cast<double>(sGs[r]);
+              }
+              double z1d = 0, z2d = 0;
+              group_sum3<double, W>(pd, z1d, z2d, reinterpret_cast<double*>(sRes));
+              schur = static_cast<T>(1.0 / (static_cast<double>(sGs[i]) - pd));
+              __syncthreads();
+            }
             // Schur-complement update of the inverse (decomp/decomp.h:1756-1770)
             for (int r = tid; r < i; r += GT) {
               const T vr = sV[r];
@@ -354,6 +376,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         step = step < step_max2 ? step : step_max2;
         step = step < step_max ? step : step_max;
         if (step == INF) break;
+        reached = (a.mode == SPX_PENALTY) && (step == step_max2);
 
         // -------------------------------------------------- updates
         for (int j = tid; j < s; j += GT) {
@@ -427,6 +450,38 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
             (a.mode == SPX_L1COEFFS && (lam - thrs < 1e-15)))
           break;
       }
+    }
+
+    // ------------------------------- optional FP64 refit (PENALTY, at lambda)
+    if (a.refit && reached && !overflow && s_active > 0) {
+      __syncthreads();
+      const int s = s_active;
+      T* Mf = Mc;  // the inverse of the final active block
+      for (int r = tid; r < s; r += GT) sAlpha[r] = static_cast<double>(sCoef[r]);
+      __syncthreads();
+      for (int it = 0; it < 2; ++it) {
+        // res = (D'x)_A - lam sign(alpha) - (G_AA + d I) alpha   (FP64)
+        for (int r = tid; r < s; r += GT) {
+          const int ir = sInd[r];
+          const T* gcol = a.G + (size_t)ir * ldg;
+          double acc = static_cast<double>(dtx[ir]) -
+                       static_cast<double>(lam) * (sCoef[r] > 0 ? 1.0 : -1.0);
+          for (int q = 0; q < s; ++q) {
+            const double g = static_cast<double>(gcol[sInd[q]]) + (q == r ? static_cast<double>(diag) : 0.0);
+            acc -= g * sAlpha[q];
+          }
+          sRes[r] = acc;
+        }
+        __syncthreads();
+        // alpha += invGs res
+        for (int r = tid; r < s; r += GT) {
+          double acc = 0;
+          for (int q = 0; q < s; ++q) acc += static_cast<double>(Mf[r + q * ldm]) * sRes[q];
+          sAlpha[r] += acc;
+        }
+        __syncthreads();
+      }
+      for (int r = tid; r < s; r += GT) sCoef[r] = static_cast<T>(sAlpha[r]);
     }
 
     // ------------------------------------------------------- outputs

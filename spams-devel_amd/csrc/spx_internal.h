@@ -45,6 +45,7 @@ struct LarsArgs {
   const int* list_count;
   int chunk;        // signals taken per atomic grab of the work counter
   const int* order; // work order (signal index per work item) or null
+  int refit;        // FP64 refit of the final active set (PENALTY mode)
 };
 
 // Kernel arguments of the ORMP solver (coreORMP semantics).

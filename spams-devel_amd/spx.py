@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "libspx.so")
 
 # constraint_type (decomp/decomp.h:95)
 L1COEFFS, L2ERROR, PENALTY = 0, 1, 2
-PREC = {"fp32": 0, "bf16x3": 1}
+PREC = {"fp32": 0, "bf16x3": 1, "fp32+refit64": 2, "bf16x3+refit64": 3}
 FORM = {"batched": 0, "plain": 1}
 
 _P = ctypes.c_void_p
@@ -182,6 +182,14 @@ class Dictionary:
             f.argtypes = [_P, _P, _I, _I, _I, _P]
             _check(f(ctx.h, ptr, m, p, on_dev, ctypes.byref(self.h)))
         del keep
+
+    def lipschitz(self, iters: int = 100) -> float:
+        """Largest eigenvalue of D'D by the device power iteration."""
+        out = ctypes.c_double()
+        f = lib().spx_dict_lipschitz
+        f.argtypes = [_P, _P, _I, ctypes.POINTER(ctypes.c_double)]
+        _check(f(self.ctx.h, self.h, iters, ctypes.byref(out)))
+        return out.value
 
     @property
     def nbytes(self) -> int:

@@ -245,3 +245,44 @@ def test_fista_f32_against_oracle(spx, orc):
                                 L0=L0, fixed_step=True, tol=0.0, it0=1000)
         assert np.abs(Wg - Wo).max() <= 1e-4
         np.testing.assert_allclose(ig[0], io[0], rtol=1e-5)
+
+
+# -------------------------------------------------------------- precision
+
+def test_refit64_moves_fp32_results_to_the_fp64_solution(spx, orc):
+    """The FP64 refit of the final active set brings FP32 coefficients to
+    the FP64 oracle's where the supports agree (tolerance 1e-6 absolute)."""
+    print("case lasso refit64", flush=True)
+    X, D, lam = W.lars_headline(seed=79, n=400, dtype=np.float32)
+    ref = dense(orc.lasso(X.astype(np.float64), D.astype(np.float64), lam.astype(np.float64)))
+    plain = dense(spx.lasso(X, D, lambda1=lam))
+    refit = dense(spx.lasso(X, D, lambda1=lam, precision="fp32+refit64"))
+    same = np.all((refit != 0) == (ref != 0), axis=0)
+    assert same.mean() >= 0.98
+    err_plain = np.abs(plain[:, same] - ref[:, same]).max()
+    err_refit = np.abs(refit[:, same] - ref[:, same]).max()
+    assert err_refit <= 1e-6
+    assert err_refit <= err_plain
+
+
+def test_bf16x3_lasso_within_stated_tolerance(spx, orc):
+    print("case lasso bf16x3", flush=True)
+    X, D, lam = W.lars_headline(seed=80, n=300, dtype=np.float32)
+    ref = dense(orc.lasso(X.astype(np.float64), D.astype(np.float64), lam.astype(np.float64)))
+    A = dense(spx.lasso(X, D, lambda1=lam, precision="bf16x3"))
+    same = np.all((A != 0) == (ref != 0), axis=0)
+    assert same.mean() >= 0.95
+    oa = lasso_obj(X, D, A, lam)
+    ob = lasso_obj(X, D, ref, lam)
+    assert np.all(np.abs(oa - ob) <= 1e-4 * np.abs(ob))
+
+
+def test_device_power_iteration_lipschitz(spx):
+    print("case lipschitz", flush=True)
+    rng = np.random.default_rng(81)
+    D = W.unit_dict(rng, 64, 200, np.float32)
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    L = d.lipschitz(300)
+    ev = np.linalg.eigvalsh(D.astype(np.float64).T @ D)[-1]
+    assert abs(L - ev) <= 1e-4 * ev
