@@ -70,16 +70,36 @@ __device__ __forceinline__ void store_block(const V& acc, int i0, int j0,
 }
 
 // ----------------------------------------------------------- FP32 / FP64
+//
+// K is staged in chunks of BK = 16.  Within a chunk, MFMA step t (0..3) of
+// lane l uses k = 4*(l/16) + t, so a lane's four k values are contiguous and
+// are read from LDS as one 16-byte (FP32) / two 16-byte (FP64) loads; the
+// four steps together cover the 16 k of the chunk.  LDS tiles are [row][k]
+// with a row stride of BK + 4 elements.  The next chunk is loaded into
+// registers while the current one is multiplied (one LDS buffer, register
+// double buffering).
 
 template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec4<double> {
+  typedef double type __attribute__((ext_vector_type(4)));
+};
+
+template <typename T, bool VECLOAD>
 __global__ __launch_bounds__(256) void gemm_tn_kernel(
     int M, int N, int K, const T* __restrict__ A, int lda,
     const T* __restrict__ B, int ldb, T* __restrict__ C, int ldc, T alpha,
     T beta, const T* __restrict__ E, int lde) {
   constexpr int BK = 16;
-  constexpr int LDS_LD = BM + 4;  // [k][i] layout, padded
-  __shared__ T As[BK * LDS_LD];
-  __shared__ T Bs[BK * LDS_LD];
+  constexpr int LD = BK + 4;  // LDS row stride (elements)
+  typedef typename Vec4<T>::type V4;
+  __shared__ __attribute__((aligned(16))) T As[BM * LD];
+  __shared__ __attribute__((aligned(16))) T Bs[BN * LD];
   typedef typename Acc<T>::type V;
 
   const int tid = threadIdx.x;
@@ -96,42 +116,52 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
 
-  // loader mapping: thread t loads 8 consecutive k of row (t >> 1)
-  const int lr = tid >> 1, lk = (tid & 1) * 8;
-
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    {
-      const int i = bm + lr;
-      const T* src = A + (size_t)i * lda + k0 + lk;
+  // loader: thread t loads rows (t >> 2) and (t >> 2) + 64, k quad (t & 3)
+  const int lr = tid >> 2, lq = (tid & 3) * 4;
+  V4 ra[2], rb[2];
+  auto load = [&](int k0) {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = k0 + lk + e;
-        As[(lk + e) * LDS_LD + lr] = (i < M && k < K) ? src[e] : T(0);
-      }
-      const int j = bn + lr;
-      const T* srcb = B + (size_t)j * ldb + k0 + lk;
+    for (int h = 0; h < 2; ++h) {
+      const int i = bm + lr + 64 * h, j = bn + lr + 64 * h, k = k0 + lq;
+      if (VECLOAD && k + 3 < K) {
+        ra[h] = i < M ? *reinterpret_cast<const V4*>(A + (size_t)i * lda + k) : V4(0);
+        rb[h] = j < N ? *reinterpret_cast<const V4*>(B + (size_t)j * ldb + k) : V4(0);
+      } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = k0 + lk + e;
-        Bs[(lk + e) * LDS_LD + lr] = (j < N && k < K) ? srcb[e] : T(0);
+        for (int e = 0; e < 4; ++e) {
+          ra[h][e] = (i < M && k + e < K) ? A[(size_t)i * lda + k + e] : T(0);
+          rb[h][e] = (j < N && k + e < K) ? B[(size_t)j * ldb + k + e] : T(0);
+        }
       }
     }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      *reinterpret_cast<V4*>(&As[(lr + 64 * h) * LD + lq]) = ra[h];
+      *reinterpret_cast<V4*>(&Bs[(lr + 64 * h) * LD + lq]) = rb[h];
+    }
+  };
+
+  load(0);
+  const int kq = 4 * (lane >> 4);
+  for (int k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();
+    store();
+    __syncthreads();
+    if (k0 + BK < K) load(k0 + BK);
+    V4 a4[4], b4[4];
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      T a[4], b[4];
+    for (int x = 0; x < 4; ++x) {
+      a4[x] = *reinterpret_cast<const V4*>(&As[(wm + 16 * x + (lane & 15)) * LD + kq]);
+      b4[x] = *reinterpret_cast<const V4*>(&Bs[(wn + 16 * x + (lane & 15)) * LD + kq]);
+    }
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        a[x] = As[kr * LDS_LD + wm + 16 * x + (lane & 15)];
-        b[x] = Bs[kr * LDS_LD + wn + 16 * x + (lane & 15)];
-      }
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = Acc<T>::mfma(a[x], b[y], acc[x][y]);
-    }
-    __syncthreads();
+        for (int y = 0; y < 4; ++y) acc[x][y] = Acc<T>::mfma(a4[x][t], b4[y][t], acc[x][y]);
   }
 #pragma unroll
   for (int x = 0; x < 4; ++x)
@@ -273,8 +303,16 @@ int gemm_tn(hipStream_t st, int M, int N, int K, const T* A, int lda,
       return SPX_OK;
     }
   }
-  hipLaunchKernelGGL(gemm_tn_kernel<T>, grid, dim3(256), 0, st, M, N, K, A,
-                     lda, B, ldb, C, ldc, alpha, beta, E, lde);
+  // 16-byte-aligned rows of whole quads allow the vector loads
+  const bool vec = (lda % 4 == 0) && (ldb % 4 == 0) &&
+                   (reinterpret_cast<uintptr_t>(A) % (4 * sizeof(T)) == 0) &&
+                   (reinterpret_cast<uintptr_t>(B) % (4 * sizeof(T)) == 0);
+  if (vec)
+    hipLaunchKernelGGL((gemm_tn_kernel<T, true>), grid, dim3(256), 0, st, M, N, K, A,
+                       lda, B, ldb, C, ldc, alpha, beta, E, lde);
+  else
+    hipLaunchKernelGGL((gemm_tn_kernel<T, false>), grid, dim3(256), 0, st, M, N, K, A,
+                       lda, B, ldb, C, ldc, alpha, beta, E, lde);
   SPX_HIP(hipGetLastError());
   return SPX_OK;
 }

@@ -59,7 +59,7 @@ struct LarsLds {
   static constexpr int PT = VEC * GT * NQ;
   // byte size for active-set capacity LC
   static size_t bytes(int LC) {
-    size_t b = sizeof(T) * (PT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC + PT;
+    size_t b = sizeof(T) * (PT + GT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC + PT;
     b = (b + 15) & ~size_t(15);
     b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
     b = (b + 15) & ~size_t(15);
@@ -70,12 +70,43 @@ struct LarsLds {
   }
 };
 
+// y(0:n) = M(0:n, 0:n) x for a symmetric M (leading dimension ldm), with
+// all GT threads: row r is split into P = GT / n parts whose partial sums
+// are combined in part order (deterministic).  x and y in LDS, part a
+// GT-entry LDS buffer.  Ends with a barrier.
+template <typename T, int GT, bool SPLIT>
+__device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
+                                          T* part, int n) {
+  const int tid = threadIdx.x;
+  // rows are split only when M is in HBM (several loads in flight per row);
+  // with M in LDS one thread per row is faster
+  const int P = (!SPLIT || n >= GT) ? 1 : GT / n;
+  const int seg = (n + P - 1) / P;
+  for (int t = tid; t < n * P; t += GT) {
+    const int r = t % n, k = t / n;
+    const int q0 = k * seg, q1 = q0 + seg < n ? q0 + seg : n;
+    T acc = 0;
+    for (int q = q0; q < q1; ++q) acc += M[r + q * ldm] * x[q];
+    if (P == 1) y[r] = acc;
+    else part[k * n + r] = acc;
+  }
+  __syncthreads();
+  if (P > 1) {
+    for (int r = tid; r < n; r += GT) {
+      T acc = part[r];
+      for (int k = 1; k < P; ++k) acc += part[k * n + r];
+      y[r] = acc;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T, int W, int NQ, int CAP>
 __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   constexpr int GT = 64 * W;
   constexpr int VEC = Vec16<T>::N;
   constexpr int PT = VEC * GT * NQ;  // rows covered by the group
-  constexpr int UNR = NQ >= 4 ? 2 : 4;  // Gram columns in flight per round
+  constexpr int UNR = NQ >= 4 ? 2 : (NQ == 2 ? 4 : 8);  // Gram columns in flight
   typedef typename Vec16<T>::type V;
   const T INF = dev_inf<T>();
 
@@ -85,7 +116,8 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   const int p = a.p;
   const int ldg = a.ldg;
   T* sDtR = reinterpret_cast<T*>(smem);  // PT
-  T* sCoef = sDtR + PT;                   // LC
+  T* sPart = sDtR + PT;                   // GT partial sums
+  T* sCoef = sPart + GT;                  // LC
   T* sU = sCoef + LC;                     // LC
   T* sW = sU + LC;                        // LC
   T* sGs = sW + LC;                       // LC
@@ -212,12 +244,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
             if (tid == 0) Mc[0] = T(1) / sGs[0];
           } else {
             // v = invGs(i x i) * gs(0:i)
-            for (int r = tid; r < i; r += GT) {
-              T acc = 0;
-              for (int q = 0; q < i; ++q) acc += Mc[r + q * ldm] * sGs[q];
-              sV[r] = acc;
-            }
-            __syncthreads();
+            symv_dist<T, GT, CAP == 0>(Mc, ldm, sGs, sV, sPart, i);
             T dv = 0, z1 = 0, z2 = 0;
             for (int r = tid; r < i; r += GT) dv += sV[r] * sGs[r];
             group_sum3<T, W>(dv, z1, z2, sRed3);
@@ -239,17 +266,28 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
               schur = static_cast<T>(1.0 / (static_cast<double>(sGs[i]) - pd));
               __syncthreads();
             }
-            // Schur-complement update of the inverse (decomp/decomp.h:1756-1770)
-            for (int r = tid; r < i; r += GT) {
-              const T vr = sV[r];
-              for (int q = 0; q < i; ++q) {
-                const T vq = sV[q];
-                // symmetric rounding: element (r,q) and (q,r) get the
-                // identical update v[min] * (schur * v[max])
+            // Schur-complement update of the inverse (decomp/decomp.h:1756-1770),
+            // one element per thread; element (r,q) and (q,r) get the
+            // identical update v[min] * (schur * v[max]) (exact symmetry)
+            if (CAP == 0) {
+              for (int e = tid; e < i * i; e += GT) {
+                const int r = e % i, q = e / i;
+                const T vr = sV[r], vq = sV[q];
                 const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
                 Mc[r + q * ldm] += upd;
               }
-              const T off = -schur * vr;
+            } else {
+              for (int r = tid; r < i; r += GT) {
+                const T vr = sV[r];
+                for (int q = 0; q < i; ++q) {
+                  const T vq = sV[q];
+                  const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
+                  Mc[r + q * ldm] += upd;
+                }
+              }
+            }
+            for (int r = tid; r < i; r += GT) {
+              const T off = -schur * sV[r];
               Mc[r + i * ldm] = off;
               Mc[i + r * ldm] = off;
             }
@@ -262,12 +300,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         // ------------------------------------------ direction u = invGs*w
         for (int j = tid; j < s; j += GT) sW[j] = sDtR[sInd[j]] > 0 ? T(1) : T(-1);
         __syncthreads();
-        for (int r = tid; r < s; r += GT) {
-          T acc = 0;
-          for (int q = 0; q < s; ++q) acc += Mc[r + q * ldm] * sW[q];
-          sU[r] = acc;
-        }
-        __syncthreads();
+        symv_dist<T, GT, CAP == 0>(Mc, ldm, sW, sU, sPart, s);
 
         // ------------------ zero-crossing step, polynomial coefficients
         T step_max;
@@ -405,14 +438,24 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
           }
           __syncthreads();
           const T alpha = T(-1) / schur;
-          for (int r = tid; r < i; r += GT) {
-            const int r2 = r + (r >= fz);
-            const T ur = sV[r];
-            for (int q = 0; q < i; ++q) {
-              const int q2 = q + (q >= fz);
-              const T uq = sV[q];
+          if (CAP == 0) {
+            for (int e = tid; e < i * i; e += GT) {
+              const int r = e % i, q = e / i;
+              const int r2 = r + (r >= fz), q2 = q + (q >= fz);
+              const T ur = sV[r], uq = sV[q];
               const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
               Mn[r + q * ldm] = Mc[r2 + q2 * ldm] + upd;
+            }
+          } else {
+            for (int r = tid; r < i; r += GT) {
+              const int r2 = r + (r >= fz);
+              const T ur = sV[r];
+              for (int q = 0; q < i; ++q) {
+                const int q2 = q + (q >= fz);
+                const T uq = sV[q];
+                const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
+                Mn[r + q * ldm] = Mc[r2 + q2 * ldm] + upd;
+              }
             }
           }
           for (int j = tid; j < i; j += GT) {
