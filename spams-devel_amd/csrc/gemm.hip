@@ -178,14 +178,16 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
+template <bool VECLOAD>
 __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
     int M, int N, int K, const float* __restrict__ A, int lda,
     const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
     float alpha, float beta, const float* __restrict__ E, int lde) {
   constexpr int BK = 32;
-  constexpr int LDS_LD = BK + 8;  // [i][k] layout (k contiguous), padded
-  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM * LDS_LD], Al[BM * LDS_LD];
-  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * LDS_LD], Bl[BN * LDS_LD];
+  constexpr int LD = BK + 8;  // [row][k] layout (k contiguous), padded
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM * LD], Al[BM * LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * LD], Bl[BN * LD];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -203,44 +205,71 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
 
   // loader: thread t loads 16 consecutive k of row (t >> 1)
   const int lr = tid >> 1, lk = (tid & 1) * 16;
-
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    {
-      const int i = bm + lr;
-      const float* src = A + (size_t)i * lda + k0 + lk;
+  f4 ra[4], rb[4];
+  auto load = [&](int k0) {
+    const int i = bm + lr, j = bn + lr;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int k = k0 + lk + e;
-        float x = (i < M && k < K) ? src[e] : 0.f;
-        __bf16 h, l;
-        split_bf16(x, h, l);
-        Ah[lr * LDS_LD + lk + e] = h;
-        Al[lr * LDS_LD + lk + e] = l;
-      }
-      const int j = bn + lr;
-      const float* srcb = B + (size_t)j * ldb + k0 + lk;
+    for (int v = 0; v < 4; ++v) {
+      const int k = k0 + lk + 4 * v;
+      if (VECLOAD && k + 3 < K) {
+        ra[v] = i < M ? *reinterpret_cast<const f4*>(A + (size_t)i * lda + k) : f4(0);
+        rb[v] = j < N ? *reinterpret_cast<const f4*>(B + (size_t)j * ldb + k) : f4(0);
+      } else {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int k = k0 + lk + e;
-        float x = (j < N && k < K) ? srcb[e] : 0.f;
-        __bf16 h, l;
-        split_bf16(x, h, l);
-        Bh[lr * LDS_LD + lk + e] = h;
-        Bl[lr * LDS_LD + lk + e] = l;
+        for (int e = 0; e < 4; ++e) {
+          ra[v][e] = (i < M && k + e < K) ? A[(size_t)i * lda + k + e] : 0.f;
+          rb[v][e] = (j < N && k + e < K) ? B[(size_t)j * ldb + k + e] : 0.f;
+        }
       }
     }
+  };
+  auto store = [&]() {
+    bf16x8 h[2], l[2];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        __bf16 hh, ll;
+        split_bf16(ra[v][e], hh, ll);
+        h[v >> 1][(v & 1) * 4 + e] = hh;
+        l[v >> 1][(v & 1) * 4 + e] = ll;
+      }
+    *reinterpret_cast<bf16x8*>(&Ah[lr * LD + lk]) = h[0];
+    *reinterpret_cast<bf16x8*>(&Ah[lr * LD + lk + 8]) = h[1];
+    *reinterpret_cast<bf16x8*>(&Al[lr * LD + lk]) = l[0];
+    *reinterpret_cast<bf16x8*>(&Al[lr * LD + lk + 8]) = l[1];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        __bf16 hh, ll;
+        split_bf16(rb[v][e], hh, ll);
+        h[v >> 1][(v & 1) * 4 + e] = hh;
+        l[v >> 1][(v & 1) * 4 + e] = ll;
+      }
+    *reinterpret_cast<bf16x8*>(&Bh[lr * LD + lk]) = h[0];
+    *reinterpret_cast<bf16x8*>(&Bh[lr * LD + lk + 8]) = h[1];
+    *reinterpret_cast<bf16x8*>(&Bl[lr * LD + lk]) = l[0];
+    *reinterpret_cast<bf16x8*>(&Bl[lr * LD + lk + 8]) = l[1];
+  };
+
+  load(0);
+  // 16x16x32 operand layout: lane l holds row (l % 16), k = 8*(l/16)+0..7
+  const int ko = 8 * (lane >> 4);
+  for (int k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();
-    // 16x16x32 operand layout: lane l holds row (l % 16), k = 8*(l/16)+0..7
-    const int ko = 8 * (lane >> 4);
+    store();
+    __syncthreads();
+    if (k0 + BK < K) load(k0 + BK);
     bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
-      const int ra = (wm + 16 * x + (lane & 15)) * LDS_LD + ko;
-      const int rb = (wn + 16 * x + (lane & 15)) * LDS_LD + ko;
-      ah[x] = *reinterpret_cast<const bf16x8*>(&Ah[ra]);
-      al[x] = *reinterpret_cast<const bf16x8*>(&Al[ra]);
-      bh[x] = *reinterpret_cast<const bf16x8*>(&Bh[rb]);
-      bl[x] = *reinterpret_cast<const bf16x8*>(&Bl[rb]);
+      const int ra_ = (wm + 16 * x + (lane & 15)) * LD + ko;
+      const int rb_ = (wn + 16 * x + (lane & 15)) * LD + ko;
+      ah[x] = *reinterpret_cast<const bf16x8*>(&Ah[ra_]);
+      al[x] = *reinterpret_cast<const bf16x8*>(&Al[ra_]);
+      bh[x] = *reinterpret_cast<const bf16x8*>(&Bh[rb_]);
+      bl[x] = *reinterpret_cast<const bf16x8*>(&Bl[rb_]);
     }
 #pragma unroll
     for (int x = 0; x < 4; ++x)
@@ -250,7 +279,6 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
         acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
         acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
       }
-    __syncthreads();
   }
 #pragma unroll
   for (int x = 0; x < 4; ++x)
@@ -296,9 +324,16 @@ int gemm_tn(hipStream_t st, int M, int N, int K, const T* A, int lda,
     return SPX_OK;
   }
   if constexpr (sizeof(T) == 4) {
-    if (precision == SPX_PREC_BF16X3) {
-      hipLaunchKernelGGL(gemm_tn_bf16x3_kernel, grid, dim3(256), 0, st, M, N,
-                         K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde);
+    if (precision & SPX_PREC_BF16X3) {
+      const bool vec = (lda % 4 == 0) && (ldb % 4 == 0) &&
+                       (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
+                       (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+      if (vec)
+        hipLaunchKernelGGL(gemm_tn_bf16x3_kernel<true>, grid, dim3(256), 0, st, M, N,
+                           K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde);
+      else
+        hipLaunchKernelGGL(gemm_tn_bf16x3_kernel<false>, grid, dim3(256), 0, st, M, N,
+                           K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde);
       SPX_HIP(hipGetLastError());
       return SPX_OK;
     }
