@@ -60,22 +60,34 @@ def timed_sample(run, n_total, seconds):
     return ns, dt
 
 
-def measured_traffic(prefix):
-    """HBM bytes per launch of the kernels whose names start with `prefix`,
-    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the
-    headline workload (null when absent)."""
+def measured_traffic(names):
+    """HBM bytes per step of the kernels `names` (one launch each), from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the headline
+    workload; null when the summary lacks any of them (e.g. the profile was
+    taken at another group width)."""
     path = os.path.join(ROOT, "profiles", "headline_counters.json")
     try:
         with open(path) as f:
             d = json.load(f)["bytes_per_dispatch"]
     except (OSError, ValueError, KeyError):
         return None
-    tot, hit = 0.0, False
-    for k, v in d.items():
-        if k.startswith(prefix):
-            tot += v
-            hit = True
-    return tot if hit else None
+    if not all(k in d for k in names):
+        return None
+    return float(sum(d[k] for k in names))
+
+
+def lars_kernel_names(p, w, elem=4):
+    """Template names of the two LARS launches at group width w: the LDS pass
+    (capacity 24 in FP32, 16 in FP64) and the full-capacity pass (width >= 4);
+    NQ = chunks of 16 bytes per lane covering p, rounded up to a power of 2."""
+    t = "float" if elem == 4 else "double"
+    cap = 24 if elem == 4 else 16
+
+    def nq(w_):
+        c = -(-p // ((16 // elem) * 64 * w_))
+        return next(q for q in (1, 2, 4, 8, 16) if q >= c)
+    w2 = max(4, w)
+    return [f"lars_kernel<{t}, {w}, {nq(w)}, {cap}>", f"lars_kernel<{t}, {w2}, {nq(w2)}, 0>"]
 
 
 class Engine:
@@ -137,11 +149,13 @@ def served_report(n_jobs=120, rate=150.0, max_n=20000, lanes=4):
     jobs = [server.Job(t, di, kind, X, prm) for (t, di, kind, X, prm) in trace]
     srv = server.SparseCodingServer(dicts, lanes=lanes, hbm_budget=32 << 30)
     try:
+        tuned = srv.autotune(jobs[:32])
         srv.run_trace(jobs[:8], realtime=False)  # load the kernels once
         fresh = [server.Job(j.arrival, j.dict_id, j.kind, j.X, j.param) for j in jobs]
         done = srv.run_trace(fresh, realtime=True)
         rep = server.latency_report(done)
         rep["batches"] = srv.n_batches
+        rep["autotune"] = {f"dict{k[0]}/{k[1]}": v[0] for k, v in tuned.items()}
     finally:
         srv.close()
     rep.update({"offered_jobs_per_s": rate, "lanes": lanes,
@@ -182,6 +196,13 @@ def run_lars(args):
         if ev:
             ev[2].record(E.stream)
 
+    tune = None
+    if args.waves <= 0:
+        log("bench: autotune")
+        ns = min(n, 16384)
+        w, trial = d.autotune_lasso(Xt[:ns], lam_t[:ns])
+        tune = {"waves_per_signal": w, "trial_ms": trial, "sample_signals": ns}
+
     if args.calib_copy:
         a = torch.ones((64 << 20,), dtype=torch.float32, device=E.dev)
         b = torch.empty_like(a)
@@ -194,13 +215,19 @@ def run_lars(args):
     torch.cuda.synchronize()
     evs = [E.events(3) for _ in range(args.steps)]
     log("bench: timing %d steps" % args.steps)
+    E.ctx.reset_timing()
+    E.ctx.set_timing(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
         step("batched", evs[k])
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
+    E.ctx.set_timing(False)
+    passes = {k: v[0] / max(v[1], 1) for k, v in E.ctx.read_timing().items() if v[1]}
     gemm_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     solve_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    kern_ms = passes.get("lars_fast", 0.0) + passes.get("lars_full", 0.0)
+    waves_used = args.waves if args.waves > 0 else d.waves("lasso")
     value = n / wall
 
     st = stats.cpu().numpy()
@@ -267,17 +294,22 @@ def run_lars(args):
                                "5-sparse + 0.05 noise normalised, lambda log-U[0.05,0.5] "
                                "per signal, FP32 MFMA D'X" % n,
                    "m": m, "p": p, "n": n, "form": "batched"},
-        "roofline": {"bound": "hbm", "kernel": "lars_kernel (fast + full-capacity passes)",
-                     "achieved": solver_bytes / (solve_ms * 1e-3) / 1e9,
+        "roofline": {"bound": "hbm",
+                     "kernel": "lars_kernel: LDS pass over all signals + full-capacity pass "
+                               "over the overflow list (one launch each per step)",
+                     "achieved": solver_bytes / (kern_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": solver_bytes / (solve_ms * 1e-3) / HBM_PEAK,
-                     "traffic": measured_traffic("lars_kernel"),
-                     "algorithmic_bytes": solver_bytes},
+                     "frac": solver_bytes / (kern_ms * 1e-3) / HBM_PEAK,
+                     "traffic": measured_traffic(lars_kernel_names(p, waves_used)),
+                     "algorithmic_bytes": solver_bytes,
+                     "launch_ms": {k: passes[k] for k in ("lars_fast", "lars_full") if k in passes}},
         "roofline_gemm": {"bound": "mfma", "kernel": "gemm_tn_kernel<float>",
                           "achieved": gemm_flops / (gemm_ms * 1e-3) / 1e12,
                           "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
                           "frac": gemm_flops / (gemm_ms * 1e-3) / FP32_MFMA_PEAK},
-        "kernel_ms": {"dtx_and_norms": gemm_ms, "lars_solver": solve_ms},
+        "kernel_ms": {"dtx_and_norms": gemm_ms, "lars_solver": solve_ms,
+                      "lars_order_sort": passes.get("lars_order", 0.0)},
+        "autotune": tune,
         "solver_stats": {"mean_admissions": float(admissions.mean()),
                          "max_admissions": int(admissions.max()),
                          "mean_steps": float(st[:, 0].mean()),
@@ -319,6 +351,11 @@ def run_omp(args):
         if ev:
             ev[1].record(E.stream)
 
+    tune = None
+    if args.waves <= 0:
+        ns = min(n, 16384)
+        w, trial = d.autotune_omp(Xt[:ns], L=L_t, eps=e_t)
+        tune = {"waves_per_signal": w, "trial_ms": trial, "sample_signals": ns}
     for _ in range(args.warmup):
         step("batched")
     torch.cuda.synchronize()
@@ -369,7 +406,7 @@ def run_omp(args):
             "roofline_gemm": {"bound": "mfma",
                               "achieved": 2.0 * m * p * n / (gemm_ms * 1e-3) / 1e12,
                               "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s"},
-            "kernel_ms": {"dtx": gemm_ms, "ormp_solver": solve_ms},
+            "kernel_ms": {"dtx": gemm_ms, "ormp_solver": solve_ms}, "autotune": tune,
             "mean_nnz": float(counts.mean()), "plain_gpu": plain, "cpu_baseline": cpu}
 
 

@@ -123,6 +123,21 @@ void spx_ctx_destroy(spx_ctx* ctx);
 int spx_ctx_set_stream(spx_ctx* ctx, void* hip_stream);
 int spx_ctx_own_stream(spx_ctx* ctx);
 int spx_ctx_synchronize(spx_ctx* ctx);
+
+/* Launch timing.  While on, the batched solvers record HIP events on the
+ * context's stream around each of their launches, under these timers: */
+enum {
+  SPX_TIMER_LARS_ORDER = 0, /* work-order sort by lambda (longest path first) */
+  SPX_TIMER_LARS_FAST = 1,  /* LARS pass with the active set in LDS */
+  SPX_TIMER_LARS_FULL = 2,  /* LARS pass over the signals that outgrew LDS */
+  SPX_TIMER_ORMP = 3,       /* ORMP solver */
+  SPX_TIMER_COUNT = 8
+};
+int spx_ctx_set_timing(spx_ctx* ctx, int on);
+/* Totals since the last reset: ms[i] and launches[i] for i < count (either
+ * may be NULL).  Waits for the recorded events. */
+int spx_ctx_read_timing(spx_ctx* ctx, double* ms, long long* launches, int count);
+int spx_ctx_reset_timing(spx_ctx* ctx);
 /* Thread-local message of the last failing call on this thread. */
 const char* spx_last_error(void);
 /* Library version string. */
@@ -143,6 +158,37 @@ void spx_dict_destroy(spx_dict* dict);
 int spx_dict_dims(const spx_dict* dict, int* m, int* p);
 /* Device bytes held by the dictionary (D, D', G). */
 size_t spx_dict_bytes(const spx_dict* dict);
+
+/* Autotuner (no counterpart in the reference, whose OpenMP loop has one
+ * thread per signal): times the batched solver at 1, 2, 4 and 16 wavefronts
+ * per signal on the first min(n, 16384) signals of a sample batch and stores
+ * the fastest width on the dictionary; later batched calls on the dictionary
+ * with waves_per_signal <= 0 use it (results depend on the width only through
+ * FP32 summation order).  `ms` (optional, 4 entries) receives each width's
+ * time, 0 for widths that cannot cover p.  Synchronous. */
+int spx_dict_autotune_lasso_f32(spx_ctx* ctx, spx_dict* dict, const float* X,
+                                int ldx, int n, const float* lambda,
+                                int n_lambda, float lambda2,
+                                const spx_lasso_opts* opts, int* waves,
+                                float* ms);
+int spx_dict_autotune_lasso_f64(spx_ctx* ctx, spx_dict* dict, const double* X,
+                                int ldx, int n, const double* lambda,
+                                int n_lambda, double lambda2,
+                                const spx_lasso_opts* opts, int* waves,
+                                float* ms);
+int spx_dict_autotune_omp_f32(spx_ctx* ctx, spx_dict* dict, const float* X,
+                              int ldx, int n, const int* L, int nL,
+                              const float* eps, int n_eps, const float* lambda,
+                              int n_lambda, int* waves, float* ms);
+int spx_dict_autotune_omp_f64(spx_ctx* ctx, spx_dict* dict, const double* X,
+                              int ldx, int n, const int* L, int nL,
+                              const double* eps, int n_eps,
+                              const double* lambda, int n_lambda, int* waves,
+                              float* ms);
+/* Width used by batched calls on the dictionary (solver 0 = lasso, 1 = omp)
+ * when waves_per_signal <= 0; spx_dict_set_waves pins it (0 = size rule). */
+int spx_dict_get_waves(const spx_dict* dict, int solver, int* waves);
+int spx_dict_set_waves(spx_dict* dict, int solver, int waves);
 
 /* ------------------------------------------------------- host entry points */
 

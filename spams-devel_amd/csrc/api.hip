@@ -139,19 +139,28 @@ int zero_ints(hipStream_t st, int* p, int n) {
   return SPX_OK;
 }
 
-int tuned_waves(Ctx* ctx, int solver, int p, int elem_bytes) {
-  const long key = (long)solver * 1000000000L + (long)elem_bytes * 100000000L + p;
-  {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    for (auto& kv : ctx->tuned)
-      if (kv.first == key) return kv.second;
-  }
-  // default before tuning: one wavefront per signal while a wavefront holds
-  // at most 16 chunks of 16 bytes per lane (p <= 4096 in FP32)
-  const int per_wave = 64 * 16 * (16 / elem_bytes);
-  if (p <= per_wave) return 1;
-  if (p <= 4 * per_wave) return 4;
+int tuned_waves(const Dict* d, int solver) {
+  if (d->waves[solver] > 0) return d->waves[solver];
+  const int per_wave = 64 * 16 * (16 / d->elem);
+  if (d->p <= per_wave) return 1;
+  if (d->p <= 4 * per_wave) return 4;
   return 16;
+}
+
+// ------------------------------------------------------------------ timers
+
+ScopedTimer::ScopedTimer(Ctx* ctx, int id) : ctx_(ctx), id_(id) {
+  if (!ctx_->timing) return;
+  a_ = ctx_->event_get();
+  if (a_ && hipEventRecord(a_, ctx_->stream) != hipSuccess) a_ = nullptr;
+}
+
+ScopedTimer::~ScopedTimer() {
+  if (!a_) return;
+  hipEvent_t b = ctx_->event_get();
+  if (!b || hipEventRecord(b, ctx_->stream) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(ctx_->mu);
+  ctx_->pending.push_back({id_, a_, b});
 }
 
 template <typename T>
@@ -263,7 +272,9 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   a.outCount = out->count;
   a.stats = out->stats;
   a.refit = (o->precision & SPX_PREC_REFIT64) ? 1 : 0;
-  return lars_solve<T>(ctx, a, o->form, o->waves_per_signal);
+  const int w = o->waves_per_signal > 0 ? o->waves_per_signal
+                : o->form == SPX_FORM_PLAIN ? 4 : tuned_waves(d, 0);
+  return lars_solve<T>(ctx, a, o->form, w);
 }
 
 template <typename T>
@@ -287,6 +298,11 @@ static int lasso_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx,
   if (rc) return rc;
   return lasso_dtx_dev<T>(ctx, d, DtX, d->ldg, norms, n, lambda, nlam, lambda2, o, out);
 }
+
+template <typename T>
+static int omp_solve_dtx(spx_ctx* ctx, const spx_dict* d, const T* DtX, const T* norms, int n,
+                         const int* L, int nL, const T* eps, int nEps, const T* lambda,
+                         int nLambda, const spx_omp_opts* o, spx_lists_dev* out);
 
 template <typename T>
 static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
@@ -321,6 +337,15 @@ static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
   T* norms = nullptr;
   int rc = products<T>(ctx, d, X, ldx, n, o->precision, &DtX, &norms);
   if (rc) return rc;
+  return omp_solve_dtx<T>(ctx, d, DtX, norms, n, L, nL, eps, nEps, lambda, nLambda, o, out);
+}
+
+// ORMP from the correlations D'X (leading dimension ldg) and ||x||^2 after
+// omp_dev's argument checks.
+template <typename T>
+static int omp_solve_dtx(spx_ctx* ctx, const spx_dict* d, const T* DtX, const T* norms, int n,
+                         const int* L, int nL, const T* eps, int nEps, const T* lambda,
+                         int nLambda, const spx_omp_opts* o, spx_lists_dev* out) {
   OmpArgs<T> a{};
   a.G = static_cast<const T*>(d->G);
   a.ldg = d->ldg;
@@ -340,7 +365,165 @@ static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
   a.outV = static_cast<T*>(out->v);
   a.outR = out->r;
   a.outCount = out->count;
-  return omp_solve<T>(ctx, a, o->form, o->waves_per_signal);
+  const int w = o->waves_per_signal > 0 ? o->waves_per_signal
+                : o->form == SPX_FORM_PLAIN ? 4 : tuned_waves(d, 1);
+  return omp_solve<T>(ctx, a, o->form, w);
+}
+
+// ------------------------------------------------------------- autotuner
+
+constexpr int kTuneSignals = 16384;
+constexpr int kTuneWidths[4] = {1, 2, 4, 16};
+
+// Device lists for the autotuner's trial runs (freed on scope exit).
+template <typename T>
+struct TrialLists {
+  spx_lists_dev l{};
+  int alloc(int L, int n) {
+    l.L = L;
+    SPX_HIP(hipMalloc(&l.v, sizeof(T) * (size_t)L * n));
+    SPX_HIP(hipMalloc(reinterpret_cast<void**>(&l.r), sizeof(int) * (size_t)L * n));
+    SPX_HIP(hipMalloc(reinterpret_cast<void**>(&l.count), sizeof(int) * (size_t)n));
+    return SPX_OK;
+  }
+  ~TrialLists() {
+    if (l.v) (void)hipFree(l.v);
+    if (l.r) (void)hipFree(l.r);
+    if (l.count) (void)hipFree(l.count);
+  }
+};
+
+// Times run(w) for each width that covers p (one warm-up launch, then the
+// best of two timed launches on the context's stream) and stores the fastest
+// width on the dictionary.
+template <typename Run>
+static int tune_widths(spx_ctx* ctx, spx_dict* d, int solver, bool (*fits)(int, int), Run run,
+                       int* waves, float* ms_out) {
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  hipEvent_t e0 = ctx->event_get(), e1 = ctx->event_get();
+  int rc = (e0 && e1) ? SPX_OK : SPX_ERR_HIP;
+  if (rc) set_error("autotune: event creation failed");
+  float ms[4] = {0, 0, 0, 0};
+  float best = 0.f;
+  int bw = 0;
+  for (int i = 0; i < 4 && rc == SPX_OK; ++i) {
+    const int w = kTuneWidths[i];
+    if (!fits(w, d->p)) continue;
+    rc = run(w);
+    if (rc) break;
+    float t = 0.f;
+    for (int rep = 0; rep < 2 && rc == SPX_OK; ++rep) {
+      float x = 0.f;
+      if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = SPX_ERR_HIP;
+      if (!rc) rc = run(w);
+      if (!rc && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                  hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&x, e0, e1) != hipSuccess)) {
+        set_error("autotune: event timing failed");
+        rc = SPX_ERR_HIP;
+      }
+      t = rep == 0 ? x : (x < t ? x : t);
+    }
+    ms[i] = t;
+    if (!rc && (bw == 0 || t < best)) {
+      best = t;
+      bw = w;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (e0) ctx->free_events.push_back(e0);
+    if (e1) ctx->free_events.push_back(e1);
+  }
+  ctx->timing = timing;
+  if (rc) return rc;
+  if (bw == 0) {
+    set_error("autotune: p=%d exceeds every supported width", d->p);
+    return SPX_ERR_UNSUP;
+  }
+  d->waves[solver] = bw;
+  for (int i = 0; i < 4; ++i) d->tune_ms[solver][i] = ms[i];
+  if (waves) *waves = bw;
+  if (ms_out)
+    for (int i = 0; i < 4; ++i) ms_out[i] = ms[i];
+  return SPX_OK;
+}
+
+template <typename T>
+static int lasso_autotune(spx_ctx* ctx, spx_dict* d, const T* X, int ldx, int n,
+                          const T* lambda, int nlam, T lambda2, const spx_lasso_opts* o,
+                          int* waves, float* ms) {
+  if (!ctx || !d || !o || !X || !lambda || n <= 0 || ldx < d->m ||
+      !(nlam == 1 || nlam == n) || (int)sizeof(T) != d->elem) {
+    set_error("autotune_lasso: invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  DeviceGuard g(ctx->device);
+  const int ns = n < kTuneSignals ? n : kTuneSignals;
+  int L = o->L < 0 ? d->p : o->L;
+  if (L > d->m) L = d->m;
+  if (L > d->p) L = d->p;
+  if (L < 1) L = 1;
+  TrialLists<T> tl;
+  int rc = tl.alloc(L, ns);
+  if (rc) return rc;
+  T* DtX = nullptr;
+  T* norms = nullptr;
+  rc = products<T>(ctx, d, X, ldx, ns, o->precision & SPX_PREC_BF16X3, &DtX, &norms);
+  if (rc) return rc;
+  spx_lasso_opts oo = *o;
+  oo.form = SPX_FORM_BATCHED;
+  auto run = [&](int w) {
+    oo.waves_per_signal = w;
+    return lasso_dtx_dev<T>(ctx, d, DtX, d->ldg, norms, ns, lambda, nlam == 1 ? 1 : ns,
+                            lambda2, &oo, &tl.l);
+  };
+  return tune_widths(ctx, d, 0, &lars_fits<T>, run, waves, ms);
+}
+
+template <typename T>
+static int omp_autotune(spx_ctx* ctx, spx_dict* d, const T* X, int ldx, int n, const int* L,
+                        int nL, const T* eps, int nEps, const T* lambda, int nLambda,
+                        int* waves, float* ms) {
+  if (!ctx || !d || !X || n <= 0 || ldx < d->m || (int)sizeof(T) != d->elem) {
+    set_error("autotune_omp: invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  if (!L) nL = 0;
+  if (!eps) nEps = 0;
+  if (!lambda) nLambda = 0;
+  if ((nL == 0 && nEps == 0 && nLambda == 0) || (nL > 1 && nL != n) ||
+      (nEps > 1 && nEps != n) || (nLambda > 1 && nLambda != n)) {
+    set_error("autotune_omp: invalid stopping-rule vectors");
+    return SPX_ERR_ARG;
+  }
+  DeviceGuard g(ctx->device);
+  const int ns = n < kTuneSignals ? n : kTuneSignals;
+  // list width: the largest L any signal may use (the scalar default when L
+  // is not given), read back from the device vector
+  int Lmax = d->m < d->p ? d->m : d->p;
+  if (nL > 0) {
+    std::vector<int> hl((size_t)(nL > 1 ? ns : 1));
+    SPX_HIP(hipMemcpy(hl.data(), L, sizeof(int) * hl.size(), hipMemcpyDeviceToHost));
+    Lmax = spx_omp_L(d->m, d->p, hl.data(), (int)hl.size(), nL > 1 ? ns : 1);
+  }
+  if (Lmax < 1) Lmax = 1;
+  TrialLists<T> tl;
+  int rc = tl.alloc(Lmax, ns);
+  if (rc) return rc;
+  T* DtX = nullptr;
+  T* norms = nullptr;
+  rc = products<T>(ctx, d, X, ldx, ns, SPX_PREC_FP32, &DtX, &norms);
+  if (rc) return rc;
+  spx_omp_opts oo{SPX_PREC_FP32, SPX_FORM_BATCHED, 0};
+  auto run = [&](int w) {
+    oo.waves_per_signal = w;
+    return omp_solve_dtx<T>(ctx, d, DtX, norms, ns, L, nL > 1 ? ns : nL, eps,
+                            nEps > 1 ? ns : nEps, lambda, nLambda > 1 ? ns : nLambda, &oo,
+                            &tl.l);
+  };
+  return tune_widths(ctx, d, 1, &ormp_fits<T>, run, waves, ms);
 }
 
 // Device CSC of the lists, then download into a host spx_csc.
@@ -588,7 +771,49 @@ int spx_ctx::scratch_get(int id, size_t bytes, void** out) {
   return SPX_OK;
 }
 
+hipEvent_t spx_ctx::event_get() {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!free_events.empty()) {
+      hipEvent_t e = free_events.back();
+      free_events.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int spx_ctx::timers_fold() {
+  std::vector<Pending> p;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    p.swap(pending);
+  }
+  int rc = SPX_OK;
+  for (auto& e : p) {
+    float ms = 0.f;
+    if (hipEventSynchronize(e.b) != hipSuccess || hipEventElapsedTime(&ms, e.a, e.b) != hipSuccess) {
+      spx::set_error("timing: event query failed");
+      rc = SPX_ERR_HIP;
+    } else if (e.id >= 0 && e.id < SPX_TIMER_COUNT) {
+      timer_ms[e.id] += ms;
+      timer_n[e.id] += 1;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    free_events.push_back(e.a);
+    free_events.push_back(e.b);
+  }
+  return rc;
+}
+
 spx_ctx::~spx_ctx() {
+  for (auto& e : pending) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  for (auto e : free_events) (void)hipEventDestroy(e);
   for (auto& s : slots)
     if (s.ptr) (void)hipFree(s.ptr);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -662,6 +887,34 @@ int spx_ctx_synchronize(spx_ctx* ctx) {
   return SPX_OK;
 }
 
+int spx_ctx_set_timing(spx_ctx* ctx, int on) {
+  if (!ctx) return SPX_ERR_ARG;
+  ctx->timing = on != 0;
+  return SPX_OK;
+}
+
+int spx_ctx_read_timing(spx_ctx* ctx, double* ms, long long* launches, int count) {
+  if (!ctx || count < 0) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  int rc = ctx->timers_fold();
+  for (int i = 0; i < count; ++i) {
+    if (ms) ms[i] = i < SPX_TIMER_COUNT ? ctx->timer_ms[i] : 0.0;
+    if (launches) launches[i] = i < SPX_TIMER_COUNT ? ctx->timer_n[i] : 0;
+  }
+  return rc;
+}
+
+int spx_ctx_reset_timing(spx_ctx* ctx) {
+  if (!ctx) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  int rc = ctx->timers_fold();
+  for (int i = 0; i < SPX_TIMER_COUNT; ++i) {
+    ctx->timer_ms[i] = 0.0;
+    ctx->timer_n[i] = 0;
+  }
+  return rc;
+}
+
 int spx_dict_create_f32(spx_ctx* ctx, const float* D, int m, int p, int on_device,
                         int precision, spx_dict** out) {
   return spx::dict_create<float>(ctx, D, m, p, on_device, precision, out);
@@ -693,6 +946,45 @@ int spx_dict_dims(const spx_dict* d, int* m, int* p) {
 size_t spx_dict_bytes(const spx_dict* d) {
   if (!d) return 0;
   return (size_t)d->elem * ((size_t)2 * d->m * d->p + (size_t)d->ldg * d->p);
+}
+
+int spx_dict_autotune_lasso_f32(spx_ctx* ctx, spx_dict* d, const float* X, int ldx, int n,
+                                const float* lambda, int nlam, float lambda2,
+                                const spx_lasso_opts* o, int* waves, float* ms) {
+  return spx::lasso_autotune<float>(ctx, d, X, ldx, n, lambda, nlam, lambda2, o, waves, ms);
+}
+int spx_dict_autotune_lasso_f64(spx_ctx* ctx, spx_dict* d, const double* X, int ldx, int n,
+                                const double* lambda, int nlam, double lambda2,
+                                const spx_lasso_opts* o, int* waves, float* ms) {
+  return spx::lasso_autotune<double>(ctx, d, X, ldx, n, lambda, nlam, lambda2, o, waves, ms);
+}
+int spx_dict_autotune_omp_f32(spx_ctx* ctx, spx_dict* d, const float* X, int ldx, int n,
+                              const int* L, int nL, const float* eps, int nEps,
+                              const float* lambda, int nLambda, int* waves, float* ms) {
+  return spx::omp_autotune<float>(ctx, d, X, ldx, n, L, nL, eps, nEps, lambda, nLambda, waves,
+                                  ms);
+}
+int spx_dict_autotune_omp_f64(spx_ctx* ctx, spx_dict* d, const double* X, int ldx, int n,
+                              const int* L, int nL, const double* eps, int nEps,
+                              const double* lambda, int nLambda, int* waves, float* ms) {
+  return spx::omp_autotune<double>(ctx, d, X, ldx, n, L, nL, eps, nEps, lambda, nLambda, waves,
+                                   ms);
+}
+
+int spx_dict_get_waves(const spx_dict* d, int solver, int* waves) {
+  if (!d || solver < 0 || solver > 1 || !waves) return SPX_ERR_ARG;
+  *waves = spx::tuned_waves(d, solver);
+  return SPX_OK;
+}
+
+int spx_dict_set_waves(spx_dict* d, int solver, int waves) {
+  if (!d || solver < 0 || solver > 1 || !(waves == 0 || waves == 1 || waves == 2 ||
+                                          waves == 4 || waves == 16)) {
+    spx::set_error("spx_dict_set_waves: solver must be 0 or 1 and waves one of 0, 1, 2, 4, 16");
+    return SPX_ERR_ARG;
+  }
+  d->waves[solver] = waves;
+  return SPX_OK;
 }
 
 int spx_lasso_L(int m, int p, int L, int mode, double lambda1, int pos, double lambda2) {

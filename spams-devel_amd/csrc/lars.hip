@@ -681,16 +681,21 @@ int lars_batched(Ctx* ctx, const LarsArgs<T>& a0, int w) {
     int* order = nullptr;
     rc = ctx->scratch_get(10, sizeof(int) * (size_t)a0.n, reinterpret_cast<void**>(&order));
     if (rc) return rc;
+    ScopedTimer tm(ctx, SPX_TIMER_LARS_ORDER);
     rc = sort_order<T>(ctx, a0.lambda, a0.n, order);
     if (rc) return rc;
     a.order = order;
   }
-  rc = lars_pass_w<T>(ctx, w, true, a, true, cnt, a0.n);
-  if (rc) return rc;
+  {
+    ScopedTimer tm(ctx, SPX_TIMER_LARS_FAST);
+    rc = lars_pass_w<T>(ctx, w, true, a, true, cnt, a0.n);
+    if (rc) return rc;
+  }
   if (a0.L <= FastCap<T>::C) return SPX_OK;  // every active set fitted
   a.list = ovf;
   a.list_count = cnt + 1;
   int w2 = w < 4 ? 4 : w;
+  ScopedTimer tm(ctx, SPX_TIMER_LARS_FULL);
   rc = lars_pass_w<T>(ctx, w2, false, a, true, cnt + 2, a0.n);
   if (rc < 0) rc = lars_pass_w<T>(ctx, 16, false, a, true, cnt + 2, a0.n);
   return rc;
@@ -698,12 +703,12 @@ int lars_batched(Ctx* ctx, const LarsArgs<T>& a0, int w) {
 
 }  // namespace
 
-// Group width: explicit (waves_per_signal), else the autotuner's choice for
-// the batched form, else 4 wavefronts for the plain form; widened until the
+// Group width `waves` as resolved by the caller (explicit, the dictionary's
+// autotuned width or the size rule; 4 for the plain form), widened until the
 // group covers p.
 template <typename T>
 int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves) {
-  if (waves <= 0) waves = form == SPX_FORM_PLAIN ? 4 : tuned_waves(ctx, 0, a.p, sizeof(T));
+  if (waves <= 0) waves = 4;
   static const int widths[] = {1, 2, 4, 16};
   for (int w : widths) {
     if (w < waves) continue;
@@ -727,6 +732,19 @@ int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves) {
   return SPX_ERR_UNSUP;
 }
 
+template <typename T>
+bool lars_fits(int w, int p) {
+  switch (w) {
+    case 1: return lars_nq<T, 1>(p) > 0;
+    case 2: return lars_nq<T, 2>(p) > 0;
+    case 4: return lars_nq<T, 4>(p) > 0;
+    case 16: return lars_nq<T, 16>(p) > 0;
+    default: return false;
+  }
+}
+
+template bool lars_fits<float>(int, int);
+template bool lars_fits<double>(int, int);
 template int lars_solve<float>(Ctx*, const LarsArgs<float>&, int, int);
 template int lars_solve<double>(Ctx*, const LarsArgs<double>&, int, int);
 

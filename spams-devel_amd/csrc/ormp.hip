@@ -255,6 +255,7 @@ int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
     if (rc) return rc;
     a.scratch = scratch;
     a.queue = queue;
+    ScopedTimer tm(ctx, SPX_TIMER_ORMP);
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, a);
     SPX_HIP(hipGetLastError());
     return SPX_OK;
@@ -304,7 +305,7 @@ int ormp_dispatch_nq(Ctx* ctx, const OmpArgs<T>& a, int form) {
 
 template <typename T>
 int omp_solve(Ctx* ctx, const OmpArgs<T>& a, int form, int waves) {
-  if (waves <= 0) waves = form == SPX_FORM_PLAIN ? 4 : tuned_waves(ctx, 1, a.p, sizeof(T));
+  if (waves <= 0) waves = 4;
   static const int widths[] = {1, 2, 4, 16};
   for (int w : widths) {
     if (w < waves) continue;
@@ -321,6 +322,14 @@ int omp_solve(Ctx* ctx, const OmpArgs<T>& a, int form, int waves) {
   return SPX_ERR_UNSUP;
 }
 
+template <typename T>
+bool ormp_fits(int w, int p) {
+  const int per = Vec16o<T>::N * 64 * w;
+  return (p + per - 1) / per <= 16;
+}
+
+template bool ormp_fits<float>(int, int);
+template bool ormp_fits<double>(int, int);
 template int omp_solve<float>(Ctx*, const OmpArgs<float>&, int, int);
 template int omp_solve<double>(Ctx*, const OmpArgs<double>&, int, int);
 

@@ -101,9 +101,30 @@ int sort_order(Ctx* ctx, const T* key, int n, int* order);
 // Clears n ints with a kernel (capture-safe replacement for a memset node).
 int zero_ints(hipStream_t st, int* p, int n);
 
-// Wavefronts per signal chosen by the autotuner for a solver (0 = LARS,
-// 1 = OMP) at dictionary size p and element size.
-int tuned_waves(Ctx* ctx, int solver, int p, int elem_bytes);
+// Wavefronts per signal for a batched solve (solver 0 = LARS, 1 = OMP): the
+// width the autotuner stored on the dictionary, else a size rule (one
+// wavefront while a wavefront covers p with at most 16 chunks per lane).
+int tuned_waves(const Dict* d, int solver);
+
+// True when a workgroup of w wavefronts covers p rows (the solver's register
+// and LDS layout), i.e. the width can run this dictionary.
+template <typename T>
+bool lars_fits(int w, int p);
+template <typename T>
+bool ormp_fits(int w, int p);
+
+// Records the duration of the enclosed launches on the context's stream
+// under timer `id` (SPX_TIMER_*) while timing is switched on; no-op else.
+class ScopedTimer {
+ public:
+  ScopedTimer(Ctx* ctx, int id);
+  ~ScopedTimer();
+
+ private:
+  Ctx* ctx_;
+  int id_;
+  hipEvent_t a_ = nullptr;
+};
 
 }  // namespace spx
 
@@ -118,9 +139,20 @@ struct spx_ctx {
     size_t bytes = 0;
   };
   std::vector<Slot> slots;
-  // autotuner cache: key -> waves per signal
-  std::vector<std::pair<long, int>> tuned;
   std::mutex mu;
+  // launch timing (spx_ctx_set_timing): pending event pairs per timer id,
+  // folded into the totals when read
+  bool timing = false;
+  struct Pending {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+  double timer_ms[SPX_TIMER_COUNT] = {};
+  long long timer_n[SPX_TIMER_COUNT] = {};
+  hipEvent_t event_get();
+  int timers_fold();
   // Device scratch slot `id` of at least `bytes` bytes (contents undefined).
   int scratch_get(int id, size_t bytes, void** out);
   ~spx_ctx();
@@ -138,4 +170,6 @@ struct spx_dict {
   void* Dt = nullptr;   // p x m
   void* G = nullptr;    // ldg x p
   double lipschitz = 0; // largest eigenvalue of G (power iteration), lazily
+  int waves[2] = {0, 0};     // autotuned wavefronts per signal (LARS, OMP)
+  float tune_ms[2][4] = {};  // autotuner trial times for 1, 2, 4, 16 waves
 };

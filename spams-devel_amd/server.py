@@ -84,6 +84,32 @@ class SparseCodingServer:
         self.lock = threading.Condition()
         self.n_batches = 0
 
+    # ----------------------------------------------------------- autotune
+    def autotune(self, sample_jobs, max_signals=16384):
+        """Tunes the wavefronts-per-signal of each (dictionary, solver) pair
+        present in `sample_jobs` on their signals, before serving.  The width
+        is then fixed for every later batch on that pair, so a job's result
+        does not depend on which batch it lands in.  Returns
+        {(dict_id, kind): (waves, {width: ms})}."""
+        torch = self.torch
+        groups = {}
+        for j in sample_jobs:
+            groups.setdefault((j.dict_id, j.kind), []).append(j)
+        out = {}
+        for (di, kind), js in sorted(groups.items()):
+            d = self.dicts[di]
+            X = np.concatenate([j.X for j in js], axis=1)[:, :max_signals]
+            n = X.shape[1]
+            Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(self.dev)
+            par = np.concatenate([np.full(j.n, j.param) for j in js])[:n]
+            if kind == "lasso":
+                lam = torch.from_numpy(par.astype(np.float32)).to(self.dev)
+                out[(di, kind)] = d.autotune_lasso(Xt, lam)
+            else:
+                L = torch.from_numpy(par.astype(np.int32)).to(self.dev)
+                out[(di, kind)] = d.autotune_omp(Xt, L=L)
+        return out
+
     # ---------------------------------------------------------- footprint
     def footprint(self, dict_id: int, kind: str, n: int) -> int:
         d = self.dicts[dict_id]

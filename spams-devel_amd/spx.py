@@ -25,6 +25,9 @@ LIB_PATH = os.path.join(HERE, "libspx.so")
 L1COEFFS, L2ERROR, PENALTY = 0, 1, 2
 PREC = {"fp32": 0, "bf16x3": 1, "fp32+refit64": 2, "bf16x3+refit64": 3}
 FORM = {"batched": 0, "plain": 1}
+SOLVERS = {"lasso": 0, "omp": 1}
+# launch timers of spx_ctx_read_timing, in SPX_TIMER_* order
+TIMERS = ("lars_order", "lars_fast", "lars_full", "ormp")
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -131,6 +134,25 @@ class Context:
         lib().spx_ctx_synchronize.argtypes = [_P]
         _check(lib().spx_ctx_synchronize(self.h))
 
+    def set_timing(self, on: bool = True) -> None:
+        """Record HIP events around the batched solvers' launches."""
+        lib().spx_ctx_set_timing.argtypes = [_P, _I]
+        _check(lib().spx_ctx_set_timing(self.h, int(bool(on))))
+
+    def read_timing(self) -> dict:
+        """{timer: (total ms, launches)} since the last reset (waits for
+        the recorded events)."""
+        ms = (ctypes.c_double * len(TIMERS))()
+        cnt = (ctypes.c_longlong * len(TIMERS))()
+        f = lib().spx_ctx_read_timing
+        f.argtypes = [_P, _P, _P, _I]
+        _check(f(self.h, ms, cnt, len(TIMERS)))
+        return {name: (ms[i], int(cnt[i])) for i, name in enumerate(TIMERS)}
+
+    def reset_timing(self) -> None:
+        lib().spx_ctx_reset_timing.argtypes = [_P]
+        _check(lib().spx_ctx_reset_timing(self.h))
+
     def close(self) -> None:
         if self.h:
             lib().spx_ctx_destroy(self.h)
@@ -194,6 +216,49 @@ class Dictionary:
     @property
     def nbytes(self) -> int:
         return int(lib().spx_dict_bytes(self.h))
+
+    def waves(self, solver: str) -> int:
+        """Wavefronts per signal that batched calls use ("lasso" | "omp")."""
+        w = ctypes.c_int()
+        f = lib().spx_dict_get_waves
+        f.argtypes = [_P, _I, ctypes.POINTER(ctypes.c_int)]
+        _check(f(self.h, SOLVERS[solver], ctypes.byref(w)))
+        return w.value
+
+    def set_waves(self, solver: str, waves: int) -> None:
+        """Pins the width for a solver (0 = back to the size rule)."""
+        f = lib().spx_dict_set_waves
+        f.argtypes = [_P, _I, _I]
+        _check(f(self.h, SOLVERS[solver], waves))
+
+    def autotune_lasso(self, X, lam, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
+                       max_length_path=-1, precision="fp32"):
+        """Times the batched LARS solver at 1, 2, 4 and 16 wavefronts per
+        signal on a sample batch (X torch (n, m), lam torch (1,) or (n,)),
+        keeps the fastest on the dictionary and returns
+        (waves, {width: ms})."""
+        o = spx_lasso_opts(L, mode, int(pos), max_length_path, PREC[precision],
+                           FORM["batched"], 0)
+        w = ctypes.c_int()
+        ms = (ctypes.c_float * 4)()
+        f = getattr(lib(), "spx_dict_autotune_lasso_" + _sfx(self.dtype))
+        f.argtypes = [_P, _P, _P, _I, _I, _P, _I, _ct(self.dtype),
+                      ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(ctypes.c_int), _P]
+        _check(f(self.ctx.h, self.h, _tptr(X), X.shape[1], X.shape[0], _tptr(lam),
+                 lam.numel(), lambda2, ctypes.byref(o), ctypes.byref(w), ms))
+        return w.value, {k: ms[i] for i, k in enumerate((1, 2, 4, 16)) if ms[i] > 0}
+
+    def autotune_omp(self, X, L=None, eps=None, lambda1=None):
+        """As autotune_lasso for the batched ORMP solver."""
+        w = ctypes.c_int()
+        ms = (ctypes.c_float * 4)()
+        nn = lambda t: 0 if t is None else t.numel()
+        f = getattr(lib(), "spx_dict_autotune_omp_" + _sfx(self.dtype))
+        f.argtypes = [_P, _P, _P, _I, _I, _P, _I, _P, _I, _P, _I,
+                      ctypes.POINTER(ctypes.c_int), _P]
+        _check(f(self.ctx.h, self.h, _tptr(X), X.shape[1], X.shape[0], _tptr(L), nn(L),
+                 _tptr(eps), nn(eps), _tptr(lambda1), nn(lambda1), ctypes.byref(w), ms))
+        return w.value, {k: ms[i] for i, k in enumerate((1, 2, 4, 16)) if ms[i] > 0}
 
     def close(self) -> None:
         if self.h:

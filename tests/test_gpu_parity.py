@@ -286,3 +286,81 @@ def test_device_power_iteration_lipschitz(spx):
     L = d.lipschitz(300)
     ev = np.linalg.eigvalsh(D.astype(np.float64).T @ D)[-1]
     assert abs(L - ev) <= 1e-4 * ev
+
+
+# ------------------------------------------------------ autotuner, timers
+
+def test_autotuner_stores_a_width_that_batched_calls_use(spx, orc):
+    """The tuner times each width that covers p, keeps the fastest on the
+    dictionary, and later batched calls without an explicit width run at it:
+    their results equal an explicit call at that width bit for bit, and
+    match the oracle within the FP32 tolerances."""
+    import torch
+    print("case autotune lasso+omp", flush=True)
+    ctx = gpu_ctx(spx)
+    X, D, lam = W.lars_headline(seed=79, n=3000)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    lt = torch.from_numpy(lam).cuda()
+    assert d.waves("lasso") == 1  # size rule before tuning (p=1024 FP32)
+    w, trial = d.autotune_lasso(Xt, lt)
+    assert w in (1, 2, 4, 16) and set(trial) == {1, 2, 4, 16} and w in trial
+    assert trial[w] == min(trial.values())
+    assert d.waves("lasso") == w
+    n, Lout = X.shape[1], spx.lasso_L(256, 1024, -1, spx.PENALTY, float(lam.max()))
+
+    def run(waves):
+        V = torch.zeros((n, Lout), dtype=torch.float32, device="cuda")
+        R = torch.zeros((n, Lout), dtype=torch.int32, device="cuda")
+        c = torch.zeros((n,), dtype=torch.int32, device="cuda")
+        spx.lasso_dev(ctx, d, Xt, lt, V, R, c, waves_per_signal=waves)
+        torch.cuda.synchronize()
+        return V.cpu().numpy(), R.cpu().numpy(), c.cpu().numpy()
+
+    a, b = run(0), run(w)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    A = dense(spx.lasso(X[:, :400], D, lambda1=lam[:400]))
+    B = dense(orc.lasso(X[:, :400], D, lam[:400]))
+    same = np.all((A != 0) == (B != 0), axis=0)
+    assert same.mean() >= 0.98
+
+    Xo, Do, L, eps = W.omp_batch(seed=80, n=2000)
+    do = spx.Dictionary(ctx, Do)
+    Xot = torch.from_numpy(np.ascontiguousarray(Xo.T)).cuda()
+    Lt = torch.tensor([L], dtype=torch.int32, device="cuda")
+    et = torch.tensor([eps], dtype=torch.float32, device="cuda")
+    wo, trial_o = do.autotune_omp(Xot, L=Lt, eps=et)
+    assert wo in trial_o and do.waves("omp") == wo
+    do.set_waves("omp", 0)
+    assert do.waves("omp") == 1
+    with pytest.raises(ValueError):
+        do.set_waves("omp", 3)
+    d.close()
+    do.close()
+
+
+def test_launch_timers_count_solver_launches(spx):
+    import torch
+    print("case timers", flush=True)
+    ctx = gpu_ctx(spx)
+    X, D, lam = W.lars_headline(seed=81, n=2000)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    lt = torch.from_numpy(lam).cuda()
+    n, Lout = X.shape[1], spx.lasso_L(256, 1024, -1, spx.PENALTY, float(lam.max()))
+    V = torch.zeros((n, Lout), dtype=torch.float32, device="cuda")
+    R = torch.zeros((n, Lout), dtype=torch.int32, device="cuda")
+    c = torch.zeros((n,), dtype=torch.int32, device="cuda")
+    ctx.set_timing(True)
+    for _ in range(3):
+        spx.lasso_dev(ctx, d, Xt, lt, V, R, c)
+    ctx.set_timing(False)
+    spx.lasso_dev(ctx, d, Xt, lt, V, R, c)  # not recorded
+    t = ctx.read_timing()
+    assert t["lars_fast"][1] == 3 and t["lars_full"][1] == 3 and t["lars_order"][1] == 3
+    assert t["lars_fast"][0] > 0 and t["lars_full"][0] > 0
+    assert t["ormp"][1] == 0
+    ctx.reset_timing()
+    assert ctx.read_timing()["lars_fast"] == (0.0, 0)
+    d.close()
