@@ -1,7 +1,8 @@
 // LARS-Lasso solver kernel: coreLARS2 semantics (decomp/decomp.h:1706-1915).
 //
 // One group of W wavefronts (one workgroup) solves one signal at a time:
-//   - the correlation vector DtR (p) and the active-atom flags live in LDS;
+//   - the correlation vector DtR (p) lives in LDS, the active-atom flags in a
+//     bit mask of the thread that owns the rows;
 //   - the correlation change A = (G + diag I)(:, active) * u is accumulated
 //     in registers (each thread owns NQ chunks of VEC = 16 bytes of rows),
 //     streaming the active Gram columns from the resident G (L2 / MALL),
@@ -25,6 +26,7 @@
 // (fast pass over 0..n-1, then the full pass over the overflow list, on the
 // same stream).  Plain form: one workgroup per signal, full capacity.
 #include <climits>
+#include <type_traits>
 
 #include "spx_common.h"
 #include "spx_internal.h"
@@ -59,7 +61,7 @@ struct LarsLds {
   static constexpr int PT = VEC * GT * NQ;
   // byte size for active-set capacity LC
   static size_t bytes(int LC) {
-    size_t b = sizeof(T) * (PT + GT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC + PT;
+    size_t b = sizeof(T) * (PT + GT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC;
     b = (b + 15) & ~size_t(15);
     b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
     b = (b + 15) & ~size_t(15);
@@ -108,6 +110,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   constexpr int PT = VEC * GT * NQ;  // rows covered by the group
   constexpr int UNR = NQ >= 4 ? 2 : (NQ == 2 ? 4 : 8);  // Gram columns in flight
   typedef typename Vec16<T>::type V;
+  typedef typename std::conditional<(NQ * VEC <= 32), unsigned, unsigned long long>::type ActMask;
   const T INF = dev_inf<T>();
 
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -124,9 +127,8 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   T* sV = sGs + LC;                       // LC
   int* sInd = reinterpret_cast<int*>(sV + LC);  // LC
   int* sTmpI = sInd + LC;                       // LC
-  unsigned char* sAct = reinterpret_cast<unsigned char*>(sTmpI + LC);  // PT
   KeyIdx<T>* sRed = reinterpret_cast<KeyIdx<T>*>(
-      (reinterpret_cast<uintptr_t>(sAct + PT) + 15) & ~uintptr_t(15));
+      (reinterpret_cast<uintptr_t>(sTmpI + LC) + 15) & ~uintptr_t(15));
   T* sRed3 = reinterpret_cast<T*>(sRed + W);  // 3W
   int* sSig = reinterpret_cast<int*>(sRed3 + 3 * W);
   T* sM0 = reinterpret_cast<T*>(
@@ -147,15 +149,23 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   // work distribution: chunks of `chunk` consecutive work items per atomic
   // grab (one device-scope atomic per signal serialises on its address)
   const int chunk = a.chunk > 0 ? a.chunk : 1;
+  // The LDS pass (many short signals) requests its next grab as soon as it
+  // starts on the current one, so the atomic's latency overlaps the set-up
+  // loads; the full-capacity pass (few long signals) grabs when free, so no
+  // long signal waits behind another.
+  constexpr bool PREFETCH = CAP > 0;
   int w = 0, wend = 0;
+  int pre = 0;  // thread 0: the prefetched grab
+  if (PREFETCH && a.queue && tid == 0) pre = atomicAdd(a.queue, chunk);
   for (int round = 0;; ++round) {
     if (a.queue) {
       if (w >= wend) {
         __syncthreads();
-        if (tid == 0) *sSig = atomicAdd(a.queue, chunk);
+        if (tid == 0) *sSig = PREFETCH ? pre : atomicAdd(a.queue, chunk);
         __syncthreads();
         w = *sSig;
         wend = w + chunk;
+        if (PREFETCH && tid == 0 && w < nwork) pre = atomicAdd(a.queue, chunk);
       }
     } else {
       w = round == 0 ? (int)blockIdx.x : nwork;
@@ -168,16 +178,22 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
     // ---------------------------------------------------------- set-up
     __syncthreads();  // the previous signal's LDS state is no longer read
     const T* dtx = a.DtX + (size_t)sig * a.ldd;
+    const bool vec_ok = (a.ldd % VEC == 0) &&
+                        (reinterpret_cast<uintptr_t>(a.DtX) % (sizeof(T) * VEC) == 0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int j0 = VEC * (tid + GT * q);
+      V v;
+      if (vec_ok && j0 + VEC <= p) {
+        v = *reinterpret_cast<const V*>(dtx + j0);
+      } else {
 #pragma unroll
-      for (int e = 0; e < VEC; ++e) {
-        const int j = j0 + e;
-        sDtR[j] = j < p ? dtx[j] : T(0);
-        sAct[j] = j < p ? 0 : 1;
+        for (int e = 0; e < VEC; ++e) v[e] = j0 + e < p ? dtx[j0 + e] : T(0);
       }
+      *reinterpret_cast<V*>(&sDtR[j0]) = v;
     }
+    // active flags of this thread's rows (bit q * VEC + e)
+    ActMask act = 0;
     for (int j = tid; j < LC; j += GT) {
       sCoef[j] = 0;
       sInd[j] = -1;
@@ -229,10 +245,8 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
           const int c = currentInd;
           ++admissions;
           ++s_active;
-          if (tid == 0) {
-            sInd[i] = c;
-            sAct[c] = 1;
-          }
+          if (tid == 0) sInd[i] = c;
+          if (tid == (c / VEC) % GT) act |= ActMask(1) << ((c / VEC / GT) * VEC + c % VEC);
           __syncthreads();
           const T* gcol = a.G + (size_t)c * ldg;
           for (int j = tid; j <= i; j += GT) {
@@ -369,27 +383,111 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         }
 
         // ------------------------------- next atom: custom iamin over 2p
+        // candidates (decomp/decomp.h:1803-1820): c1 = (d + cc) / (1 + A) and
+        // c2 = (cc - d) / (1 - A) over the inactive atoms, key |c|, first
+        // index of the minimum, IEEE quotients.
         T bestk = INF, bestv = INF;
         int besti = INT_MAX;
+        if constexpr (sizeof(T) == 4) {
+          // pass 1: keys from the hardware reciprocal, within 3 ulp of the
+          // IEEE quotient; each thread keeps its smallest key (m1, slot i1)
+          // and its second smallest (m2)
+          T m1 = INF, m2 = INF;
+          int i1 = 0;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+          for (int q = 0; q < NQ; ++q)
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) {
-            const int j = VEC * (tid + GT * q) + e;
-            const bool cand = (j < p) & !sAct[j];
-            const T Aj = Areg[q][e];
+            for (int e = 0; e < VEC; ++e) {
+              const int j = VEC * (tid + GT * q) + e;
+              const bool cand = (j < p) & !((act >> (q * VEC + e)) & 1);
+              const T Aj = Areg[q][e];
+              const T d = sDtR[j];
+              const bool ok1 = cand & (Aj < INF) & (Aj > T(-1)) & !a.pos;
+              const bool ok2 = cand & (Aj < INF) & (Aj < T(1));
+              const T k1 = ok1 ? T(fabs((d + cc) * __builtin_amdgcn_rcpf(T(1) + Aj))) : INF;
+              const T k2 = ok2 ? T(fabs((cc - d) * __builtin_amdgcn_rcpf(T(1) - Aj))) : INF;
+              const int sl = 2 * (q * VEC + e);
+              bool lt = k1 < m1;
+              m2 = lt ? m1 : fminf(m2, k1);
+              i1 = lt ? sl : i1;
+              m1 = lt ? k1 : m1;
+              lt = k2 < m1;
+              m2 = lt ? m1 : fminf(m2, k2);
+              i1 = lt ? sl + 1 : i1;
+              m1 = lt ? k2 : m1;
+            }
+          // pass 2 recomputes what it needs: no pass-1 temporaries stay live
+          // (register pressure sets the signals in flight per CU)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              T x = Areg[q][e];
+              asm volatile("" : "+v"(x));
+              Areg[q][e] = x;
+            }
+          asm volatile("" ::: "memory");
+          // every key within the margin of the group minimum may be the
+          // minimum of the IEEE keys: those get the exact quotient
+          T amin = m1, zp = 0;
+          int zi = 0;
+          group_argmin<T, W>(amin, zi, zp, sRed);
+          const T thr = amin * T(1.0f + 0x1.0p-18f) + T(1e-37f);
+          if (m2 <= thr) {
+            // two of this thread's keys are near the minimum (or no key is
+            // finite): all of its keys exactly (a superset of the keys
+            // within the margin; the group arg-min is unchanged).  Rare, so
+            // one row at a time (few registers live).
+#pragma unroll 1
+            for (int sl = 0; sl < NQ * VEC; ++sl) {
+              T Aj = 0;
+#pragma unroll
+              for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) Aj = sl == q * VEC + e ? Areg[q][e] : Aj;
+              const int j = VEC * (tid + GT * (sl / VEC)) + sl % VEC;
+              const bool cand = (j < p) & !((act >> sl) & 1);
+              const T d = sDtR[j < PT ? j : 0];
+              T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
+              c1 = a.pos ? INF : c1;
+              const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
+              take_if_better(bestk, besti, bestv, cand ? T(fabs(c1)) : INF,
+                             cand ? j : INT_MAX, c1);
+              take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
+                             cand ? p + j : INT_MAX, c2);
+            }
+          } else if (m1 <= thr) {
+            // one key of this thread near the minimum (finite, so valid)
+            const int sl = i1 >> 1;
+            T Aj = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+              for (int e = 0; e < VEC; ++e) Aj = sl == q * VEC + e ? Areg[q][e] : Aj;
+            const int j = VEC * (tid + GT * (sl / VEC)) + sl % VEC;
             const T d = sDtR[j];
-            T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
-            c1 = a.pos ? INF : c1;
-            const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
-            take_if_better(bestk, besti, bestv, cand ? T(fabs(c1)) : INF,
-                           cand ? j : INT_MAX, c1);
-            take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
-                           cand ? p + j : INT_MAX, c2);
-            // keep one row's division chain live at a time (register
-            // pressure sets the number of signals in flight per CU)
-            __builtin_amdgcn_sched_barrier(0);
+            const bool second = i1 & 1;
+            const T c = second ? (cc - d) / (T(1) - Aj) : (d + cc) / (T(1) + Aj);
+            take_if_better(bestk, besti, bestv, T(fabs(c)), second ? p + j : j, c);
           }
+        } else {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              const int j = VEC * (tid + GT * q) + e;
+              const bool cand = (j < p) & !((act >> (q * VEC + e)) & 1);
+              const T Aj = Areg[q][e];
+              const T d = sDtR[j];
+              T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
+              c1 = a.pos ? INF : c1;
+              const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
+              take_if_better(bestk, besti, bestv, cand ? T(fabs(c1)) : INF,
+                             cand ? j : INT_MAX, c1);
+              take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
+                             cand ? p + j : INT_MAX, c2);
+            }
+        }
         group_argmin<T, W>(bestk, besti, bestv, sRed);
         T step = besti == INT_MAX ? INF : bestv;
         const int nextInd = besti == INT_MAX ? 0 : besti % p;
@@ -464,7 +562,8 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
             sGs[j] = sCoef[j2];
           }
           __syncthreads();
-          if (tid == 0) sAct[dropped] = 0;
+          if (tid == (dropped / VEC) % GT)
+            act &= ~(ActMask(1) << ((dropped / VEC / GT) * VEC + dropped % VEC));
           for (int j = tid; j < i; j += GT) {
             sInd[j] = sTmpI[j];
             sCoef[j] = sGs[j];
