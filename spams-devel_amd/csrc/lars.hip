@@ -16,11 +16,12 @@
 // The kernel is instantiated with an active-set capacity CAP:
 //   CAP > 0  the active-set arrays and invGs live in LDS (capacity CAP); a
 //            signal whose active set would outgrow CAP is appended to an
-//            overflow list and left to the full-capacity pass;
+//            overflow list, with its state (DtR, coefficients, invGs,
+//            counters) saved in the signal's hand-over slot;
 //   CAP == 0 capacity L: active-set arrays in LDS, invGs in a per-workgroup
-//            HBM slot.
-// Both instantiations run the same arithmetic for a given group width, so a
-// signal's result does not depend on which pass solved it.
+//            HBM slot; listed signals resume from their hand-over state.
+// A signal's arithmetic depends only on the two group widths, never on the
+// batch around it.
 //
 // Batched form: persistent grids pulling signal indices from atomic counters
 // (fast pass over 0..n-1, then the full pass over the overflow list, on the
@@ -52,6 +53,35 @@ struct Vec16<double> {
 template <typename T>
 struct FastCap {
   static constexpr int C = sizeof(T) == 4 ? 24 : 16;
+};
+
+// State of a signal handed from the LDS pass to the full-capacity pass,
+// taken just before the admission that would outgrow the LDS capacity C:
+// DtR (ldg), coefficients (C), the inverse Gram block (C x C, leading
+// dimension C), normX and thrs, the active indices (C) and the counters
+// (iterations, admissions, active-set size, atom to admit).  The full pass
+// resumes from it instead of restarting the path.
+template <typename T>
+struct LarsState {
+  static constexpr int C = FastCap<T>::C;
+  static size_t bytes(int ldg) {
+    const size_t b = sizeof(T) * ((size_t)ldg + C + (size_t)C * C + 2) + sizeof(int) * (C + 4);
+    return (b + 15) & ~size_t(15);
+  }
+  T* dtr;
+  T* coef;
+  T* M;
+  T* sc;
+  int* ind;
+  int* isc;
+  __device__ LarsState(void* base, int ldg) {
+    dtr = static_cast<T*>(base);
+    coef = dtr + ldg;
+    M = coef + C;
+    sc = M + C * C;
+    ind = reinterpret_cast<int*>(sc + 2);
+    isc = ind + C;
+  }
 };
 
 template <typename T, int W, int NQ, int CAP>
@@ -178,67 +208,103 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
     // ---------------------------------------------------------- set-up
     __syncthreads();  // the previous signal's LDS state is no longer read
     const T* dtx = a.DtX + (size_t)sig * a.ldd;
-    const bool vec_ok = (a.ldd % VEC == 0) &&
-                        (reinterpret_cast<uintptr_t>(a.DtX) % (sizeof(T) * VEC) == 0);
+    // the full-capacity pass resumes the signals handed over by the LDS pass
+    const bool resume = CAP == 0 && a.state && a.list;
+    const LarsState<T> st(resume ? static_cast<void*>(static_cast<char*>(a.state) +
+                                                      (size_t)sig * a.state_bytes)
+                                 : nullptr, ldg);
+    const bool vec_ok = resume || ((a.ldd % VEC == 0) &&
+                                   (reinterpret_cast<uintptr_t>(a.DtX) % (sizeof(T) * VEC) == 0));
+    const T* src = resume ? st.dtr : dtx;
+    const int nsrc = resume ? ldg : p;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int j0 = VEC * (tid + GT * q);
       V v;
-      if (vec_ok && j0 + VEC <= p) {
-        v = *reinterpret_cast<const V*>(dtx + j0);
+      if (vec_ok && j0 + VEC <= nsrc) {
+        v = *reinterpret_cast<const V*>(src + j0);
       } else {
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) v[e] = j0 + e < p ? dtx[j0 + e] : T(0);
+        for (int e = 0; e < VEC; ++e) v[e] = j0 + e < p ? src[j0 + e] : T(0);
       }
       *reinterpret_cast<V*>(&sDtR[j0]) = v;
     }
     // active flags of this thread's rows (bit q * VEC + e)
     ActMask act = 0;
-    for (int j = tid; j < LC; j += GT) {
-      sCoef[j] = 0;
-      sInd[j] = -1;
-    }
-    T normX = a.norms[sig];
     const T lam = a.lambda[a.nlam > 1 ? sig : 0];
-    __syncthreads();
-
-    // first atom: i?amax of DtR, or the plain max when pos
+    T normX;
+    T thrs0 = 0;
     int currentInd;
-    {
-      T key = INF, pay = 0;
-      int idx = INT_MAX;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) {
-          const int j = VEC * (tid + GT * q) + e;
-          const T d = sDtR[j];
-          const T k = j < p ? (a.pos ? -d : -fabs(d)) : INF;
-          take_if_better(key, idx, pay, k, j < p ? j : INT_MAX, T(0));
-        }
-      group_argmin<T, W>(key, idx, pay, sRed);
-      currentInd = idx == INT_MAX ? 0 : idx;
-    }
+    int i0 = 0;
     int s_active = 0;
     int iter = 0, admissions = 0;
     bool empty = false, overflow = false, reached = false;
-    {
+    if (resume) {
+      const int s0 = st.isc[2];
+      for (int j = tid; j < LC; j += GT) {
+        sCoef[j] = j < s0 ? st.coef[j] : T(0);
+        sInd[j] = j < s0 ? st.ind[j] : -1;
+      }
+      for (int e = tid; e < s0 * s0; e += GT) {
+        const int r = e % s0, q = e / s0;
+        M0[r + q * ldm] = st.M[r + q * LarsState<T>::C];
+      }
+      for (int k = 0; k < s0; ++k) {
+        const int c = st.ind[k];
+        if (tid == (c / VEC) % GT) act |= ActMask(1) << ((c / VEC / GT) * VEC + c % VEC);
+      }
+      normX = st.sc[0];
+      thrs0 = st.sc[1];
+      iter = st.isc[0];
+      admissions = st.isc[1];
+      i0 = s0;
+      s_active = s0;
+      currentInd = st.isc[3];
+      __syncthreads();
+    } else {
+      for (int j = tid; j < LC; j += GT) {
+        sCoef[j] = 0;
+        sInd[j] = -1;
+      }
+      normX = a.norms[sig];
+      __syncthreads();
+
+      // first atom: i?amax of DtR, or the plain max when pos
+      {
+        T key = INF, pay = 0;
+        int idx = INT_MAX;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+            const int j = VEC * (tid + GT * q) + e;
+            const T d = sDtR[j];
+            const T k = j < p ? (a.pos ? -d : -fabs(d)) : INF;
+            take_if_better(key, idx, pay, k, j < p ? j : INT_MAX, T(0));
+          }
+        group_argmin<T, W>(key, idx, pay, sRed);
+        currentInd = idx == INT_MAX ? 0 : idx;
+      }
       const T dc = sDtR[currentInd];
       if (a.mode == SPX_PENALTY && fabs(dc) < lam) empty = true;
       if (a.mode == SPX_L2ERROR && normX < lam) empty = true;
     }
+    int ovf_i = 0;
+    T ovf_thrs = 0;
 
     T* Mc = M0;
     T* Mn = M1;
     if (!empty) {
       bool newAtom = true;
-      T thrs = 0;
-      for (int i = 0; i < Lm; ++i) {
+      T thrs = thrs0;
+      for (int i = i0; i < Lm; ++i) {
         ++iter;
         if (newAtom) {
           if (CAP > 0 && i >= LC) {
             // the active set outgrows the fast pass: hand the signal over
             overflow = true;
+            ovf_i = i;
+            ovf_thrs = thrs;
             break;
           }
           // ---------------------------------------------- admit an atom
@@ -633,6 +699,31 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         const int pos = atomicAdd(a.ovf_count, 1);
         a.ovf_list[pos] = sig;
       }
+      if (CAP > 0 && a.state) {
+        // hand-over state for the full-capacity pass
+        LarsState<T> sv(static_cast<char*>(a.state) + (size_t)sig * a.state_bytes, ldg);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int j0 = VEC * (tid + GT * q);
+          if (j0 < ldg) *reinterpret_cast<V*>(sv.dtr + j0) = *reinterpret_cast<const V*>(&sDtR[j0]);
+        }
+        for (int j = tid; j < ovf_i; j += GT) {
+          sv.coef[j] = sCoef[j];
+          sv.ind[j] = sInd[j];
+        }
+        for (int e = tid; e < ovf_i * ovf_i; e += GT) {
+          const int r = e % ovf_i, q = e / ovf_i;
+          sv.M[r + q * LarsState<T>::C] = Mc[r + q * ldm];
+        }
+        if (tid == 0) {
+          sv.sc[0] = normX;
+          sv.sc[1] = ovf_thrs;
+          sv.isc[0] = iter - 1;  // the interrupted iteration is redone
+          sv.isc[1] = admissions;
+          sv.isc[2] = ovf_i;
+          sv.isc[3] = currentInd;
+        }
+      }
       continue;
     }
     T* ov = a.outV + (size_t)sig * L;
@@ -775,6 +866,16 @@ int lars_batched(Ctx* ctx, const LarsArgs<T>& a0, int w) {
   a.ovf_count = cnt + 1;
   a.list = nullptr;
   a.list_count = nullptr;
+  if (a0.L > FastCap<T>::C) {
+    // one hand-over slot per signal (only overflowing signals write theirs),
+    // so whether a signal resumes never depends on the batch around it
+    const size_t sb = LarsState<T>::bytes(a0.ldg);
+    void* stp = nullptr;
+    rc = ctx->scratch_get(13, sb * (size_t)a0.n, &stp);
+    if (rc) return rc;
+    a.state = stp;
+    a.state_bytes = sb;
+  }
   if (a0.nlam > 1 && !a.order) {
     // longest paths first: the homotopy is longest for the smallest lambda
     int* order = nullptr;

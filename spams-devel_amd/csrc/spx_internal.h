@@ -46,6 +46,11 @@ struct LarsArgs {
   int chunk;        // signals taken per atomic grab of the work counter
   const int* order; // work order (signal index per work item) or null
   int refit;        // FP64 refit of the final active set (PENALTY mode)
+  // Per-signal state slots (state_bytes each) through which the LDS pass
+  // hands its overflowing signals to the full-capacity pass, which resumes
+  // them; null = the full pass solves every listed signal from the start.
+  void* state;
+  size_t state_bytes;
 };
 
 // Kernel arguments of the ORMP solver (coreORMP semantics).

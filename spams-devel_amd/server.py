@@ -121,7 +121,9 @@ class SparseCodingServer:
         # solver scratch: one slot per resident workgroup, bounded by n
         slot = (2 if kind == "lasso" else 1) * L * L * 4
         wg = min(n, 256 * 16)
-        return per + wg * slot + (1 << 20)
+        # LARS hand-over state between its two passes: one slot per signal
+        handover = n * (4 * (ldg + 24 + 24 * 24 + 2) + 4 * 28) if kind == "lasso" else 0
+        return per + wg * slot + handover + (1 << 20)
 
     # --------------------------------------------------------------- run
     def _run_batch(self, lane: _Lane, jobs):
