@@ -118,7 +118,18 @@ __device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
     const int r = t % n, k = t / n;
     const int q0 = k * seg, q1 = q0 + seg < n ? q0 + seg : n;
     T acc = 0;
-    for (int q = q0; q < q1; ++q) acc += M[r + q * ldm] * x[q];
+    int q = q0;
+    if (SPLIT) {
+      // M in HBM: eight loads in flight, then the sums in order
+      for (; q + 8 <= q1; q += 8) {
+        T mv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) mv[u] = M[r + (q + u) * ldm];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += mv[u] * x[q + u];
+      }
+    }
+    for (; q < q1; ++q) acc += M[r + q * ldm] * x[q];
     if (P == 1) y[r] = acc;
     else part[k * n + r] = acc;
   }
@@ -350,11 +361,25 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
             // one element per thread; element (r,q) and (q,r) get the
             // identical update v[min] * (schur * v[max]) (exact symmetry)
             if (CAP == 0) {
-              for (int e = tid; e < i * i; e += GT) {
-                const int r = e % i, q = e / i;
-                const T vr = sV[r], vq = sV[q];
-                const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-                Mc[r + q * ldm] += upd;
+              // M in HBM: each thread loads eight elements before storing
+              const int nn = i * i;
+              for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
+                T mv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                  const int e = e0 + u * GT;
+                  mv[u] = e < nn ? Mc[e % i + (e / i) * ldm] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                  const int e = e0 + u * GT;
+                  if (e < nn) {
+                    const int r = e % i, q = e / i;
+                    const T vr = sV[r], vq = sV[q];
+                    const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
+                    Mc[r + q * ldm] = mv[u] + upd;
+                  }
+                }
               }
             } else {
               for (int r = tid; r < i; r += GT) {
@@ -603,12 +628,25 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
           __syncthreads();
           const T alpha = T(-1) / schur;
           if (CAP == 0) {
-            for (int e = tid; e < i * i; e += GT) {
-              const int r = e % i, q = e / i;
-              const int r2 = r + (r >= fz), q2 = q + (q >= fz);
-              const T ur = sV[r], uq = sV[q];
-              const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
-              Mn[r + q * ldm] = Mc[r2 + q2 * ldm] + upd;
+            const int nn = i * i;
+            for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
+              T mv[8];
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * GT;
+                const int r = e % i, q = e / i;
+                mv[u] = e < nn ? Mc[r + (r >= fz) + (q + (q >= fz)) * ldm] : T(0);
+              }
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * GT;
+                if (e < nn) {
+                  const int r = e % i, q = e / i;
+                  const T ur = sV[r], uq = sV[q];
+                  const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
+                  Mn[r + q * ldm] = mv[u] + upd;
+                }
+              }
             }
           } else {
             for (int r = tid; r < i; r += GT) {
