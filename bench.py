@@ -308,7 +308,8 @@ def run_lars(args):
                           "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
                           "frac": gemm_flops / (gemm_ms * 1e-3) / FP32_MFMA_PEAK},
         "kernel_ms": {"dtx_and_norms": gemm_ms, "lars_solver": solve_ms,
-                      "lars_order_sort": passes.get("lars_order", 0.0)},
+                      "lars_order_sort": passes.get("lars_order", 0.0),
+                      "lars_overflow_sort": passes.get("lars_list", 0.0)},
         "autotune": tune,
         "solver_stats": {"mean_admissions": float(admissions.mean()),
                          "max_admissions": int(admissions.max()),

@@ -131,6 +131,7 @@ enum {
   SPX_TIMER_LARS_FAST = 1,  /* LARS pass with the active set in LDS */
   SPX_TIMER_LARS_FULL = 2,  /* LARS pass over the signals that outgrew LDS */
   SPX_TIMER_ORMP = 3,       /* ORMP solver */
+  SPX_TIMER_LARS_LIST = 4,  /* ordering of the overflow list (longest first) */
   SPX_TIMER_COUNT = 8
 };
 int spx_ctx_set_timing(spx_ctx* ctx, int on);

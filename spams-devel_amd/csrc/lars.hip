@@ -736,6 +736,12 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
       if (tid == 0) {
         const int pos = atomicAdd(a.ovf_count, 1);
         a.ovf_list[pos] = sig;
+        // remaining path: from the current correlation down to lambda (the
+        // smaller the ratio, the longer; the full pass takes these first)
+        if (a.ovf_key) {
+          const T kk = lam / fabs(sDtR[sInd[0]]);
+          a.ovf_key[pos] = kk == kk ? kk : T(0);  // NaN sorts first, not past the list
+        }
       }
       if (CAP > 0 && a.state) {
         // hand-over state for the full-capacity pass
@@ -902,6 +908,12 @@ int lars_batched(Ctx* ctx, const LarsArgs<T>& a0, int w) {
   if (a.chunk <= 0) a.chunk = 1;
   a.ovf_list = ovf;
   a.ovf_count = cnt + 1;
+  if (a0.L > FastCap<T>::C && a0.n > 1) {
+    T* key = nullptr;
+    rc = ctx->scratch_get(14, sizeof(T) * (size_t)a0.n, reinterpret_cast<void**>(&key));
+    if (rc) return rc;
+    a.ovf_key = key;
+  }
   a.list = nullptr;
   a.list_count = nullptr;
   if (a0.L > FastCap<T>::C) {
@@ -932,6 +944,17 @@ int lars_batched(Ctx* ctx, const LarsArgs<T>& a0, int w) {
   if (a0.L <= FastCap<T>::C) return SPX_OK;  // every active set fitted
   a.list = ovf;
   a.list_count = cnt + 1;
+  if (a.ovf_key) {
+    // longest remaining path first (stable sort of the listed signals by
+    // their key; unused entries sort last)
+    ScopedTimer tm(ctx, SPX_TIMER_LARS_LIST);
+    int* sorted = nullptr;
+    rc = ctx->scratch_get(15, sizeof(int) * (size_t)a0.n, reinterpret_cast<void**>(&sorted));
+    if (rc) return rc;
+    rc = sort_list<T>(ctx, a.ovf_key, ovf, cnt + 1, a0.n, sorted);
+    if (rc) return rc;
+    a.list = sorted;
+  }
   int w2 = w < 4 ? 4 : w;
   ScopedTimer tm(ctx, SPX_TIMER_LARS_FULL);
   rc = lars_pass_w<T>(ctx, w2, false, a, true, cnt + 2, a0.n);

@@ -51,6 +51,7 @@ struct LarsArgs {
   // them; null = the full pass solves every listed signal from the start.
   void* state;
   size_t state_bytes;
+  T* ovf_key;       // per list position: work key of the overflowing signal
 };
 
 // Kernel arguments of the ORMP solver (coreORMP semantics).
@@ -102,6 +103,11 @@ int fista_run(Ctx* ctx, const Dict* dict, const T* X, int n, const T* W0,
 // order[i] = indices 0..n-1 sorted by key ascending (stable); scratch slot 9.
 template <typename T>
 int sort_order(Ctx* ctx, const T* key, int n, int* order);
+
+// out[0:count) = list[0:count) stably sorted by key[0:count) ascending,
+// where count = *count_dev (device value <= n); scratch slots 9, 11.
+template <typename T>
+int sort_list(Ctx* ctx, const T* key, const int* list, const int* count_dev, int n, int* out);
 
 // Clears n ints with a kernel (capture-safe replacement for a memset node).
 int zero_ints(hipStream_t st, int* p, int n);

@@ -27,7 +27,7 @@ PREC = {"fp32": 0, "bf16x3": 1, "fp32+refit64": 2, "bf16x3+refit64": 3}
 FORM = {"batched": 0, "plain": 1}
 SOLVERS = {"lasso": 0, "omp": 1}
 # launch timers of spx_ctx_read_timing, in SPX_TIMER_* order
-TIMERS = ("lars_order", "lars_fast", "lars_full", "ormp")
+TIMERS = ("lars_order", "lars_fast", "lars_full", "ormp", "lars_list")
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int

@@ -359,6 +359,7 @@ def test_launch_timers_count_solver_launches(spx):
     spx.lasso_dev(ctx, d, Xt, lt, V, R, c)  # not recorded
     t = ctx.read_timing()
     assert t["lars_fast"][1] == 3 and t["lars_full"][1] == 3 and t["lars_order"][1] == 3
+    assert t["lars_list"][1] == 3
     assert t["lars_fast"][0] > 0 and t["lars_full"][0] > 0
     assert t["ormp"][1] == 0
     ctx.reset_timing()
