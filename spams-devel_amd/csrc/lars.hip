@@ -90,7 +90,7 @@ struct LarsLds {
   static constexpr int VEC = Vec16<T>::N;
   static constexpr int PT = VEC * GT * NQ;
   // byte size for active-set capacity LC
-  static size_t bytes(int LC) {
+  static size_t bytes(int LC, int CL = 0) {
     size_t b = sizeof(T) * (PT + GT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC;
     b = (b + 15) & ~size_t(15);
     b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
@@ -98,6 +98,8 @@ struct LarsLds {
     if (CAP > 0) b += 2 * sizeof(T) * CAP * CAP + 16;
     b = (b + 15) & ~size_t(15);
     b += sizeof(double) * (size_t)(LC + (LC > 64 ? LC : 64)) + 16;  // FP64 vectors
+    b = (b + 15) & ~size_t(15);
+    b += sizeof(T) * (size_t)CL * CL + 16;  // LDS copy of the inverse (full pass)
     return b;
   }
 };
@@ -144,8 +146,136 @@ __device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
   }
 }
 
+// Admits the i-th active atom (i >= 1) into the explicit inverse M of the
+// active Gram block, i x i -> (i+1) x (i+1), by its Schur complement
+// (decomp/decomp.h:1756-1770); sGs(0:i+1) = (G + diag I)(active, new atom).
+// v = M gs lands in sV.  An FP32 pivot below 1e-3 of the diagonal (or not
+// positive) is recomputed with FP64 accumulation.  ELEM: the rank-one update
+// runs one element per thread with eight loads in flight (M in HBM, or a
+// wide group); else one row per thread.  Elements (r,q) and (q,r) get the
+// identical update v[min] * (schur * v[max]), so M stays exactly symmetric.
+// Ends with a barrier.
+template <typename T, int W, bool SPLIT, bool ELEM>
+__device__ __forceinline__ void inverse_admit(T* M, int ldm, int i, const T* sGs, T* sV,
+                                              T* sPart, T* sRed3, double* sRes) {
+  constexpr int GT = 64 * W;
+  const int tid = threadIdx.x;
+  // v = invGs(i x i) * gs(0:i)
+  symv_dist<T, GT, SPLIT>(M, ldm, sGs, sV, sPart, i);
+  T dv = 0, z1 = 0, z2 = 0;
+  for (int r = tid; r < i; r += GT) dv += sV[r] * sGs[r];
+  group_sum3<T, W>(dv, z1, z2, sRed3);
+  T schur = T(1) / (sGs[i] - dv);
+  if (sizeof(T) == 4 && !(sGs[i] - dv > T(1e-3) * sGs[i])) {
+    __syncthreads();
+    double pd = 0;
+    for (int r = tid; r < i; r += GT) {
+      double acc = 0;
+      for (int q = 0; q < i; ++q)
+        acc += static_cast<double>(M[r + q * ldm]) * static_cast<double>(sGs[q]);
+      sV[r] = static_cast<T>(acc);
+      pd += acc * static_cast<double>(sGs[r]);
+    }
+    double z1d = 0, z2d = 0;
+    group_sum3<double, W>(pd, z1d, z2d, sRes);
+    schur = static_cast<T>(1.0 / (static_cast<double>(sGs[i]) - pd));
+    __syncthreads();
+  }
+  if (ELEM) {
+    const int nn = i * i;
+    for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
+      T mv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * GT;
+        mv[u] = e < nn ? M[e % i + (e / i) * ldm] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * GT;
+        if (e < nn) {
+          const int r = e % i, q = e / i;
+          const T vr = sV[r], vq = sV[q];
+          const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
+          M[r + q * ldm] = mv[u] + upd;
+        }
+      }
+    }
+  } else {
+    for (int r = tid; r < i; r += GT) {
+      const T vr = sV[r];
+      for (int q = 0; q < i; ++q) {
+        const T vq = sV[q];
+        const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
+        M[r + q * ldm] += upd;
+      }
+    }
+  }
+  for (int r = tid; r < i; r += GT) {
+    const T off = -schur * sV[r];
+    M[r + i * ldm] = off;
+    M[i + r * ldm] = off;
+  }
+  if (tid == 0) M[i + i * ldm] = schur;
+  __syncthreads();
+}
+
+// Md (i x i) = Ms ((i+1) x (i+1)) without row and column fz, downdated by
+// -1/Ms(fz,fz) times the outer product of that column
+// (decomp/decomp.h:1875-1903).  Ms and Md are distinct buffers.  No barrier
+// after the writes.
+template <typename T, int W, bool ELEM>
+__device__ __forceinline__ void inverse_drop(const T* Ms, int lds, T* Md, int ldd, int i, int fz,
+                                             T* sV) {
+  constexpr int GT = 64 * W;
+  const int tid = threadIdx.x;
+  const T schur = Ms[fz + fz * lds];
+  for (int r = tid; r < i; r += GT) {
+    const int r2 = r + (r >= fz);
+    sV[r] = Ms[r2 + fz * lds];
+  }
+  __syncthreads();
+  const T alpha = T(-1) / schur;
+  if (ELEM) {
+    const int nn = i * i;
+    for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
+      T mv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * GT;
+        const int r = e % i, q = e / i;
+        mv[u] = e < nn ? Ms[r + (r >= fz) + (q + (q >= fz)) * lds] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * GT;
+        if (e < nn) {
+          const int r = e % i, q = e / i;
+          const T ur = sV[r], uq = sV[q];
+          const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
+          Md[r + q * ldd] = mv[u] + upd;
+        }
+      }
+    }
+  } else {
+    for (int r = tid; r < i; r += GT) {
+      const int r2 = r + (r >= fz);
+      const T ur = sV[r];
+      for (int q = 0; q < i; ++q) {
+        const int q2 = q + (q >= fz);
+        const T uq = sV[q];
+        const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
+        Md[r + q * ldd] = Ms[r2 + q2 * lds] + upd;
+      }
+    }
+  }
+}
+
+// The full-capacity pass at 4 wavefronts is held to 4 workgroups per CU
+// (128 VGPRs): its signals are latency-bound, so occupancy pays more than
+// registers.
 template <typename T, int W, int NQ, int CAP>
-__global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
+__global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_kernel(LarsArgs<T> a) {
   constexpr int GT = 64 * W;
   constexpr int VEC = Vec16<T>::N;
   constexpr int PT = VEC * GT * NQ;  // rows covered by the group
@@ -178,6 +308,11 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
   double* sAlpha = reinterpret_cast<double*>(
       (reinterpret_cast<uintptr_t>(sM1 + (CAP > 0 ? CAP * CAP : 0)) + 15) & ~uintptr_t(15));
   double* sRes = sAlpha + LC;  // FP64 refit vectors; sRes has max(LC, 64)
+  // full-capacity pass: LDS copy of the inverse while the active set has at
+  // most CL atoms (CL x CL, leading dimension CL)
+  const int CL = CAP == 0 ? a.lds_cap : 0;
+  T* sMl = reinterpret_cast<T*>(
+      (reinterpret_cast<uintptr_t>(sRes + (LC > 64 ? LC : 64)) + 15) & ~uintptr_t(15));
 
   const int tid = threadIdx.x;
   T* const M0 = CAP > 0 ? sM0 : a.scratch + (size_t)blockIdx.x * 2 * L * L;
@@ -258,7 +393,8 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
       }
       for (int e = tid; e < s0 * s0; e += GT) {
         const int r = e % s0, q = e / s0;
-        M0[r + q * ldm] = st.M[r + q * LarsState<T>::C];
+        if (CL >= s0) sMl[r + q * CL] = st.M[r + q * LarsState<T>::C];
+        else M0[r + q * ldm] = st.M[r + q * LarsState<T>::C];
       }
       for (int k = 0; k < s0; ++k) {
         const int c = st.ind[k];
@@ -305,6 +441,9 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
 
     T* Mc = M0;
     T* Mn = M1;
+    // full-capacity pass: the inverse starts in LDS (capacity CL) and moves
+    // to the HBM slot when the active set outgrows it
+    bool inl = CAP == 0 && CL > 0 && (!resume || CL >= i0);
     if (!empty) {
       bool newAtom = true;
       T thrs = thrs0;
@@ -332,72 +471,22 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
           }
           __syncthreads();
           if (i == 0) {
-            if (tid == 0) Mc[0] = T(1) / sGs[0];
+            if (tid == 0) {
+              if (inl) sMl[0] = T(1) / sGs[0];
+              else Mc[0] = T(1) / sGs[0];
+            }
           } else {
-            // v = invGs(i x i) * gs(0:i)
-            symv_dist<T, GT, CAP == 0>(Mc, ldm, sGs, sV, sPart, i);
-            T dv = 0, z1 = 0, z2 = 0;
-            for (int r = tid; r < i; r += GT) dv += sV[r] * sGs[r];
-            group_sum3<T, W>(dv, z1, z2, sRed3);
-            T schur = T(1) / (sGs[i] - dv);
-            if (sizeof(T) == 4 && !(sGs[i] - dv > T(1e-3) * sGs[i])) {
-              // low pivot (below 1e-3 of the diagonal, or not positive):
-              // recompute v = invGs gs and the pivot with FP64 accumulation
+            if (inl && i >= CL) {
+              // the block outgrows the LDS copy: continue in the HBM slot
+              for (int e = tid; e < i * i; e += GT) {
+                const int r = e % i, q = e / i;
+                Mc[r + q * ldm] = sMl[r + q * CL];
+              }
               __syncthreads();
-              double pd = 0;
-              for (int r = tid; r < i; r += GT) {
-                double acc = 0;
-                for (int q = 0; q < i; ++q)
-                  acc += static_cast<double>(Mc[r + q * ldm]) * static_cast<double>(sGs[q]);
-                sV[r] = static_cast<T>(acc);
-                pd += acc * static_cast<double>(sGs[r]);
-              }
-              double z1d = 0, z2d = 0;
-              group_sum3<double, W>(pd, z1d, z2d, reinterpret_cast<double*>(sRes));
-              schur = static_cast<T>(1.0 / (static_cast<double>(sGs[i]) - pd));
-              __syncthreads();
+              inl = false;
             }
-            // Schur-complement update of the inverse (decomp/decomp.h:1756-1770),
-            // one element per thread; element (r,q) and (q,r) get the
-            // identical update v[min] * (schur * v[max]) (exact symmetry)
-            if (CAP == 0) {
-              // M in HBM: each thread loads eight elements before storing
-              const int nn = i * i;
-              for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
-                T mv[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                  const int e = e0 + u * GT;
-                  mv[u] = e < nn ? Mc[e % i + (e / i) * ldm] : T(0);
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                  const int e = e0 + u * GT;
-                  if (e < nn) {
-                    const int r = e % i, q = e / i;
-                    const T vr = sV[r], vq = sV[q];
-                    const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-                    Mc[r + q * ldm] = mv[u] + upd;
-                  }
-                }
-              }
-            } else {
-              for (int r = tid; r < i; r += GT) {
-                const T vr = sV[r];
-                for (int q = 0; q < i; ++q) {
-                  const T vq = sV[q];
-                  const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-                  Mc[r + q * ldm] += upd;
-                }
-              }
-            }
-            for (int r = tid; r < i; r += GT) {
-              const T off = -schur * sV[r];
-              Mc[r + i * ldm] = off;
-              Mc[i + r * ldm] = off;
-            }
-            if (tid == 0) Mc[i + i * ldm] = schur;
-            __syncthreads();
+            if (inl) inverse_admit<T, W, true, true>(sMl, CL, i, sGs, sV, sPart, sRed3, sRes);
+            else inverse_admit<T, W, CAP == 0, CAP == 0>(Mc, ldm, i, sGs, sV, sPart, sRed3, sRes);
           }
         }
 
@@ -405,7 +494,8 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         // ------------------------------------------ direction u = invGs*w
         for (int j = tid; j < s; j += GT) sW[j] = sDtR[sInd[j]] > 0 ? T(1) : T(-1);
         __syncthreads();
-        symv_dist<T, GT, CAP == 0>(Mc, ldm, sW, sU, sPart, s);
+        if (inl) symv_dist<T, GT, true>(sMl, CL, sW, sU, sPart, s);
+        else symv_dist<T, GT, CAP == 0>(Mc, ldm, sW, sU, sPart, s);
 
         // ------------------ zero-crossing step, polynomial coefficients
         T step_max;
@@ -620,45 +710,16 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
           // ------------------- drop atom first_zero (decomp/decomp.h:1875-1903)
           const int fz = first_zero;
           const int dropped = sInd[fz];
-          const T schur = Mc[fz + fz * ldm];
-          for (int r = tid; r < i; r += GT) {
-            const int r2 = r + (r >= fz);
-            sV[r] = Mc[r2 + fz * ldm];
-          }
-          __syncthreads();
-          const T alpha = T(-1) / schur;
-          if (CAP == 0) {
-            const int nn = i * i;
-            for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
-              T mv[8];
-#pragma unroll
-              for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * GT;
-                const int r = e % i, q = e / i;
-                mv[u] = e < nn ? Mc[r + (r >= fz) + (q + (q >= fz)) * ldm] : T(0);
-              }
-#pragma unroll
-              for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * GT;
-                if (e < nn) {
-                  const int r = e % i, q = e / i;
-                  const T ur = sV[r], uq = sV[q];
-                  const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
-                  Mn[r + q * ldm] = mv[u] + upd;
-                }
-              }
+          if (inl) {
+            // LDS copy: stage the block in the HBM slot, downdate back
+            for (int e = tid; e < (i + 1) * (i + 1); e += GT) {
+              const int r = e % (i + 1), q = e / (i + 1);
+              Mn[r + q * ldm] = sMl[r + q * CL];
             }
+            __syncthreads();
+            inverse_drop<T, W, true>(Mn, ldm, sMl, CL, i, fz, sV);
           } else {
-            for (int r = tid; r < i; r += GT) {
-              const int r2 = r + (r >= fz);
-              const T ur = sV[r];
-              for (int q = 0; q < i; ++q) {
-                const int q2 = q + (q >= fz);
-                const T uq = sV[q];
-                const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
-                Mn[r + q * ldm] = Mc[r2 + q2 * ldm] + upd;
-              }
-            }
+            inverse_drop<T, W, CAP == 0>(Mc, ldm, Mn, ldm, i, fz, sV);
           }
           for (int j = tid; j < i; j += GT) {
             const int j2 = j + (j >= fz);
@@ -677,9 +738,11 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
             sCoef[i] = 0;
           }
           __syncthreads();
-          T* t = Mc;
-          Mc = Mn;
-          Mn = t;
+          if (!inl) {
+            T* t = Mc;
+            Mc = Mn;
+            Mn = t;
+          }
           --s_active;
           newAtom = false;
           i = i - 2;
@@ -702,7 +765,8 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
     if (a.refit && reached && !overflow && s_active > 0) {
       __syncthreads();
       const int s = s_active;
-      T* Mf = Mc;  // the inverse of the final active block
+      const T* Mf = inl ? sMl : Mc;  // the inverse of the final active block
+      const int ldf = inl ? CL : ldm;
       for (int r = tid; r < s; r += GT) sAlpha[r] = static_cast<double>(sCoef[r]);
       __syncthreads();
       for (int it = 0; it < 2; ++it) {
@@ -722,7 +786,7 @@ __global__ __launch_bounds__(64 * W) void lars_kernel(LarsArgs<T> a) {
         // alpha += invGs res
         for (int r = tid; r < s; r += GT) {
           double acc = 0;
-          for (int q = 0; q < s; ++q) acc += static_cast<double>(Mf[r + q * ldm]) * sRes[q];
+          for (int q = 0; q < s; ++q) acc += static_cast<double>(Mf[r + q * ldf]) * sRes[q];
           sAlpha[r] += acc;
         }
         __syncthreads();
@@ -795,16 +859,32 @@ template <typename T, int W, int NQ, int CAP>
 int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
                 long max_blocks) {
   const int LC = CAP > 0 ? (CAP < a.L ? CAP : a.L) : a.L;
-  const size_t lds = LarsLds<T, W, NQ, CAP>::bytes(LC);
+  size_t lds = LarsLds<T, W, NQ, CAP>::bytes(LC);
   if (lds > 160 * 1024) {
     set_error("lasso: L=%d, p=%d need %zu bytes of LDS per signal", a.L, a.p, lds);
     return SPX_ERR_UNSUP;
   }
   auto kern = lars_kernel<T, W, NQ, CAP>;
   SPX_HIP(hipFuncSetAttribute((const void*)kern,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   hipStream_t st = ctx->stream;
   LarsArgs<T> b = a;
+  b.lds_cap = 0;
+  if (CAP == 0) {
+    // LDS copy of the inverse: the LDS the register-limited occupancy leaves
+    // free, if it holds at least the LDS pass's capacity
+    int per_cu = 0;
+    SPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds));
+    if (per_cu < 1) per_cu = 1;
+    const size_t budget = (size_t)156 * 1024 / per_cu;
+    int cl = 0;
+    while (cl + 4 <= a.L && LarsLds<T, W, NQ, CAP>::bytes(LC, cl + 4) <= budget) cl += 4;
+    if (cl >= a.L) cl = a.L;
+    if (cl >= FastCap<T>::C) {
+      b.lds_cap = cl;
+      lds = LarsLds<T, W, NQ, CAP>::bytes(LC, cl);
+    }
+  }
   if (persistent) {
     int per_cu = 0;
     SPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds));

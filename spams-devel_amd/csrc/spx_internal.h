@@ -52,6 +52,7 @@ struct LarsArgs {
   void* state;
   size_t state_bytes;
   T* ovf_key;       // per list position: work key of the overflowing signal
+  int lds_cap;      // full-capacity pass: LDS capacity of the inverse (0 = HBM only)
 };
 
 // Kernel arguments of the ORMP solver (coreORMP semantics).
