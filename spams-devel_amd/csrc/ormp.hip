@@ -2,16 +2,25 @@
 //
 // One group of W wavefronts (one workgroup) per signal at a time.  Per-row
 // state (the residual correlations Rdn and the projected atom norms `norm`)
-// lives in registers of the owning thread; the selected-atom flags, RUn and
-// the selection order live in LDS; the inverse Cholesky factor Un (upper
-// triangular, L x L) lives in a per-workgroup HBM slot.
+// lives in registers of the owning thread, the selected-atom flags in a bit
+// mask of that thread; RUn, the selection order and (for L up to the LDS
+// capacity) the inverse Cholesky factor Un (upper triangular, L x L) live in
+// LDS, a larger Un in a per-workgroup HBM slot.
 //
 // The reference keeps Undn = Gs * Un (p x L) only to read row c of it when
 // atom c is selected (decomp/decomp.h:596).  That row equals
 // G(c, active) * Un, an O(j^2) product, so it is recomputed from the resident
 // Gram column instead of storing p x L per signal.  Each step streams the
-// j+1 selected Gram columns once to form Undn_j = G(:, active) * Un_j.
+// j+1 selected Gram columns once (several in flight) to form
+// Undn_j = G(:, active) * Un_j.
+//
+// The arg-max of the scores Rdn^2 / norm (i?amax: first index of the largest
+// magnitude) takes two passes in FP32: keys from the hardware reciprocal
+// (within 3 ulp of the IEEE quotient), then the IEEE quotient for the keys
+// within a margin of the largest -- the same winner as IEEE quotients
+// everywhere, at one division per thread instead of one per row.
 #include <climits>
+#include <type_traits>
 
 #include "spx_common.h"
 #include "spx_internal.h"
@@ -33,12 +42,18 @@ struct Vec16o<double> {
   static constexpr int N = 2;
 };
 
-template <typename T, int W, int NQ>
-__global__ __launch_bounds__(64 * W) void ormp_kernel(OmpArgs<T> a) {
+// LDS capacity for Un: L x L elements of at most this many bytes.
+constexpr size_t kUnLdsBytes = 32 * 1024;
+
+// Held to 4 wavefronts per SIMD up to 16 rows per lane: the signals are
+// latency-bound, so occupancy pays more than registers.
+template <typename T, int W, int NQ, bool ULDS>
+__global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T> a) {
   constexpr int GT = 64 * W;
   constexpr int VEC = Vec16o<T>::N;
-  constexpr int PT = VEC * GT * NQ;
+  constexpr int UNR = NQ >= 4 ? 2 : (NQ == 2 ? 4 : 8);  // Gram columns in flight
   typedef typename Vec16o<T>::type V;
+  typedef typename std::conditional<(NQ * VEC <= 32), unsigned, unsigned long long>::type ActMask;
   const T INF = dev_inf<T>();
 
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -50,39 +65,47 @@ __global__ __launch_bounds__(64 * W) void ormp_kernel(OmpArgs<T> a) {
   T* sGc = sX + L;                        // L  (G(c, active))
   T* sScal = sGc + L;                     // 4  (Rdn[c], norm[c])
   int* sInd = reinterpret_cast<int*>(sScal + 4);  // L
-  unsigned char* sAct = reinterpret_cast<unsigned char*>(sInd + L);  // PT
   KeyIdx<T>* sRed = reinterpret_cast<KeyIdx<T>*>(
-      (reinterpret_cast<uintptr_t>(sAct + PT) + 15) & ~uintptr_t(15));
+      (reinterpret_cast<uintptr_t>(sInd + L) + 15) & ~uintptr_t(15));
   int* sSig = reinterpret_cast<int*>(sRed + W);
+  T* sUn = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(sSig + 1) + 15) & ~uintptr_t(15));
 
   const int tid = threadIdx.x;
-  T* Un = a.scratch + (size_t)blockIdx.x * L * L;  // column-major L x L
+  T* const Un = ULDS ? sUn : a.scratch + (size_t)blockIdx.x * L * L;  // column-major L x L
 
+  // the next grab is requested when the current signal starts, so its
+  // latency overlaps the set-up loads
+  int pre = 0;
+  if (a.queue && tid == 0) pre = atomicAdd(a.queue, 1);
   for (int round = 0;; ++round) {
     int sig;
     if (a.queue) {
       __syncthreads();
-      if (tid == 0) *sSig = atomicAdd(a.queue, 1);
+      if (tid == 0) *sSig = pre;
       __syncthreads();
       sig = *sSig;
+      if (tid == 0 && sig < a.n) pre = atomicAdd(a.queue, 1);
     } else {
       sig = round == 0 ? (int)blockIdx.x : a.n;
     }
     if (sig >= a.n) break;
 
     const T* dtx = a.DtX + (size_t)sig * a.ldd;
+    const bool vec_ok = (a.ldd % VEC == 0) &&
+                        (reinterpret_cast<uintptr_t>(a.DtX) % (sizeof(T) * VEC) == 0);
     V Rdn[NQ], nrm[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int j0 = VEC * (tid + GT * q);
+      if (vec_ok && j0 + VEC <= p) {
+        Rdn[q] = *reinterpret_cast<const V*>(dtx + j0);
+      } else {
 #pragma unroll
-      for (int e = 0; e < VEC; ++e) {
-        const int j = j0 + e;
-        Rdn[q][e] = j < p ? dtx[j] : T(0);
-        nrm[q][e] = T(1);
-        sAct[j] = j < p ? 0 : 1;
+        for (int e = 0; e < VEC; ++e) Rdn[q][e] = j0 + e < p ? dtx[j0 + e] : T(0);
       }
+      nrm[q] = V(1);
     }
+    ActMask act = 0;  // selected atoms among this thread's rows (bit q * VEC + e)
     for (int j = tid; j < L; j += GT) sInd[j] = -1;
     T normX = a.norms[sig];
     const T eps = fabs(a.nEps == 0 ? T(0) : a.eps[a.nEps > 1 ? sig : 0]);
@@ -95,26 +118,91 @@ __global__ __launch_bounds__(64 * W) void ormp_kernel(OmpArgs<T> a) {
     if (!((normX <= eps) || Li <= 0)) {
       for (j = 0; j < Li; ++j) {
         // ------------------------------------------ c = i?amax(scores)
+        // scores = Rdn before the first update, then Rdn^2 / norm with the
+        // selected atoms zeroed (decomp/decomp.h:550, :626-629); key -|score|
         T key = INF, pay = 0;
         int idx = INT_MAX;
+        if (j == 0 || sizeof(T) != 4) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+          for (int q = 0; q < NQ; ++q)
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) {
-            const int r = VEC * (tid + GT * q) + e;
-            if (r < p) {
-              // scores = Rdn before the first update, then Rdn^2 / norm with
-              // the selected atoms zeroed (decomp/decomp.h:550, :626-629)
-              T sc;
-              if (j == 0) sc = Rdn[q][e];
-              else sc = sAct[r] ? T(0) : (Rdn[q][e] * Rdn[q][e]) / nrm[q][e];
-              const T k = -fabs(sc);
-              if (key_better(k, r, key, idx)) {
-                key = k;
-                idx = r;
-              }
+            for (int e = 0; e < VEC; ++e) {
+              const int r = VEC * (tid + GT * q) + e;
+              const bool sel = (act >> (q * VEC + e)) & 1;
+              const T sc = j == 0 ? Rdn[q][e] : (sel ? T(0) : (Rdn[q][e] * Rdn[q][e]) / nrm[q][e]);
+              take_if_better(key, idx, pay, r < p ? T(-fabs(sc)) : INF, r < p ? r : INT_MAX, T(0));
             }
+        } else {
+          // pass 1: approximate magnitudes; per thread the largest (slot i1)
+          // and the second largest
+          T m1 = -INF, m2 = -INF;
+          int i1 = 0;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              const int r = VEC * (tid + GT * q) + e;
+              const bool sel = (act >> (q * VEC + e)) & 1;
+              const T ka = (r < p) ? (sel ? T(0)
+                                          : T(fabs(Rdn[q][e] * Rdn[q][e] *
+                                                   __builtin_amdgcn_rcpf(nrm[q][e]))))
+                                   : -INF;
+              const bool gt = ka > m1;
+              m2 = gt ? m1 : fmaxf(m2, ka);
+              i1 = gt ? q * VEC + e : i1;
+              m1 = gt ? ka : m1;
+            }
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            V x = Rdn[q], y = nrm[q];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              T u = x[e], w = y[e];
+              asm volatile("" : "+v"(u), "+v"(w));
+              x[e] = u;
+              y[e] = w;
+            }
+            Rdn[q] = x;
+            nrm[q] = y;
           }
+          T amax = -m1, zp = 0;
+          int zi = 0;
+          group_argmin<T, W>(amax, zi, zp, sRed);  // minimum of -m1
+          amax = -amax;
+          const T thr = amax * T(1.0f - 0x1.0p-18f);
+          if (m2 >= thr) {
+            // several of this thread's rows near the maximum (or ties at
+            // zero): every row exactly, one at a time
+#pragma unroll 1
+            for (int sl = 0; sl < NQ * VEC; ++sl) {
+              T rd = 0, nr = 1;
+#pragma unroll
+              for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                  rd = sl == q * VEC + e ? Rdn[q][e] : rd;
+                  nr = sl == q * VEC + e ? nrm[q][e] : nr;
+                }
+              const int r = VEC * (tid + GT * (sl / VEC)) + sl % VEC;
+              const bool sel = (act >> sl) & 1;
+              const T sc = sel ? T(0) : (rd * rd) / nr;
+              take_if_better(key, idx, pay, r < p ? T(-fabs(sc)) : INF, r < p ? r : INT_MAX, T(0));
+            }
+          } else if (m1 >= thr) {
+            T rd = 0, nr = 1;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+              for (int e = 0; e < VEC; ++e) {
+                rd = i1 == q * VEC + e ? Rdn[q][e] : rd;
+                nr = i1 == q * VEC + e ? nrm[q][e] : nr;
+              }
+            const int r = VEC * (tid + GT * (i1 / VEC)) + i1 % VEC;
+            const bool sel = (act >> i1) & 1;
+            const T sc = sel ? T(0) : (rd * rd) / nr;
+            take_if_better(key, idx, pay, T(-fabs(sc)), r, T(0));
+          }
+        }
         group_argmin<T, W>(key, idx, pay, sRed);
         const int c = idx == INT_MAX ? 0 : idx;
         // owner publishes Rdn[c], norm[c]
@@ -162,8 +250,8 @@ __global__ __launch_bounds__(64 * W) void ormp_kernel(OmpArgs<T> a) {
           Un[j + (size_t)j * L] = invNorm;
           sRUn[j] = RU;
           sInd[j] = c;
-          sAct[c] = 1;
         }
+        if (tid == (c / VEC) % GT) act |= ActMask(1) << ((c / VEC / GT) * VEC + c % VEC);
         __syncthreads();
         if (j == Li - 1 || normX <= eps) {
           ++j;
@@ -173,17 +261,31 @@ __global__ __launch_bounds__(64 * W) void ormp_kernel(OmpArgs<T> a) {
         V und[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) und[q] = V(0);
-        for (int k = 0; k <= j; ++k) {
-          const T uk = Un[k + (size_t)j * L];
-          const T* col = a.G + (size_t)sInd[k] * ldg;
+        // UNR Gram columns per round so their loads are in flight together;
+        // the sum over k keeps its sequential order
+        for (int k0 = 0; k0 <= j; k0 += UNR) {
+          T uk[UNR];
+          const T* col[UNR];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const int j0 = VEC * (tid + GT * q);
-            if (j0 < ldg) {
-              const V g = *reinterpret_cast<const V*>(col + j0);
-              und[q] += uk * g;
-            }
+          for (int e = 0; e < UNR; ++e) {
+            const int k = k0 + e <= j ? k0 + e : k0;
+            uk[e] = k0 + e <= j ? Un[k + (size_t)j * L] : T(0);
+            col[e] = a.G + (size_t)sInd[k] * ldg;
           }
+          V g[UNR][NQ];
+#pragma unroll
+          for (int e = 0; e < UNR; ++e)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              const int j0 = VEC * (tid + GT * q);
+              g[e][q] = j0 < ldg ? *reinterpret_cast<const V*>(col[e] + j0) : V(0);
+            }
+#pragma unroll
+          for (int e = 0; e < UNR; ++e)
+            if (k0 + e <= j) {
+#pragma unroll
+              for (int q = 0; q < NQ; ++q) und[q] += uk[e] * g[e][q];
+            }
         }
         // Rdn -= RU * Undn_j ; norm -= Undn_j^2
 #pragma unroll
@@ -217,26 +319,25 @@ __global__ __launch_bounds__(64 * W) void ormp_kernel(OmpArgs<T> a) {
   }
 }
 
-template <typename T, int W, int NQ>
+template <typename T, int W, int NQ, bool ULDS>
 size_t ormp_lds_bytes(int L) {
-  constexpr int GT = 64 * W;
-  constexpr int VEC = Vec16o<T>::N;
-  constexpr int PT = VEC * GT * NQ;
-  size_t b = sizeof(T) * (3 * (size_t)L + 4) + sizeof(int) * (size_t)L + PT;
+  size_t b = sizeof(T) * (3 * (size_t)L + 4) + sizeof(int) * (size_t)L;
   b = (b + 15) & ~size_t(15);
   b += sizeof(KeyIdx<T>) * W + 16;
+  b = (b + 15) & ~size_t(15);
+  if (ULDS) b += sizeof(T) * (size_t)L * L + 16;
   return b;
 }
 
-template <typename T, int W, int NQ>
+template <typename T, int W, int NQ, bool ULDS>
 int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
   OmpArgs<T> a = a0;
-  const size_t lds = ormp_lds_bytes<T, W, NQ>(a.L);
-  auto kern = ormp_kernel<T, W, NQ>;
+  const size_t lds = ormp_lds_bytes<T, W, NQ, ULDS>(a.L);
+  auto kern = ormp_kernel<T, W, NQ, ULDS>;
   SPX_HIP(hipFuncSetAttribute((const void*)kern,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
-  const size_t slot = sizeof(T) * (size_t)a.L * a.L;
+  const size_t slot = ULDS ? 0 : sizeof(T) * (size_t)a.L * a.L;
   hipStream_t st = ctx->stream;
   if (form == SPX_FORM_BATCHED) {
     int per_cu = 0;
@@ -247,8 +348,11 @@ int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
     if (blocks < 1) blocks = 1;
     T* scratch = nullptr;
     int* queue = nullptr;
-    int rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
-    if (rc) return rc;
+    int rc = SPX_OK;
+    if (slot) {
+      rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
+      if (rc) return rc;
+    }
     rc = ctx->scratch_get(1, sizeof(int), reinterpret_cast<void**>(&queue));
     if (rc) return rc;
     rc = zero_ints(st, queue, 1);
@@ -265,9 +369,11 @@ int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
   if (chunk < 1) chunk = 1;
   const long alloc_chunk = chunk < a0.n ? chunk : a0.n;
   T* scratch = nullptr;
-  int rc = ctx->scratch_get(0, slot * (alloc_chunk > 0 ? alloc_chunk : 1),
-                            reinterpret_cast<void**>(&scratch));
-  if (rc) return rc;
+  if (slot) {
+    int rc = ctx->scratch_get(0, slot * (alloc_chunk > 0 ? alloc_chunk : 1),
+                              reinterpret_cast<void**>(&scratch));
+    if (rc) return rc;
+  }
   for (long s0 = 0; s0 < a0.n; s0 += chunk) {
     const long cnt = (a0.n - s0) < chunk ? (a0.n - s0) : chunk;
     a = a0;
@@ -293,11 +399,19 @@ int ormp_dispatch_nq(Ctx* ctx, const OmpArgs<T>& a, int form) {
   constexpr int VEC = Vec16o<T>::N;
   const int per = VEC * 64 * W;
   const int nq = (a.p + per - 1) / per;
-  if (nq <= 1) return ormp_run<T, W, 1>(ctx, a, form);
-  if (nq <= 2) return ormp_run<T, W, 2>(ctx, a, form);
-  if (nq <= 4) return ormp_run<T, W, 4>(ctx, a, form);
-  if (nq <= 8) return ormp_run<T, W, 8>(ctx, a, form);
-  if (nq <= 16) return ormp_run<T, W, 16>(ctx, a, form);
+  if (sizeof(T) * (size_t)a.L * a.L <= kUnLdsBytes) {
+    if (nq <= 1) return ormp_run<T, W, 1, true>(ctx, a, form);
+    if (nq <= 2) return ormp_run<T, W, 2, true>(ctx, a, form);
+    if (nq <= 4) return ormp_run<T, W, 4, true>(ctx, a, form);
+    if (nq <= 8) return ormp_run<T, W, 8, true>(ctx, a, form);
+    if (nq <= 16) return ormp_run<T, W, 16, true>(ctx, a, form);
+    return -1;
+  }
+  if (nq <= 1) return ormp_run<T, W, 1, false>(ctx, a, form);
+  if (nq <= 2) return ormp_run<T, W, 2, false>(ctx, a, form);
+  if (nq <= 4) return ormp_run<T, W, 4, false>(ctx, a, form);
+  if (nq <= 8) return ormp_run<T, W, 8, false>(ctx, a, form);
+  if (nq <= 16) return ormp_run<T, W, 16, false>(ctx, a, form);
   return -1;
 }
 
