@@ -109,6 +109,9 @@ typedef struct {
   int pos;
   int ista;
   int precision;       /* SPX_PREC_* for the two products per iteration */
+  int graph;           /* fixed step: replay the whole iteration loop as one
+                        * hipGraph, captured once per (buffers, sizes,
+                        * options) and cached on the context */
 } spx_fista_opts;
 
 /* ---------------------------------------------------------------- context */

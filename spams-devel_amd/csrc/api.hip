@@ -809,6 +809,9 @@ int spx_ctx::timers_fold() {
 }
 
 spx_ctx::~spx_ctx() {
+  for (auto& g : graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (capture_stream) (void)hipStreamDestroy(capture_stream);
   for (auto& e : pending) {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);

@@ -16,6 +16,9 @@
 #include <cmath>
 #include <vector>
 
+#include <algorithm>
+#include <cstring>
+
 #include "spx_common.h"
 #include "spx_internal.h"
 
@@ -337,89 +340,167 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
   const int it0 = o->it0 > 1 ? o->it0 : 1;
   const int ista = o->ista;
 
-  SPX_HIP(hipMemcpyAsync(W, W0, sizeof(T) * pn, hipMemcpyDeviceToDevice, st));
-  SPX_HIP(hipMemcpyAsync(y, W0, sizeof(T) * pn, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(init_state_kernel<T>, lin(n), dim3(256), 0, st, n,
-                     static_cast<T>(o->L0), Lsig, active, best, info);
-  SPX_HIP(hipGetLastError());
-
-  // gradient of the square loss at `at`: R = D*at - X, grad = D'R
-  auto gradient = [&](const T* at) -> int {
-    int r = gemm_tn<T>(st, m, n, p, Dt, p, at, p, R, m, T(1), T(-1), X, m, prec);
-    if (r) return r;
-    return gemm_tn<T>(st, p, n, m, D, m, R, m, grad, p, T(1), T(0), nullptr, 0, prec);
-  };
-  // objective / gap evaluation at W
-  auto evaluate = [&](int mode, int it) -> int {
-    int r = gemm_tn<T>(st, m, n, p, Dt, p, W, p, R, m, T(1), T(-1), X, m, prec);
-    if (r) return r;
-    if (duality) {
-      r = gemm_tn<T>(st, p, n, m, D, m, R, m, grad, p, T(1), T(0), nullptr, 0, prec);
-      if (r) return r;
-    }
-    hipLaunchKernelGGL(eval_kernel<T>, cols4(n), dim3(256), 0, st, m, p, n, X,
-                       W, R, grad, pp, duality, mode, o->tol, it, active, best, info);
+  // everything after the workspace set-up, on stream `qs` (the context's
+  // stream, or the capture stream when the loop is recorded as a graph)
+  auto enqueue = [&](hipStream_t qs) -> int {
+    SPX_HIP(hipMemcpyAsync(W, W0, sizeof(T) * pn, hipMemcpyDeviceToDevice, qs));
+    SPX_HIP(hipMemcpyAsync(y, W0, sizeof(T) * pn, hipMemcpyDeviceToDevice, qs));
+    hipLaunchKernelGGL(init_state_kernel<T>, lin(n), dim3(256), 0, qs, n,
+                       static_cast<T>(o->L0), Lsig, active, best, info);
     SPX_HIP(hipGetLastError());
-    return SPX_OK;
+
+    // gradient of the square loss at `at`: R = D*at - X, grad = D'R
+    auto gradient = [&](const T* at) -> int {
+      int r = gemm_tn<T>(qs, m, n, p, Dt, p, at, p, R, m, T(1), T(-1), X, m, prec);
+      if (r) return r;
+      return gemm_tn<T>(qs, p, n, m, D, m, R, m, grad, p, T(1), T(0), nullptr, 0, prec);
+    };
+    // objective / gap evaluation at W
+    auto evaluate = [&](int mode, int it) -> int {
+      int r = gemm_tn<T>(qs, m, n, p, Dt, p, W, p, R, m, T(1), T(-1), X, m, prec);
+      if (r) return r;
+      if (duality) {
+        r = gemm_tn<T>(qs, p, n, m, D, m, R, m, grad, p, T(1), T(0), nullptr, 0, prec);
+        if (r) return r;
+      }
+      hipLaunchKernelGGL(eval_kernel<T>, cols4(n), dim3(256), 0, qs, m, p, n, X,
+                         W, R, grad, pp, duality, mode, o->tol, it, active, best, info);
+      SPX_HIP(hipGetLastError());
+      return SPX_OK;
+    };
+
+    double t = 1.0;
+    int it;
+    int rc2;
+    for (it = 1; it <= o->max_it; ++it) {
+      rc2 = gradient(ista ? W : y);
+      if (rc2) return rc2;
+      T beta = 0;
+      if (!ista) {
+        const T told = static_cast<T>(t);
+        const T sq = std::sqrt(static_cast<T>(1 + 4 * told * told));
+        const T tn = static_cast<T>((1.0 + static_cast<double>(sq)) / 2);
+        beta = (T(1) - told) / tn;
+        t = tn;
+      }
+      if (!bt) {
+        hipLaunchKernelGGL(prox_step_kernel<T>, cols4(n), dim3(256), 0, qs, p, n,
+                           W, y, grad, Lsig, active, pp, beta, ista, stopsum);
+        SPX_HIP(hipGetLastError());
+      } else {
+        const T* base = ista ? W : y;
+        hipLaunchKernelGGL(set_pending_kernel, lin(n), dim3(256), 0, qs, n, active, pending);
+        SPX_HIP(hipGetLastError());
+        for (int trial = 1;; ++trial) {
+          const int last = trial + 1 >= o->max_iter_backtracking;
+          hipLaunchKernelGGL(prox_trial_kernel<T>, cols4(n), dim3(256), 0, qs, p,
+                             n, base, grad, Lsig, pending, pp, zb, diff, stopsum);
+          SPX_HIP(hipGetLastError());
+          rc2 = gemm_tn<T>(qs, m, n, p, Dt, p, diff, p, R, m, T(1), T(0), nullptr, 0, prec);
+          if (rc2) return rc2;
+          hipLaunchKernelGGL(zero_one_kernel, dim3(1), dim3(1), 0, qs, npend);
+          hipLaunchKernelGGL(backtrack_check_kernel<T>, cols4(n), dim3(256), 0, qs,
+                             n, m, R, stopsum, static_cast<T>(o->gamma), last,
+                             Lsig, pending, npend);
+          SPX_HIP(hipGetLastError());
+          int h = 0;
+          SPX_HIP(hipMemcpyAsync(&h, npend, sizeof(int), hipMemcpyDeviceToHost, qs));
+          SPX_HIP(hipStreamSynchronize(qs));
+          if (h == 0 || last) break;
+        }
+        hipLaunchKernelGGL(momentum_kernel<T>, cols4(n), dim3(256), 0, qs, p, n,
+                           W, y, zb, active, beta, ista, stopsum);
+        SPX_HIP(hipGetLastError());
+      }
+      if (duality) {
+        if (it % it0 == 0) {
+          rc2 = evaluate(0, it);
+          if (rc2) return rc2;
+        }
+      } else {
+        hipLaunchKernelGGL(relstop_kernel<T>, lin(n), dim3(256), 0, qs, n, stopsum,
+                           o->tol, it, active, info);
+        SPX_HIP(hipGetLastError());
+      }
+    }
+    hipLaunchKernelGGL(finish_it_kernel<T>, lin(n), dim3(256), 0, qs, n, active, it, info);
+    SPX_HIP(hipGetLastError());
+    return evaluate(1, it);
   };
 
-  double t = 1.0;
-  int it;
-  for (it = 1; it <= o->max_it; ++it) {
-    rc = gradient(ista ? W : y);
-    if (rc) return rc;
-    T beta = 0;
-    if (!ista) {
-      const T told = static_cast<T>(t);
-      const T sq = std::sqrt(static_cast<T>(1 + 4 * told * told));
-      const T tn = static_cast<T>((1.0 + static_cast<double>(sq)) / 2);
-      beta = (T(1) - told) / tn;
-      t = tn;
-    }
-    if (!bt) {
-      hipLaunchKernelGGL(prox_step_kernel<T>, cols4(n), dim3(256), 0, st, p, n,
-                         W, y, grad, Lsig, active, pp, beta, ista, stopsum);
-      SPX_HIP(hipGetLastError());
-    } else {
-      const T* base = ista ? W : y;
-      hipLaunchKernelGGL(set_pending_kernel, lin(n), dim3(256), 0, st, n, active, pending);
-      SPX_HIP(hipGetLastError());
-      for (int trial = 1;; ++trial) {
-        const int last = trial + 1 >= o->max_iter_backtracking;
-        hipLaunchKernelGGL(prox_trial_kernel<T>, cols4(n), dim3(256), 0, st, p,
-                           n, base, grad, Lsig, pending, pp, zb, diff, stopsum);
-        SPX_HIP(hipGetLastError());
-        rc = gemm_tn<T>(st, m, n, p, Dt, p, diff, p, R, m, T(1), T(0), nullptr, 0, prec);
-        if (rc) return rc;
-        hipLaunchKernelGGL(zero_one_kernel, dim3(1), dim3(1), 0, st, npend);
-        hipLaunchKernelGGL(backtrack_check_kernel<T>, cols4(n), dim3(256), 0, st,
-                           n, m, R, stopsum, static_cast<T>(o->gamma), last,
-                           Lsig, pending, npend);
-        SPX_HIP(hipGetLastError());
-        int h = 0;
-        SPX_HIP(hipMemcpyAsync(&h, npend, sizeof(int), hipMemcpyDeviceToHost, st));
-        SPX_HIP(hipStreamSynchronize(st));
-        if (h == 0 || last) break;
+  if (!(o->graph && !bt)) return enqueue(st);
+
+  // ---- fixed step: the whole loop as one graph, captured once per key
+  struct Key {
+    int elem, m, p, n, max_it, regul, pos, ista, it0, prec;
+    const void *X, *W0, *W, *info, *D, *Dt, *ws;
+    double L0, lambda1, lambda2, tol;
+  } key;
+  std::memset(&key, 0, sizeof(key));
+  key.elem = sizeof(T);
+  key.m = m;
+  key.p = p;
+  key.n = n;
+  key.max_it = o->max_it;
+  key.regul = o->regul;
+  key.pos = o->pos;
+  key.ista = ista;
+  key.it0 = it0;
+  key.prec = prec;
+  key.X = X;
+  key.W0 = W0;
+  key.W = W;
+  key.info = info;
+  key.D = D;
+  key.Dt = Dt;
+  key.ws = ws;
+  key.L0 = o->L0;
+  key.lambda1 = o->lambda1;
+  key.lambda2 = o->lambda2;
+  key.tol = o->tol;
+  const char* kb = reinterpret_cast<const char*>(&key);
+  hipGraphExec_t exec = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (size_t i = 0; i < ctx->graphs.size(); ++i) {
+      auto& g = ctx->graphs[i];
+      if (g.key.size() == sizeof(key) && std::memcmp(g.key.data(), kb, sizeof(key)) == 0) {
+        exec = g.exec;
+        std::rotate(ctx->graphs.begin() + i, ctx->graphs.begin() + i + 1, ctx->graphs.end());
+        break;
       }
-      hipLaunchKernelGGL(momentum_kernel<T>, cols4(n), dim3(256), 0, st, p, n,
-                         W, y, zb, active, beta, ista, stopsum);
-      SPX_HIP(hipGetLastError());
-    }
-    if (duality) {
-      if (it % it0 == 0) {
-        rc = evaluate(0, it);
-        if (rc) return rc;
-      }
-    } else {
-      hipLaunchKernelGGL(relstop_kernel<T>, lin(n), dim3(256), 0, st, n, stopsum,
-                         o->tol, it, active, info);
-      SPX_HIP(hipGetLastError());
     }
   }
-  hipLaunchKernelGGL(finish_it_kernel<T>, lin(n), dim3(256), 0, st, n, active, it, info);
-  SPX_HIP(hipGetLastError());
-  rc = evaluate(1, it);
-  if (rc) return rc;
+  if (!exec) {
+    if (!ctx->capture_stream)
+      SPX_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+    hipStream_t cs = ctx->capture_stream;
+    SPX_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    rc = enqueue(cs);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    if (rc || ec != hipSuccess || !graph) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (!rc) set_error("fistaFlat: graph capture failed: %s", hipGetErrorString(ec));
+      return rc ? rc : SPX_ERR_HIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      set_error("fistaFlat: graph instantiation failed: %s", hipGetErrorString(ei));
+      return SPX_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->graphs.size() >= 8) {
+      (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+      ctx->graphs.erase(ctx->graphs.begin());
+    }
+    Ctx::CachedGraph cg;
+    cg.key.assign(kb, kb + sizeof(key));
+    cg.exec = exec;
+    ctx->graphs.push_back(std::move(cg));
+  }
+  SPX_HIP(hipGraphLaunch(exec, st));
   return SPX_OK;
 }
 

@@ -165,6 +165,14 @@ struct spx_ctx {
   long long timer_n[SPX_TIMER_COUNT] = {};
   hipEvent_t event_get();
   int timers_fold();
+  // captured FISTA iteration loops (spx_fista_opts.graph): key bytes ->
+  // instantiated graph, most recent last; capture stream created on demand
+  struct CachedGraph {
+    std::vector<char> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<CachedGraph> graphs;
+  hipStream_t capture_stream = nullptr;
   // Device scratch slot `id` of at least `bytes` bytes (contents undefined).
   int scratch_get(int id, size_t bytes, void** out);
   ~spx_ctx();

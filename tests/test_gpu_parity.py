@@ -365,3 +365,41 @@ def test_launch_timers_count_solver_launches(spx):
     ctx.reset_timing()
     assert ctx.read_timing()["lars_fast"] == (0.0, 0)
     d.close()
+
+
+def test_fista_graph_replays_equal_eager(spx, orc):
+    """The fixed-step loop captured as a hipGraph: every replay (with eager
+    engine and torch work between the replays, on changed inputs) equals the
+    eager launches bit for bit, and the eager result matches the oracle."""
+    import torch
+    print("case fista graph", flush=True)
+    ctx = gpu_ctx(spx)
+    X, D = rand_problem(21, 64, 256, 48, np.float32, k=8, noise=0.02, norm=False)
+    L0 = float(np.linalg.eigvalsh(D.astype(np.float64).T @ D)[-1])
+    d = spx.Dictionary(ctx, D)
+    n, p = X.shape[1], D.shape[1]
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    W0 = torch.zeros((n, p), dtype=torch.float32, device="cuda")
+    for regul, it0 in (("l1", 7), ("elastic-net", 1000)):
+        kw = dict(regul=regul, lambda2=0.01, max_it=60, L0=L0, fixed_step=True, tol=0.0,
+                  it0=it0)
+        We = torch.zeros((n, p), dtype=torch.float32, device="cuda")
+        ie = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+        spx.fista_flat_dev(ctx, d, Xt, W0, We, ie, 0.05, graph=False, **kw)
+        torch.cuda.synchronize()
+        Wg = torch.zeros((n, p), dtype=torch.float32, device="cuda")
+        ig = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+        for rep in range(4):
+            Wg.fill_(float(rep + 1))  # stale outputs must be overwritten
+            ig.fill_(-1.0)
+            spx.fista_flat_dev(ctx, d, Xt, W0, Wg, ig, 0.05, graph=True, **kw)
+            torch.cuda.synchronize()
+            assert torch.equal(Wg, We) and torch.equal(ig, ie), (regul, rep)
+            # eager engine work between replays
+            spx.fista_flat_dev(ctx, d, Xt, W0, We, ie, 0.05, graph=False, **kw)
+            torch.cuda.synchronize()
+        Wo, io = orc.fista_flat(X, D, np.zeros((p, n), np.float32, order="F"), 0.05,
+                                regul=regul, lambda2=0.01, max_it=60, L0=L0, fixed_step=True,
+                                tol=0.0, it0=it0)
+        assert np.abs(We.cpu().numpy().T - Wo).max() <= 1e-4
+    d.close()
