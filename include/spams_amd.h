@@ -159,21 +159,44 @@ inline void check(int rc) {
 }
 
 inline int lasso_call(const float* X, int m, int n, const float* D, int p, const float* lam,
-                      int nlam, float l2, const spx_lasso_opts* o, spx_csc* out) {
-  return spx_lasso_f32(context(), X, m, n, D, p, lam, nlam, l2, o, out);
+                      int nlam, float l2, const spx_lasso_opts* o, spx_csc* out,
+                      float* path) {
+  return spx_lasso_path_f32(context(), X, m, n, D, p, lam, nlam, l2, o, out, path);
 }
 inline int lasso_call(const double* X, int m, int n, const double* D, int p,
                       const double* lam, int nlam, double l2, const spx_lasso_opts* o,
-                      spx_csc* out) {
-  return spx_lasso_f64(context(), X, m, n, D, p, lam, nlam, l2, o, out);
+                      spx_csc* out, double* path) {
+  return spx_lasso_path_f64(context(), X, m, n, D, p, lam, nlam, l2, o, out, path);
+}
+inline int lasso_qq_call(const float* X, int m, int n, const float* Q, const float* q, int p,
+                         const float* lam, float l2, const spx_lasso_opts* o, spx_csc* out,
+                         float* path) {
+  return spx_lasso_qq_f32(context(), X, m, n, Q, q, p, lam, 1, l2, o, out, path);
+}
+inline int lasso_qq_call(const double* X, int m, int n, const double* Q, const double* q,
+                         int p, const double* lam, double l2, const spx_lasso_opts* o,
+                         spx_csc* out, double* path) {
+  return spx_lasso_qq_f64(context(), X, m, n, Q, q, p, lam, 1, l2, o, out, path);
 }
 inline int omp_call(const float* X, int m, int n, const float* D, int p, const int* L, int nL,
-                    const float* e, int ne, const float* la, int nla, spx_csc* out) {
-  return spx_omp_f32(context(), X, m, n, D, p, L, nL, e, ne, la, nla, nullptr, out);
+                    const float* e, int ne, const float* la, int nla, spx_csc* out,
+                    float* path) {
+  return spx_omp_path_f32(context(), X, m, n, D, p, L, nL, e, ne, la, nla, nullptr, out, path);
 }
 inline int omp_call(const double* X, int m, int n, const double* D, int p, const int* L,
-                    int nL, const double* e, int ne, const double* la, int nla, spx_csc* out) {
-  return spx_omp_f64(context(), X, m, n, D, p, L, nL, e, ne, la, nla, nullptr, out);
+                    int nL, const double* e, int ne, const double* la, int nla, spx_csc* out,
+                    double* path) {
+  return spx_omp_path_f64(context(), X, m, n, D, p, L, nL, e, ne, la, nla, nullptr, out, path);
+}
+
+// The path matrix the host entry points fill: p x length, length = the
+// facade's normalisation of max_length_path.
+template <typename T>
+T* path_buffer(Matrix<T>* path, int p, int L, int length_path) {
+  if (!path) return nullptr;
+  const int cols = length_path < 0 ? 4 * L : length_path;
+  if (path->m() != p || path->n() < cols) throw "lasso : the path matrix must be p x max_length_path";
+  return path->rawX();
 }
 inline int fista_call(const float* X, int m, int n, const float* D, int p, const float* W0,
                       float* W, const spx_fista_opts* o, float* info) {
@@ -193,17 +216,18 @@ void lasso2(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, int L,
             const bool pos = false, const int numThreads = -1, Matrix<T>* path = nullptr,
             const int length_path = -1) {
   (void)numThreads;
-  if (path) throw "lasso2: the regularisation path output is not provided by spx";
   spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, length_path, SPX_PREC_FP32, SPX_FORM_BATCHED, 0};
   spx_csc out{};
+  T* pp = spams_amd_detail::path_buffer(path, D.n(), L, length_path);
   spams_amd_detail::check(spams_amd_detail::lasso_call(X.rawX(), X.m(), X.n(), D.rawX(), D.n(),
-                                                       &constraint, 1, lambda2, &o, &out));
+                                                       &constraint, 1, lambda2, &o, &out, pp));
   spalpha.adopt(out);
 }
 
 // _lassoD (cpp_library/spams.h:107-143).  cholevsky selects the Cholesky
 // variant in the reference; both variants solve the same problem and spx
-// runs its single inverse-Gram homotopy for either.
+// runs its single inverse-Gram homotopy for either.  ols returns the
+// least-squares fit on each final active set.
 template <typename T>
 SpMatrix<T>* _lassoD(Matrix<T>* X, Matrix<T>* D, Matrix<T>** path, bool return_reg_path, int L,
                      const T constraint, const T lambda2, constraint_type mode, const bool pos,
@@ -211,18 +235,59 @@ SpMatrix<T>* _lassoD(Matrix<T>* X, Matrix<T>* D, Matrix<T>** path, bool return_r
                      const bool verbose, bool cholevsky) {
   (void)verbose;
   (void)cholevsky;
+  (void)numThreads;
   if (X->m() != D->m()) throw "lasso : incompatible matrix dimensions";
-  if (return_reg_path) throw "lasso : return_reg_path is not provided by spx";
-  if (ols) throw "lasso : ols is not provided by spx";
   if (L < 0) L = D->n();
   if (max_length_path < 0) max_length_path = 4 * L;
-  *path = nullptr;
+  *path = return_reg_path ? new Matrix<T>(D->n(), max_length_path) : nullptr;
   SpMatrix<T>* alpha = new SpMatrix<T>();
   try {
-    lasso2(*X, *D, *alpha, L, constraint, lambda2, mode, pos, numThreads, (Matrix<T>*)nullptr,
-           max_length_path);
+    spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, max_length_path, SPX_PREC_FP32,
+                     SPX_FORM_BATCHED, 0, ols ? 1 : 0};
+    spx_csc out{};
+    spams_amd_detail::check(spams_amd_detail::lasso_call(
+        X->rawX(), X->m(), X->n(), D->rawX(), D->n(), &constraint, 1, lambda2, &o, &out,
+        *path ? (*path)->rawX() : (T*)nullptr));
+    alpha->adopt(out);
   } catch (...) {
     delete alpha;
+    delete *path;
+    *path = nullptr;
+    throw;
+  }
+  return alpha;
+}
+
+// _lassoQq (cpp_library/spams.h:145-188): Gram matrix Q (p x p) and
+// correlations q (p x n) given; X only gives the signals' norms.
+template <typename T>
+SpMatrix<T>* _lassoQq(Matrix<T>* X, Matrix<T>* Q, Matrix<T>* q, Matrix<T>** path,
+                      bool return_reg_path, int L, const T constraint, const T lambda2,
+                      constraint_type mode, const bool pos, const bool ols,
+                      const int numThreads, int max_length_path, const bool verbose,
+                      bool cholevsky) {
+  (void)verbose;
+  (void)cholevsky;
+  (void)numThreads;
+  if (Q->m() != Q->n()) throw "lasso : Q must be square";
+  const int K = Q->m();
+  if (q->m() != K || q->n() != X->n()) throw "lasso : incompatible matrix dimensions";
+  if (L < 0) L = K;
+  if (max_length_path < 0) max_length_path = 4 * L;
+  *path = return_reg_path ? new Matrix<T>(K, max_length_path) : nullptr;
+  SpMatrix<T>* alpha = new SpMatrix<T>();
+  try {
+    spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, max_length_path, SPX_PREC_FP32,
+                     SPX_FORM_BATCHED, 0, ols ? 1 : 0};
+    spx_csc out{};
+    spams_amd_detail::check(spams_amd_detail::lasso_qq_call(
+        X->rawX(), X->m(), X->n(), Q->rawX(), q->rawX(), K, &constraint, lambda2, &o, &out,
+        *path ? (*path)->rawX() : (T*)nullptr));
+    alpha->adopt(out);
+  } catch (...) {
+    delete alpha;
+    delete *path;
+    *path = nullptr;
     throw;
   }
   return alpha;
@@ -234,12 +299,15 @@ void omp(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, const int
          const T* eps, const T* lambda, const bool vecL = false, const bool vecEps = false,
          const bool Lambda = false, const int numThreads = -1, Matrix<T>* path = nullptr) {
   (void)numThreads;
-  if (path) throw "omp: the regularisation path output is not provided by spx";
   const int n = X.n();
+  const int cols = X.m() < D.n() ? X.m() : D.n();
+  if (path && (path->m() != D.n() || path->n() < cols))
+    throw "omp : the path matrix must be p x min(m, p)";
   spx_csc out{};
   spams_amd_detail::check(spams_amd_detail::omp_call(
       X.rawX(), X.m(), n, D.rawX(), D.n(), L, L ? (vecL ? n : 1) : 0, eps,
-      eps ? (vecEps ? n : 1) : 0, lambda, lambda ? (Lambda ? n : 1) : 0, &out));
+      eps ? (vecEps ? n : 1) : 0, lambda, lambda ? (Lambda ? n : 1) : 0, &out,
+      path ? path->rawX() : (T*)nullptr));
   spalpha.adopt(out);
 }
 
@@ -251,16 +319,17 @@ SpMatrix<T>* _omp(Matrix<T>* X, Matrix<T>* D, Matrix<T>** path, bool return_reg_
   if (X->m() != D->m()) throw "omp : incompatible matrix dimensions";
   if (!given_L && !given_eps && !given_Lambda)
     throw "omp : You should either provide L, eps or lambda";
-  if (return_reg_path) throw "omp : return_reg_path is not provided by spx";
-  *path = nullptr;
+  const int scalar_L = X->m() < D->n() ? X->m() : D->n();
+  *path = return_reg_path ? new Matrix<T>(D->n(), scalar_L) : nullptr;
   SpMatrix<T>* alpha = new SpMatrix<T>();
   try {
     omp(*X, *D, *alpha, given_L ? L->rawX() : nullptr, given_eps ? eps->rawX() : nullptr,
         given_Lambda ? Lambda->rawX() : nullptr, given_L && L->n() > 1,
-        given_eps && eps->n() > 1, given_Lambda && Lambda->n() > 1, numThreads,
-        (Matrix<T>*)nullptr);
+        given_eps && eps->n() > 1, given_Lambda && Lambda->n() > 1, numThreads, *path);
   } catch (...) {
     delete alpha;
+    delete *path;
+    *path = nullptr;
     throw;
   }
   return alpha;

@@ -82,6 +82,9 @@ typedef struct {
   int precision;       /* SPX_PREC_* (f32 entry points only)                 */
   int form;            /* SPX_FORM_*                                         */
   int waves_per_signal;/* 0 = autotuned; else 1, 2, 4 or 16                  */
+  int ols;             /* 1: least-squares refit on the final active set (the
+                        * facade's ols, decomp/decomp.h:1087-1091); the path
+                        * itself is the same homotopy */
 } spx_lasso_opts;
 
 /* Options of spx_omp_* (the _omp facade's vector arguments are passed
@@ -158,6 +161,14 @@ int spx_dict_create_f32(spx_ctx* ctx, const float* D, int m, int p,
                         int on_device, int precision, spx_dict** out);
 int spx_dict_create_f64(spx_ctx* ctx, const double* D, int m, int p,
                         int on_device, spx_dict** out);
+/* A resident Gram matrix Q (p x p, column-major) without a dictionary, for
+ * the Gram-given LARS (spx_lasso_dtx_dev_* with correlations the caller
+ * computed): Q is used as given, no ridge is added; m is the signal length.
+ * Entry points that need D (products, FISTA, OMP) reject it. */
+int spx_dict_create_gram_f32(spx_ctx* ctx, const float* Q, int m, int p,
+                             int on_device, spx_dict** out);
+int spx_dict_create_gram_f64(spx_ctx* ctx, const double* Q, int m, int p,
+                             int on_device, spx_dict** out);
 void spx_dict_destroy(spx_dict* dict);
 int spx_dict_dims(const spx_dict* dict, int* m, int* p);
 /* Device bytes held by the dictionary (D, D', G). */
@@ -203,6 +214,30 @@ int spx_dict_set_waves(spx_dict* dict, int solver, int waves);
 int spx_lasso_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D,
                   int p, const float* lambda, int n_lambda, float lambda2,
                   const spx_lasso_opts* opts, spx_csc* out);
+/* Gram-given LARS: replaces _lassoQq (cpp_library/spams.h:145-188), i.e.
+ * lasso2(X, Q, q, ...) (decomp/decomp.h:1574-1646).  X (m x n) only gives
+ * ||x_i||^2; Q (p x p) is the Gram matrix used as given (no ridge; lambda2
+ * only enters the facade's L clamp); q (p x n) holds the correlations.
+ * `path` as in spx_lasso_path_* (may be NULL). */
+int spx_lasso_qq_f32(spx_ctx* ctx, const float* X, int m, int n, const float* Q,
+                     const float* q, int p, const float* lambda, int n_lambda,
+                     float lambda2, const spx_lasso_opts* opts, spx_csc* out,
+                     float* path);
+int spx_lasso_qq_f64(spx_ctx* ctx, const double* X, int m, int n,
+                     const double* Q, const double* q, int p,
+                     const double* lambda, int n_lambda, double lambda2,
+                     const spx_lasso_opts* opts, spx_csc* out, double* path);
+/* As spx_lasso_*, with _lassoD's return_reg_path: `path` (host, p x
+ * max_length_path after the facade's normalisation, column-major) receives
+ * the first signal's regularisation path. */
+int spx_lasso_path_f32(spx_ctx* ctx, const float* X, int m, int n,
+                       const float* D, int p, const float* lambda,
+                       int n_lambda, float lambda2, const spx_lasso_opts* opts,
+                       spx_csc* out, float* path);
+int spx_lasso_path_f64(spx_ctx* ctx, const double* X, int m, int n,
+                       const double* D, int p, const double* lambda,
+                       int n_lambda, double lambda2,
+                       const spx_lasso_opts* opts, spx_csc* out, double* path);
 int spx_lasso_f64(spx_ctx* ctx, const double* X, int m, int n,
                   const double* D, int p, const double* lambda, int n_lambda,
                   double lambda2, const spx_lasso_opts* opts, spx_csc* out);
@@ -218,6 +253,19 @@ int spx_omp_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D,
                 int p, const int* L, int nL, const double* eps, int n_eps,
                 const double* lambda, int n_lambda, const spx_omp_opts* opts,
                 spx_csc* out);
+/* As spx_omp_*, with _omp's return_reg_path: `path` (host, p x min(m, p),
+ * column-major) receives the first signal's path -- column j holds its
+ * coefficients after j+1 selections, the last column written its final
+ * coefficients (decomp/decomp.h:605-613, :634-639). */
+int spx_omp_path_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D,
+                     int p, const int* L, int nL, const float* eps, int n_eps,
+                     const float* lambda, int n_lambda, const spx_omp_opts* opts,
+                     spx_csc* out, float* path);
+int spx_omp_path_f64(spx_ctx* ctx, const double* X, int m, int n,
+                     const double* D, int p, const int* L, int nL,
+                     const double* eps, int n_eps, const double* lambda,
+                     int n_lambda, const spx_omp_opts* opts, spx_csc* out,
+                     double* path);
 
 /* FISTA-flat, square loss + l1 / elastic-net: replaces _fistaFlat
  * (cpp_library/spams.h:460-608).  W (p x n) is written; optim_info is 4 x n
@@ -243,6 +291,10 @@ typedef struct {
   int* count;  /* int[n] */
   int L;       /* leading dimension of v/r (see spx_lasso_L / spx_omp_L) */
   int* stats;  /* optional int[2 x n]: homotopy steps, atom admissions */
+  void* path;  /* optional T[p x path_cols]: signal 0's regularisation path,
+                * column `iter` = its coefficients after homotopy step iter
+                * (decomp/decomp.h:1868-1871); entries never written are 0 */
+  int path_cols;
 } spx_lists_dev;
 
 /* Effective L after the facade's clamping (cpp_library/spams.h:117-131 and

@@ -98,9 +98,11 @@ def _csc_from_lists(vM, rM, L, n, p, dtype):
 # ------------------------------------------------------------------ restatement
 
 def lasso(X, D, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
-          max_length_path=-1, num_threads=-1, return_stats=False):
+          max_length_path=-1, num_threads=-1, return_stats=False, return_reg_path=False):
     """_lassoD + lasso2 + coreLARS2 (restated).  ``lambda1`` may be a scalar
-    or a length-n vector (one constraint per signal)."""
+    or a length-n vector (one constraint per signal).  ``return_reg_path``:
+    also the first signal's path (p x max_length_path), NaN where coreLARS2
+    never writes (the reference leaves those entries uninitialised)."""
     dtype = X.dtype
     X = _f(X, dtype)
     D = _f(D, dtype)
@@ -118,13 +120,50 @@ def lasso(X, D, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
     vM = np.zeros((Le, n), dtype, order="F")
     rM = np.full((Le, n), -1, np.int32, order="F")
     stats = np.zeros((2, n), np.int32, order="F")
+    path = None
+    if return_reg_path:
+        mlp = max_length_path if max_length_path >= 0 else 4 * (p if L < 0 else L)
+        path = np.full((p, mlp), np.nan, dtype, order="F")
     f = getattr(lib(), "orc_lasso_" + sfx)
     f.restype = _I
-    f.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, ct, _I, _I, _I, _I, _P, _P, _P]
+    f.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, ct, _I, _I, _I, _I, _P, _P, _P, _P]
     f(_ptr(X), m, n, _ptr(D), p, L, _ptr(lam), lam.size, lambda2, mode,
-      int(pos), max_length_path, num_threads, _ptr(vM), _ptr(rM), _ptr(stats))
+      int(pos), max_length_path, num_threads, _ptr(vM), _ptr(rM), _ptr(stats), _ptr(path))
     out = _csc_from_lists(vM, rM, Le, n, p, dtype)
-    return (out, stats) if return_stats else out
+    res = (out,)
+    if return_stats:
+        res += (stats,)
+    if return_reg_path:
+        res += (path,)
+    return res[0] if len(res) == 1 else res
+
+
+def lasso_qq(X, Q, q, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
+             max_length_path=-1, num_threads=-1):
+    """_lassoQq + lasso2 + coreLARS2 (restated): Gram Q used as given."""
+    dtype = X.dtype
+    X = _f(X, dtype)
+    Q = _f(Q, dtype)
+    q = _f(q, dtype)
+    m, n = X.shape
+    p = Q.shape[1]
+    lam = np.atleast_1d(np.asarray(lambda1, dtype=dtype))
+    sfx = _sfx(dtype)
+    ct = _ct(dtype)
+    fL = getattr(lib(), "orc_lasso_L_" + sfx)
+    fL.restype = _I
+    fL.argtypes = [_I, _I, _I, _I, ct, _I, ct]
+    Le = fL(m, p, L, mode, lam[0], int(pos), lambda2)
+    if Le <= 0:
+        return sp.csc_matrix((p, n), dtype=dtype)
+    vM = np.zeros((Le, n), dtype, order="F")
+    rM = np.full((Le, n), -1, np.int32, order="F")
+    f = getattr(lib(), "orc_lasso_qq_" + sfx)
+    f.restype = _I
+    f.argtypes = [_P, _I, _I, _P, _P, _I, _I, _P, _I, ct, _I, _I, _I, _I, _P, _P, _P]
+    f(_ptr(X), m, n, _ptr(Q), _ptr(q), p, L, _ptr(lam), lam.size, lambda2, mode,
+      int(pos), max_length_path, num_threads, _ptr(vM), _ptr(rM), None)
+    return _csc_from_lists(vM, rM, Le, n, p, dtype)
 
 
 def omp(X, D, L=None, eps=None, lambda1=None, num_threads=-1):
@@ -249,6 +288,29 @@ def ref_lasso(X, D, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
     h = f(_ptr(X), m, n, _ptr(D), p, L, lambda1, lambda2, mode, int(pos),
           int(ols), int(cholesky), num_threads, max_length_path,
           int(return_reg_path))
+    if return_reg_path:
+        Lp = p if L < 0 else L
+        mlp = 4 * Lp if max_length_path < 0 else max_length_path
+        return _ref_sp(h, n, p, dtype, path_shape=(p, mlp))
+    return _ref_sp(h, n, p, dtype)
+
+
+def ref_lasso_qq(X, Q, q, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
+                 ols=False, cholesky=False, max_length_path=-1, num_threads=-1,
+                 return_reg_path=False):
+    """The reference _lassoQq (cpp_library/spams.h:145-188), compiled in place."""
+    dtype = X.dtype
+    X = _f(X, dtype)
+    Q = _f(Q, dtype)
+    q = _f(q, dtype)
+    m, n = X.shape
+    p = Q.shape[1]
+    ct = _ct(dtype)
+    f = getattr(ref(), "ref_lasso_qq_" + _sfx(dtype))
+    f.restype = _P
+    f.argtypes = [_P, _I, _I, _P, _P, _I, _I, ct, ct] + [_I] * 7
+    h = f(_ptr(X), m, n, _ptr(Q), _ptr(q), p, L, lambda1, lambda2, mode, int(pos),
+          int(ols), int(cholesky), num_threads, max_length_path, int(return_reg_path))
     if return_reg_path:
         Lp = p if L < 0 else L
         mlp = 4 * Lp if max_length_path < 0 else max_length_path

@@ -11,6 +11,7 @@
 //   _lassoD     cpp_library/spams.h:107-143
 //   _omp        cpp_library/spams.h:250-293
 //   _fistaFlat  cpp_library/spams.h:460-608
+//   _lassoQq    cpp_library/spams.h:145-188
 // plus one per-signal-lambda LARS loop (ref_lasso_pervec_*) that mirrors the
 // OpenMP loop of lasso2 (decomp/decomp.h:1574-1646) with the constraint taken
 // per signal, calling the reference coreLARS2 (decomp/decomp.h:1706-1915).
@@ -41,6 +42,26 @@ void* lasso_impl(const T* X, int m, int n, const T* D, int p, int L, T lambda1,
                     num_threads, max_length_path, false, cholesky != 0);
   } catch (const char* msg) {
     fprintf(stderr, "reference lasso: %s\n", msg);
+    return nullptr;
+  }
+  return new SpOut<T>{sp, path};
+}
+
+template <typename T>
+void* lasso_qq_impl(const T* X, int m, int n, const T* Q, const T* q, int p, int L,
+                    T lambda1, T lambda2, int mode, int pos, int ols, int cholesky,
+                    int num_threads, int max_length_path, int return_path) {
+  Matrix<T> Xm(const_cast<T*>(X), m, n);
+  Matrix<T> Qm(const_cast<T*>(Q), p, p);
+  Matrix<T> qm(const_cast<T*>(q), p, n);
+  Matrix<T>* path = nullptr;
+  SpMatrix<T>* sp = nullptr;
+  try {
+    sp = _lassoQq<T>(&Xm, &Qm, &qm, &path, return_path != 0, L, lambda1, lambda2,
+                     static_cast<constraint_type>(mode), pos != 0, ols != 0, num_threads,
+                     max_length_path, false, cholesky != 0);
+  } catch (const char* msg) {
+    fprintf(stderr, "reference lassoQq: %s\n", msg);
     return nullptr;
   }
   return new SpOut<T>{sp, path};
@@ -193,6 +214,14 @@ extern "C" {
                         int return_path) {                                     \
     return lasso_impl<T>(X, m, n, D, p, L, lambda1, lambda2, mode, pos, ols,   \
                          cholesky, num_threads, max_length_path, return_path); \
+  }                                                                            \
+  void* ref_lasso_qq_##SUF(const T* X, int m, int n, const T* Q, const T* q,  \
+                           int p, int L, T lambda1, T lambda2, int mode,       \
+                           int pos, int ols, int cholesky, int num_threads,    \
+                           int max_length_path, int return_path) {             \
+    return lasso_qq_impl<T>(X, m, n, Q, q, p, L, lambda1, lambda2, mode, pos,  \
+                            ols, cholesky, num_threads, max_length_path,       \
+                            return_path);                                      \
   }                                                                            \
   void* ref_omp_##SUF(const T* X, int m, int n, const T* D, int p,             \
                       const int* L, int nL, const T* eps, int nEps,            \

@@ -9,6 +9,9 @@
  *   orc_corelars2          coreLARS2              decomp/decomp.h:1706-1915
  *   orc_lasso              _lassoD + lasso2       cpp_library/spams.h:107-143,
  *                                                 decomp/decomp.h:1563-1646
+ *                          (+ signal 0's path, decomp/decomp.h:1868-1871)
+ *   orc_lasso_qq           _lassoQq + lasso2      cpp_library/spams.h:145-188,
+ *                                                 decomp/decomp.h:1574-1646
  *   orc_coreormp           coreORMP               decomp/decomp.h:538-640
  *   orc_omp                _omp + omp             cpp_library/spams.h:250-293,
  *                                                 decomp/decomp.h:322-404

@@ -208,10 +208,53 @@ static int dict_create(spx_ctx* ctx, const T* D, int m, int p, int on_device,
   return SPX_OK;
 }
 
+// A resident Gram matrix without a dictionary (the Gram-given form of the
+// reference, _lassoQq / lasso2(X, Q, q, ...), cpp_library/spams.h:145-188):
+// G = Q (p x p, leading dimension p), used as given (no diagonal ridge);
+// m is the signal length (for the facade's L clamp).
+template <typename T>
+static int dict_create_gram(spx_ctx* ctx, const T* Q, int m, int p, int on_device,
+                            spx_dict** out) {
+  if (!ctx || !out || !Q || m <= 0 || p <= 0) {
+    set_error("dict_create_gram: invalid arguments (m=%d, p=%d)", m, p);
+    return SPX_ERR_ARG;
+  }
+  DeviceGuard g(ctx->device);
+  spx_dict* d = new spx_dict();
+  d->ctx = ctx;
+  d->device = ctx->device;
+  d->m = m;
+  d->p = p;
+  d->ldg = pad_ld(p);
+  d->elem = sizeof(T);
+  d->gram_only = 1;
+  hipStream_t st = ctx->stream;
+  if (hipMalloc(&d->G, (size_t)d->ldg * p * sizeof(T)) != hipSuccess) {
+    spx_dict_destroy(d);
+    set_error("dict_create_gram: device allocation failed (p=%d)", p);
+    return SPX_ERR_NOMEM;
+  }
+  SPX_HIP(hipMemsetAsync(d->G, 0, (size_t)d->ldg * p * sizeof(T), st));
+  SPX_HIP(hipMemcpy2DAsync(d->G, sizeof(T) * d->ldg, Q, sizeof(T) * p, sizeof(T) * p, p,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  SPX_HIP(hipStreamSynchronize(st));
+  *out = d;
+  return SPX_OK;
+}
+
+static int need_dictionary(const spx_dict* d) {
+  if (d && d->gram_only) {
+    set_error("this entry point needs the dictionary D; the object holds only a Gram matrix");
+    return SPX_ERR_ARG;
+  }
+  return SPX_OK;
+}
+
 // D'X and ||x||^2 for n signals into context scratch; returns pointers.
 template <typename T>
 static int products(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
                     int precision, T** DtX, T** norms) {
+  if (int rc = need_dictionary(d)) return rc;
   void* b1 = nullptr;
   void* b2 = nullptr;
   int rc = ctx->scratch_get(3, sizeof(T) * (size_t)d->ldg * (n > 0 ? n : 1), &b1);
@@ -252,7 +295,9 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   LarsArgs<T> a{};
   a.G = static_cast<const T*>(d->G);
   a.ldg = d->ldg;
-  a.diag = static_cast<T>(lambda2 > T(1e-10) ? double(lambda2) : 1e-10);
+  // G + max(lambda2, 1e-10) I (decomp/decomp.h:1566); a Gram given as such
+  // is used unchanged (decomp/decomp.h:1574)
+  a.diag = d->gram_only ? T(0) : static_cast<T>(lambda2 > T(1e-10) ? double(lambda2) : 1e-10);
   a.DtX = DtX;
   a.ldd = ldd;
   a.norms = norms;
@@ -272,6 +317,12 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   a.outCount = out->count;
   a.stats = out->stats;
   a.refit = (o->precision & SPX_PREC_REFIT64) ? 1 : 0;
+  a.ols = o->ols ? 1 : 0;
+  if (out->path && out->path_cols > 0) {
+    a.path = static_cast<T*>(out->path);
+    a.path_cols = out->path_cols;
+    SPX_HIP(hipMemsetAsync(out->path, 0, sizeof(T) * (size_t)d->p * out->path_cols, ctx->stream));
+  }
   const int w = o->waves_per_signal > 0 ? o->waves_per_signal
                 : o->form == SPX_FORM_PLAIN ? 4 : tuned_waves(d, 0);
   return lars_solve<T>(ctx, a, o->form, w);
@@ -365,6 +416,11 @@ static int omp_solve_dtx(spx_ctx* ctx, const spx_dict* d, const T* DtX, const T*
   a.outV = static_cast<T*>(out->v);
   a.outR = out->r;
   a.outCount = out->count;
+  if (out->path && out->path_cols > 0) {
+    a.path = static_cast<T*>(out->path);
+    a.path_cols = out->path_cols;
+    SPX_HIP(hipMemsetAsync(out->path, 0, sizeof(T) * (size_t)d->p * out->path_cols, ctx->stream));
+  }
   const int w = o->waves_per_signal > 0 ? o->waves_per_signal
                 : o->form == SPX_FORM_PLAIN ? 4 : tuned_waves(d, 1);
   return omp_solve<T>(ctx, a, o->form, w);
@@ -585,7 +641,7 @@ struct DevBuf {
 template <typename T>
 static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
                       const T* lambda, int nlam, T lambda2,
-                      const spx_lasso_opts* opts, spx_csc* out) {
+                      const spx_lasso_opts* opts, spx_csc* out, T* path = nullptr) {
   if (!ctx || !out || !D || !lambda || m <= 0 || p <= 0 || n < 0 || (n > 0 && !X)) {
     set_error("lasso : invalid arguments");
     return SPX_ERR_ARG;
@@ -601,6 +657,8 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   if (o.max_length_path < 0) o.max_length_path = 4 * (o.L < 0 ? p : o.L);
   o.L = Le;
   DeviceGuard g(ctx->device);
+  const int pcols = o.max_length_path > 0 ? o.max_length_path : 0;
+  if (path && pcols > 0) memset(path, 0, sizeof(T) * (size_t)p * pcols);
   if (Le <= 0 || n == 0) {
     // decomp/decomp.h:1587: L <= 0 -> empty result
     out->m = p;
@@ -614,10 +672,11 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   spx_dict* d = nullptr;
   int rc = dict_create<T>(ctx, D, m, p, 0, o.precision & SPX_PREC_BF16X3, &d);
   if (rc) return rc;
-  DevBuf bX, bL, bV, bR, bC;
+  DevBuf bX, bL, bV, bR, bC, bP;
   if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bL.alloc(sizeof(T) * nlam)) ||
       (rc = bV.alloc(sizeof(T) * (size_t)Le * n)) ||
-      (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) || (rc = bC.alloc(sizeof(int) * (size_t)n))) {
+      (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) || (rc = bC.alloc(sizeof(int) * (size_t)n)) ||
+      (path && pcols > 0 && (rc = bP.alloc(sizeof(T) * (size_t)p * pcols)))) {
     spx_dict_destroy(d);
     return rc;
   }
@@ -629,9 +688,89 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
     return SPX_ERR_HIP;
   }
   spx_lists_dev l{bV.p, static_cast<int*>(bR.p), static_cast<int*>(bC.p), Le, nullptr};
+  if (path && pcols > 0) {
+    l.path = bP.p;
+    l.path_cols = pcols;
+  }
   rc = lasso_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
                     static_cast<const T*>(bL.p), nlam, lambda2, &o, &l);
   if (!rc) rc = lists_to_host_csc<T>(ctx, l, p, n, out);
+  if (!rc && path && pcols > 0 &&
+      hipMemcpy(path, bP.p, sizeof(T) * (size_t)p * pcols, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("lasso: path download failed");
+    rc = SPX_ERR_HIP;
+  }
+  spx_dict_destroy(d);
+  return rc;
+}
+
+// Gram-given LARS (_lassoQq, cpp_library/spams.h:145-188): X (m x n) only
+// gives ||x_i||^2, Q (p x p) is the Gram matrix used as given (lambda2 is
+// ignored except in the facade's L clamp), q (p x n) the correlations.
+template <typename T>
+static int lasso_qq_host(spx_ctx* ctx, const T* X, int m, int n, const T* Q, const T* q,
+                         int p, const T* lambda, int nlam, T lambda2,
+                         const spx_lasso_opts* opts, spx_csc* out, T* path) {
+  if (!ctx || !out || !Q || !lambda || m <= 0 || p <= 0 || n < 0 || (n > 0 && (!X || !q))) {
+    set_error("lasso : invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  if (!(nlam == 1 || nlam == n)) {
+    set_error("lasso : lambda must have 1 or n entries");
+    return SPX_ERR_ARG;
+  }
+  memset(out, 0, sizeof(*out));
+  spx_lasso_opts o{-1, SPX_PENALTY, 0, -1, SPX_PREC_FP32, SPX_FORM_BATCHED, 0};
+  if (opts) o = *opts;
+  const int Le = spx_lasso_L(m, p, o.L, o.mode, double(lambda[0]), o.pos, double(lambda2));
+  if (o.max_length_path < 0) o.max_length_path = 4 * (o.L < 0 ? p : o.L);
+  o.L = Le;
+  DeviceGuard g(ctx->device);
+  const int pcols = o.max_length_path > 0 ? o.max_length_path : 0;
+  if (path && pcols > 0) memset(path, 0, sizeof(T) * (size_t)p * pcols);
+  if (Le <= 0 || n == 0) {
+    out->m = p;
+    out->n = n;
+    out->pB = static_cast<int*>(calloc((size_t)n + 1, sizeof(int)));
+    out->r = static_cast<int*>(malloc(sizeof(int)));
+    out->v = malloc(sizeof(T));
+    return SPX_OK;
+  }
+  spx_dict* d = nullptr;
+  int rc = dict_create_gram<T>(ctx, Q, m, p, 0, &d);
+  if (rc) return rc;
+  DevBuf bX, bq, bN, bL, bV, bR, bC, bP;
+  if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bq.alloc(sizeof(T) * (size_t)p * n)) ||
+      (rc = bN.alloc(sizeof(T) * (size_t)n)) || (rc = bL.alloc(sizeof(T) * nlam)) ||
+      (rc = bV.alloc(sizeof(T) * (size_t)Le * n)) ||
+      (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) || (rc = bC.alloc(sizeof(int) * (size_t)n)) ||
+      (path && pcols > 0 && (rc = bP.alloc(sizeof(T) * (size_t)p * pcols)))) {
+    spx_dict_destroy(d);
+    return rc;
+  }
+  hipStream_t st = ctx->stream;
+  if (hipMemcpyAsync(bX.p, X, sizeof(T) * (size_t)m * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(bq.p, q, sizeof(T) * (size_t)p * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(bL.p, lambda, sizeof(T) * nlam, hipMemcpyHostToDevice, st) != hipSuccess) {
+    spx_dict_destroy(d);
+    set_error("lasso: upload failed");
+    return SPX_ERR_HIP;
+  }
+  rc = col_sqnorms<T>(st, m, n, static_cast<const T*>(bX.p), m, static_cast<T*>(bN.p));
+  spx_lists_dev l{bV.p, static_cast<int*>(bR.p), static_cast<int*>(bC.p), Le, nullptr};
+  if (path && pcols > 0) {
+    l.path = bP.p;
+    l.path_cols = pcols;
+  }
+  if (!rc)
+    rc = lasso_dtx_dev<T>(ctx, d, static_cast<const T*>(bq.p), p, static_cast<const T*>(bN.p), n,
+                          static_cast<const T*>(bL.p), nlam, lambda2, &o, &l);
+  if (!rc) rc = lists_to_host_csc<T>(ctx, l, p, n, out);
+  if (!rc && path && pcols > 0 &&
+      hipMemcpy(path, bP.p, sizeof(T) * (size_t)p * pcols, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("lasso: path download failed");
+    rc = SPX_ERR_HIP;
+  }
   spx_dict_destroy(d);
   return rc;
 }
@@ -639,12 +778,16 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
 template <typename T>
 static int omp_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
                     const int* L, int nL, const T* eps, int nEps, const T* lambda,
-                    int nLambda, const spx_omp_opts* opts, spx_csc* out) {
+                    int nLambda, const spx_omp_opts* opts, spx_csc* out,
+                    T* path = nullptr) {
   if (!ctx || !out || !D || m <= 0 || p <= 0 || n < 0 || (n > 0 && !X)) {
     set_error("omp : invalid arguments");
     return SPX_ERR_ARG;
   }
   memset(out, 0, sizeof(*out));
+  // path: p x min(m, p), zero where never written (cpp_library/spams.h:283-285)
+  const int pcols = m < p ? m : p;
+  if (path) memset(path, 0, sizeof(T) * (size_t)p * pcols);
   if (!L) nL = 0;
   if (!eps) nEps = 0;
   if (!lambda) nLambda = 0;
@@ -671,11 +814,12 @@ static int omp_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   spx_dict* d = nullptr;
   int rc = dict_create<T>(ctx, D, m, p, 0, o.precision, &d);
   if (rc) return rc;
-  DevBuf bX, bLs, bE, bLa, bV, bR, bC;
+  DevBuf bX, bLs, bE, bLa, bV, bR, bC, bP;
   if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bLs.alloc(sizeof(int) * (nL ? nL : 1))) ||
       (rc = bE.alloc(sizeof(T) * (nEps ? nEps : 1))) || (rc = bLa.alloc(sizeof(T) * (nLambda ? nLambda : 1))) ||
       (rc = bV.alloc(sizeof(T) * (size_t)Le * n)) || (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) ||
-      (rc = bC.alloc(sizeof(int) * (size_t)n))) {
+      (rc = bC.alloc(sizeof(int) * (size_t)n)) ||
+      (path && (rc = bP.alloc(sizeof(T) * (size_t)p * pcols)))) {
     spx_dict_destroy(d);
     return rc;
   }
@@ -690,11 +834,20 @@ static int omp_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
     return SPX_ERR_HIP;
   }
   spx_lists_dev l{bV.p, static_cast<int*>(bR.p), static_cast<int*>(bC.p), Le, nullptr};
+  if (path) {
+    l.path = bP.p;
+    l.path_cols = pcols;
+  }
   rc = omp_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
                   nL ? static_cast<const int*>(bLs.p) : nullptr, nL,
                   nEps ? static_cast<const T*>(bE.p) : nullptr, nEps,
                   nLambda ? static_cast<const T*>(bLa.p) : nullptr, nLambda, &o, &l);
   if (!rc) rc = lists_to_host_csc<T>(ctx, l, p, n, out);
+  if (!rc && path &&
+      hipMemcpy(path, bP.p, sizeof(T) * (size_t)p * pcols, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("omp: path download failed");
+    rc = SPX_ERR_HIP;
+  }
   spx_dict_destroy(d);
   return rc;
 }
@@ -1023,6 +1176,37 @@ int spx_lasso_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, 
                   const spx_lasso_opts* opts, spx_csc* out) {
   return spx::lasso_host<double>(ctx, X, m, n, D, p, lambda, n_lambda, lambda2, opts, out);
 }
+int spx_lasso_qq_f32(spx_ctx* ctx, const float* X, int m, int n, const float* Q,
+                     const float* q, int p, const float* lambda, int n_lambda, float lambda2,
+                     const spx_lasso_opts* opts, spx_csc* out, float* path) {
+  return spx::lasso_qq_host<float>(ctx, X, m, n, Q, q, p, lambda, n_lambda, lambda2, opts, out,
+                                   path);
+}
+int spx_lasso_qq_f64(spx_ctx* ctx, const double* X, int m, int n, const double* Q,
+                     const double* q, int p, const double* lambda, int n_lambda,
+                     double lambda2, const spx_lasso_opts* opts, spx_csc* out, double* path) {
+  return spx::lasso_qq_host<double>(ctx, X, m, n, Q, q, p, lambda, n_lambda, lambda2, opts, out,
+                                    path);
+}
+int spx_dict_create_gram_f32(spx_ctx* ctx, const float* Q, int m, int p, int on_device,
+                             spx_dict** out) {
+  return spx::dict_create_gram<float>(ctx, Q, m, p, on_device, out);
+}
+int spx_dict_create_gram_f64(spx_ctx* ctx, const double* Q, int m, int p, int on_device,
+                             spx_dict** out) {
+  return spx::dict_create_gram<double>(ctx, Q, m, p, on_device, out);
+}
+int spx_lasso_path_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
+                       const float* lambda, int n_lambda, float lambda2,
+                       const spx_lasso_opts* opts, spx_csc* out, float* path) {
+  return spx::lasso_host<float>(ctx, X, m, n, D, p, lambda, n_lambda, lambda2, opts, out, path);
+}
+int spx_lasso_path_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, int p,
+                       const double* lambda, int n_lambda, double lambda2,
+                       const spx_lasso_opts* opts, spx_csc* out, double* path) {
+  return spx::lasso_host<double>(ctx, X, m, n, D, p, lambda, n_lambda, lambda2, opts, out,
+                                 path);
+}
 
 int spx_omp_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
                 const int* L, int nL, const float* eps, int n_eps, const float* lambda,
@@ -1033,6 +1217,18 @@ int spx_omp_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, in
                 const int* L, int nL, const double* eps, int n_eps, const double* lambda,
                 int n_lambda, const spx_omp_opts* opts, spx_csc* out) {
   return spx::omp_host<double>(ctx, X, m, n, D, p, L, nL, eps, n_eps, lambda, n_lambda, opts, out);
+}
+int spx_omp_path_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
+                     const int* L, int nL, const float* eps, int n_eps, const float* lambda,
+                     int n_lambda, const spx_omp_opts* opts, spx_csc* out, float* path) {
+  return spx::omp_host<float>(ctx, X, m, n, D, p, L, nL, eps, n_eps, lambda, n_lambda, opts, out,
+                              path);
+}
+int spx_omp_path_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, int p,
+                     const int* L, int nL, const double* eps, int n_eps, const double* lambda,
+                     int n_lambda, const spx_omp_opts* opts, spx_csc* out, double* path) {
+  return spx::omp_host<double>(ctx, X, m, n, D, p, L, nL, eps, n_eps, lambda, n_lambda, opts,
+                               out, path);
 }
 
 int spx_fista_flat_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
@@ -1108,6 +1304,7 @@ int spx_fista_flat_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int 
     spx::set_error("fista_dev: invalid arguments");
     return SPX_ERR_ARG;
   }
+  if (int rc = spx::need_dictionary(d)) return rc;
   spx::DeviceGuard g(ctx->device);
   return spx::fista_run<float>(ctx, d, X, n, W0, W, o, info);
 }
@@ -1117,6 +1314,7 @@ int spx_fista_flat_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int
     spx::set_error("fista_dev: invalid arguments");
     return SPX_ERR_ARG;
   }
+  if (int rc = spx::need_dictionary(d)) return rc;
   spx::DeviceGuard g(ctx->device);
   return spx::fista_run<double>(ctx, d, X, n, W0, W, o, info);
 }
@@ -1141,6 +1339,7 @@ int spx_lists_to_csc_dev_f64(spx_ctx* ctx, const spx_lists_dev* l, int n, int* p
 int spx_dtx_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, int n,
                     float* DtX, int ldc, int precision) {
   if (!ctx || !d || d->elem != 4 || ldc < d->p || ldx < d->m) return SPX_ERR_ARG;
+  if (int rc = spx::need_dictionary(d)) return rc;
   spx::DeviceGuard g(ctx->device);
   return spx::gemm_tn<float>(ctx->stream, d->p, n, d->m, static_cast<const float*>(d->D),
                              d->m, X, ldx, DtX, ldc, 1.f, 0.f, nullptr, 0, precision);
@@ -1148,6 +1347,7 @@ int spx_dtx_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, in
 int spx_dtx_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx, int n,
                     double* DtX, int ldc) {
   if (!ctx || !d || d->elem != 8 || ldc < d->p || ldx < d->m) return SPX_ERR_ARG;
+  if (int rc = spx::need_dictionary(d)) return rc;
   spx::DeviceGuard g(ctx->device);
   return spx::gemm_tn<double>(ctx->stream, d->p, n, d->m, static_cast<const double*>(d->D),
                               d->m, X, ldx, DtX, ldc, 1.0, 0.0, nullptr, 0, 0);

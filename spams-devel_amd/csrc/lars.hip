@@ -705,6 +705,10 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         normX += coeff1 * step * step - 2 * coeff2 * step;
         thrs += step * coeff1;
         __syncthreads();
+        if (a.path && sig == 0 && iter < a.path_cols) {
+          // regularisation path of the first signal (decomp/decomp.h:1868-1871)
+          for (int k = tid; k <= i; k += GT) a.path[(size_t)iter * p + sInd[k]] = sCoef[k];
+        }
 
         if (step == step_max) {
           // ------------------- drop atom first_zero (decomp/decomp.h:1875-1903)
@@ -761,21 +765,23 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
       }
     }
 
-    // ------------------------------- optional FP64 refit (PENALTY, at lambda)
-    if (a.refit && reached && !overflow && s_active > 0) {
+    // ---------- optional FP64 refit (PENALTY, at lambda) or least-squares fit
+    // ols: alpha_A = (G_AA + diag I)^-1 (D'x)_A, the coefficients the
+    // reference's ols option returns (decomp/decomp.h:1087-1091)
+    if (((a.refit && reached) || a.ols) && !overflow && s_active > 0) {
       __syncthreads();
       const int s = s_active;
       const T* Mf = inl ? sMl : Mc;  // the inverse of the final active block
       const int ldf = inl ? CL : ldm;
-      for (int r = tid; r < s; r += GT) sAlpha[r] = static_cast<double>(sCoef[r]);
+      for (int r = tid; r < s; r += GT) sAlpha[r] = a.ols ? 0.0 : static_cast<double>(sCoef[r]);
       __syncthreads();
-      for (int it = 0; it < 2; ++it) {
+      for (int it = 0; it < (a.ols ? 3 : 2); ++it) {
         // res = (D'x)_A - lam sign(alpha) - (G_AA + d I) alpha   (FP64)
         for (int r = tid; r < s; r += GT) {
           const int ir = sInd[r];
           const T* gcol = a.G + (size_t)ir * ldg;
-          double acc = static_cast<double>(dtx[ir]) -
-                       static_cast<double>(lam) * (sCoef[r] > 0 ? 1.0 : -1.0);
+          double acc = static_cast<double>(dtx[ir]);
+          if (!a.ols) acc -= static_cast<double>(lam) * (sCoef[r] > 0 ? 1.0 : -1.0);
           for (int q = 0; q < s; ++q) {
             const double g = static_cast<double>(gcol[sInd[q]]) + (q == r ? static_cast<double>(diag) : 0.0);
             acc -= g * sAlpha[q];

@@ -257,6 +257,18 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
           ++j;
           break;
         }
+        if (a.path && sig == 0 && j < a.path_cols) {
+          // path of the first signal: column j = Un(0:j+1,0:j+1) RUn(0:j+1)
+          // scattered to the selected atoms (decomp/decomp.h:605-613)
+          for (int r = tid; r <= j; r += GT) {
+            T acc = 0;
+            for (int q = r; q <= j; ++q) acc += Un[r + (size_t)q * L] * sRUn[q];
+            sX[r] = acc;
+          }
+          __syncthreads();
+          for (int k = tid; k <= j; k += GT) a.path[(size_t)j * p + sInd[k]] = sX[k];
+          __syncthreads();
+        }
         // ----------------------- Undn_j = G(:, active) * Un(0:j+1, j)
         V und[NQ];
 #pragma unroll
@@ -310,6 +322,11 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
       const int id = k < j ? sInd[k] : -1;
       orr[k] = id;
       ov[k] = k < j ? sX[k] : T(0);
+    }
+    if (a.path && sig == 0 && j > 0 && j - 1 < a.path_cols) {
+      // final coefficients in column j-1 (decomp/decomp.h:634-639)
+      for (int k = tid; k < j; k += GT)
+        if (sInd[k] >= 0) a.path[(size_t)(j - 1) * p + sInd[k]] = sX[k];
     }
     if (tid == 0) {
       int cnt = 0;

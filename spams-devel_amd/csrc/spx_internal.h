@@ -53,6 +53,9 @@ struct LarsArgs {
   size_t state_bytes;
   T* ovf_key;       // per list position: work key of the overflowing signal
   int lds_cap;      // full-capacity pass: LDS capacity of the inverse (0 = HBM only)
+  T* path;          // signal 0's regularisation path (p x path_cols) or null
+  int path_cols;
+  int ols;          // least-squares refit of the final active set
 };
 
 // Kernel arguments of the ORMP solver (coreORMP semantics).
@@ -77,6 +80,8 @@ struct OmpArgs {
   T* scratch;       // per workgroup: L*L (Un) + p*L (Undn)
   int* queue;
   int chunk;        // signals taken per atomic grab of the work counter
+  T* path;          // signal 0's path (p x path_cols) or null
+  int path_cols;
 };
 
 template <typename T>
@@ -191,5 +196,6 @@ struct spx_dict {
   void* G = nullptr;    // ldg x p
   double lipschitz = 0; // largest eigenvalue of G (power iteration), lazily
   int waves[2] = {0, 0};     // autotuned wavefronts per signal (LARS, OMP)
+  int gram_only = 0;         // created from a Gram matrix (no D; no ridge)
   float tune_ms[2][4] = {};  // autotuner trial times for 1, 2, 4, 16 waves
 };

@@ -41,7 +41,7 @@ class spx_csc(ctypes.Structure):
 
 class spx_lasso_opts(ctypes.Structure):
     _fields_ = [("L", _I), ("mode", _I), ("pos", _I), ("max_length_path", _I),
-                ("precision", _I), ("form", _I), ("waves_per_signal", _I)]
+                ("precision", _I), ("form", _I), ("waves_per_signal", _I), ("ols", _I)]
 
 
 class spx_omp_opts(ctypes.Structure):
@@ -57,7 +57,8 @@ class spx_fista_opts(ctypes.Structure):
 
 
 class spx_lists_dev(ctypes.Structure):
-    _fields_ = [("v", _P), ("r", _P), ("count", _P), ("L", _I), ("stats", _P)]
+    _fields_ = [("v", _P), ("r", _P), ("count", _P), ("L", _I), ("stats", _P),
+                ("path", _P), ("path_cols", _I)]
 
 
 _lib = None
@@ -297,48 +298,67 @@ def lasso(X, D=None, Q=None, q=None, return_reg_path=False, L=-1,
     ``lambda1`` is the reference's scalar constraint, or a length-n vector
     (one constraint per signal).  ``cholesky=True`` solves the same problem
     with the same homotopy (the engine keeps an explicit inverse of the active
-    Gram block).  ``numThreads`` is accepted and ignored (no CPU threads)."""
+    Gram block); ``ols=True`` returns the least-squares fit on each final
+    active set.  With ``Q``/``q`` (Gram matrix and correlations) the Gram is
+    used as given.  ``return_reg_path`` also returns the first signal's path
+    (p x max_length_path).  ``numThreads`` is accepted and ignored."""
     if Q is not None:
         if q is None:
             raise ValueError("lasso : q is needed when Q is given")
-        raise NotImplementedError("lasso: the (Q, q) form is not provided by spx")
-    if D is None:
+    elif D is None:
         raise ValueError("lasso : you must give D or Q and q")
     if lambda1 is None:
         raise ValueError("lasso : lambda1 must be defined")
-    if return_reg_path:
-        raise NotImplementedError("lasso: return_reg_path is not provided by spx")
-    if ols:
-        raise NotImplementedError("lasso: ols=True is not provided by spx")
     dtype = np.asarray(X).dtype
     sfx = _sfx(dtype)
     X = np.asfortranarray(X, dtype=dtype)
-    D = np.asfortranarray(D, dtype=dtype)
     m, n = X.shape
-    if D.shape[0] != m:
-        raise ValueError("lasso : incompatible matrix dimensions")
-    p = D.shape[1]
+    if Q is not None:
+        Q = np.asfortranarray(Q, dtype=dtype)
+        q = np.asfortranarray(q, dtype=dtype)
+        p = Q.shape[1]
+        if Q.shape[0] != p:
+            raise ValueError("lasso : Q must be square")
+        if q.shape != (p, n):
+            raise ValueError("lasso : incompatible matrix dimensions")
+    else:
+        D = np.asfortranarray(D, dtype=dtype)
+        if D.shape[0] != m:
+            raise ValueError("lasso : incompatible matrix dimensions")
+        p = D.shape[1]
     lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lambda1, dtype=dtype)))
     if lam.size not in (1, n):
         raise ValueError("lasso : lambda1 must be a scalar or have one entry per signal")
     o = spx_lasso_opts(L, mode, int(pos), max_length_path, PREC[precision],
-                       FORM[form], waves_per_signal)
+                       FORM[form], waves_per_signal, int(bool(ols)))
+    path = None
+    if return_reg_path:
+        mlp = max_length_path if max_length_path >= 0 else 4 * (p if L < 0 else L)
+        path = np.zeros((p, max(mlp, 0)), dtype=dtype, order="F")
     out = spx_csc()
     ctx = ctx or default_context()
-    f = getattr(lib(), "spx_lasso_" + sfx)
-    f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _I, _ct(dtype),
-                  ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_csc)]
-    _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(lam), lam.size, lambda2,
-             ctypes.byref(o), ctypes.byref(out)))
-    return _csc_to_scipy(out, dtype)
+    if Q is not None:
+        f = getattr(lib(), "spx_lasso_qq_" + sfx)
+        f.argtypes = [_P, _P, _I, _I, _P, _P, _I, _P, _I, _ct(dtype),
+                      ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_csc), _P]
+        _check(f(ctx.h, _ptr(X), m, n, _ptr(Q), _ptr(q), p, _ptr(lam), lam.size, lambda2,
+                 ctypes.byref(o), ctypes.byref(out), _ptr(path)))
+    else:
+        f = getattr(lib(), "spx_lasso_path_" + sfx)
+        f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _I, _ct(dtype),
+                      ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_csc), _P]
+        _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(lam), lam.size, lambda2,
+                 ctypes.byref(o), ctypes.byref(out), _ptr(path)))
+    A = _csc_to_scipy(out, dtype)
+    return (A, path) if return_reg_path else A
 
 
 def omp(X, D, L=None, eps=None, lambda1=None, return_reg_path=False,
         numThreads=-1, precision="fp32", form="batched", waves_per_signal=0,
         ctx=None):
-    """Cholesky-based ORMP (spams.omp, swig/python/spams.py:167-199)."""
-    if return_reg_path:
-        raise NotImplementedError("omp: return_reg_path is not provided by spx")
+    """Cholesky-based ORMP (spams.omp, swig/python/spams.py:167-199).
+    ``return_reg_path`` also returns the first signal's path
+    (p x min(m, p))."""
     dtype = np.asarray(X).dtype
     sfx = _sfx(dtype)
     X = np.asfortranarray(X, dtype=dtype)
@@ -360,12 +380,14 @@ def omp(X, D, L=None, eps=None, lambda1=None, return_reg_path=False,
     o = spx_omp_opts(PREC[precision], FORM[form], waves_per_signal)
     out = spx_csc()
     ctx = ctx or default_context()
-    f = getattr(lib(), "spx_omp_" + sfx)
+    path = np.zeros((p, min(m, p)), dtype=dtype, order="F") if return_reg_path else None
+    f = getattr(lib(), "spx_omp_path_" + sfx)
     f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _I, _P, _I, _P, _I,
-                  ctypes.POINTER(spx_omp_opts), ctypes.POINTER(spx_csc)]
+                  ctypes.POINTER(spx_omp_opts), ctypes.POINTER(spx_csc), _P]
     _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(Ls), nL, _ptr(E), nE,
-             _ptr(La), nLa, ctypes.byref(o), ctypes.byref(out)))
-    return _csc_to_scipy(out, dtype)
+             _ptr(La), nLa, ctypes.byref(o), ctypes.byref(out), _ptr(path)))
+    A = _csc_to_scipy(out, dtype)
+    return (A, path) if return_reg_path else A
 
 
 def fistaFlat(Y, X, W0, return_optim_info=False, numThreads=-1, max_it=1000,

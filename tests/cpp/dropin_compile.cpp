@@ -11,6 +11,16 @@ void use_lasso() {
   SpMatrix<float>* s2 = _lassoD<float>(&Xf, &Df, reinterpret_cast<Matrix<float>**>(&path), false,
                                        -1, 0.15f, 0.f, PENALTY, false, false, -1, -1, false, false);
   delete s2;
+  // return_reg_path and ols, and the Gram-given form
+  SpMatrix<float>* s3 = _lassoD<float>(&Xf, &Df, reinterpret_cast<Matrix<float>**>(&path), true,
+                                       -1, 0.15f, 0.f, PENALTY, false, true, -1, -1, false, false);
+  delete s3;
+  delete reinterpret_cast<Matrix<float>*>(path);
+  Matrix<float> Q(256, 256), q(256, 5);
+  SpMatrix<float>* s4 = _lassoQq<float>(&Xf, &Q, &q, reinterpret_cast<Matrix<float>**>(&path),
+                                        false, -1, 0.15f, 0.f, PENALTY, false, false, -1, -1,
+                                        false, false);
+  delete s4;
 }
 
 void use_omp() {
@@ -25,6 +35,9 @@ void use_omp() {
   Matrix<double>* path;
   SpMatrix<double>* s = _omp(&X, &D, &path, false, true, &vL, false, &ve, false, &vl, -1);
   delete s;
+  SpMatrix<double>* s2 = _omp(&X, &D, &path, true, true, &vL, false, &ve, false, &vl, -1);
+  delete s2;
+  delete path;
 }
 
 void use_fista() {

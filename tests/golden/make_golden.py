@@ -5,7 +5,7 @@ oracle/Makefile from the reference's own headers) on small seeded inputs and
 stores inputs and outputs as .npz files next to this script.  Needs the
 reference tree at build time; the fixtures themselves are plain data.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--all]
 """
 from __future__ import annotations
 
@@ -94,9 +94,41 @@ def main():
                 lambda1=np.asarray(0.05, dt), lambda2=np.asarray(0.01, dt),
                 L0=np.asarray(L0, dt), fixed=fixed, max_it=200, it0=10,
                 tol=np.asarray(1e-10, dt))))
+        # regularisation paths, Gram-given form, least-squares refit
+        X, D = small(17, 64, 256, 10, 5, 0.05, True, dt)
+        A, path = oracle.ref_lasso(X, D, 0.05, return_reg_path=True)
+        # the reference leaves the path entries coreLARS2 never writes
+        # uninitialised; the restatement (bit-exact on the written entries,
+        # tests/test_oracle.py) marks them, and they are stored as NaN
+        _, opath = oracle.lasso(X, D, 0.05, return_reg_path=True)
+        path = np.where(np.isnan(opath), np.nan, path)
+        pB, r, v = csc_parts(A)
+        cases.append((f"lassopath_{tag}", dict(kind="lasso", X=X, D=D, lam=np.asarray(0.05, dt),
+                                               pB=pB, r=r, v=v, path=path)))
+        Q = (D.T.astype(np.float64) @ D).astype(dt)
+        q = (D.T.astype(np.float64) @ X).astype(dt)
+        A = oracle.ref_lasso_qq(X, Q, q, 0.05)
+        pB, r, v = csc_parts(A)
+        cases.append((f"lassoqq_{tag}", dict(kind="lasso", X=X, Q=Q, q=q,
+                                             lam=np.asarray(0.05, dt), pB=pB, r=r, v=v)))
+        A = oracle.ref_lasso(X, D, 0.05, ols=True)
+        pB, r, v = csc_parts(A)
+        cases.append((f"lassools_{tag}", dict(kind="lasso", X=X, D=D, lam=np.asarray(0.05, dt),
+                                              pB=pB, r=r, v=v)))
+        X, D = small(18, 64, 256, 10, 8, 0.01, False, dt)
+        A, path = oracle.ref_omp(X, D, L=12, return_reg_path=True)
+        pB, r, v = csc_parts(A)
+        cases.append((f"omppath_{tag}", dict(kind="omp", X=X, D=D, L=np.asarray(12, np.int32),
+                                             pB=pB, r=r, v=v, path=path)))
+    only_new = "--all" not in sys.argv
+    wrote = 0
     for name, d in cases:
-        np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
-    print(f"wrote {len(cases)} fixtures to {OUT}")
+        fn = os.path.join(OUT, name + ".npz")
+        if only_new and os.path.exists(fn):
+            continue
+        np.savez_compressed(fn, **d)
+        wrote += 1
+    print(f"wrote {wrote} of {len(cases)} fixtures to {OUT} (existing kept; --all rewrites)")
 
 
 if __name__ == "__main__":

@@ -403,3 +403,97 @@ def test_fista_graph_replays_equal_eager(spx, orc):
                                 tol=0.0, it0=it0)
         assert np.abs(We.cpu().numpy().T - Wo).max() <= 1e-4
     d.close()
+
+
+# ------------------------------ paths, Gram-given form, least-squares refit
+
+@pytest.mark.parametrize("name", golden_files("lassopath_"))
+def test_lasso_path_matches_golden(spx, name):
+    """Signal 0's regularisation path against the reference's: the entries
+    it writes (finite in the fixture) agree, the others are zero."""
+    print("case", name, flush=True)
+    g = load_golden(name)
+    A, path = spx.lasso(g["X"], g["D"], lambda1=g["lam"].item(), return_reg_path=True)
+    w = ~np.isnan(g["path"])
+    tol = 1e-9 if g["X"].dtype == np.float64 else 2e-4
+    assert np.abs(path[w] - g["path"][w]).max() <= tol
+    assert not path[~w].any()
+    ref = np.zeros(dense(A).shape)
+    for j in range(ref.shape[1]):
+        ref[g["r"][g["pB"][j]:g["pB"][j + 1]], j] = g["v"][g["pB"][j]:g["pB"][j + 1]]
+    assert np.abs(dense(A) - ref).max() <= tol
+
+
+@pytest.mark.parametrize("name", golden_files("lassoqq_"))
+def test_lasso_qq_matches_golden(spx, name):
+    print("case", name, flush=True)
+    g = load_golden(name)
+    A = dense(spx.lasso(g["X"], Q=g["Q"], q=g["q"], lambda1=g["lam"].item()))
+    ref = np.zeros(A.shape)
+    for j in range(ref.shape[1]):
+        ref[g["r"][g["pB"][j]:g["pB"][j + 1]], j] = g["v"][g["pB"][j]:g["pB"][j + 1]]
+    assert np.array_equal(A != 0, ref != 0)
+    tol = 1e-9 if g["X"].dtype == np.float64 else 2e-4
+    assert np.abs(A - ref).max() <= tol
+
+
+def test_lasso_ols_matches_golden_f64(spx):
+    print("case lasso ols f64", flush=True)
+    g = load_golden("lassools_f64.npz")
+    A = dense(spx.lasso(g["X"], g["D"], lambda1=g["lam"].item(), ols=True))
+    ref = np.zeros(A.shape)
+    for j in range(ref.shape[1]):
+        ref[g["r"][g["pB"][j]:g["pB"][j + 1]], j] = g["v"][g["pB"][j]:g["pB"][j + 1]]
+    assert np.array_equal(A != 0, ref != 0)
+    np.testing.assert_allclose(A, ref, atol=1e-9)
+
+
+def test_lasso_ols_f32_is_least_squares_on_its_support(spx, orc):
+    """FP32 ols: the reference's own FP32 ols run leaves its FP64 result (its
+    Cholesky path changes supports in FP32), so the FP32 result is held to
+    the FP64 semantics: the FP64 path's support and the least-squares fit on
+    it, within the FP32 tolerance."""
+    print("case lasso ols f32", flush=True)
+    g = load_golden("lassools_f64.npz")
+    X, D = g["X"].astype(np.float32), g["D"].astype(np.float32)
+    A = dense(spx.lasso(X, D, lambda1=0.05, ols=True))
+    B = dense(orc.lasso(X.astype(np.float64), D.astype(np.float64), 0.05))
+    same = np.all((A != 0) == (B != 0), axis=0)
+    assert same.mean() >= 0.9
+    for j in np.nonzero(same)[0]:
+        S = np.nonzero(A[:, j])[0]
+        Ds = D[:, S].astype(np.float64)
+        c = np.linalg.solve(Ds.T @ Ds + 1e-10 * np.eye(len(S)), Ds.T @ X[:, j].astype(np.float64))
+        assert np.abs(A[S, j] - c).max() <= 2e-4
+
+
+@pytest.mark.parametrize("name", golden_files("omppath_"))
+def test_omp_path_matches_golden(spx, name):
+    print("case", name, flush=True)
+    g = load_golden(name)
+    A, path = spx.omp(g["X"], g["D"], L=int(g["L"]), return_reg_path=True)
+    tol = 1e-9 if g["X"].dtype == np.float64 else 5e-4
+    assert path.shape == g["path"].shape
+    assert np.array_equal(path != 0, g["path"] != 0)
+    assert np.abs(path - g["path"]).max() <= tol
+
+
+def test_gram_only_dictionary_is_rejected_where_D_is_needed(spx):
+    import ctypes
+    import torch
+    print("case gram-only", flush=True)
+    ctx = gpu_ctx(spx)
+    Q = np.eye(8, dtype=np.float32)
+    h = ctypes.c_void_p()
+    f = spx.lib().spx_dict_create_gram_f32
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.c_void_p]
+    assert f(ctx.h, Q.ctypes.data, 8, 8, 0, ctypes.byref(h)) == 0
+    X = torch.zeros((4, 8), dtype=torch.float32, device="cuda")
+    out = torch.zeros((4, 8), dtype=torch.float32, device="cuda")
+    g = spx.lib().spx_dtx_dev_f32
+    g.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                  ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    assert g(ctx.h, h, X.data_ptr(), 8, 4, out.data_ptr(), 8, 0) == 1  # SPX_ERR_ARG
+    assert b"Gram" in spx.lib().spx_last_error()
+    spx.lib().spx_dict_destroy(h)

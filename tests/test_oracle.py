@@ -235,3 +235,111 @@ def test_fista_l1_converges_to_lars(orc):
                               max_it=20000, L0=L0, fixed_step=True, tol=1e-14, it0=100)
     np.testing.assert_allclose(Wt, A, atol=1e-6)
     assert np.all(info[2] < 1e-6)
+
+
+# ------------------------------------ paths, Gram-given form, least squares
+
+@pytest.mark.parametrize("name", golden_files("lassopath_"))
+def test_lasso_path_matches_golden(orc, name):
+    """Signal 0's regularisation path: the restatement writes exactly the
+    entries the reference writes (NaN in the fixture marks the others,
+    which the reference leaves uninitialised) with the same values."""
+    g = load_golden(name)
+    A, path = orc.lasso(g["X"], g["D"], g["lam"].item(), return_reg_path=True)
+    import scipy.sparse as sp
+    ref = sp.csc_matrix((g["v"], g["r"], g["pB"]), shape=(g["D"].shape[1], g["X"].shape[1]))
+    assert csc_equal(A, ref)
+    assert np.array_equal(np.isnan(path), np.isnan(g["path"]))
+    w = ~np.isnan(path)
+    assert w.sum() > 10 and np.array_equal(path[w], g["path"][w])
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_lasso_path_matches_reference_build(orc, dt):
+    if not orc.has_ref():
+        pytest.skip("reference build not present")
+    rng = np.random.default_rng(31)
+    D = W.unit_dict(rng, 32, 96, dt)
+    X = W.sparse_signals(rng, D, 6, 4, 0.05, True, dt)
+    for lam, mlp in ((0.02, -1), (0.1, 7)):
+        _, path = orc.lasso(X, D, lam, max_length_path=mlp, return_reg_path=True)
+        _, rpath = orc.ref_lasso(X, D, lam, max_length_path=mlp, return_reg_path=True)
+        w = ~np.isnan(path)
+        assert w.any() and np.array_equal(path[w], rpath[w])
+
+
+@pytest.mark.parametrize("name", golden_files("lassoqq_"))
+def test_lasso_qq_matches_golden(orc, name):
+    g = load_golden(name)
+    A = orc.lasso_qq(g["X"], g["Q"], g["q"], g["lam"].item())
+    import scipy.sparse as sp
+    ref = sp.csc_matrix((g["v"], g["r"], g["pB"]), shape=(g["Q"].shape[1], g["X"].shape[1]))
+    assert csc_equal(A, ref)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_lasso_qq_matches_reference_build_and_uses_q_as_given(orc, dt):
+    """Bit-exact with the reference build; and Q is used as given (no
+    ridge added): Q = D'D + 0.1 I reproduces the D form with lambda2 = 0.1,
+    whose Gram carries that ridge."""
+    if not orc.has_ref():
+        pytest.skip("reference build not present")
+    rng = np.random.default_rng(32)
+    D = W.unit_dict(rng, 32, 96, dt)
+    X = W.sparse_signals(rng, D, 8, 4, 0.05, True, dt)
+    Q = (D.T.astype(np.float64) @ D).astype(dt)
+    q = (D.T.astype(np.float64) @ X).astype(dt)
+    a = orc.lasso_qq(X, Q, q, 0.05)
+    b = orc.ref_lasso_qq(X, Q, q, 0.05)
+    assert csc_equal(a, b)
+    if dt == np.float64:
+        Qs = Q + 0.1 * np.eye(96)
+        c = orc.lasso_qq(X, Qs, q, 0.05).toarray()
+        d = orc.lasso(X, D, 0.05, lambda2=0.1).toarray()
+        np.testing.assert_allclose(c, d, atol=1e-10)
+
+
+def test_reference_ols_is_least_squares_on_the_lasso_support(orc):
+    """ols (decomp/decomp.h:1087-1091) returns, on each signal's final
+    active set A, (D_A'D_A + 1e-10 I)^-1 D_A'x; the active set is the one
+    of the plain path (FP64)."""
+    if not orc.has_ref():
+        pytest.skip("reference build not present")
+    rng = np.random.default_rng(33)
+    D = W.unit_dict(rng, 48, 128, np.float64)
+    X = W.sparse_signals(rng, D, 12, 5, 0.05, True, np.float64)
+    base = orc.lasso(X, D, 0.05).toarray()
+    ols = orc.ref_lasso(X, D, 0.05, ols=True).toarray()
+    assert np.array_equal(base != 0, ols != 0)
+    for j in range(X.shape[1]):
+        S = np.nonzero(ols[:, j])[0]
+        Ds = D[:, S]
+        x = np.linalg.solve(Ds.T @ Ds + 1e-10 * np.eye(len(S)), Ds.T @ X[:, j])
+        np.testing.assert_allclose(ols[S, j], x, rtol=1e-9, atol=1e-12)
+    # a planted error (the lasso coefficients) is far from the fit
+    assert np.abs(base - ols).max() > 1e-3
+
+
+@pytest.mark.parametrize("name", golden_files("omppath_"))
+def test_omp_path_fixture_is_consistent(orc, name):
+    """The reference's OMP path (decomp/decomp.h:605-613, :634-639): the
+    last written column holds the final coefficients, earlier column j holds
+    a (j+1)-atom least-squares fit on the first j+1 selected atoms."""
+    g = load_golden(name)
+    A = orc.omp(g["X"], g["D"], L=int(g["L"]))
+    import scipy.sparse as sp
+    ref = sp.csc_matrix((g["v"], g["r"], g["pB"]), shape=(g["D"].shape[1], g["X"].shape[1]))
+    assert csc_equal(A, ref)
+    path = g["path"].astype(np.float64)
+    a0 = ref.toarray()[:, 0]
+    k = int((a0 != 0).sum())
+    np.testing.assert_array_equal(path[:, k - 1], ref.toarray()[:, 0])
+    D = g["D"].astype(np.float64)
+    x = g["X"][:, 0].astype(np.float64)
+    for j in range(k - 1):
+        S = np.nonzero(path[:, j])[0]
+        assert len(S) == j + 1
+        c = np.linalg.lstsq(D[:, S], x, rcond=None)[0]
+        tol = 1e-9 if g["X"].dtype == np.float64 else 1e-3
+        np.testing.assert_allclose(path[S, j], c, rtol=tol, atol=tol)
+    assert not path[:, k:].any()
