@@ -757,7 +757,12 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           newAtom = true;
           currentInd = nextInd;
         }
-        if ((iter >= a.length_path - 1) || fabs(step) < 1e-15 ||
+        // Stop rules of decomp/decomp.h:1907-1911.  In FP32 a vanishing step
+        // that admits an atom is a tie between two atoms at FP32 resolution
+        // (both were due at the same point of the path): the atom is admitted
+        // and the path goes on, where the rule would end it short of lambda.
+        const bool tie = sizeof(T) == 4 && newAtom;
+        if ((iter >= a.length_path - 1) || (fabs(step) < 1e-15 && !tie) ||
             step == step_max2 || (normX < 1e-15) || (i == (Lm - 1)) ||
             (a.mode == SPX_L2ERROR && normX - lam < 1e-15) ||
             (a.mode == SPX_L1COEFFS && (lam - thrs < 1e-15)))

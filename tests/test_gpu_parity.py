@@ -497,3 +497,20 @@ def test_gram_only_dictionary_is_rejected_where_D_is_needed(spx):
     assert g(ctx.h, h, X.data_ptr(), 8, 4, out.data_ptr(), 8, 0) == 1  # SPX_ERR_ARG
     assert b"Gram" in spx.lib().spx_last_error()
     spx.lib().spx_dict_destroy(h)
+
+
+def test_fp32_objective_on_the_headline_batch(spx, orc):
+    """Every signal of a 3000-signal slice of the benchmark batch ends within
+    1e-5 (relative) of the FP64 optimum, in the batched form (LDS pass, hand
+    over, full pass) and the plain form.  The slice holds a pair of atoms
+    tied at FP32 resolution (signal 996) whose vanishing admission step used
+    to end that path early."""
+    print("case fp32 objective headline slice", flush=True)
+    X, D, lam = W.lars_headline(n=200000)
+    Xs, ls = np.asfortranarray(X[:, :3000]), lam[:3000]
+    ref = dense(orc.lasso(Xs.astype(np.float64), D.astype(np.float64), ls.astype(np.float64)))
+    fref = lasso_obj(Xs, D, ref, ls)
+    for form in ("batched", "plain"):
+        A = dense(spx.lasso(Xs, D, lambda1=ls, form=form)).astype(np.float64)
+        gap = (lasso_obj(Xs, D, A, ls) - fref) / np.abs(fref)
+        assert np.abs(gap).max() <= 1e-5, (form, int(np.argmax(np.abs(gap))), float(np.abs(gap).max()))
