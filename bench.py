@@ -155,6 +155,14 @@ def served_report(n_jobs=120, rate=150.0, max_n=20000, lanes=4):
         done = srv.run_trace(fresh, realtime=True)
         rep = server.latency_report(done)
         rep["batches"] = srv.n_batches
+        # capacity: the same jobs all queued at once (continuous batching at
+        # saturation); throughput = signals / makespan
+        nb0 = srv.n_batches
+        sat = [server.Job(0.0, j.dict_id, j.kind, j.X, j.param) for j in jobs]
+        sat_done = srv.run_trace(sat, realtime=False)
+        srep = server.latency_report(sat_done)
+        rep["saturated"] = {"throughput_signals_per_s": srep["throughput_signals_per_s"],
+                            "makespan_s": srep["makespan_s"], "batches": srv.n_batches - nb0}
         rep["autotune"] = {f"dict{k[0]}/{k[1]}": v[0] for k, v in tuned.items()}
     finally:
         srv.close()

@@ -25,6 +25,21 @@ namespace {
 
 constexpr int BM = 128, BN = 128;
 
+// Output tile of this block.  Blocks are dispatched round-robin over the 8
+// XCDs (block b on XCD b % 8), each with its own L2; the remap gives each XCD
+// a contiguous run of tiles, row tiles fastest, so the row tiles that share
+// one column tile of B (a panel of signals) read it into the same L2.
+__device__ __forceinline__ void tile_origin(int& bm, int& bn) {
+  constexpr int kXcd = 8;
+  const int mt = gridDim.x;
+  const int total = mt * gridDim.y;
+  int b = blockIdx.x + blockIdx.y * mt;
+  const int full = total - total % kXcd;
+  if (b < full) b = (b % kXcd) * (full / kXcd) + b / kXcd;
+  bm = (b % mt) * BM;
+  bn = (b / mt) * BN;
+}
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -106,7 +121,8 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(
   const int lane = tid & 63;
   const int wid = tid >> 6;
   const int wm = (wid & 1) * 64, wn = (wid >> 1) * 64;
-  const int bm = blockIdx.x * BM, bn = blockIdx.y * BN;
+  int bm, bn;
+  tile_origin(bm, bn);
 
   V acc[4][4];
 #pragma unroll
@@ -193,7 +209,8 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
   const int lane = tid & 63;
   const int wid = tid >> 6;
   const int wm = (wid & 1) * 64, wn = (wid >> 1) * 64;
-  const int bm = blockIdx.x * BM, bn = blockIdx.y * BN;
+  int bm, bn;
+  tile_origin(bm, bn);
 
   f32x4 acc[4][4];
 #pragma unroll
