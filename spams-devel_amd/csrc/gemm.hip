@@ -73,6 +73,27 @@ __device__ __forceinline__ void store_block(const V& acc, int i0, int j0,
   const int lane = threadIdx.x & 63;
   const int j = j0 + (lane & 15);
   if (j >= N) return;
+  if constexpr (sizeof(T) == 4) {
+    // FP32: the lane's four rows are consecutive; one 16-byte store when
+    // they are in range and aligned
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int ib = i0 + 4 * (lane >> 4);
+    T* cp = C + (size_t)ib + (size_t)j * ldc;
+    const T* ep = E ? E + (size_t)ib + (size_t)j * lde : nullptr;
+    if (ib + 3 < M && (reinterpret_cast<uintptr_t>(cp) & 15) == 0 &&
+        (!ep || (reinterpret_cast<uintptr_t>(ep) & 15) == 0)) {
+      f4 v;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[r] = alpha * static_cast<T>(acc[r]);
+      if (ep) {
+        const f4 e = *reinterpret_cast<const f4*>(ep);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] += beta * e[r];
+      }
+      *reinterpret_cast<f4*>(cp) = v;
+      return;
+    }
+  }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int i = sizeof(T) == 8 ? i0 + (lane >> 4) + 4 * r : i0 + 4 * (lane >> 4) + r;
@@ -86,10 +107,11 @@ __device__ __forceinline__ void store_block(const V& acc, int i0, int j0,
 
 // ----------------------------------------------------------- FP32 / FP64
 //
-// K is staged in chunks of BK = 16.  Within a chunk, MFMA step t (0..3) of
-// lane l uses k = 4*(l/16) + t, so a lane's four k values are contiguous and
-// are read from LDS as one 16-byte (FP32) / two 16-byte (FP64) loads; the
-// four steps together cover the 16 k of the chunk.  LDS tiles are [row][k]
+// K is staged in chunks of BK (32 in FP32, 16 in FP64), each taken as groups
+// of 16.  Within a group, MFMA step t (0..3) of lane l uses k = 4*(l/16) + t,
+// so a lane's four k values are contiguous and are read from LDS as one
+// 16-byte (FP32) / two 16-byte (FP64) loads; the four steps together cover the
+// 16 k of the group.  LDS tiles are [row][k]
 // with a row stride of BK + 4 elements.  The next chunk is loaded into
 // registers while the current one is multiplied (one LDS buffer, register
 // double buffering).
@@ -105,13 +127,18 @@ struct Vec4<double> {
   typedef double type __attribute__((ext_vector_type(4)));
 };
 
+// FP32: held to 3 workgroups per CU (the registers of the 32-deep chunk
+// would allow 2).
 template <typename T, bool VECLOAD>
-__global__ __launch_bounds__(256) void gemm_tn_kernel(
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void gemm_tn_kernel(
     int M, int N, int K, const T* __restrict__ A, int lda,
     const T* __restrict__ B, int ldb, T* __restrict__ C, int ldc, T alpha,
     T beta, const T* __restrict__ E, int lde) {
-  constexpr int BK = 16;
+  // K chunk per barrier pair: 32 in FP32 (16 KiB of operands per chunk),
+  // 16 in FP64 (same bytes)
+  constexpr int BK = sizeof(T) == 4 ? 32 : 16;
   constexpr int LD = BK + 4;  // LDS row stride (elements)
+  constexpr int NV = BK / 16;  // float4 (double4) loads per row half and operand
   typedef typename Vec4<T>::type V4;
   __shared__ __attribute__((aligned(16))) T As[BM * LD];
   __shared__ __attribute__((aligned(16))) T Bs[BN * LD];
@@ -132,31 +159,36 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
 
-  // loader: thread t loads rows (t >> 2) and (t >> 2) + 64, k quad (t & 3)
+  // loader: thread t loads rows (t >> 2) and (t >> 2) + 64, k quads
+  // (t & 3) + 4 v for v < NV
   const int lr = tid >> 2, lq = (tid & 3) * 4;
-  V4 ra[2], rb[2];
+  V4 ra[2][NV], rb[2][NV];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int i = bm + lr + 64 * h, j = bn + lr + 64 * h, k = k0 + lq;
-      if (VECLOAD && k + 3 < K) {
-        ra[h] = i < M ? *reinterpret_cast<const V4*>(A + (size_t)i * lda + k) : V4(0);
-        rb[h] = j < N ? *reinterpret_cast<const V4*>(B + (size_t)j * ldb + k) : V4(0);
-      } else {
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          ra[h][e] = (i < M && k + e < K) ? A[(size_t)i * lda + k + e] : T(0);
-          rb[h][e] = (j < N && k + e < K) ? B[(size_t)j * ldb + k + e] : T(0);
+      for (int v = 0; v < NV; ++v) {
+        const int i = bm + lr + 64 * h, j = bn + lr + 64 * h, k = k0 + lq + 16 * v;
+        if (VECLOAD && k + 3 < K) {
+          ra[h][v] = i < M ? *reinterpret_cast<const V4*>(A + (size_t)i * lda + k) : V4(0);
+          rb[h][v] = j < N ? *reinterpret_cast<const V4*>(B + (size_t)j * ldb + k) : V4(0);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            ra[h][v][e] = (i < M && k + e < K) ? A[(size_t)i * lda + k + e] : T(0);
+            rb[h][v][e] = (j < N && k + e < K) ? B[(size_t)j * ldb + k + e] : T(0);
+          }
         }
       }
-    }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      *reinterpret_cast<V4*>(&As[(lr + 64 * h) * LD + lq]) = ra[h];
-      *reinterpret_cast<V4*>(&Bs[(lr + 64 * h) * LD + lq]) = rb[h];
-    }
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        *reinterpret_cast<V4*>(&As[(lr + 64 * h) * LD + lq + 16 * v]) = ra[h][v];
+        *reinterpret_cast<V4*>(&Bs[(lr + 64 * h) * LD + lq + 16 * v]) = rb[h][v];
+      }
   };
 
   load(0);
@@ -166,18 +198,21 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(
     store();
     __syncthreads();
     if (k0 + BK < K) load(k0 + BK);
-    V4 a4[4], b4[4];
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      a4[x] = *reinterpret_cast<const V4*>(&As[(wm + 16 * x + (lane & 15)) * LD + kq]);
-      b4[x] = *reinterpret_cast<const V4*>(&Bs[(wn + 16 * x + (lane & 15)) * LD + kq]);
+    for (int v = 0; v < NV; ++v) {
+      V4 a4[4], b4[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        a4[x] = *reinterpret_cast<const V4*>(&As[(wm + 16 * x + (lane & 15)) * LD + kq + 16 * v]);
+        b4[x] = *reinterpret_cast<const V4*>(&Bs[(wn + 16 * x + (lane & 15)) * LD + kq + 16 * v]);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) acc[x][y] = Acc<T>::mfma(a4[x][t], b4[y][t], acc[x][y]);
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = Acc<T>::mfma(a4[x][t], b4[y][t], acc[x][y]);
   }
 #pragma unroll
   for (int x = 0; x < 4; ++x)
