@@ -263,6 +263,26 @@ def run_lars(args):
     if not args.no_precision:
         log("bench: precision report")
         prec = precision_report(X, D, lam)
+        # the same batched step with D'X and G as BF16x3 products
+        db = spx.Dictionary(E.ctx, D, precision="bf16x3")
+        for _ in range(2):
+            spx.dtx_dev(E.ctx, db, Xt, DtX, precision="bf16x3")
+        torch.cuda.synchronize()
+        ev = E.events(3)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ev[0].record(E.stream)
+            spx.dtx_dev(E.ctx, db, Xt, DtX, precision="bf16x3")
+            spx.sqnorms_dev(E.ctx, Xt, norms)
+            ev[1].record(E.stream)
+            spx.lasso_dtx_dev(E.ctx, db, DtX, norms, lam_t, V, R, cnt, form="batched",
+                              waves_per_signal=waves_used, stats=stats)
+            ev[2].record(E.stream)
+        torch.cuda.synchronize()
+        wb = (time.perf_counter() - t0) / args.steps
+        prec["bf16x3_batched_step"] = {"signals_per_s": n / wb, "ms_per_step": wb * 1e3,
+                                       "dtx_and_norms_ms": ev[0].elapsed_time(ev[1])}
+        db.close()
 
     served = None
     if not args.no_served:
