@@ -5,6 +5,8 @@
 // error conventions, and run on the GPU through include/spx.h.
 //
 //   _lassoD     cpp_library/spams.h:107-143   (lasso2 / coreLARS2 homotopy)
+//   _lassoQq    cpp_library/spams.h:145-188   (Gram matrix given)
+//   _lassoWeighted  cpp_library/spams.h:217-249  (lassoWeight / coreLARS2W)
 //   _omp        cpp_library/spams.h:250-293   (omp / coreORMP)
 //   _fistaFlat  cpp_library/spams.h:460-608   (square loss; l1, elastic-net)
 //   lasso2      decomp/decomp.h:126-135        omp  decomp/decomp.h:53-57
@@ -178,6 +180,16 @@ inline int lasso_qq_call(const double* X, int m, int n, const double* Q, const d
                          spx_csc* out, double* path) {
   return spx_lasso_qq_f64(context(), X, m, n, Q, q, p, lam, 1, l2, o, out, path);
 }
+inline int lasso_weighted_call(const float* X, int m, int n, const float* D, int p,
+                               const float* W, const float* lam, const spx_lasso_opts* o,
+                               spx_csc* out) {
+  return spx_lasso_weighted_f32(context(), X, m, n, D, p, W, lam, 1, o, out);
+}
+inline int lasso_weighted_call(const double* X, int m, int n, const double* D, int p,
+                               const double* W, const double* lam, const spx_lasso_opts* o,
+                               spx_csc* out) {
+  return spx_lasso_weighted_f64(context(), X, m, n, D, p, W, lam, 1, o, out);
+}
 inline int omp_call(const float* X, int m, int n, const float* D, int p, const int* L, int nL,
                     const float* e, int ne, const float* la, int nla, spx_csc* out,
                     float* path) {
@@ -288,6 +300,31 @@ SpMatrix<T>* _lassoQq(Matrix<T>* X, Matrix<T>* Q, Matrix<T>* q, Matrix<T>** path
     delete alpha;
     delete *path;
     *path = nullptr;
+    throw;
+  }
+  return alpha;
+}
+
+// _lassoWeighted (cpp_library/spams.h:217-249): signal i's penalty is
+// constraint * sum_j W(j, i) |alpha_j|, W p x n.
+template <typename T>
+SpMatrix<T>* _lassoWeighted(Matrix<T>* X, Matrix<T>* D, Matrix<T>* W, int L, const T constraint,
+                            constraint_type mode, const bool pos, const int numThreads,
+                            bool verbose) {
+  (void)verbose;
+  (void)numThreads;
+  if (X->m() != D->m()) throw "lassoWeighted : incompatible matrix dimensions";
+  if (W->m() != D->n() || W->n() != X->n())
+    throw "lassoWeighted : inconsistent dimensions of matrix W";
+  SpMatrix<T>* alpha = new SpMatrix<T>();
+  try {
+    spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, -1, SPX_PREC_FP32, SPX_FORM_BATCHED, 0, 0};
+    spx_csc out{};
+    spams_amd_detail::check(spams_amd_detail::lasso_weighted_call(
+        X->rawX(), X->m(), X->n(), D->rawX(), D->n(), W->rawX(), &constraint, &o, &out));
+    alpha->adopt(out);
+  } catch (...) {
+    delete alpha;
     throw;
   }
   return alpha;

@@ -241,6 +241,21 @@ int spx_lasso_path_f64(spx_ctx* ctx, const double* X, int m, int n,
 int spx_lasso_f64(spx_ctx* ctx, const double* X, int m, int n,
                   const double* D, int p, const double* lambda, int n_lambda,
                   double lambda2, const spx_lasso_opts* opts, spx_csc* out);
+/* Weighted LARS: replaces _lassoWeighted (cpp_library/spams.h:217-249), i.e.
+ * lassoWeight + coreLARS2W (decomp/decomp.h:1315-1391, :1917-2107): signal i
+ * minimises 0.5 ||x_i - D a||^2 + lambda sum_j W(j, i) |a_j| (PENALTY; the
+ * other modes bound the weighted l1 norm / the residual).  W: host p x n,
+ * positive.  Gram ridge 1e-10, stop tolerance 1e-10, at most 4 min(L, p) + 1
+ * homotopy steps; opts->max_length_path and opts->ols do not apply (ols and
+ * SPX_PREC_REFIT64 are rejected). */
+int spx_lasso_weighted_f32(spx_ctx* ctx, const float* X, int m, int n,
+                           const float* D, int p, const float* W,
+                           const float* lambda, int n_lambda,
+                           const spx_lasso_opts* opts, spx_csc* out);
+int spx_lasso_weighted_f64(spx_ctx* ctx, const double* X, int m, int n,
+                           const double* D, int p, const double* W,
+                           const double* lambda, int n_lambda,
+                           const spx_lasso_opts* opts, spx_csc* out);
 
 /* ORMP: replaces _omp (cpp_library/spams.h:250-293).  L/eps/lambda follow the
  * facade: pointer NULL or size 0 = not given, size 1 = scalar, size n = per
@@ -313,6 +328,18 @@ int spx_lasso_dev_f64(spx_ctx* ctx, const spx_dict* dict, const double* X,
                       int ldx, int n, const double* lambda, int n_lambda,
                       double lambda2, const spx_lasso_opts* opts,
                       spx_lists_dev* out);
+
+/* Device weighted LARS (spx_lasso_weighted_* on a resident dictionary): W is
+ * a device p x n weight matrix with leading dimension ldw >= p. */
+int spx_lasso_weighted_dev_f32(spx_ctx* ctx, const spx_dict* dict,
+                               const float* X, int ldx, int n, const float* W,
+                               int ldw, const float* lambda, int n_lambda,
+                               const spx_lasso_opts* opts, spx_lists_dev* out);
+int spx_lasso_weighted_dev_f64(spx_ctx* ctx, const spx_dict* dict,
+                               const double* X, int ldx, int n,
+                               const double* W, int ldw, const double* lambda,
+                               int n_lambda, const spx_lasso_opts* opts,
+                               spx_lists_dev* out);
 
 /* Device LARS from precomputed correlations (the Gram-given form of the
  * reference, lasso2(X, G, DtX, ...) decomp/decomp.h:1574-1646 and _lassoQq

@@ -166,6 +166,35 @@ def lasso_qq(X, Q, q, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
     return _csc_from_lists(vM, rM, Le, n, p, dtype)
 
 
+def lasso_weighted(X, D, W, lambda1, L=-1, mode=PENALTY, pos=False, num_threads=-1):
+    """_lassoWeighted + lassoWeight + coreLARS2W (restated).  W: p x n
+    weights; ``lambda1`` scalar or length-n vector."""
+    dtype = X.dtype
+    X = _f(X, dtype)
+    D = _f(D, dtype)
+    W = _f(W, dtype)
+    m, n = X.shape
+    p = D.shape[1]
+    assert W.shape == (p, n)
+    lam = np.atleast_1d(np.asarray(lambda1, dtype=dtype))
+    sfx = _sfx(dtype)
+    ct = _ct(dtype)
+    fL = getattr(lib(), "orc_lasso_L_" + sfx)
+    fL.restype = _I
+    fL.argtypes = [_I, _I, _I, _I, ct, _I, ct]
+    Le = fL(m, p, L, mode, lam[0], int(pos), 0.0)
+    if Le <= 0:
+        return sp.csc_matrix((p, n), dtype=dtype)
+    vM = np.zeros((Le, n), dtype, order="F")
+    rM = np.full((Le, n), -1, np.int32, order="F")
+    f = getattr(lib(), "orc_lasso_weighted_" + sfx)
+    f.restype = _I
+    f.argtypes = [_P, _I, _I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _P, _P]
+    f(_ptr(X), m, n, _ptr(D), p, _ptr(W), L, _ptr(lam), lam.size, mode, int(pos),
+      num_threads, _ptr(vM), _ptr(rM))
+    return _csc_from_lists(vM, rM, Le, n, p, dtype)
+
+
 def omp(X, D, L=None, eps=None, lambda1=None, num_threads=-1):
     """_omp + omp + coreORMP (restated).  L/eps/lambda1: None (not given),
     scalar, or length-n vector."""
@@ -335,6 +364,71 @@ def ref_lasso_pervec(X, D, lambdas, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
     f.argtypes = [_P, _I, _I, _P, _I, _I, _P, ct, _I, _I, _I, _I]
     h = f(_ptr(X), m, n, _ptr(D), p, L, _ptr(lam), lambda2, mode, int(pos),
           num_threads, max_length_path)
+    return _ref_sp(h, n, p, dtype)
+
+
+def ref_lasso_weighted(X, D, W, lambda1, L=-1, mode=PENALTY, pos=False, num_threads=-1):
+    """The reference _lassoWeighted (cpp_library/spams.h:217-249), compiled in
+    place."""
+    dtype = X.dtype
+    X = _f(X, dtype)
+    D = _f(D, dtype)
+    W = _f(W, dtype)
+    m, n = X.shape
+    p = D.shape[1]
+    ct = _ct(dtype)
+    f = getattr(ref(), "ref_lasso_weighted_" + _sfx(dtype))
+    f.restype = _P
+    f.argtypes = [_P, _I, _I, _P, _I, _P, _I, ct, _I, _I, _I]
+    h = f(_ptr(X), m, n, _ptr(D), p, _ptr(W), L, lambda1, mode, int(pos), num_threads)
+    return _ref_sp(h, n, p, dtype)
+
+
+def ref_lasso_mask(X, D, B, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
+                   num_threads=-1):
+    """The reference _lassoMask (cpp_library/spams.h:190-215), compiled in
+    place.  B: m x n mask (nonzero = observed)."""
+    dtype = X.dtype
+    X = _f(X, dtype)
+    D = _f(D, dtype)
+    Bm = np.asfortranarray(B != 0, dtype=np.uint8)
+    m, n = X.shape
+    p = D.shape[1]
+    ct = _ct(dtype)
+    f = getattr(ref(), "ref_lasso_mask_" + _sfx(dtype))
+    f.restype = _P
+    f.argtypes = [_P, _I, _I, _P, _I, _P, _I, ct, ct, _I, _I, _I]
+    h = f(_ptr(X), m, n, _ptr(D), p, _ptr(Bm), L, lambda1, lambda2, mode, int(pos),
+          num_threads)
+    return _ref_sp(h, n, p, dtype)
+
+
+def ref_omp_mask(X, D, B, L=None, eps=None, lambda1=None, num_threads=-1,
+                 return_reg_path=False):
+    """The reference _ompMask (cpp_library/spams.h:295-343), compiled in place."""
+    dtype = X.dtype
+    X = _f(X, dtype)
+    D = _f(D, dtype)
+    Bm = np.asfortranarray(B != 0, dtype=np.uint8)
+    m, n = X.shape
+    p = D.shape[1]
+
+    def vec(a, dt):
+        if a is None:
+            return np.zeros(1, dt), 0
+        a = np.atleast_1d(np.asarray(a, dtype=dt))
+        return a, a.size
+
+    Ls, nL = vec(L, np.int32)
+    E, nE = vec(eps, dtype)
+    La, nLa = vec(lambda1, dtype)
+    f = getattr(ref(), "ref_omp_mask_" + _sfx(dtype))
+    f.restype = _P
+    f.argtypes = [_P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I, _I, _I]
+    h = f(_ptr(X), m, n, _ptr(D), p, _ptr(Bm), _ptr(Ls), nL, _ptr(E), nE, _ptr(La),
+          nLa, num_threads, int(return_reg_path))
+    if return_reg_path:
+        return _ref_sp(h, n, p, dtype, path_shape=(p, min(m, p)))
     return _ref_sp(h, n, p, dtype)
 
 

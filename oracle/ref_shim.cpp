@@ -12,6 +12,8 @@
 //   _omp        cpp_library/spams.h:250-293
 //   _fistaFlat  cpp_library/spams.h:460-608
 //   _lassoQq    cpp_library/spams.h:145-188
+//   _lassoMask / _lassoWeighted   cpp_library/spams.h:190-249
+//   _ompMask    cpp_library/spams.h:295-343
 // plus one per-signal-lambda LARS loop (ref_lasso_pervec_*) that mirrors the
 // OpenMP loop of lasso2 (decomp/decomp.h:1574-1646) with the constraint taken
 // per signal, calling the reference coreLARS2 (decomp/decomp.h:1706-1915).
@@ -62,6 +64,73 @@ void* lasso_qq_impl(const T* X, int m, int n, const T* Q, const T* q, int p, int
                      max_length_path, false, cholesky != 0);
   } catch (const char* msg) {
     fprintf(stderr, "reference lassoQq: %s\n", msg);
+    return nullptr;
+  }
+  return new SpOut<T>{sp, path};
+}
+
+template <typename T>
+void* lasso_weighted_impl(const T* X, int m, int n, const T* D, int p, const T* W, int L,
+                          T lambda1, int mode, int pos, int num_threads) {
+  Matrix<T> Xm(const_cast<T*>(X), m, n);
+  Matrix<T> Dm(const_cast<T*>(D), m, p);
+  Matrix<T> Wm(const_cast<T*>(W), p, n);
+  SpMatrix<T>* sp = nullptr;
+  try {
+    sp = _lassoWeighted<T>(&Xm, &Dm, &Wm, L, lambda1, static_cast<constraint_type>(mode),
+                           pos != 0, num_threads, false);
+  } catch (const char* msg) {
+    fprintf(stderr, "reference lassoWeighted: %s\n", msg);
+    return nullptr;
+  }
+  return new SpOut<T>{sp, nullptr};
+}
+
+// masks arrive as bytes (0 / nonzero), m x n column-major
+struct BoolMat {
+  bool* b;
+  BoolMat(const unsigned char* B, size_t cnt) : b(new bool[cnt ? cnt : 1]) {
+    for (size_t k = 0; k < cnt; ++k) b[k] = B[k] != 0;
+  }
+  ~BoolMat() { delete[] b; }
+};
+
+template <typename T>
+void* lasso_mask_impl(const T* X, int m, int n, const T* D, int p, const unsigned char* B,
+                      int L, T lambda1, T lambda2, int mode, int pos, int num_threads) {
+  Matrix<T> Xm(const_cast<T*>(X), m, n);
+  Matrix<T> Dm(const_cast<T*>(D), m, p);
+  BoolMat bm(B, (size_t)m * n);
+  Matrix<bool> Bm(bm.b, m, n);
+  SpMatrix<T>* sp = nullptr;
+  try {
+    sp = _lassoMask<T>(&Xm, &Dm, &Bm, L, lambda1, lambda2, static_cast<constraint_type>(mode),
+                       pos != 0, num_threads, false);
+  } catch (const char* msg) {
+    fprintf(stderr, "reference lassoMask: %s\n", msg);
+    return nullptr;
+  }
+  return new SpOut<T>{sp, nullptr};
+}
+
+template <typename T>
+void* omp_mask_impl(const T* X, int m, int n, const T* D, int p, const unsigned char* B,
+                    const int* L, int nL, const T* eps, int nEps, const T* lambda, int nLambda,
+                    int num_threads, int return_path) {
+  Matrix<T> Xm(const_cast<T*>(X), m, n);
+  Matrix<T> Dm(const_cast<T*>(D), m, p);
+  BoolMat bm(B, (size_t)m * n);
+  Matrix<bool> Bm(bm.b, m, n);
+  Vector<int> vL(const_cast<int*>(L), nL > 0 ? nL : 1);
+  Vector<T> vE(const_cast<T*>(eps), nEps > 0 ? nEps : 1);
+  Vector<T> vLa(const_cast<T*>(lambda), nLambda > 0 ? nLambda : 1);
+  Matrix<T>* path = nullptr;
+  SpMatrix<T>* sp = nullptr;
+  try {
+    sp = _ompMask<T>(&Xm, &Dm, &Bm, &path, return_path != 0, nL > 0, &vL, nEps > 0, &vE,
+                     nLambda > 0, &vLa, num_threads);
+  } catch (const char* msg) {
+    fprintf(stderr, "reference ompMask: %s\n", msg);
     return nullptr;
   }
   return new SpOut<T>{sp, path};
@@ -236,6 +305,25 @@ extern "C" {
                                int max_length_path) {                          \
     return lasso_pervec_impl<T>(X, m, n, D, p, L, lambdas, lambda2, mode, pos, \
                                 num_threads, max_length_path);                 \
+  }                                                                            \
+  void* ref_lasso_weighted_##SUF(const T* X, int m, int n, const T* D, int p, \
+                                 const T* W, int L, T lambda1, int mode,       \
+                                 int pos, int num_threads) {                   \
+    return lasso_weighted_impl<T>(X, m, n, D, p, W, L, lambda1, mode, pos,     \
+                                  num_threads);                                \
+  }                                                                            \
+  void* ref_lasso_mask_##SUF(const T* X, int m, int n, const T* D, int p,      \
+                             const unsigned char* B, int L, T lambda1,         \
+                             T lambda2, int mode, int pos, int num_threads) {  \
+    return lasso_mask_impl<T>(X, m, n, D, p, B, L, lambda1, lambda2, mode,     \
+                              pos, num_threads);                               \
+  }                                                                            \
+  void* ref_omp_mask_##SUF(const T* X, int m, int n, const T* D, int p,        \
+                           const unsigned char* B, const int* L, int nL,       \
+                           const T* eps, int nEps, const T* lambda,            \
+                           int nLambda, int num_threads, int return_path) {    \
+    return omp_mask_impl<T>(X, m, n, D, p, B, L, nL, eps, nEps, lambda,        \
+                            nLambda, num_threads, return_path);                \
   }                                                                            \
   int ref_sp_nnz_##SUF(void* h) { return sp_nnz<T>(h); }                       \
   int ref_sp_n_##SUF(void* h) {                                                \

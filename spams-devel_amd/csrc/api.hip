@@ -269,10 +269,13 @@ static int products(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
   return col_sqnorms<T>(ctx->stream, d->m, n, X, ldx, *norms);
 }
 
+// W (device, p x n, leading dimension ldw) selects the weighted penalty of
+// lassoWeight / coreLARS2W (decomp/decomp.h:1315-1391, :1917-2107).
 template <typename T>
 static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
                          const T* norms, int n, const T* lambda, int nlam, T lambda2,
-                         const spx_lasso_opts* o, spx_lists_dev* out) {
+                         const spx_lasso_opts* o, spx_lists_dev* out,
+                         const T* W = nullptr, int ldw = 0) {
   if (!ctx || !d || !o || !out || n < 0 || (n > 0 && (!DtX || !norms || !lambda)) ||
       ldd < d->p || !(nlam == 1 || nlam == n)) {
     set_error("lasso_dev: invalid arguments");
@@ -290,9 +293,15 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
     set_error("lasso_dev: lists have L=%d", out->L);
     return SPX_ERR_ARG;
   }
+  if (W && (ldw < d->p || o->ols || (o->precision & SPX_PREC_REFIT64) ||
+            (out->path && out->path_cols > 0))) {
+    set_error("lassoWeighted: invalid arguments (ldw < p, or ols / refit / path requested)");
+    return SPX_ERR_ARG;
+  }
   if (n == 0) return SPX_OK;
   DeviceGuard g(ctx->device);
   LarsArgs<T> a{};
+  a.tol = T(1e-15);
   a.G = static_cast<const T*>(d->G);
   a.ldg = d->ldg;
   // G + max(lambda2, 1e-10) I (decomp/decomp.h:1566); a Gram given as such
@@ -323,6 +332,15 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
     a.path_cols = out->path_cols;
     SPX_HIP(hipMemsetAsync(out->path, 0, sizeof(T) * (size_t)d->p * out->path_cols, ctx->stream));
   }
+  if (W) {
+    // lassoWeight: G + 1e-10 I (decomp/decomp.h:1337), tolerance 1e-10 and
+    // the step cap iter > 4 min(L, p) (decomp/decomp.h:2099)
+    a.weights = W;
+    a.ldw = ldw;
+    a.diag = d->gram_only ? T(0) : T(1e-10);
+    a.tol = T(1e-10);
+    a.length_path = 4 * (a.L < a.p ? a.L : a.p) + 2;
+  }
   const int w = o->waves_per_signal > 0 ? o->waves_per_signal
                 : o->form == SPX_FORM_PLAIN ? 4 : tuned_waves(d, 0);
   return lars_solve<T>(ctx, a, o->form, w);
@@ -331,7 +349,8 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
 template <typename T>
 static int lasso_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx,
                      int n, const T* lambda, int nlam, T lambda2,
-                     const spx_lasso_opts* o, spx_lists_dev* out) {
+                     const spx_lasso_opts* o, spx_lists_dev* out,
+                     const T* W = nullptr, int ldw = 0) {
   if (!ctx || !d || !o || !out || n < 0 || (n > 0 && (!X || !lambda)) ||
       ldx < d->m || !(nlam == 1 || nlam == n)) {
     set_error("lasso_dev: invalid arguments");
@@ -347,7 +366,7 @@ static int lasso_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx,
   T* norms = nullptr;
   int rc = products<T>(ctx, d, X, ldx, n, o->precision & SPX_PREC_BF16X3, &DtX, &norms);
   if (rc) return rc;
-  return lasso_dtx_dev<T>(ctx, d, DtX, d->ldg, norms, n, lambda, nlam, lambda2, o, out);
+  return lasso_dtx_dev<T>(ctx, d, DtX, d->ldg, norms, n, lambda, nlam, lambda2, o, out, W, ldw);
 }
 
 template <typename T>
@@ -638,10 +657,12 @@ struct DevBuf {
   }
 };
 
+// W: host p x n weights of the weighted penalty (_lassoWeighted), or null.
 template <typename T>
 static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
                       const T* lambda, int nlam, T lambda2,
-                      const spx_lasso_opts* opts, spx_csc* out, T* path = nullptr) {
+                      const spx_lasso_opts* opts, spx_csc* out, T* path = nullptr,
+                      const T* W = nullptr) {
   if (!ctx || !out || !D || !lambda || m <= 0 || p <= 0 || n < 0 || (n > 0 && !X)) {
     set_error("lasso : invalid arguments");
     return SPX_ERR_ARG;
@@ -672,8 +693,9 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   spx_dict* d = nullptr;
   int rc = dict_create<T>(ctx, D, m, p, 0, o.precision & SPX_PREC_BF16X3, &d);
   if (rc) return rc;
-  DevBuf bX, bL, bV, bR, bC, bP;
+  DevBuf bX, bL, bV, bR, bC, bP, bW;
   if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bL.alloc(sizeof(T) * nlam)) ||
+      (W && (rc = bW.alloc(sizeof(T) * (size_t)p * n))) ||
       (rc = bV.alloc(sizeof(T) * (size_t)Le * n)) ||
       (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) || (rc = bC.alloc(sizeof(int) * (size_t)n)) ||
       (path && pcols > 0 && (rc = bP.alloc(sizeof(T) * (size_t)p * pcols)))) {
@@ -682,7 +704,9 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   }
   hipStream_t st = ctx->stream;
   if (hipMemcpyAsync(bX.p, X, sizeof(T) * (size_t)m * n, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(bL.p, lambda, sizeof(T) * nlam, hipMemcpyHostToDevice, st) != hipSuccess) {
+      hipMemcpyAsync(bL.p, lambda, sizeof(T) * nlam, hipMemcpyHostToDevice, st) != hipSuccess ||
+      (W && hipMemcpyAsync(bW.p, W, sizeof(T) * (size_t)p * n, hipMemcpyHostToDevice, st) !=
+                hipSuccess)) {
     spx_dict_destroy(d);
     set_error("lasso: upload failed");
     return SPX_ERR_HIP;
@@ -693,7 +717,8 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
     l.path_cols = pcols;
   }
   rc = lasso_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
-                    static_cast<const T*>(bL.p), nlam, lambda2, &o, &l);
+                    static_cast<const T*>(bL.p), nlam, lambda2, &o, &l,
+                    static_cast<const T*>(bW.p), p);
   if (!rc) rc = lists_to_host_csc<T>(ctx, l, p, n, out);
   if (!rc && path && pcols > 0 &&
       hipMemcpy(path, bP.p, sizeof(T) * (size_t)p * pcols, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -1196,6 +1221,24 @@ int spx_dict_create_gram_f64(spx_ctx* ctx, const double* Q, int m, int p, int on
                              spx_dict** out) {
   return spx::dict_create_gram<double>(ctx, Q, m, p, on_device, out);
 }
+int spx_lasso_weighted_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
+                           const float* W, const float* lambda, int n_lambda,
+                           const spx_lasso_opts* opts, spx_csc* out) {
+  if (!W) {
+    spx::set_error("lassoWeighted: W is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::lasso_host<float>(ctx, X, m, n, D, p, lambda, n_lambda, 0.f, opts, out, nullptr, W);
+}
+int spx_lasso_weighted_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, int p,
+                           const double* W, const double* lambda, int n_lambda,
+                           const spx_lasso_opts* opts, spx_csc* out) {
+  if (!W) {
+    spx::set_error("lassoWeighted: W is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::lasso_host<double>(ctx, X, m, n, D, p, lambda, n_lambda, 0.0, opts, out, nullptr, W);
+}
 int spx_lasso_path_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
                        const float* lambda, int n_lambda, float lambda2,
                        const spx_lasso_opts* opts, spx_csc* out, float* path) {
@@ -1261,6 +1304,24 @@ int spx_lasso_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx,
                       const double* lambda, int n_lambda, double lambda2,
                       const spx_lasso_opts* o, spx_lists_dev* out) {
   return spx::lasso_dev<double>(ctx, d, X, ldx, n, lambda, n_lambda, lambda2, o, out);
+}
+int spx_lasso_weighted_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, int n,
+                               const float* W, int ldw, const float* lambda, int n_lambda,
+                               const spx_lasso_opts* o, spx_lists_dev* out) {
+  if (!W) {
+    spx::set_error("lassoWeighted: W is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::lasso_dev<float>(ctx, d, X, ldx, n, lambda, n_lambda, 0.f, o, out, W, ldw);
+}
+int spx_lasso_weighted_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx, int n,
+                               const double* W, int ldw, const double* lambda, int n_lambda,
+                               const spx_lasso_opts* o, spx_lists_dev* out) {
+  if (!W) {
+    spx::set_error("lassoWeighted: W is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::lasso_dev<double>(ctx, d, X, ldx, n, lambda, n_lambda, 0.0, o, out, W, ldw);
 }
 
 int spx_lasso_dtx_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* DtX, int ldd,

@@ -1,984 +1,17 @@
-// LARS-Lasso solver kernel: coreLARS2 semantics (decomp/decomp.h:1706-1915).
-//
-// One group of W wavefronts (one workgroup) solves one signal at a time:
-//   - the correlation vector DtR (p) lives in LDS, the active-atom flags in a
-//     bit mask of the thread that owns the rows;
-//   - the correlation change A = (G + diag I)(:, active) * u is accumulated
-//     in registers (each thread owns NQ chunks of VEC = 16 bytes of rows),
-//     streaming the active Gram columns from the resident G (L2 / MALL),
-//     several columns in flight per round;
-//   - the explicit inverse of the active Gram block, invGs (s x s, stored full
-//     and exactly symmetric), is double buffered for the downdate;
-//   - arg-min / arg-max searches are deterministic (value, index) wavefront
-//     reductions with the reference's tie rules (first index for i?amax and
-//     the custom iamin; last index for the zero-crossing search).
-//
-// The kernel is instantiated with an active-set capacity CAP:
-//   CAP > 0  the active-set arrays and invGs live in LDS (capacity CAP); a
-//            signal whose active set would outgrow CAP is appended to an
-//            overflow list, with its state (DtR, coefficients, invGs,
-//            counters) saved in the signal's hand-over slot;
-//   CAP == 0 capacity L: active-set arrays in LDS, invGs in a per-workgroup
-//            HBM slot; listed signals resume from their hand-over state.
-// A signal's arithmetic depends only on the two group widths, never on the
-// batch around it.
-//
-// Batched form: persistent grids pulling signal indices from atomic counters
-// (fast pass over 0..n-1, then the full pass over the overflow list, on the
-// same stream).  Plain form: one workgroup per signal, full capacity.
-#include <climits>
-#include <type_traits>
-
-#include "spx_common.h"
-#include "spx_internal.h"
+// LARS-Lasso solver: launch logic of the batched and plain forms over the
+// kernels of lars_impl.h (plain penalty here; the weighted penalty's kernels
+// are compiled in lars_w.hip).
+#include "lars_impl.h"
 
 namespace spx {
 
 namespace {
 
 template <typename T>
-struct Vec16;
-template <>
-struct Vec16<float> {
-  typedef float type __attribute__((ext_vector_type(4)));
-  static constexpr int N = 4;
-};
-template <>
-struct Vec16<double> {
-  typedef double type __attribute__((ext_vector_type(2)));
-  static constexpr int N = 2;
-};
-
-// LDS capacity of the fast pass (active-set size it solves without HBM).
-template <typename T>
-struct FastCap {
-  static constexpr int C = sizeof(T) == 4 ? 24 : 16;
-};
-
-// State of a signal handed from the LDS pass to the full-capacity pass,
-// taken just before the admission that would outgrow the LDS capacity C:
-// DtR (ldg), coefficients (C), the inverse Gram block (C x C, leading
-// dimension C), normX and thrs, the active indices (C) and the counters
-// (iterations, admissions, active-set size, atom to admit).  The full pass
-// resumes from it instead of restarting the path.
-template <typename T>
-struct LarsState {
-  static constexpr int C = FastCap<T>::C;
-  static size_t bytes(int ldg) {
-    const size_t b = sizeof(T) * ((size_t)ldg + C + (size_t)C * C + 2) + sizeof(int) * (C + 4);
-    return (b + 15) & ~size_t(15);
-  }
-  T* dtr;
-  T* coef;
-  T* M;
-  T* sc;
-  int* ind;
-  int* isc;
-  __device__ LarsState(void* base, int ldg) {
-    dtr = static_cast<T*>(base);
-    coef = dtr + ldg;
-    M = coef + C;
-    sc = M + C * C;
-    ind = reinterpret_cast<int*>(sc + 2);
-    isc = ind + C;
-  }
-};
-
-template <typename T, int W, int NQ, int CAP>
-struct LarsLds {
-  static constexpr int GT = 64 * W;
-  static constexpr int VEC = Vec16<T>::N;
-  static constexpr int PT = VEC * GT * NQ;
-  // byte size for active-set capacity LC
-  static size_t bytes(int LC, int CL = 0) {
-    size_t b = sizeof(T) * (PT + GT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC;
-    b = (b + 15) & ~size_t(15);
-    b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
-    b = (b + 15) & ~size_t(15);
-    if (CAP > 0) b += 2 * sizeof(T) * CAP * CAP + 16;
-    b = (b + 15) & ~size_t(15);
-    b += sizeof(double) * (size_t)(LC + (LC > 64 ? LC : 64)) + 16;  // FP64 vectors
-    b = (b + 15) & ~size_t(15);
-    b += sizeof(T) * (size_t)CL * CL + 16;  // LDS copy of the inverse (full pass)
-    return b;
-  }
-};
-
-// y(0:n) = M(0:n, 0:n) x for a symmetric M (leading dimension ldm), with
-// all GT threads: row r is split into P = GT / n parts whose partial sums
-// are combined in part order (deterministic).  x and y in LDS, part a
-// GT-entry LDS buffer.  Ends with a barrier.
-template <typename T, int GT, bool SPLIT>
-__device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
-                                          T* part, int n) {
-  const int tid = threadIdx.x;
-  // rows are split only when M is in HBM (several loads in flight per row);
-  // with M in LDS one thread per row is faster
-  const int P = (!SPLIT || n >= GT) ? 1 : GT / n;
-  const int seg = (n + P - 1) / P;
-  for (int t = tid; t < n * P; t += GT) {
-    const int r = t % n, k = t / n;
-    const int q0 = k * seg, q1 = q0 + seg < n ? q0 + seg : n;
-    T acc = 0;
-    int q = q0;
-    if (SPLIT) {
-      // M in HBM: eight loads in flight, then the sums in order
-      for (; q + 8 <= q1; q += 8) {
-        T mv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) mv[u] = M[r + (q + u) * ldm];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc += mv[u] * x[q + u];
-      }
-    }
-    for (; q < q1; ++q) acc += M[r + q * ldm] * x[q];
-    if (P == 1) y[r] = acc;
-    else part[k * n + r] = acc;
-  }
-  __syncthreads();
-  if (P > 1) {
-    for (int r = tid; r < n; r += GT) {
-      T acc = part[r];
-      for (int k = 1; k < P; ++k) acc += part[k * n + r];
-      y[r] = acc;
-    }
-    __syncthreads();
-  }
-}
-
-// Admits the i-th active atom (i >= 1) into the explicit inverse M of the
-// active Gram block, i x i -> (i+1) x (i+1), by its Schur complement
-// (decomp/decomp.h:1756-1770); sGs(0:i+1) = (G + diag I)(active, new atom).
-// v = M gs lands in sV.  An FP32 pivot below 1e-3 of the diagonal (or not
-// positive) is recomputed with FP64 accumulation.  ELEM: the rank-one update
-// runs one element per thread with eight loads in flight (M in HBM, or a
-// wide group); else one row per thread.  Elements (r,q) and (q,r) get the
-// identical update v[min] * (schur * v[max]), so M stays exactly symmetric.
-// Ends with a barrier.
-template <typename T, int W, bool SPLIT, bool ELEM>
-__device__ __forceinline__ void inverse_admit(T* M, int ldm, int i, const T* sGs, T* sV,
-                                              T* sPart, T* sRed3, double* sRes) {
-  constexpr int GT = 64 * W;
-  const int tid = threadIdx.x;
-  // v = invGs(i x i) * gs(0:i)
-  symv_dist<T, GT, SPLIT>(M, ldm, sGs, sV, sPart, i);
-  T dv = 0, z1 = 0, z2 = 0;
-  for (int r = tid; r < i; r += GT) dv += sV[r] * sGs[r];
-  group_sum3<T, W>(dv, z1, z2, sRed3);
-  T schur = T(1) / (sGs[i] - dv);
-  if (sizeof(T) == 4 && !(sGs[i] - dv > T(1e-3) * sGs[i])) {
-    __syncthreads();
-    double pd = 0;
-    for (int r = tid; r < i; r += GT) {
-      double acc = 0;
-      for (int q = 0; q < i; ++q)
-        acc += static_cast<double>(M[r + q * ldm]) * static_cast<double>(sGs[q]);
-      sV[r] = static_cast<T>(acc);
-      pd += acc * static_cast<double>(sGs[r]);
-    }
-    double z1d = 0, z2d = 0;
-    group_sum3<double, W>(pd, z1d, z2d, sRes);
-    schur = static_cast<T>(1.0 / (static_cast<double>(sGs[i]) - pd));
-    __syncthreads();
-  }
-  if (ELEM) {
-    const int nn = i * i;
-    for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
-      T mv[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = e0 + u * GT;
-        mv[u] = e < nn ? M[e % i + (e / i) * ldm] : T(0);
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = e0 + u * GT;
-        if (e < nn) {
-          const int r = e % i, q = e / i;
-          const T vr = sV[r], vq = sV[q];
-          const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-          M[r + q * ldm] = mv[u] + upd;
-        }
-      }
-    }
-  } else {
-    for (int r = tid; r < i; r += GT) {
-      const T vr = sV[r];
-      for (int q = 0; q < i; ++q) {
-        const T vq = sV[q];
-        const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-        M[r + q * ldm] += upd;
-      }
-    }
-  }
-  for (int r = tid; r < i; r += GT) {
-    const T off = -schur * sV[r];
-    M[r + i * ldm] = off;
-    M[i + r * ldm] = off;
-  }
-  if (tid == 0) M[i + i * ldm] = schur;
-  __syncthreads();
-}
-
-// Md (i x i) = Ms ((i+1) x (i+1)) without row and column fz, downdated by
-// -1/Ms(fz,fz) times the outer product of that column
-// (decomp/decomp.h:1875-1903).  Ms and Md are distinct buffers.  No barrier
-// after the writes.
-template <typename T, int W, bool ELEM>
-__device__ __forceinline__ void inverse_drop(const T* Ms, int lds, T* Md, int ldd, int i, int fz,
-                                             T* sV) {
-  constexpr int GT = 64 * W;
-  const int tid = threadIdx.x;
-  const T schur = Ms[fz + fz * lds];
-  for (int r = tid; r < i; r += GT) {
-    const int r2 = r + (r >= fz);
-    sV[r] = Ms[r2 + fz * lds];
-  }
-  __syncthreads();
-  const T alpha = T(-1) / schur;
-  if (ELEM) {
-    const int nn = i * i;
-    for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
-      T mv[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = e0 + u * GT;
-        const int r = e % i, q = e / i;
-        mv[u] = e < nn ? Ms[r + (r >= fz) + (q + (q >= fz)) * lds] : T(0);
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = e0 + u * GT;
-        if (e < nn) {
-          const int r = e % i, q = e / i;
-          const T ur = sV[r], uq = sV[q];
-          const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
-          Md[r + q * ldd] = mv[u] + upd;
-        }
-      }
-    }
-  } else {
-    for (int r = tid; r < i; r += GT) {
-      const int r2 = r + (r >= fz);
-      const T ur = sV[r];
-      for (int q = 0; q < i; ++q) {
-        const int q2 = q + (q >= fz);
-        const T uq = sV[q];
-        const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
-        Md[r + q * ldd] = Ms[r2 + q2 * lds] + upd;
-      }
-    }
-  }
-}
-
-// The full-capacity pass at 4 wavefronts is held to 4 workgroups per CU
-// (128 VGPRs): its signals are latency-bound, so occupancy pays more than
-// registers.
-template <typename T, int W, int NQ, int CAP>
-__global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_kernel(LarsArgs<T> a) {
-  constexpr int GT = 64 * W;
-  constexpr int VEC = Vec16<T>::N;
-  constexpr int PT = VEC * GT * NQ;  // rows covered by the group
-  constexpr int UNR = NQ >= 4 ? 2 : (NQ == 2 ? 4 : 8);  // Gram columns in flight
-  typedef typename Vec16<T>::type V;
-  typedef typename std::conditional<(NQ * VEC <= 32), unsigned, unsigned long long>::type ActMask;
-  const T INF = dev_inf<T>();
-
-  extern __shared__ __attribute__((aligned(16))) char smem[];
-  const int L = a.L;
-  const int LC = CAP > 0 ? (CAP < L ? CAP : L) : L;  // active-set capacity
-  const int p = a.p;
-  const int ldg = a.ldg;
-  T* sDtR = reinterpret_cast<T*>(smem);  // PT
-  T* sPart = sDtR + PT;                   // GT partial sums
-  T* sCoef = sPart + GT;                  // LC
-  T* sU = sCoef + LC;                     // LC
-  T* sW = sU + LC;                        // LC
-  T* sGs = sW + LC;                       // LC
-  T* sV = sGs + LC;                       // LC
-  int* sInd = reinterpret_cast<int*>(sV + LC);  // LC
-  int* sTmpI = sInd + LC;                       // LC
-  KeyIdx<T>* sRed = reinterpret_cast<KeyIdx<T>*>(
-      (reinterpret_cast<uintptr_t>(sTmpI + LC) + 15) & ~uintptr_t(15));
-  T* sRed3 = reinterpret_cast<T*>(sRed + W);  // 3W
-  int* sSig = reinterpret_cast<int*>(sRed3 + 3 * W);
-  T* sM0 = reinterpret_cast<T*>(
-      (reinterpret_cast<uintptr_t>(sSig + 1) + 15) & ~uintptr_t(15));  // CAP x CAP
-  T* sM1 = sM0 + (CAP > 0 ? CAP * CAP : 0);
-  double* sAlpha = reinterpret_cast<double*>(
-      (reinterpret_cast<uintptr_t>(sM1 + (CAP > 0 ? CAP * CAP : 0)) + 15) & ~uintptr_t(15));
-  double* sRes = sAlpha + LC;  // FP64 refit vectors; sRes has max(LC, 64)
-  // full-capacity pass: LDS copy of the inverse while the active set has at
-  // most CL atoms (CL x CL, leading dimension CL)
-  const int CL = CAP == 0 ? a.lds_cap : 0;
-  T* sMl = reinterpret_cast<T*>(
-      (reinterpret_cast<uintptr_t>(sRes + (LC > 64 ? LC : 64)) + 15) & ~uintptr_t(15));
-
-  const int tid = threadIdx.x;
-  T* const M0 = CAP > 0 ? sM0 : a.scratch + (size_t)blockIdx.x * 2 * L * L;
-  T* const M1 = CAP > 0 ? sM1 : M0 + (size_t)L * L;
-  const int ldm = CAP > 0 ? CAP : L;
-  const int Lm = L < p ? L : p;
-  const T diag = a.diag;
-  const int nwork = a.list ? *a.list_count : a.n;
-
-  // work distribution: chunks of `chunk` consecutive work items per atomic
-  // grab (one device-scope atomic per signal serialises on its address)
-  const int chunk = a.chunk > 0 ? a.chunk : 1;
-  // The LDS pass (many short signals) requests its next grab as soon as it
-  // starts on the current one, so the atomic's latency overlaps the set-up
-  // loads; the full-capacity pass (few long signals) grabs when free, so no
-  // long signal waits behind another.
-  constexpr bool PREFETCH = CAP > 0;
-  int w = 0, wend = 0;
-  int pre = 0;  // thread 0: the prefetched grab
-  if (PREFETCH && a.queue && tid == 0) pre = atomicAdd(a.queue, chunk);
-  for (int round = 0;; ++round) {
-    if (a.queue) {
-      if (w >= wend) {
-        __syncthreads();
-        if (tid == 0) *sSig = PREFETCH ? pre : atomicAdd(a.queue, chunk);
-        __syncthreads();
-        w = *sSig;
-        wend = w + chunk;
-        if (PREFETCH && tid == 0 && w < nwork) pre = atomicAdd(a.queue, chunk);
-      }
-    } else {
-      w = round == 0 ? (int)blockIdx.x : nwork;
-      wend = w + 1;
-    }
-    if (w >= nwork) break;
-    const int wcur = w++;
-    const int sig = a.list ? a.list[wcur] : (a.order ? a.order[wcur] : wcur);
-
-    // ---------------------------------------------------------- set-up
-    __syncthreads();  // the previous signal's LDS state is no longer read
-    const T* dtx = a.DtX + (size_t)sig * a.ldd;
-    // the full-capacity pass resumes the signals handed over by the LDS pass
-    const bool resume = CAP == 0 && a.state && a.list;
-    const LarsState<T> st(resume ? static_cast<void*>(static_cast<char*>(a.state) +
-                                                      (size_t)sig * a.state_bytes)
-                                 : nullptr, ldg);
-    const bool vec_ok = resume || ((a.ldd % VEC == 0) &&
-                                   (reinterpret_cast<uintptr_t>(a.DtX) % (sizeof(T) * VEC) == 0));
-    const T* src = resume ? st.dtr : dtx;
-    const int nsrc = resume ? ldg : p;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int j0 = VEC * (tid + GT * q);
-      V v;
-      if (vec_ok && j0 + VEC <= nsrc) {
-        v = *reinterpret_cast<const V*>(src + j0);
-      } else {
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) v[e] = j0 + e < p ? src[j0 + e] : T(0);
-      }
-      *reinterpret_cast<V*>(&sDtR[j0]) = v;
-    }
-    // active flags of this thread's rows (bit q * VEC + e)
-    ActMask act = 0;
-    const T lam = a.lambda[a.nlam > 1 ? sig : 0];
-    T normX;
-    T thrs0 = 0;
-    int currentInd;
-    int i0 = 0;
-    int s_active = 0;
-    int iter = 0, admissions = 0;
-    bool empty = false, overflow = false, reached = false;
-    if (resume) {
-      const int s0 = st.isc[2];
-      for (int j = tid; j < LC; j += GT) {
-        sCoef[j] = j < s0 ? st.coef[j] : T(0);
-        sInd[j] = j < s0 ? st.ind[j] : -1;
-      }
-      for (int e = tid; e < s0 * s0; e += GT) {
-        const int r = e % s0, q = e / s0;
-        if (CL >= s0) sMl[r + q * CL] = st.M[r + q * LarsState<T>::C];
-        else M0[r + q * ldm] = st.M[r + q * LarsState<T>::C];
-      }
-      for (int k = 0; k < s0; ++k) {
-        const int c = st.ind[k];
-        if (tid == (c / VEC) % GT) act |= ActMask(1) << ((c / VEC / GT) * VEC + c % VEC);
-      }
-      normX = st.sc[0];
-      thrs0 = st.sc[1];
-      iter = st.isc[0];
-      admissions = st.isc[1];
-      i0 = s0;
-      s_active = s0;
-      currentInd = st.isc[3];
-      __syncthreads();
-    } else {
-      for (int j = tid; j < LC; j += GT) {
-        sCoef[j] = 0;
-        sInd[j] = -1;
-      }
-      normX = a.norms[sig];
-      __syncthreads();
-
-      // first atom: i?amax of DtR, or the plain max when pos
-      {
-        T key = INF, pay = 0;
-        int idx = INT_MAX;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q)
-#pragma unroll
-          for (int e = 0; e < VEC; ++e) {
-            const int j = VEC * (tid + GT * q) + e;
-            const T d = sDtR[j];
-            const T k = j < p ? (a.pos ? -d : -fabs(d)) : INF;
-            take_if_better(key, idx, pay, k, j < p ? j : INT_MAX, T(0));
-          }
-        group_argmin<T, W>(key, idx, pay, sRed);
-        currentInd = idx == INT_MAX ? 0 : idx;
-      }
-      const T dc = sDtR[currentInd];
-      if (a.mode == SPX_PENALTY && fabs(dc) < lam) empty = true;
-      if (a.mode == SPX_L2ERROR && normX < lam) empty = true;
-    }
-    int ovf_i = 0;
-    T ovf_thrs = 0;
-
-    T* Mc = M0;
-    T* Mn = M1;
-    // full-capacity pass: the inverse starts in LDS (capacity CL) and moves
-    // to the HBM slot when the active set outgrows it
-    bool inl = CAP == 0 && CL > 0 && (!resume || CL >= i0);
-    if (!empty) {
-      bool newAtom = true;
-      T thrs = thrs0;
-      for (int i = i0; i < Lm; ++i) {
-        ++iter;
-        if (newAtom) {
-          if (CAP > 0 && i >= LC) {
-            // the active set outgrows the fast pass: hand the signal over
-            overflow = true;
-            ovf_i = i;
-            ovf_thrs = thrs;
-            break;
-          }
-          // ---------------------------------------------- admit an atom
-          const int c = currentInd;
-          ++admissions;
-          ++s_active;
-          if (tid == 0) sInd[i] = c;
-          if (tid == (c / VEC) % GT) act |= ActMask(1) << ((c / VEC / GT) * VEC + c % VEC);
-          __syncthreads();
-          const T* gcol = a.G + (size_t)c * ldg;
-          for (int j = tid; j <= i; j += GT) {
-            const int r = sInd[j];
-            sGs[j] = gcol[r] + (r == c ? diag : T(0));
-          }
-          __syncthreads();
-          if (i == 0) {
-            if (tid == 0) {
-              if (inl) sMl[0] = T(1) / sGs[0];
-              else Mc[0] = T(1) / sGs[0];
-            }
-          } else {
-            if (inl && i >= CL) {
-              // the block outgrows the LDS copy: continue in the HBM slot
-              for (int e = tid; e < i * i; e += GT) {
-                const int r = e % i, q = e / i;
-                Mc[r + q * ldm] = sMl[r + q * CL];
-              }
-              __syncthreads();
-              inl = false;
-            }
-            if (inl) inverse_admit<T, W, true, true>(sMl, CL, i, sGs, sV, sPart, sRed3, sRes);
-            else inverse_admit<T, W, CAP == 0, CAP == 0>(Mc, ldm, i, sGs, sV, sPart, sRed3, sRes);
-          }
-        }
-
-        const int s = i + 1;
-        // ------------------------------------------ direction u = invGs*w
-        for (int j = tid; j < s; j += GT) sW[j] = sDtR[sInd[j]] > 0 ? T(1) : T(-1);
-        __syncthreads();
-        if (inl) symv_dist<T, GT, true>(sMl, CL, sW, sU, sPart, s);
-        else symv_dist<T, GT, CAP == 0>(Mc, ldm, sW, sU, sPart, s);
-
-        // ------------------ zero-crossing step, polynomial coefficients
-        T step_max;
-        int first_zero;
-        T coeff1 = 0, coeff2 = 0;
-        {
-          T key = INF, pay = 0, z = 0;
-          int idx = INT_MAX;
-          for (int j = tid; j < s; j += GT) {
-            const T uj = sU[j];
-            const T ratio = -sCoef[j] / uj;
-            // the sequential scan keeps the LAST j on ties: index L - j
-            take_if_better(key, idx, pay, ratio > 0 ? ratio : INF,
-                           ratio > 0 ? L - j : INT_MAX, T(0));
-            const T dj = sDtR[sInd[j]];
-            coeff1 += dj > 0 ? uj : -uj;
-            coeff2 += dj * uj;
-          }
-          group_argmin<T, W>(key, idx, pay, sRed);
-          group_sum3<T, W>(coeff1, coeff2, z, sRed3);
-          step_max = key;
-          first_zero = idx == INT_MAX ? -1 : L - idx;
-        }
-        const T cc = fabs(sDtR[sInd[0]]);
-
-        // ------------------------------- A = (G + diag I)(:,active) * u
-        V Areg[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) Areg[q] = V(0);
-        // UNR Gram columns per round so their loads are in flight together;
-        // the sum over k keeps its sequential order
-        for (int k0 = 0; k0 < s; k0 += UNR) {
-          T uk[UNR];
-          const T* col[UNR];
-#pragma unroll
-          for (int e = 0; e < UNR; ++e) {
-            const int k = k0 + e < s ? k0 + e : k0;
-            uk[e] = k0 + e < s ? sU[k] : T(0);
-            col[e] = a.G + (size_t)sInd[k] * ldg;
-          }
-          V g[UNR][NQ];
-#pragma unroll
-          for (int e = 0; e < UNR; ++e)
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const int j0 = VEC * (tid + GT * q);
-              g[e][q] = j0 < ldg ? *reinterpret_cast<const V*>(col[e] + j0) : V(0);
-            }
-#pragma unroll
-          for (int e = 0; e < UNR; ++e)
-            if (k0 + e < s) {
-#pragma unroll
-              for (int q = 0; q < NQ; ++q) Areg[q] += uk[e] * g[e][q];
-            }
-        }
-        for (int k = 0; k < s; ++k) {
-          const int r = sInd[k];
-          const int rc = r / VEC;
-          if (rc % GT == tid) {
-            const int q = rc / GT;
-            const T add = diag * sU[k];
-#pragma unroll
-            for (int qq = 0; qq < NQ; ++qq)
-              if (qq == q) Areg[qq][r % VEC] += add;
-          }
-        }
-
-        // ------------------------------- next atom: custom iamin over 2p
-        // candidates (decomp/decomp.h:1803-1820): c1 = (d + cc) / (1 + A) and
-        // c2 = (cc - d) / (1 - A) over the inactive atoms, key |c|, first
-        // index of the minimum, IEEE quotients.
-        T bestk = INF, bestv = INF;
-        int besti = INT_MAX;
-        if constexpr (sizeof(T) == 4) {
-          // pass 1: keys from the hardware reciprocal, within 3 ulp of the
-          // IEEE quotient; each thread keeps its smallest key (m1, slot i1)
-          // and its second smallest (m2)
-          T m1 = INF, m2 = INF;
-          int i1 = 0;
-#pragma unroll
-          for (int q = 0; q < NQ; ++q)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-              const int j = VEC * (tid + GT * q) + e;
-              const bool cand = (j < p) & !((act >> (q * VEC + e)) & 1);
-              const T Aj = Areg[q][e];
-              const T d = sDtR[j];
-              const bool ok1 = cand & (Aj < INF) & (Aj > T(-1)) & !a.pos;
-              const bool ok2 = cand & (Aj < INF) & (Aj < T(1));
-              const T k1 = ok1 ? T(fabs((d + cc) * __builtin_amdgcn_rcpf(T(1) + Aj))) : INF;
-              const T k2 = ok2 ? T(fabs((cc - d) * __builtin_amdgcn_rcpf(T(1) - Aj))) : INF;
-              const int sl = 2 * (q * VEC + e);
-              bool lt = k1 < m1;
-              m2 = lt ? m1 : fminf(m2, k1);
-              i1 = lt ? sl : i1;
-              m1 = lt ? k1 : m1;
-              lt = k2 < m1;
-              m2 = lt ? m1 : fminf(m2, k2);
-              i1 = lt ? sl + 1 : i1;
-              m1 = lt ? k2 : m1;
-            }
-          // pass 2 recomputes what it needs: no pass-1 temporaries stay live
-          // (register pressure sets the signals in flight per CU)
-#pragma unroll
-          for (int q = 0; q < NQ; ++q)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-              T x = Areg[q][e];
-              asm volatile("" : "+v"(x));
-              Areg[q][e] = x;
-            }
-          asm volatile("" ::: "memory");
-          // every key within the margin of the group minimum may be the
-          // minimum of the IEEE keys: those get the exact quotient
-          T amin = m1, zp = 0;
-          int zi = 0;
-          group_argmin<T, W>(amin, zi, zp, sRed);
-          const T thr = amin * T(1.0f + 0x1.0p-18f) + T(1e-37f);
-          if (m2 <= thr) {
-            // two of this thread's keys are near the minimum (or no key is
-            // finite): all of its keys exactly (a superset of the keys
-            // within the margin; the group arg-min is unchanged).  Rare, so
-            // one row at a time (few registers live).
-#pragma unroll 1
-            for (int sl = 0; sl < NQ * VEC; ++sl) {
-              T Aj = 0;
-#pragma unroll
-              for (int q = 0; q < NQ; ++q)
-#pragma unroll
-                for (int e = 0; e < VEC; ++e) Aj = sl == q * VEC + e ? Areg[q][e] : Aj;
-              const int j = VEC * (tid + GT * (sl / VEC)) + sl % VEC;
-              const bool cand = (j < p) & !((act >> sl) & 1);
-              const T d = sDtR[j < PT ? j : 0];
-              T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
-              c1 = a.pos ? INF : c1;
-              const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
-              take_if_better(bestk, besti, bestv, cand ? T(fabs(c1)) : INF,
-                             cand ? j : INT_MAX, c1);
-              take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
-                             cand ? p + j : INT_MAX, c2);
-            }
-          } else if (m1 <= thr) {
-            // one key of this thread near the minimum (finite, so valid)
-            const int sl = i1 >> 1;
-            T Aj = 0;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q)
-#pragma unroll
-              for (int e = 0; e < VEC; ++e) Aj = sl == q * VEC + e ? Areg[q][e] : Aj;
-            const int j = VEC * (tid + GT * (sl / VEC)) + sl % VEC;
-            const T d = sDtR[j];
-            const bool second = i1 & 1;
-            const T c = second ? (cc - d) / (T(1) - Aj) : (d + cc) / (T(1) + Aj);
-            take_if_better(bestk, besti, bestv, T(fabs(c)), second ? p + j : j, c);
-          }
-        } else {
-#pragma unroll
-          for (int q = 0; q < NQ; ++q)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-              const int j = VEC * (tid + GT * q) + e;
-              const bool cand = (j < p) & !((act >> (q * VEC + e)) & 1);
-              const T Aj = Areg[q][e];
-              const T d = sDtR[j];
-              T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
-              c1 = a.pos ? INF : c1;
-              const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
-              take_if_better(bestk, besti, bestv, cand ? T(fabs(c1)) : INF,
-                             cand ? j : INT_MAX, c1);
-              take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
-                             cand ? p + j : INT_MAX, c2);
-            }
-        }
-        group_argmin<T, W>(bestk, besti, bestv, sRed);
-        T step = besti == INT_MAX ? INF : bestv;
-        const int nextInd = besti == INT_MAX ? 0 : besti % p;
-
-        const T coeff3 = normX - lam;
-        T step_max2;
-        if (a.mode == SPX_PENALTY) {
-          step_max2 = cc - lam;
-        } else if (a.mode == SPX_L2ERROR) {
-          const T delta = coeff2 * coeff2 - coeff1 * coeff3;
-          step_max2 = delta < 0 ? INF : (coeff2 - sqrt(delta)) / coeff1;
-          step_max2 = step_max2 < cc ? step_max2 : cc;
-        } else {
-          step_max2 = coeff1 < 0 ? INF : (lam - thrs) / coeff1;
-          step_max2 = step_max2 < cc ? step_max2 : cc;
-        }
-        step = step < step_max2 ? step : step_max2;
-        step = step < step_max ? step : step_max;
-        if (step == INF) break;
-        reached = (a.mode == SPX_PENALTY) && (step == step_max2);
-
-        // -------------------------------------------------- updates
-        for (int j = tid; j < s; j += GT) {
-          T cj = sCoef[j] + step * sU[j];
-          if (a.pos && cj < 0) cj = 0;
-          sCoef[j] = cj;
-        }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int j0 = VEC * (tid + GT * q);
-#pragma unroll
-          for (int e = 0; e < VEC; ++e) sDtR[j0 + e] -= step * Areg[q][e];
-        }
-        normX += coeff1 * step * step - 2 * coeff2 * step;
-        thrs += step * coeff1;
-        __syncthreads();
-        if (a.path && sig == 0 && iter < a.path_cols) {
-          // regularisation path of the first signal (decomp/decomp.h:1868-1871)
-          for (int k = tid; k <= i; k += GT) a.path[(size_t)iter * p + sInd[k]] = sCoef[k];
-        }
-
-        if (step == step_max) {
-          // ------------------- drop atom first_zero (decomp/decomp.h:1875-1903)
-          const int fz = first_zero;
-          const int dropped = sInd[fz];
-          if (inl) {
-            // LDS copy: stage the block in the HBM slot, downdate back
-            for (int e = tid; e < (i + 1) * (i + 1); e += GT) {
-              const int r = e % (i + 1), q = e / (i + 1);
-              Mn[r + q * ldm] = sMl[r + q * CL];
-            }
-            __syncthreads();
-            inverse_drop<T, W, true>(Mn, ldm, sMl, CL, i, fz, sV);
-          } else {
-            inverse_drop<T, W, CAP == 0>(Mc, ldm, Mn, ldm, i, fz, sV);
-          }
-          for (int j = tid; j < i; j += GT) {
-            const int j2 = j + (j >= fz);
-            sTmpI[j] = sInd[j2];
-            sGs[j] = sCoef[j2];
-          }
-          __syncthreads();
-          if (tid == (dropped / VEC) % GT)
-            act &= ~(ActMask(1) << ((dropped / VEC / GT) * VEC + dropped % VEC));
-          for (int j = tid; j < i; j += GT) {
-            sInd[j] = sTmpI[j];
-            sCoef[j] = sGs[j];
-          }
-          if (tid == 0) {
-            sInd[i] = -1;
-            sCoef[i] = 0;
-          }
-          __syncthreads();
-          if (!inl) {
-            T* t = Mc;
-            Mc = Mn;
-            Mn = t;
-          }
-          --s_active;
-          newAtom = false;
-          i = i - 2;
-          // an emptied active set has no direction: end the path (the
-          // reference would read ind[-1] here)
-          if (s_active == 0) break;
-        } else {
-          newAtom = true;
-          currentInd = nextInd;
-        }
-        // Stop rules of decomp/decomp.h:1907-1911.  In FP32 a vanishing step
-        // that admits an atom is a tie between two atoms at FP32 resolution
-        // (both were due at the same point of the path): the atom is admitted
-        // and the path goes on, where the rule would end it short of lambda.
-        const bool tie = sizeof(T) == 4 && newAtom;
-        if ((iter >= a.length_path - 1) || (fabs(step) < 1e-15 && !tie) ||
-            step == step_max2 || (normX < 1e-15) || (i == (Lm - 1)) ||
-            (a.mode == SPX_L2ERROR && normX - lam < 1e-15) ||
-            (a.mode == SPX_L1COEFFS && (lam - thrs < 1e-15)))
-          break;
-      }
-    }
-
-    // ---------- optional FP64 refit (PENALTY, at lambda) or least-squares fit
-    // ols: alpha_A = (G_AA + diag I)^-1 (D'x)_A, the coefficients the
-    // reference's ols option returns (decomp/decomp.h:1087-1091)
-    if (((a.refit && reached) || a.ols) && !overflow && s_active > 0) {
-      __syncthreads();
-      const int s = s_active;
-      const T* Mf = inl ? sMl : Mc;  // the inverse of the final active block
-      const int ldf = inl ? CL : ldm;
-      for (int r = tid; r < s; r += GT) sAlpha[r] = a.ols ? 0.0 : static_cast<double>(sCoef[r]);
-      __syncthreads();
-      for (int it = 0; it < (a.ols ? 3 : 2); ++it) {
-        // res = (D'x)_A - lam sign(alpha) - (G_AA + d I) alpha   (FP64)
-        for (int r = tid; r < s; r += GT) {
-          const int ir = sInd[r];
-          const T* gcol = a.G + (size_t)ir * ldg;
-          double acc = static_cast<double>(dtx[ir]);
-          if (!a.ols) acc -= static_cast<double>(lam) * (sCoef[r] > 0 ? 1.0 : -1.0);
-          for (int q = 0; q < s; ++q) {
-            const double g = static_cast<double>(gcol[sInd[q]]) + (q == r ? static_cast<double>(diag) : 0.0);
-            acc -= g * sAlpha[q];
-          }
-          sRes[r] = acc;
-        }
-        __syncthreads();
-        // alpha += invGs res
-        for (int r = tid; r < s; r += GT) {
-          double acc = 0;
-          for (int q = 0; q < s; ++q) acc += static_cast<double>(Mf[r + q * ldf]) * sRes[q];
-          sAlpha[r] += acc;
-        }
-        __syncthreads();
-      }
-      for (int r = tid; r < s; r += GT) sCoef[r] = static_cast<T>(sAlpha[r]);
-    }
-
-    // ------------------------------------------------------- outputs
-    __syncthreads();
-    if (overflow) {
-      if (tid == 0) {
-        const int pos = atomicAdd(a.ovf_count, 1);
-        a.ovf_list[pos] = sig;
-        // remaining path: from the current correlation down to lambda (the
-        // smaller the ratio, the longer; the full pass takes these first)
-        if (a.ovf_key) {
-          const T kk = lam / fabs(sDtR[sInd[0]]);
-          a.ovf_key[pos] = kk == kk ? kk : T(0);  // NaN sorts first, not past the list
-        }
-      }
-      if (CAP > 0 && a.state) {
-        // hand-over state for the full-capacity pass
-        LarsState<T> sv(static_cast<char*>(a.state) + (size_t)sig * a.state_bytes, ldg);
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int j0 = VEC * (tid + GT * q);
-          if (j0 < ldg) *reinterpret_cast<V*>(sv.dtr + j0) = *reinterpret_cast<const V*>(&sDtR[j0]);
-        }
-        for (int j = tid; j < ovf_i; j += GT) {
-          sv.coef[j] = sCoef[j];
-          sv.ind[j] = sInd[j];
-        }
-        for (int e = tid; e < ovf_i * ovf_i; e += GT) {
-          const int r = e % ovf_i, q = e / ovf_i;
-          sv.M[r + q * LarsState<T>::C] = Mc[r + q * ldm];
-        }
-        if (tid == 0) {
-          sv.sc[0] = normX;
-          sv.sc[1] = ovf_thrs;
-          sv.isc[0] = iter - 1;  // the interrupted iteration is redone
-          sv.isc[1] = admissions;
-          sv.isc[2] = ovf_i;
-          sv.isc[3] = currentInd;
-        }
-      }
-      continue;
-    }
-    T* ov = a.outV + (size_t)sig * L;
-    int* orr = a.outR + (size_t)sig * L;
-    // the live part of the list and its -1 terminator (entries past the
-    // terminator are left unspecified; consumers read `count` entries)
-    const int nout = s_active < L ? s_active + 1 : L;
-    for (int j = tid; j < nout; j += GT) {
-      ov[j] = j < s_active ? sCoef[j] : T(0);
-      orr[j] = j < s_active ? sInd[j] : -1;
-    }
-    if (tid == 0) {
-      a.outCount[sig] = s_active;
-      if (a.stats) {
-        a.stats[2 * (size_t)sig] = iter;
-        a.stats[2 * (size_t)sig + 1] = admissions;
-      }
-    }
-  }
-}
-
-// Launches one instantiation: persistent grid over a counter (batched) or
-// one workgroup per signal (plain; CAP must be 0).
-template <typename T, int W, int NQ, int CAP>
-int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
-                long max_blocks) {
-  const int LC = CAP > 0 ? (CAP < a.L ? CAP : a.L) : a.L;
-  size_t lds = LarsLds<T, W, NQ, CAP>::bytes(LC);
-  if (lds > 160 * 1024) {
-    set_error("lasso: L=%d, p=%d need %zu bytes of LDS per signal", a.L, a.p, lds);
-    return SPX_ERR_UNSUP;
-  }
-  auto kern = lars_kernel<T, W, NQ, CAP>;
-  SPX_HIP(hipFuncSetAttribute((const void*)kern,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-  hipStream_t st = ctx->stream;
-  LarsArgs<T> b = a;
-  b.lds_cap = 0;
-  if (CAP == 0) {
-    // LDS copy of the inverse: the LDS the register-limited occupancy leaves
-    // free, if it holds at least the LDS pass's capacity
-    int per_cu = 0;
-    SPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds));
-    if (per_cu < 1) per_cu = 1;
-    const size_t budget = (size_t)156 * 1024 / per_cu;
-    int cl = 0;
-    while (cl + 4 <= a.L && LarsLds<T, W, NQ, CAP>::bytes(LC, cl + 4) <= budget) cl += 4;
-    if (cl >= a.L) cl = a.L;
-    if (cl >= FastCap<T>::C) {
-      b.lds_cap = cl;
-      lds = LarsLds<T, W, NQ, CAP>::bytes(LC, cl);
-    }
-  }
-  if (persistent) {
-    int per_cu = 0;
-    SPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds));
-    if (per_cu < 1) per_cu = 1;
-    long blocks = (long)per_cu * ctx->num_cus;
-    if (blocks > max_blocks) blocks = max_blocks;
-    if (blocks < 1) blocks = 1;
-    if (CAP == 0) {
-      const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
-      T* scratch = nullptr;
-      int rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
-      if (rc) return rc;
-      b.scratch = scratch;
-    }
-    b.queue = counter;
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, b);
-    SPX_HIP(hipGetLastError());
-    return SPX_OK;
-  }
-  // plain form: one workgroup per signal, chunks bounded by scratch size
-  const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
-  long chunk = (long)(512ull * 1024 * 1024 / (slot ? slot : 1));
-  if (chunk > 65535) chunk = 65535;
-  if (chunk < 1) chunk = 1;
-  const long alloc_chunk = chunk < a.n ? chunk : a.n;
-  T* scratch = nullptr;
-  int rc = ctx->scratch_get(0, slot * (alloc_chunk > 0 ? alloc_chunk : 1),
-                            reinterpret_cast<void**>(&scratch));
-  if (rc) return rc;
-  for (long s0 = 0; s0 < a.n; s0 += chunk) {
-    const long cnt = (a.n - s0) < chunk ? (a.n - s0) : chunk;
-    b = a;
-    b.queue = nullptr;
-    b.list = nullptr;
-    b.scratch = scratch;
-    b.n = (int)cnt;
-    b.DtX = a.DtX + (size_t)s0 * a.ldd;
-    b.norms = a.norms + s0;
-    if (a.nlam > 1) b.lambda = a.lambda + s0;
-    b.outV = a.outV + (size_t)s0 * a.L;
-    b.outR = a.outR + (size_t)s0 * a.L;
-    b.outCount = a.outCount + s0;
-    if (a.stats) b.stats = a.stats + 2 * s0;
-    hipLaunchKernelGGL(kern, dim3((unsigned)cnt), dim3(64 * W), lds, st, b);
-    SPX_HIP(hipGetLastError());
-  }
-  return SPX_OK;
-}
-
-template <typename T, int W>
-int lars_nq(int p) {
-  const int per = Vec16<T>::N * 64 * W;
-  const int nq = (p + per - 1) / per;
-  return nq <= 1 ? 1 : nq <= 2 ? 2 : nq <= 4 ? 4 : nq <= 8 ? 8 : nq <= 16 ? 16 : -1;
-}
-
-// One pass of group width W: CAP = FastCap (fast) or 0 (full capacity).
-template <typename T, int W, bool FAST>
-int lars_pass(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter, long max_blocks) {
-  constexpr int C = FAST ? FastCap<T>::C : 0;
-  switch (lars_nq<T, W>(a.p)) {
-    case 1: return lars_launch<T, W, 1, C>(ctx, a, persistent, counter, max_blocks);
-    case 2: return lars_launch<T, W, 2, C>(ctx, a, persistent, counter, max_blocks);
-    case 4: return lars_launch<T, W, 4, C>(ctx, a, persistent, counter, max_blocks);
-    case 8: return lars_launch<T, W, 8, C>(ctx, a, persistent, counter, max_blocks);
-    case 16: return lars_launch<T, W, 16, C>(ctx, a, persistent, counter, max_blocks);
-    default: return -1;  // does not fit this group width
-  }
-}
-
-template <typename T>
 int lars_pass_w(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
                 int* counter, long max_blocks) {
-  switch (w) {
-    case 1: return fast ? lars_pass<T, 1, true>(ctx, a, persistent, counter, max_blocks)
-                        : lars_pass<T, 1, false>(ctx, a, persistent, counter, max_blocks);
-    case 2: return fast ? lars_pass<T, 2, true>(ctx, a, persistent, counter, max_blocks)
-                        : lars_pass<T, 2, false>(ctx, a, persistent, counter, max_blocks);
-    case 4: return fast ? lars_pass<T, 4, true>(ctx, a, persistent, counter, max_blocks)
-                        : lars_pass<T, 4, false>(ctx, a, persistent, counter, max_blocks);
-    default: return fast ? lars_pass<T, 16, true>(ctx, a, persistent, counter, max_blocks)
-                         : lars_pass<T, 16, false>(ctx, a, persistent, counter, max_blocks);
-  }
+  if (a.weights) return lars_weighted_pass<T>(ctx, w, fast, a, persistent, counter, max_blocks);
+  return lars_pass_wt<T, false>(ctx, w, fast, a, persistent, counter, max_blocks);
 }
 
 // Batched form: fast pass (capacity FastCap) over every signal in work order,

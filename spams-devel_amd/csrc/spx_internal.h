@@ -56,6 +56,11 @@ struct LarsArgs {
   T* path;          // signal 0's regularisation path (p x path_cols) or null
   int path_cols;
   int ols;          // least-squares refit of the final active set
+  // Weighted penalty (coreLARS2W): per-signal atom weights, p x n with
+  // leading dimension ldw, or null for the plain penalty.
+  const T* weights;
+  int ldw;
+  T tol;            // stop tolerance of |step|, normX and the mode margins
 };
 
 // Kernel arguments of the ORMP solver (coreORMP semantics).
@@ -92,6 +97,11 @@ template <typename T>
 int col_sqnorms(hipStream_t st, int m, int n, const T* X, int ldx, T* out);
 template <typename T>
 int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves);
+// One LARS pass of the weighted-penalty kernels (group width w; fast = LDS
+// capacity pass); -1 when the width does not cover p.
+template <typename T>
+int lars_weighted_pass(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
+                       int* counter, long max_blocks);
 template <typename T>
 int omp_solve(Ctx* ctx, const OmpArgs<T>& a, int form, int waves);
 template <typename T>

@@ -353,6 +353,41 @@ def lasso(X, D=None, Q=None, q=None, return_reg_path=False, L=-1,
     return (A, path) if return_reg_path else A
 
 
+def lassoWeighted(X, D, W, L=-1, lambda1=None, mode=PENALTY, pos=False,
+                  numThreads=-1, verbose=False, precision="fp32", form="batched",
+                  waves_per_signal=0, ctx=None):
+    """Weighted LARS-Lasso (spams.lassoWeighted, swig/python/spams.py:155-164):
+    signal i's penalty is lambda1 * sum_j W[j, i] |a_j| (W: p x n, positive).
+    ``lambda1`` may also be a length-n vector.  ``numThreads`` is accepted and
+    ignored."""
+    if lambda1 is None:
+        raise ValueError("lassoWeighted : lambda1 must be defined")
+    dtype = np.asarray(X).dtype
+    sfx = _sfx(dtype)
+    X = np.asfortranarray(X, dtype=dtype)
+    D = np.asfortranarray(D, dtype=dtype)
+    W = np.asfortranarray(W, dtype=dtype)
+    m, n = X.shape
+    if D.shape[0] != m:
+        raise ValueError("lassoWeighted : incompatible matrix dimensions")
+    p = D.shape[1]
+    if W.shape != (p, n):
+        raise ValueError("lassoWeighted : inconsistent dimensions of matrix W")
+    lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lambda1, dtype=dtype)))
+    if lam.size not in (1, n):
+        raise ValueError("lassoWeighted : lambda1 must be a scalar or have one entry per signal")
+    o = spx_lasso_opts(L, mode, int(pos), -1, PREC[precision], FORM[form],
+                       waves_per_signal, 0)
+    out = spx_csc()
+    ctx = ctx or default_context()
+    f = getattr(lib(), "spx_lasso_weighted_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _P, _I,
+                  ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_csc)]
+    _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(W), _ptr(lam), lam.size,
+             ctypes.byref(o), ctypes.byref(out)))
+    return _csc_to_scipy(out, dtype)
+
+
 def omp(X, D, L=None, eps=None, lambda1=None, return_reg_path=False,
         numThreads=-1, precision="fp32", form="batched", waves_per_signal=0,
         ctx=None):
@@ -470,6 +505,24 @@ def lasso_dev(ctx, d: Dictionary, X, lam, V, R, count, lambda2=0.0, L=-1,
                   ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_lists_dev)]
     _check(f(ctx.h, d.h, _tptr(X), X.shape[1], n, _tptr(lam), lam.numel(),
              lambda2, ctypes.byref(o), ctypes.byref(lists)))
+
+
+def lasso_weighted_dev(ctx, d: Dictionary, X, W, lam, V, R, count, L=-1,
+                       mode=PENALTY, pos=False, precision="fp32", form="batched",
+                       waves_per_signal=0, stats=None):
+    """Device weighted LARS: X torch (n, m), W torch (n, p) (column-major
+    p x n weights), other arguments as lasso_dev."""
+    n = X.shape[0]
+    Lout = V.shape[1]
+    sfx = "f32" if d.dtype == np.float32 else "f64"
+    o = spx_lasso_opts(L, mode, int(pos), -1, PREC[precision], FORM[form],
+                       waves_per_signal)
+    lists = spx_lists_dev(_tptr(V), _tptr(R), _tptr(count), Lout, _tptr(stats))
+    f = getattr(lib(), "spx_lasso_weighted_dev_" + sfx)
+    f.argtypes = [_P, _P, _P, _I, _I, _P, _I, _P, _I,
+                  ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_lists_dev)]
+    _check(f(ctx.h, d.h, _tptr(X), X.shape[1], n, _tptr(W), W.stride(0), _tptr(lam),
+             lam.numel(), ctypes.byref(o), ctypes.byref(lists)))
 
 
 def lasso_dtx_dev(ctx, d: Dictionary, DtX, norms, lam, V, R, count, lambda2=0.0,

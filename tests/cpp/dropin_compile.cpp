@@ -21,6 +21,9 @@ void use_lasso() {
                                         false, -1, 0.15f, 0.f, PENALTY, false, false, -1, -1,
                                         false, false);
   delete s4;
+  Matrix<float> Wf(256, 5);
+  SpMatrix<float>* s5 = _lassoWeighted<float>(&Xf, &Df, &Wf, -1, 0.15f, PENALTY, false, -1, false);
+  delete s5;
 }
 
 void use_omp() {

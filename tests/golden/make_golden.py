@@ -120,6 +120,16 @@ def main():
         pB, r, v = csc_parts(A)
         cases.append((f"omppath_{tag}", dict(kind="omp", X=X, D=D, L=np.asarray(12, np.int32),
                                              pB=pB, r=r, v=v, path=path)))
+        # weighted penalty (_lassoWeighted): positive weights per atom and signal
+        X, D = small(19, 64, 256, 40, 5, 0.05, True, dt)
+        Wt = np.random.default_rng(20).uniform(0.5, 2.0, (256, 40)).astype(dt)
+        Wt = np.asfortranarray(Wt)
+        for mode, lam, pos in ((2, 0.15, 0), (2, 0.05, 0), (2, 0.1, 1), (0, 0.8, 0), (1, 0.3, 0)):
+            A = oracle.ref_lasso_weighted(X, D, Wt, lam, mode=mode, pos=bool(pos))
+            pB, r, v = csc_parts(A)
+            cases.append((f"lassoweighted_{tag}_mode{mode}_lam{lam}_pos{pos}", dict(
+                kind="lasso", X=X, D=D, W=Wt, lam=np.asarray(lam, dt), mode=mode, pos=pos,
+                pB=pB, r=r, v=v)))
     only_new = "--all" not in sys.argv
     wrote = 0
     for name, d in cases:

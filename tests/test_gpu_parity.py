@@ -514,3 +514,83 @@ def test_fp32_objective_on_the_headline_batch(spx, orc):
         A = dense(spx.lasso(Xs, D, lambda1=ls, form=form)).astype(np.float64)
         gap = (lasso_obj(Xs, D, A, ls) - fref) / np.abs(fref)
         assert np.abs(gap).max() <= 1e-5, (form, int(np.argmax(np.abs(gap))), float(np.abs(gap).max()))
+
+
+# -------------------------------------------------------- weighted penalty
+
+def _golden_dense(g):
+    ref = np.zeros((g["D"].shape[1], g["X"].shape[1]))
+    for j in range(ref.shape[1]):
+        ref[g["r"][g["pB"][j]:g["pB"][j + 1]], j] = g["v"][g["pB"][j]:g["pB"][j + 1]]
+    return ref
+
+
+@pytest.mark.parametrize("name", golden_files("lassoweighted_"))
+def test_lasso_weighted_matches_golden(spx, name):
+    print("case", name, flush=True)
+    g = load_golden(name)
+    ref = _golden_dense(g)
+    f64 = "f64" in name
+    for form, w in (("batched", 0), ("batched", 4), ("plain", 0)):
+        A = dense(spx.lassoWeighted(g["X"], g["D"], g["W"], lambda1=g["lam"].item(),
+                                    mode=int(g["mode"]), pos=bool(g["pos"]), form=form,
+                                    waves_per_signal=w))
+        same = np.all((A != 0) == (ref != 0), axis=0)
+        if f64:
+            assert same.all(), (form, w)
+            np.testing.assert_allclose(A, ref, atol=1e-9)
+        else:
+            assert same.mean() >= 0.9, (form, w)
+            assert np.abs(A[:, same] - ref[:, same]).max(initial=0) <= 2e-4
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_lasso_weighted_headline_shape_with_handover(spx, orc, dt):
+    """m=256, p=1024, per-signal lambda down to 0.02 and weights U[0.5, 2]:
+    many active sets outgrow the LDS pass and resume in the full pass."""
+    print("case lasso weighted headline-shape", dt, flush=True)
+    X, D, lam = W.lars_headline(seed=78, n=500, dtype=dt)
+    lam = (lam * 0.4).astype(dt)
+    Wt = np.asfortranarray(np.random.default_rng(79).uniform(0.5, 2.0, (1024, 500)).astype(dt))
+    A = dense(spx.lassoWeighted(X, D, Wt, lambda1=lam))
+    B = dense(orc.lasso_weighted(X, D, Wt, lam))
+    assert (B != 0).sum(0).max() > 24  # the hand-over is exercised
+    same = np.all((A != 0) == (B != 0), axis=0)
+    if dt == np.float64:
+        assert same.all()
+        np.testing.assert_allclose(A, B, atol=1e-9)
+    else:
+        assert same.mean() >= 0.95
+        assert np.abs(A[:, same] - B[:, same]).max() <= 2e-4
+
+
+def test_lasso_weighted_device_entry_with_padded_weights(spx, orc):
+    import torch
+    print("case lasso weighted dev", flush=True)
+    X, D = rand_problem(80, 64, 256, 200, np.float64)
+    Wt = np.random.default_rng(81).uniform(0.5, 2.0, (256, 200))
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to("cuda:0")
+    Wpad = torch.zeros((200, 260), dtype=torch.float64, device="cuda:0")
+    Wpad[:, :256] = torch.from_numpy(np.ascontiguousarray(Wt.T)).to("cuda:0")
+    lam = torch.full((1,), 0.1, dtype=torch.float64, device="cuda:0")
+    Lout = spx.lasso_L(64, 256, lambda1=0.1)
+    V = torch.zeros((200, Lout), dtype=torch.float64, device="cuda:0")
+    R = torch.zeros((200, Lout), dtype=torch.int32, device="cuda:0")
+    cnt = torch.zeros((200,), dtype=torch.int32, device="cuda:0")
+    spx.lasso_weighted_dev(ctx, d, Xt, Wpad[:, :256], lam, V, R, cnt)
+    torch.cuda.synchronize()
+    A = np.zeros((256, 200))
+    Vh, Rh, ch = V.cpu().numpy(), R.cpu().numpy(), cnt.cpu().numpy()
+    for i in range(200):
+        A[Rh[i, :ch[i]], i] = Vh[i, :ch[i]]
+    B = dense(orc.lasso_weighted(X, D, Wt, 0.1))
+    assert np.array_equal(A != 0, B != 0)
+    np.testing.assert_allclose(A, B, atol=1e-9)
+    # weights with a leading dimension below p are rejected
+    Wbad = torch.ones((200, 250), dtype=torch.float64, device="cuda:0")
+    with pytest.raises(ValueError):
+        spx.lasso_weighted_dev(ctx, d, Xt, Wbad, lam, V, R, cnt)
+    d.close()
+    ctx.close()

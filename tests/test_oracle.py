@@ -343,3 +343,63 @@ def test_omp_path_fixture_is_consistent(orc, name):
         tol = 1e-9 if g["X"].dtype == np.float64 else 1e-3
         np.testing.assert_allclose(path[S, j], c, rtol=tol, atol=tol)
     assert not path[:, k:].any()
+
+
+# ------------------------------------------------------- weighted penalty
+
+@pytest.mark.parametrize("name", golden_files("lassoweighted_"))
+def test_lasso_weighted_matches_golden(orc, name):
+    import scipy.sparse as sp
+    g = load_golden(name)
+    A = orc.lasso_weighted(g["X"], g["D"], g["W"], g["lam"].item(), mode=int(g["mode"]),
+                           pos=bool(g["pos"]))
+    ref = sp.csc_matrix((g["v"], g["r"], g["pB"]), shape=(g["D"].shape[1], g["X"].shape[1]))
+    assert csc_equal(A, ref)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_lasso_weighted_matches_reference_build(orc, dt):
+    if not orc.has_ref():
+        pytest.skip("reference build absent")
+    X, D = _rand(31, 24, 60, 40, dt)
+    Wt = np.random.default_rng(32).uniform(0.3, 3.0, (60, 40)).astype(dt)
+    for mode, lam in ((2, 0.1), (0, 0.9), (1, 0.25)):
+        for pos in (False, True):
+            a = orc.lasso_weighted(X, D, Wt, lam, mode=mode, pos=pos)
+            b = orc.ref_lasso_weighted(X, D, Wt, lam, mode=mode, pos=pos)
+            assert csc_equal(a, b), (mode, pos)
+
+
+def test_lasso_weighted_identity_dictionary_is_weighted_soft_threshold(orc):
+    # D = I: a_j = soft(x_j, lambda w_j) / (1 + 1e-10)
+    rng = np.random.default_rng(33)
+    p = 20
+    X = np.asfortranarray(rng.standard_normal((p, 12)))
+    Wt = np.asfortranarray(rng.uniform(0.5, 2.0, (p, 12)))
+    for lam in (0.1, 0.4):
+        A = dense(orc.lasso_weighted(X, np.eye(p), Wt, lam))
+        np.testing.assert_allclose(A, soft(X, lam * Wt) / (1 + 1e-10), rtol=0, atol=1e-12)
+
+
+def _weighted_kkt_violation(X, D, Wt, A, lam):
+    # |d_j' r| <= lam w_j, with d_j' r = lam w_j sign(a_j) on the support
+    R = X - D @ A
+    C = D.T @ R
+    worst = 0.0
+    for i in range(X.shape[1]):
+        sup = A[:, i] != 0
+        worst = max(worst, np.max(np.abs(C[:, i]) - lam * Wt[:, i]))
+        if sup.any():
+            worst = max(worst, np.max(np.abs(C[sup, i] - lam * Wt[sup, i] * np.sign(A[sup, i]))))
+    return worst
+
+
+def test_lasso_weighted_kkt_conditions(orc):
+    X, D = _rand(34, 32, 80, 30, np.float64)
+    Wt = np.asfortranarray(np.random.default_rng(35).uniform(0.5, 2.0, (80, 30)))
+    lam = 0.08
+    A = dense(orc.lasso_weighted(X, D, Wt, lam))
+    assert _weighted_kkt_violation(X, D, Wt, A, lam) < 1e-8
+    # a transposed or dropped weight breaks the conditions
+    B = dense(orc.lasso_weighted(X, D, np.ones_like(Wt), lam))
+    assert _weighted_kkt_violation(X, D, Wt, B, lam) > 1e-3
