@@ -241,6 +241,22 @@ int spx_lasso_path_f64(spx_ctx* ctx, const double* X, int m, int n,
 int spx_lasso_f64(spx_ctx* ctx, const double* X, int m, int n,
                   const double* D, int p, const double* lambda, int n_lambda,
                   double lambda2, const spx_lasso_opts* opts, spx_csc* out);
+/* LARS over partly observed signals: replaces _lassoMask
+ * (cpp_library/spams.h:190-215), i.e. lasso_mask (decomp/decomp.h:1466-1561).
+ * B: host m x n bytes, nonzero = row observed.  A signal with every row
+ * observed runs the plain path; with none, its column is empty; otherwise
+ * the path over its observed rows (Gram columns of those rows, ridge
+ * max(lambda2, 1e-10)) with the constraint scaled by the observed fraction
+ * (PENALTY, L2ERROR).  Step cap 4 min(L, p); max_length_path, ols and
+ * SPX_PREC_REFIT64 do not apply (the last two are rejected). */
+int spx_lasso_mask_f32(spx_ctx* ctx, const float* X, int m, int n,
+                       const float* D, int p, const unsigned char* B,
+                       const float* lambda, int n_lambda, float lambda2,
+                       const spx_lasso_opts* opts, spx_csc* out);
+int spx_lasso_mask_f64(spx_ctx* ctx, const double* X, int m, int n,
+                       const double* D, int p, const unsigned char* B,
+                       const double* lambda, int n_lambda, double lambda2,
+                       const spx_lasso_opts* opts, spx_csc* out);
 /* Weighted LARS: replaces _lassoWeighted (cpp_library/spams.h:217-249), i.e.
  * lassoWeight + coreLARS2W (decomp/decomp.h:1315-1391, :1917-2107): signal i
  * minimises 0.5 ||x_i - D a||^2 + lambda sum_j W(j, i) |a_j| (PENALTY; the
@@ -281,6 +297,24 @@ int spx_omp_path_f64(spx_ctx* ctx, const double* X, int m, int n,
                      const double* eps, int n_eps, const double* lambda,
                      int n_lambda, const spx_omp_opts* opts, spx_csc* out,
                      double* path);
+
+/* ORMP over partly observed signals: replaces _ompMask
+ * (cpp_library/spams.h:295-343), i.e. omp_mask (decomp/decomp.h:406-519).
+ * B: host m x n bytes, nonzero = row observed.  A signal with every row
+ * observed runs the plain path; with none, its column is empty; otherwise
+ * ORMP over its observed rows (their Gram columns plus 1e-10 on the
+ * diagonal) with eps scaled by the observed fraction.  `path` as in
+ * spx_omp_path_* (may be NULL). */
+int spx_omp_mask_f32(spx_ctx* ctx, const float* X, int m, int n,
+                     const float* D, int p, const unsigned char* B,
+                     const int* L, int nL, const float* eps, int n_eps,
+                     const float* lambda, int n_lambda,
+                     const spx_omp_opts* opts, spx_csc* out, float* path);
+int spx_omp_mask_f64(spx_ctx* ctx, const double* X, int m, int n,
+                     const double* D, int p, const unsigned char* B,
+                     const int* L, int nL, const double* eps, int n_eps,
+                     const double* lambda, int n_lambda,
+                     const spx_omp_opts* opts, spx_csc* out, double* path);
 
 /* FISTA-flat, square loss + l1 / elastic-net: replaces _fistaFlat
  * (cpp_library/spams.h:460-608).  W (p x n) is written; optim_info is 4 x n
@@ -341,6 +375,18 @@ int spx_lasso_weighted_dev_f64(spx_ctx* ctx, const spx_dict* dict,
                                int n_lambda, const spx_lasso_opts* opts,
                                spx_lists_dev* out);
 
+/* Device masked LARS (spx_lasso_mask_* on a resident dictionary): B is a
+ * device m x n byte mask with leading dimension ldb >= m. */
+int spx_lasso_mask_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* X,
+                           int ldx, int n, const unsigned char* B, int ldb,
+                           const float* lambda, int n_lambda, float lambda2,
+                           const spx_lasso_opts* opts, spx_lists_dev* out);
+int spx_lasso_mask_dev_f64(spx_ctx* ctx, const spx_dict* dict,
+                           const double* X, int ldx, int n,
+                           const unsigned char* B, int ldb,
+                           const double* lambda, int n_lambda, double lambda2,
+                           const spx_lasso_opts* opts, spx_lists_dev* out);
+
 /* Device LARS from precomputed correlations (the Gram-given form of the
  * reference, lasso2(X, G, DtX, ...) decomp/decomp.h:1574-1646 and _lassoQq
  * cpp_library/spams.h:145-188): G is the dictionary's resident Gram, DtX a
@@ -360,6 +406,20 @@ int spx_sqnorms_dev_f32(spx_ctx* ctx, const float* X, int m, int ldx, int n,
                         float* norms);
 int spx_sqnorms_dev_f64(spx_ctx* ctx, const double* X, int m, int ldx, int n,
                         double* norms);
+
+/* Device masked ORMP (spx_omp_mask_* on a resident dictionary): B is a
+ * device m x n byte mask with leading dimension ldb >= m; L/eps/lambda as in
+ * spx_omp_dev_*. */
+int spx_omp_mask_dev_f32(spx_ctx* ctx, const spx_dict* dict, const float* X,
+                         int ldx, int n, const unsigned char* B, int ldb,
+                         const int* L, int nL, const float* eps, int n_eps,
+                         const float* lambda, int n_lambda,
+                         const spx_omp_opts* opts, spx_lists_dev* out);
+int spx_omp_mask_dev_f64(spx_ctx* ctx, const spx_dict* dict, const double* X,
+                         int ldx, int n, const unsigned char* B, int ldb,
+                         const int* L, int nL, const double* eps, int n_eps,
+                         const double* lambda, int n_lambda,
+                         const spx_omp_opts* opts, spx_lists_dev* out);
 
 /* Device ORMP over a resident dictionary.  L/eps/lambda are device arrays
  * (NULL = not given; size 1 or n). */

@@ -195,6 +195,69 @@ def lasso_weighted(X, D, W, lambda1, L=-1, mode=PENALTY, pos=False, num_threads=
     return _csc_from_lists(vM, rM, Le, n, p, dtype)
 
 
+def lasso_mask(X, D, B, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
+               num_threads=-1):
+    """_lassoMask + lasso_mask + coreLARS2 (restated).  B: m x n mask
+    (nonzero = observed).  A column with no observed row is empty."""
+    dtype = X.dtype
+    X = _f(X, dtype)
+    D = _f(D, dtype)
+    Bm = np.asfortranarray(B != 0, dtype=np.uint8)
+    m, n = X.shape
+    p = D.shape[1]
+    sfx = _sfx(dtype)
+    ct = _ct(dtype)
+    fL = getattr(lib(), "orc_lasso_L_" + sfx)
+    fL.restype = _I
+    fL.argtypes = [_I, _I, _I, _I, ct, _I, ct]
+    Le = fL(m, p, L, mode, lambda1, int(pos), lambda2)
+    if Le <= 0:
+        return sp.csc_matrix((p, n), dtype=dtype)
+    vM = np.zeros((Le, n), dtype, order="F")
+    rM = np.full((Le, n), -1, np.int32, order="F")
+    f = getattr(lib(), "orc_lasso_mask_" + sfx)
+    f.restype = _I
+    f.argtypes = [_P, _I, _I, _P, _I, _P, _I, ct, ct, _I, _I, _I, _P, _P]
+    f(_ptr(X), m, n, _ptr(D), p, _ptr(Bm), L, lambda1, lambda2, mode, int(pos), num_threads,
+      _ptr(vM), _ptr(rM))
+    return _csc_from_lists(vM, rM, Le, n, p, dtype)
+
+
+def omp_mask(X, D, B, L=None, eps=None, lambda1=None, num_threads=-1):
+    """_ompMask + omp_mask + coreORMP (restated)."""
+    dtype = X.dtype
+    X = _f(X, dtype)
+    D = _f(D, dtype)
+    Bm = np.asfortranarray(B != 0, dtype=np.uint8)
+    m, n = X.shape
+    p = D.shape[1]
+    sfx = _sfx(dtype)
+
+    def vec(a, dt):
+        if a is None:
+            return np.zeros(1, dt), 0
+        a = np.atleast_1d(np.asarray(a, dtype=dt))
+        return a, a.size
+
+    Ls, nL = vec(L, np.int32)
+    E, nE = vec(eps, dtype)
+    La, nLa = vec(lambda1, dtype)
+    fL = getattr(lib(), "orc_omp_L_" + sfx)
+    fL.restype = _I
+    fL.argtypes = [_I, _I, _P, _I, _I]
+    Le = fL(m, p, _ptr(Ls), nL, n)
+    if Le <= 0:
+        return sp.csc_matrix((p, n), dtype=dtype)
+    vM = np.zeros((Le, n), dtype, order="F")
+    rM = np.full((Le, n), -1, np.int32, order="F")
+    f = getattr(lib(), "orc_omp_mask_" + sfx)
+    f.restype = _I
+    f.argtypes = [_P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I, _I, _P, _P, _P]
+    f(_ptr(X), m, n, _ptr(D), p, _ptr(Bm), _ptr(Ls), nL, _ptr(E), nE, _ptr(La), nLa,
+      num_threads, _ptr(vM), _ptr(rM), None)
+    return _csc_from_lists(vM, rM, Le, n, p, dtype)
+
+
 def omp(X, D, L=None, eps=None, lambda1=None, num_threads=-1):
     """_omp + omp + coreORMP (restated).  L/eps/lambda1: None (not given),
     scalar, or length-n vector."""

@@ -269,13 +269,22 @@ static int products(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
   return col_sqnorms<T>(ctx->stream, d->m, n, X, ldx, *norms);
 }
 
+// Masked-signal inputs of the solvers (device): class per signal (0 / 1 / 2
+// for no / all / some rows observed) and the m x n row mask.
+struct MaskIn {
+  const unsigned char* cls;
+  const unsigned char* B;
+  int ldb;
+};
+
 // W (device, p x n, leading dimension ldw) selects the weighted penalty of
-// lassoWeight / coreLARS2W (decomp/decomp.h:1315-1391, :1917-2107).
+// lassoWeight / coreLARS2W (decomp/decomp.h:1315-1391, :1917-2107); mi the
+// masked signals of lasso_mask (decomp/decomp.h:1466-1561).
 template <typename T>
 static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
                          const T* norms, int n, const T* lambda, int nlam, T lambda2,
                          const spx_lasso_opts* o, spx_lists_dev* out,
-                         const T* W = nullptr, int ldw = 0) {
+                         const T* W = nullptr, int ldw = 0, const MaskIn* mi = nullptr) {
   if (!ctx || !d || !o || !out || n < 0 || (n > 0 && (!DtX || !norms || !lambda)) ||
       ldd < d->p || !(nlam == 1 || nlam == n)) {
     set_error("lasso_dev: invalid arguments");
@@ -296,6 +305,10 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   if (W && (ldw < d->p || o->ols || (o->precision & SPX_PREC_REFIT64) ||
             (out->path && out->path_cols > 0))) {
     set_error("lassoWeighted: invalid arguments (ldw < p, or ols / refit / path requested)");
+    return SPX_ERR_ARG;
+  }
+  if (mi && (o->ols || (o->precision & SPX_PREC_REFIT64) || (out->path && out->path_cols > 0))) {
+    set_error("lassoMask: ols / refit / path are not options of the masked solver");
     return SPX_ERR_ARG;
   }
   if (n == 0) return SPX_OK;
@@ -341,6 +354,16 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
     a.tol = T(1e-10);
     a.length_path = 4 * (a.L < a.p ? a.L : a.p) + 2;
   }
+  if (mi) {
+    // coreLARS2's default step cap 4 min(L, p) (decomp/decomp.h:1724)
+    a.mclass = mi->cls;
+    a.mask = mi->B;
+    a.ldmask = mi->ldb;
+    a.Dd = static_cast<const T*>(d->D);
+    a.Dt = static_cast<const T*>(d->Dt);
+    a.m = d->m;
+    a.length_path = 4 * (a.L < a.p ? a.L : a.p);
+  }
   const int w = o->waves_per_signal > 0 ? o->waves_per_signal
                 : o->form == SPX_FORM_PLAIN ? 4 : tuned_waves(d, 0);
   return lars_solve<T>(ctx, a, o->form, w);
@@ -372,14 +395,17 @@ static int lasso_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx,
 template <typename T>
 static int omp_solve_dtx(spx_ctx* ctx, const spx_dict* d, const T* DtX, const T* norms, int n,
                          const int* L, int nL, const T* eps, int nEps, const T* lambda,
-                         int nLambda, const spx_omp_opts* o, spx_lists_dev* out);
+                         int nLambda, const spx_omp_opts* o, spx_lists_dev* out,
+                         const MaskIn* mi = nullptr);
 
 template <typename T>
+// B (device m x n bytes, leading dimension ldb, nonzero = observed): the
+// masked signals of omp_mask (decomp/decomp.h:406-519), or null.
 static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
                    const int* L, int nL, const T* eps, int nEps,
                    const T* lambda, int nLambda, const spx_omp_opts* o,
-                   spx_lists_dev* out) {
-  if (!ctx || !d || !out || n < 0 || ldx < d->m) {
+                   spx_lists_dev* out, const unsigned char* B = nullptr, int ldb = 0) {
+  if (!ctx || !d || !out || n < 0 || ldx < d->m || (B && ldb < d->m)) {
     set_error("omp_dev: invalid arguments");
     return SPX_ERR_ARG;
   }
@@ -405,6 +431,25 @@ static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
   if (!o) o = &def;
   T* DtX = nullptr;
   T* norms = nullptr;
+  if (B) {
+    // masked copies, classes and eps * |M| / m (decomp/decomp.h:489)
+    if (int rc = need_dictionary(d)) return rc;
+    const int m = d->m;
+    void *bx = nullptr, *bc = nullptr, *be = nullptr;
+    int rc = ctx->scratch_get(18, sizeof(T) * (size_t)m * n, &bx);
+    if (!rc) rc = ctx->scratch_get(19, (size_t)n, &bc);
+    if (!rc) rc = ctx->scratch_get(20, sizeof(T) * (size_t)n, &be);
+    if (rc) return rc;
+    T* Xm = static_cast<T*>(bx);
+    unsigned char* cls = static_cast<unsigned char*>(bc);
+    T* epsv = static_cast<T*>(be);
+    rc = mask_prep<T>(ctx->stream, X, ldx, m, n, B, ldb, Xm, cls, eps, nEps, epsv, 1);
+    if (!rc) rc = products<T>(ctx, d, Xm, m, n, o->precision, &DtX, &norms);
+    if (rc) return rc;
+    const MaskIn mi{cls, B, ldb};
+    return omp_solve_dtx<T>(ctx, d, DtX, norms, n, L, nL, nEps ? epsv : nullptr, nEps ? n : 0,
+                            lambda, nLambda, o, out, &mi);
+  }
   int rc = products<T>(ctx, d, X, ldx, n, o->precision, &DtX, &norms);
   if (rc) return rc;
   return omp_solve_dtx<T>(ctx, d, DtX, norms, n, L, nL, eps, nEps, lambda, nLambda, o, out);
@@ -415,8 +460,17 @@ static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
 template <typename T>
 static int omp_solve_dtx(spx_ctx* ctx, const spx_dict* d, const T* DtX, const T* norms, int n,
                          const int* L, int nL, const T* eps, int nEps, const T* lambda,
-                         int nLambda, const spx_omp_opts* o, spx_lists_dev* out) {
+                         int nLambda, const spx_omp_opts* o, spx_lists_dev* out,
+                         const MaskIn* mi) {
   OmpArgs<T> a{};
+  if (mi) {
+    a.mclass = mi->cls;
+    a.mask = mi->B;
+    a.ldmask = mi->ldb;
+    a.Dd = static_cast<const T*>(d->D);
+    a.Dt = static_cast<const T*>(d->Dt);
+    a.m = d->m;
+  }
   a.G = static_cast<const T*>(d->G);
   a.ldg = d->ldg;
   a.DtX = DtX;
@@ -602,6 +656,48 @@ static int omp_autotune(spx_ctx* ctx, spx_dict* d, const T* X, int ldx, int n, c
 }
 
 // Device CSC of the lists, then download into a host spx_csc.
+// lasso_mask (decomp/decomp.h:1466-1561): B (device, m x n bytes, leading
+// dimension ldb; nonzero = observed).  A signal with every row observed runs
+// the plain path; with none, its column is empty; otherwise coreLARS2 over
+// its observed rows with the constraint scaled by |M| / m (PENALTY and
+// L2ERROR) and the Gram columns of its rows, ridge max(lambda2, 1e-10).
+template <typename T>
+static int lasso_mask_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
+                          const unsigned char* B, int ldb, const T* lambda, int nlam, T lambda2,
+                          const spx_lasso_opts* o, spx_lists_dev* out) {
+  if (!ctx || !d || !o || !out || n < 0 || (n > 0 && (!X || !lambda || !B)) || ldx < d->m ||
+      ldb < d->m || !(nlam == 1 || nlam == n)) {
+    set_error("lassoMask: invalid arguments");
+    return SPX_ERR_ARG;
+  }
+  if ((int)sizeof(T) != d->elem) {
+    set_error("lasso_dev: dictionary precision does not match the entry point");
+    return SPX_ERR_ARG;
+  }
+  if (int rc = need_dictionary(d)) return rc;
+  if (n == 0) return SPX_OK;
+  DeviceGuard g(ctx->device);
+  const int m = d->m;
+  void *bx = nullptr, *bc = nullptr, *bl = nullptr;
+  int rc = ctx->scratch_get(18, sizeof(T) * (size_t)m * n, &bx);
+  if (!rc) rc = ctx->scratch_get(19, (size_t)n, &bc);
+  if (!rc) rc = ctx->scratch_get(20, sizeof(T) * (size_t)n, &bl);
+  if (rc) return rc;
+  T* Xm = static_cast<T*>(bx);
+  unsigned char* cls = static_cast<unsigned char*>(bc);
+  T* lamv = static_cast<T*>(bl);
+  const int scale = o->mode == SPX_PENALTY || o->mode == SPX_L2ERROR;
+  rc = mask_prep<T>(ctx->stream, X, ldx, m, n, B, ldb, Xm, cls, lambda, nlam, lamv, scale);
+  if (rc) return rc;
+  T* DtX = nullptr;
+  T* norms = nullptr;
+  rc = products<T>(ctx, d, Xm, m, n, o->precision & SPX_PREC_BF16X3, &DtX, &norms);
+  if (rc) return rc;
+  const MaskIn mi{cls, B, ldb};
+  return lasso_dtx_dev<T>(ctx, d, DtX, d->ldg, norms, n, lamv, n, lambda2, o, out, nullptr, 0,
+                          &mi);
+}
+
 template <typename T>
 static int lists_to_host_csc(spx_ctx* ctx, const spx_lists_dev& l, int p, int n,
                              spx_csc* out) {
@@ -657,12 +753,13 @@ struct DevBuf {
   }
 };
 
-// W: host p x n weights of the weighted penalty (_lassoWeighted), or null.
+// W: host p x n weights of the weighted penalty (_lassoWeighted), or null;
+// B: host m x n observation mask of _lassoMask, or null.
 template <typename T>
 static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
                       const T* lambda, int nlam, T lambda2,
                       const spx_lasso_opts* opts, spx_csc* out, T* path = nullptr,
-                      const T* W = nullptr) {
+                      const T* W = nullptr, const unsigned char* B = nullptr) {
   if (!ctx || !out || !D || !lambda || m <= 0 || p <= 0 || n < 0 || (n > 0 && !X)) {
     set_error("lasso : invalid arguments");
     return SPX_ERR_ARG;
@@ -693,9 +790,10 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   spx_dict* d = nullptr;
   int rc = dict_create<T>(ctx, D, m, p, 0, o.precision & SPX_PREC_BF16X3, &d);
   if (rc) return rc;
-  DevBuf bX, bL, bV, bR, bC, bP, bW;
+  DevBuf bX, bL, bV, bR, bC, bP, bW, bB;
   if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bL.alloc(sizeof(T) * nlam)) ||
       (W && (rc = bW.alloc(sizeof(T) * (size_t)p * n))) ||
+      (B && (rc = bB.alloc((size_t)m * n))) ||
       (rc = bV.alloc(sizeof(T) * (size_t)Le * n)) ||
       (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) || (rc = bC.alloc(sizeof(int) * (size_t)n)) ||
       (path && pcols > 0 && (rc = bP.alloc(sizeof(T) * (size_t)p * pcols)))) {
@@ -706,7 +804,8 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   if (hipMemcpyAsync(bX.p, X, sizeof(T) * (size_t)m * n, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(bL.p, lambda, sizeof(T) * nlam, hipMemcpyHostToDevice, st) != hipSuccess ||
       (W && hipMemcpyAsync(bW.p, W, sizeof(T) * (size_t)p * n, hipMemcpyHostToDevice, st) !=
-                hipSuccess)) {
+                hipSuccess) ||
+      (B && hipMemcpyAsync(bB.p, B, (size_t)m * n, hipMemcpyHostToDevice, st) != hipSuccess)) {
     spx_dict_destroy(d);
     set_error("lasso: upload failed");
     return SPX_ERR_HIP;
@@ -716,9 +815,14 @@ static int lasso_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
     l.path = bP.p;
     l.path_cols = pcols;
   }
-  rc = lasso_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
-                    static_cast<const T*>(bL.p), nlam, lambda2, &o, &l,
-                    static_cast<const T*>(bW.p), p);
+  if (B)
+    rc = lasso_mask_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
+                           static_cast<const unsigned char*>(bB.p), m,
+                           static_cast<const T*>(bL.p), nlam, lambda2, &o, &l);
+  else
+    rc = lasso_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
+                      static_cast<const T*>(bL.p), nlam, lambda2, &o, &l,
+                      static_cast<const T*>(bW.p), p);
   if (!rc) rc = lists_to_host_csc<T>(ctx, l, p, n, out);
   if (!rc && path && pcols > 0 &&
       hipMemcpy(path, bP.p, sizeof(T) * (size_t)p * pcols, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -804,7 +908,7 @@ template <typename T>
 static int omp_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
                     const int* L, int nL, const T* eps, int nEps, const T* lambda,
                     int nLambda, const spx_omp_opts* opts, spx_csc* out,
-                    T* path = nullptr) {
+                    T* path = nullptr, const unsigned char* B = nullptr) {
   if (!ctx || !out || !D || m <= 0 || p <= 0 || n < 0 || (n > 0 && !X)) {
     set_error("omp : invalid arguments");
     return SPX_ERR_ARG;
@@ -839,8 +943,9 @@ static int omp_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   spx_dict* d = nullptr;
   int rc = dict_create<T>(ctx, D, m, p, 0, o.precision, &d);
   if (rc) return rc;
-  DevBuf bX, bLs, bE, bLa, bV, bR, bC, bP;
+  DevBuf bX, bLs, bE, bLa, bV, bR, bC, bP, bB;
   if ((rc = bX.alloc(sizeof(T) * (size_t)m * n)) || (rc = bLs.alloc(sizeof(int) * (nL ? nL : 1))) ||
+      (B && (rc = bB.alloc((size_t)m * n))) ||
       (rc = bE.alloc(sizeof(T) * (nEps ? nEps : 1))) || (rc = bLa.alloc(sizeof(T) * (nLambda ? nLambda : 1))) ||
       (rc = bV.alloc(sizeof(T) * (size_t)Le * n)) || (rc = bR.alloc(sizeof(int) * (size_t)Le * n)) ||
       (rc = bC.alloc(sizeof(int) * (size_t)n)) ||
@@ -853,6 +958,7 @@ static int omp_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   if (nL) ok = ok && hipMemcpyAsync(bLs.p, L, sizeof(int) * nL, hipMemcpyHostToDevice, st) == hipSuccess;
   if (nEps) ok = ok && hipMemcpyAsync(bE.p, eps, sizeof(T) * nEps, hipMemcpyHostToDevice, st) == hipSuccess;
   if (nLambda) ok = ok && hipMemcpyAsync(bLa.p, lambda, sizeof(T) * nLambda, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (B) ok = ok && hipMemcpyAsync(bB.p, B, (size_t)m * n, hipMemcpyHostToDevice, st) == hipSuccess;
   if (!ok) {
     spx_dict_destroy(d);
     set_error("omp: upload failed");
@@ -866,7 +972,8 @@ static int omp_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
   rc = omp_dev<T>(ctx, d, static_cast<const T*>(bX.p), m, n,
                   nL ? static_cast<const int*>(bLs.p) : nullptr, nL,
                   nEps ? static_cast<const T*>(bE.p) : nullptr, nEps,
-                  nLambda ? static_cast<const T*>(bLa.p) : nullptr, nLambda, &o, &l);
+                  nLambda ? static_cast<const T*>(bLa.p) : nullptr, nLambda, &o, &l,
+                  static_cast<const unsigned char*>(bB.p), m);
   if (!rc) rc = lists_to_host_csc<T>(ctx, l, p, n, out);
   if (!rc && path &&
       hipMemcpy(path, bP.p, sizeof(T) * (size_t)p * pcols, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -1221,6 +1328,37 @@ int spx_dict_create_gram_f64(spx_ctx* ctx, const double* Q, int m, int p, int on
                              spx_dict** out) {
   return spx::dict_create_gram<double>(ctx, Q, m, p, on_device, out);
 }
+int spx_lasso_mask_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
+                       const unsigned char* B, const float* lambda, int n_lambda, float lambda2,
+                       const spx_lasso_opts* opts, spx_csc* out) {
+  if (!B) {
+    spx::set_error("lassoMask: B is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::lasso_host<float>(ctx, X, m, n, D, p, lambda, n_lambda, lambda2, opts, out,
+                                nullptr, nullptr, B);
+}
+int spx_lasso_mask_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, int p,
+                       const unsigned char* B, const double* lambda, int n_lambda, double lambda2,
+                       const spx_lasso_opts* opts, spx_csc* out) {
+  if (!B) {
+    spx::set_error("lassoMask: B is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::lasso_host<double>(ctx, X, m, n, D, p, lambda, n_lambda, lambda2, opts, out,
+                                 nullptr, nullptr, B);
+}
+int spx_lasso_mask_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, int n,
+                           const unsigned char* B, int ldb, const float* lambda, int n_lambda,
+                           float lambda2, const spx_lasso_opts* o, spx_lists_dev* out) {
+  return spx::lasso_mask_dev<float>(ctx, d, X, ldx, n, B, ldb, lambda, n_lambda, lambda2, o, out);
+}
+int spx_lasso_mask_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx, int n,
+                           const unsigned char* B, int ldb, const double* lambda, int n_lambda,
+                           double lambda2, const spx_lasso_opts* o, spx_lists_dev* out) {
+  return spx::lasso_mask_dev<double>(ctx, d, X, ldx, n, B, ldb, lambda, n_lambda, lambda2, o,
+                                     out);
+}
 int spx_lasso_weighted_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
                            const float* W, const float* lambda, int n_lambda,
                            const spx_lasso_opts* opts, spx_csc* out) {
@@ -1272,6 +1410,50 @@ int spx_omp_path_f64(spx_ctx* ctx, const double* X, int m, int n, const double* 
                      int n_lambda, const spx_omp_opts* opts, spx_csc* out, double* path) {
   return spx::omp_host<double>(ctx, X, m, n, D, p, L, nL, eps, n_eps, lambda, n_lambda, opts,
                                out, path);
+}
+int spx_omp_mask_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,
+                     const unsigned char* B, const int* L, int nL, const float* eps, int n_eps,
+                     const float* lambda, int n_lambda, const spx_omp_opts* opts, spx_csc* out,
+                     float* path) {
+  if (!B) {
+    spx::set_error("ompMask: B is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::omp_host<float>(ctx, X, m, n, D, p, L, nL, eps, n_eps, lambda, n_lambda, opts, out,
+                              path, B);
+}
+int spx_omp_mask_f64(spx_ctx* ctx, const double* X, int m, int n, const double* D, int p,
+                     const unsigned char* B, const int* L, int nL, const double* eps, int n_eps,
+                     const double* lambda, int n_lambda, const spx_omp_opts* opts, spx_csc* out,
+                     double* path) {
+  if (!B) {
+    spx::set_error("ompMask: B is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::omp_host<double>(ctx, X, m, n, D, p, L, nL, eps, n_eps, lambda, n_lambda, opts,
+                               out, path, B);
+}
+int spx_omp_mask_dev_f32(spx_ctx* ctx, const spx_dict* d, const float* X, int ldx, int n,
+                         const unsigned char* B, int ldb, const int* L, int nL, const float* eps,
+                         int n_eps, const float* lambda, int n_lambda, const spx_omp_opts* o,
+                         spx_lists_dev* out) {
+  if (!B) {
+    spx::set_error("ompMask: B is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::omp_dev<float>(ctx, d, X, ldx, n, L, nL, eps, n_eps, lambda, n_lambda, o, out, B,
+                             ldb);
+}
+int spx_omp_mask_dev_f64(spx_ctx* ctx, const spx_dict* d, const double* X, int ldx, int n,
+                         const unsigned char* B, int ldb, const int* L, int nL,
+                         const double* eps, int n_eps, const double* lambda, int n_lambda,
+                         const spx_omp_opts* o, spx_lists_dev* out) {
+  if (!B) {
+    spx::set_error("ompMask: B is required");
+    return SPX_ERR_ARG;
+  }
+  return spx::omp_dev<double>(ctx, d, X, ldx, n, L, nL, eps, n_eps, lambda, n_lambda, o, out, B,
+                              ldb);
 }
 
 int spx_fista_flat_f32(spx_ctx* ctx, const float* X, int m, int n, const float* D, int p,

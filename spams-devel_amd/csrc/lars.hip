@@ -1,6 +1,6 @@
 // LARS-Lasso solver: launch logic of the batched and plain forms over the
-// kernels of lars_impl.h (plain penalty here; the weighted penalty's kernels
-// are compiled in lars_w.hip).
+// kernels of lars_impl.h (plain penalty here; the weighted-penalty and
+// masked-signal kernels are compiled in lars_w.hip and lars_m.hip).
 #include "lars_impl.h"
 
 namespace spx {
@@ -11,7 +11,8 @@ template <typename T>
 int lars_pass_w(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
                 int* counter, long max_blocks) {
   if (a.weights) return lars_weighted_pass<T>(ctx, w, fast, a, persistent, counter, max_blocks);
-  return lars_pass_wt<T, false>(ctx, w, fast, a, persistent, counter, max_blocks);
+  if (a.mclass) return lars_masked_pass<T>(ctx, w, fast, a, persistent, counter, max_blocks);
+  return lars_pass_wt<T, 0>(ctx, w, fast, a, persistent, counter, max_blocks);
 }
 
 // Batched form: fast pass (capacity FastCap) over every signal in work order,

@@ -30,6 +30,7 @@
 #include <climits>
 #include <type_traits>
 
+#include "mask_gram.h"
 #include "spx_common.h"
 #include "spx_internal.h"
 
@@ -90,8 +91,9 @@ struct LarsLds {
   static constexpr int GT = 64 * W;
   static constexpr int VEC = Vec16<T>::N;
   static constexpr int PT = VEC * GT * NQ;
-  // byte size for active-set capacity LC (wt: the per-row weights)
-  static size_t bytes(int LC, int CL = 0, bool wt = false) {
+  // byte size for active-set capacity LC; variant 1 (weighted) adds the
+  // per-row weights, variant 2 (masked) the Gram-column slot map
+  static size_t bytes(int LC, int CL = 0, int var = 0) {
     size_t b = sizeof(T) * (PT + GT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC;
     b = (b + 15) & ~size_t(15);
     b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
@@ -102,7 +104,8 @@ struct LarsLds {
     b = (b + 15) & ~size_t(15);
     b += sizeof(T) * (size_t)CL * CL + 16;  // LDS copy of the inverse (full pass)
     b = (b + 15) & ~size_t(15);
-    if (wt) b += sizeof(T) * PT + 16;  // weights of the group's rows
+    if (var == 1) b += sizeof(T) * PT + 16;  // weights of the group's rows
+    if (var == 2) b += sizeof(int) * 2 * (size_t)LC + 16;  // slot map + shift buffer
     return b;
   }
 };
@@ -277,7 +280,9 @@ __device__ __forceinline__ void inverse_drop(const T* Ms, int lds, T* Md, int ld
 // The full-capacity pass at 4 wavefronts is held to 4 workgroups per CU
 // (128 VGPRs): its signals are latency-bound, so occupancy pays more than
 // registers.
-template <typename T, int W, int NQ, int CAP, bool WT>
+// VAR: 0 plain penalty, 1 weighted penalty (a.weights), 2 masked signals
+// (a.mclass).
+template <typename T, int W, int NQ, int CAP, int VAR>
 __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_kernel(LarsArgs<T> a) {
   constexpr int GT = 64 * W;
   constexpr int VEC = Vec16<T>::N;
@@ -319,7 +324,13 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
   // weighted penalty: the signal's weights of the group's rows (1 past p)
   T* sWt = reinterpret_cast<T*>(
       (reinterpret_cast<uintptr_t>(sMl + CL * CL) + 15) & ~uintptr_t(15));
-  constexpr bool wt = WT;  // weighted penalty (a.weights set)
+  constexpr bool wt = VAR == 1;  // weighted penalty
+  constexpr bool mk = VAR == 2;  // masked signals
+  // masked: active position -> Gram-column slot, and its shift buffer
+  int* sSlot = reinterpret_cast<int*>(
+      (reinterpret_cast<uintptr_t>(sMl + CL * CL) + 15) & ~uintptr_t(15));
+  int* sSlotT = sSlot + LC;
+  T* const gaw = mk ? a.ga + (size_t)blockIdx.x * a.ga_stride : nullptr;
 
   const int tid = threadIdx.x;
   T* const M0 = CAP > 0 ? sM0 : a.scratch + (size_t)blockIdx.x * 2 * L * L;
@@ -402,8 +413,20 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
     int s_active = 0;
     int iter = 0, admissions = 0;
     bool empty = false, overflow = false, reached = false;
+    // masked: class 2 signals use Gram columns over their observed rows
+    const int mcls = mk ? a.mclass[sig] : 1;
+    const bool own = mk && mcls == 2;
+    const unsigned char* msk = own ? a.mask + (size_t)sig * a.ldmask : nullptr;
+    if (mk)
+      for (int j = tid; j < LC; j += GT) sSlot[j] = j;
     if (resume) {
       const int s0 = st.isc[2];
+      if (own) {
+        // the Gram columns of the handed-over active set, slots 0..s0-1
+        for (int k = 0; k < s0; ++k)
+          masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.m, p, ldg, msk, st.ind[k],
+                                     gaw + (size_t)k * ldg, T(0));
+      }
       for (int j = tid; j < LC; j += GT) {
         sCoef[j] = j < s0 ? st.coef[j] : T(0);
         sInd[j] = j < s0 ? st.ind[j] : -1;
@@ -431,6 +454,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         sInd[j] = -1;
       }
       normX = a.norms[sig];
+      if (mk && mcls == 0) empty = true;  // no observed row: empty column
       __syncthreads();
 
       // first atom: i?amax of DtR, or the plain max when pos
@@ -482,6 +506,11 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           if (tid == (c / VEC) % GT) act |= ActMask(1) << ((c / VEC / GT) * VEC + c % VEC);
           __syncthreads();
           const T* gcol = a.G + (size_t)c * ldg;
+          if (own) {
+            T* slot = gaw + (size_t)sSlot[i] * ldg;
+            masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.m, p, ldg, msk, c, slot, T(0));
+            gcol = slot;
+          }
           for (int j = tid; j <= i; j += GT) {
             const int r = sInd[j];
             sGs[j] = gcol[r] + (r == c ? diag : T(0));
@@ -562,7 +591,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           for (int e = 0; e < UNR; ++e) {
             const int k = k0 + e < s ? k0 + e : k0;
             uk[e] = k0 + e < s ? sU[k] : T(0);
-            col[e] = a.G + (size_t)sInd[k] * ldg;
+            col[e] = own ? gaw + (size_t)sSlot[k] * ldg : a.G + (size_t)sInd[k] * ldg;
           }
           V g[UNR][NQ];
 #pragma unroll
@@ -765,7 +794,17 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
             sTmpI[j] = sInd[j2];
             sGs[j] = sCoef[j2];
           }
+          int freed = 0;
+          if (mk) {
+            // Gram-column slots follow their atoms; the freed one goes last
+            freed = sSlot[fz];
+            for (int j = tid; j < i; j += GT) sSlotT[j] = sSlot[j + (j >= fz)];
+          }
           __syncthreads();
+          if (mk) {
+            for (int j = tid; j < i; j += GT) sSlot[j] = sSlotT[j];
+            if (tid == 0) sSlot[i] = freed;
+          }
           if (tid == (dropped / VEC) % GT)
             act &= ~(ActMask(1) << ((dropped / VEC / GT) * VEC + dropped % VEC));
           for (int j = tid; j < i; j += GT) {
@@ -811,7 +850,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
     // ---------- optional FP64 refit (PENALTY, at lambda) or least-squares fit
     // ols: alpha_A = (G_AA + diag I)^-1 (D'x)_A, the coefficients the
     // reference's ols option returns (decomp/decomp.h:1087-1091)
-    if (((a.refit && reached) || a.ols) && !overflow && s_active > 0) {
+    if (!mk && ((a.refit && reached) || a.ols) && !overflow && s_active > 0) {
       __syncthreads();
       const int s = s_active;
       const T* Mf = inl ? sMl : Mc;  // the inverse of the final active block
@@ -904,17 +943,16 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
 
 // Launches one instantiation: persistent grid over a counter (batched) or
 // one workgroup per signal (plain; CAP must be 0).
-template <typename T, int W, int NQ, int CAP, bool WT>
+template <typename T, int W, int NQ, int CAP, int VAR>
 int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
                 long max_blocks) {
   const int LC = CAP > 0 ? (CAP < a.L ? CAP : a.L) : a.L;
-  constexpr bool wt = WT;
-  size_t lds = LarsLds<T, W, NQ, CAP>::bytes(LC, 0, wt);
+  size_t lds = LarsLds<T, W, NQ, CAP>::bytes(LC, 0, VAR);
   if (lds > 160 * 1024) {
     set_error("lasso: L=%d, p=%d need %zu bytes of LDS per signal", a.L, a.p, lds);
     return SPX_ERR_UNSUP;
   }
-  auto kern = lars_kernel<T, W, NQ, CAP, WT>;
+  auto kern = lars_kernel<T, W, NQ, CAP, VAR>;
   SPX_HIP(hipFuncSetAttribute((const void*)kern,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   hipStream_t st = ctx->stream;
@@ -928,11 +966,11 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
     if (per_cu < 1) per_cu = 1;
     const size_t budget = (size_t)156 * 1024 / per_cu;
     int cl = 0;
-    while (cl + 4 <= a.L && LarsLds<T, W, NQ, CAP>::bytes(LC, cl + 4, wt) <= budget) cl += 4;
+    while (cl + 4 <= a.L && LarsLds<T, W, NQ, CAP>::bytes(LC, cl + 4, VAR) <= budget) cl += 4;
     if (cl >= a.L) cl = a.L;
     if (cl >= FastCap<T>::C) {
       b.lds_cap = cl;
-      lds = LarsLds<T, W, NQ, CAP>::bytes(LC, cl, wt);
+      lds = LarsLds<T, W, NQ, CAP>::bytes(LC, cl, VAR);
     }
   }
   if (persistent) {
@@ -949,6 +987,13 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
       if (rc) return rc;
       b.scratch = scratch;
     }
+    if (VAR == 2) {
+      // masked signals: LC Gram columns per workgroup
+      b.ga_stride = (size_t)a.ldg * LC;
+      int rc = ctx->scratch_get(CAP > 0 ? 16 : 17, sizeof(T) * b.ga_stride * blocks,
+                                reinterpret_cast<void**>(&b.ga));
+      if (rc) return rc;
+    }
     b.queue = counter;
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, b);
     SPX_HIP(hipGetLastError());
@@ -956,7 +1001,9 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
   }
   // plain form: one workgroup per signal, chunks bounded by scratch size
   const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
-  long chunk = (long)(512ull * 1024 * 1024 / (slot ? slot : 1));
+  const size_t ga_stride = VAR == 2 ? (size_t)a.ldg * a.L : 0;  // masked Gram columns
+  const size_t per_block = slot + sizeof(T) * ga_stride;
+  long chunk = (long)(512ull * 1024 * 1024 / (per_block ? per_block : 1));
   if (chunk > 65535) chunk = 65535;
   if (chunk < 1) chunk = 1;
   const long alloc_chunk = chunk < a.n ? chunk : a.n;
@@ -964,9 +1011,17 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
   int rc = ctx->scratch_get(0, slot * (alloc_chunk > 0 ? alloc_chunk : 1),
                             reinterpret_cast<void**>(&scratch));
   if (rc) return rc;
+  T* ga = nullptr;
+  if (VAR == 2) {
+    rc = ctx->scratch_get(17, sizeof(T) * ga_stride * (alloc_chunk > 0 ? alloc_chunk : 1),
+                          reinterpret_cast<void**>(&ga));
+    if (rc) return rc;
+  }
+  const int lds_cap = b.lds_cap;
   for (long s0 = 0; s0 < a.n; s0 += chunk) {
     const long cnt = (a.n - s0) < chunk ? (a.n - s0) : chunk;
     b = a;
+    b.lds_cap = lds_cap;
     b.queue = nullptr;
     b.list = nullptr;
     b.scratch = scratch;
@@ -975,6 +1030,12 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
     b.norms = a.norms + s0;
     if (a.nlam > 1) b.lambda = a.lambda + s0;
     if (a.weights) b.weights = a.weights + (size_t)s0 * a.ldw;
+    if (VAR == 2) {
+      b.mclass = a.mclass + s0;
+      b.mask = a.mask + (size_t)s0 * a.ldmask;
+      b.ga = ga;
+      b.ga_stride = ga_stride;
+    }
     b.outV = a.outV + (size_t)s0 * a.L;
     b.outR = a.outR + (size_t)s0 * a.L;
     b.outCount = a.outCount + s0;
@@ -995,33 +1056,33 @@ int lars_nq(int p) {
 // One pass of group width W: CAP = FastCap (fast) or 0 (full capacity);
 // the weighted penalty has kernels of its own (the plain ones keep their
 // registers).
-template <typename T, int W, bool FAST, bool WT>
+template <typename T, int W, bool FAST, int VAR>
 int lars_pass_t(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter, long max_blocks) {
   constexpr int C = FAST ? FastCap<T>::C : 0;
   switch (lars_nq<T, W>(a.p)) {
-    case 1: return lars_launch<T, W, 1, C, WT>(ctx, a, persistent, counter, max_blocks);
-    case 2: return lars_launch<T, W, 2, C, WT>(ctx, a, persistent, counter, max_blocks);
-    case 4: return lars_launch<T, W, 4, C, WT>(ctx, a, persistent, counter, max_blocks);
-    case 8: return lars_launch<T, W, 8, C, WT>(ctx, a, persistent, counter, max_blocks);
-    case 16: return lars_launch<T, W, 16, C, WT>(ctx, a, persistent, counter, max_blocks);
+    case 1: return lars_launch<T, W, 1, C, VAR>(ctx, a, persistent, counter, max_blocks);
+    case 2: return lars_launch<T, W, 2, C, VAR>(ctx, a, persistent, counter, max_blocks);
+    case 4: return lars_launch<T, W, 4, C, VAR>(ctx, a, persistent, counter, max_blocks);
+    case 8: return lars_launch<T, W, 8, C, VAR>(ctx, a, persistent, counter, max_blocks);
+    case 16: return lars_launch<T, W, 16, C, VAR>(ctx, a, persistent, counter, max_blocks);
     default: return -1;  // does not fit this group width
   }
 }
 
-// Pass of group width w (1, 2, 4 or 16) of the plain (WT false) or weighted
-// kernels; -1 when the width does not cover p.
-template <typename T, bool WT>
+// Pass of group width w (1, 2, 4 or 16) of the kernels of variant VAR (0
+// plain, 1 weighted, 2 masked); -1 when the width does not cover p.
+template <typename T, int VAR>
 int lars_pass_wt(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
                  int* counter, long max_blocks) {
   switch (w) {
-    case 1: return fast ? lars_pass_t<T, 1, true, WT>(ctx, a, persistent, counter, max_blocks)
-                        : lars_pass_t<T, 1, false, WT>(ctx, a, persistent, counter, max_blocks);
-    case 2: return fast ? lars_pass_t<T, 2, true, WT>(ctx, a, persistent, counter, max_blocks)
-                        : lars_pass_t<T, 2, false, WT>(ctx, a, persistent, counter, max_blocks);
-    case 4: return fast ? lars_pass_t<T, 4, true, WT>(ctx, a, persistent, counter, max_blocks)
-                        : lars_pass_t<T, 4, false, WT>(ctx, a, persistent, counter, max_blocks);
-    default: return fast ? lars_pass_t<T, 16, true, WT>(ctx, a, persistent, counter, max_blocks)
-                         : lars_pass_t<T, 16, false, WT>(ctx, a, persistent, counter, max_blocks);
+    case 1: return fast ? lars_pass_t<T, 1, true, VAR>(ctx, a, persistent, counter, max_blocks)
+                        : lars_pass_t<T, 1, false, VAR>(ctx, a, persistent, counter, max_blocks);
+    case 2: return fast ? lars_pass_t<T, 2, true, VAR>(ctx, a, persistent, counter, max_blocks)
+                        : lars_pass_t<T, 2, false, VAR>(ctx, a, persistent, counter, max_blocks);
+    case 4: return fast ? lars_pass_t<T, 4, true, VAR>(ctx, a, persistent, counter, max_blocks)
+                        : lars_pass_t<T, 4, false, VAR>(ctx, a, persistent, counter, max_blocks);
+    default: return fast ? lars_pass_t<T, 16, true, VAR>(ctx, a, persistent, counter, max_blocks)
+                         : lars_pass_t<T, 16, false, VAR>(ctx, a, persistent, counter, max_blocks);
   }
 }
 

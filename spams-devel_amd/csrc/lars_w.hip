@@ -7,7 +7,7 @@ namespace spx {
 template <typename T>
 int lars_weighted_pass(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
                        int* counter, long max_blocks) {
-  return lars_pass_wt<T, true>(ctx, w, fast, a, persistent, counter, max_blocks);
+  return lars_pass_wt<T, 1>(ctx, w, fast, a, persistent, counter, max_blocks);
 }
 
 template int lars_weighted_pass<float>(Ctx*, int, bool, const LarsArgs<float>&, bool, int*, long);

@@ -22,6 +22,7 @@
 #include <climits>
 #include <type_traits>
 
+#include "mask_gram.h"
 #include "spx_common.h"
 #include "spx_internal.h"
 
@@ -46,8 +47,9 @@ struct Vec16o<double> {
 constexpr size_t kUnLdsBytes = 32 * 1024;
 
 // Held to 4 wavefronts per SIMD up to 16 rows per lane: the signals are
-// latency-bound, so occupancy pays more than registers.
-template <typename T, int W, int NQ, bool ULDS>
+// latency-bound, so occupancy pays more than registers.  MK: masked signals
+// (omp_mask, decomp/decomp.h:406-519; a.mclass set).
+template <typename T, int W, int NQ, bool ULDS, bool MK>
 __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T> a) {
   constexpr int GT = 64 * W;
   constexpr int VEC = Vec16o<T>::N;
@@ -72,6 +74,7 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
 
   const int tid = threadIdx.x;
   T* const Un = ULDS ? sUn : a.scratch + (size_t)blockIdx.x * L * L;  // column-major L x L
+  T* const gaw = MK ? a.ga + (size_t)blockIdx.x * a.ga_stride : nullptr;
 
   // the next grab is requested when the current signal starts, so its
   // latency overlaps the set-up loads
@@ -112,6 +115,12 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
     const T lambda = a.nLambda == 0 ? T(0) : a.lambda[a.nLambda > 1 ? sig : 0];
     int Li = a.nL == 0 ? a.scalarL : a.Ls[a.nL > 1 ? sig : 0];
     if (Li > L) Li = L;
+    // masked: class 0 (no observed row) is empty; class 2 uses the Gram
+    // columns of its observed rows plus 1e-10 on the diagonal
+    const int mcls = MK ? a.mclass[sig] : 1;
+    const bool own = MK && mcls == 2;
+    const unsigned char* msk = own ? a.mask + (size_t)sig * a.ldmask : nullptr;
+    if (MK && mcls == 0) Li = 0;
     __syncthreads();
 
     int j = 0;
@@ -231,6 +240,12 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
         normX -= delta;
         // G(c, active) for the Un column
         const T* gcol = a.G + (size_t)c * ldg;
+        if (own) {
+          // the new atom's Gram column, slot j (selection order)
+          T* slot = gaw + (size_t)j * ldg;
+          masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.m, p, ldg, msk, c, slot, T(1e-10));
+          gcol = slot;
+        }
         for (int k = tid; k < j; k += GT) sGc[k] = gcol[sInd[k]];
         __syncthreads();
         // Undn(c, k) = sum_{k' <= k} G(c, ind k') Un(k', k), k < j
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
           for (int e = 0; e < UNR; ++e) {
             const int k = k0 + e <= j ? k0 + e : k0;
             uk[e] = k0 + e <= j ? Un[k + (size_t)j * L] : T(0);
-            col[e] = a.G + (size_t)sInd[k] * ldg;
+            col[e] = own ? gaw + (size_t)k * ldg : a.G + (size_t)sInd[k] * ldg;
           }
           V g[UNR][NQ];
 #pragma unroll
@@ -346,11 +361,12 @@ size_t ormp_lds_bytes(int L) {
   return b;
 }
 
-template <typename T, int W, int NQ, bool ULDS>
+template <typename T, int W, int NQ, bool ULDS, bool MK>
 int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
   OmpArgs<T> a = a0;
   const size_t lds = ormp_lds_bytes<T, W, NQ, ULDS>(a.L);
-  auto kern = ormp_kernel<T, W, NQ, ULDS>;
+  auto kern = ormp_kernel<T, W, NQ, ULDS, MK>;
+  const size_t ga_stride = MK ? (size_t)a.ldg * a.L : 0;  // masked Gram columns
   SPX_HIP(hipFuncSetAttribute((const void*)kern,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
@@ -376,12 +392,18 @@ int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
     if (rc) return rc;
     a.scratch = scratch;
     a.queue = queue;
+    if (MK) {
+      a.ga_stride = ga_stride;
+      rc = ctx->scratch_get(21, sizeof(T) * ga_stride * blocks, reinterpret_cast<void**>(&a.ga));
+      if (rc) return rc;
+    }
     ScopedTimer tm(ctx, SPX_TIMER_ORMP);
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, a);
     SPX_HIP(hipGetLastError());
     return SPX_OK;
   }
-  long chunk = (long)(512ull * 1024 * 1024 / (slot ? slot : 1));
+  const size_t per_block = slot + sizeof(T) * ga_stride;
+  long chunk = (long)(512ull * 1024 * 1024 / (per_block ? per_block : 1));
   if (chunk > 65535) chunk = 65535;
   if (chunk < 1) chunk = 1;
   const long alloc_chunk = chunk < a0.n ? chunk : a0.n;
@@ -389,6 +411,12 @@ int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
   if (slot) {
     int rc = ctx->scratch_get(0, slot * (alloc_chunk > 0 ? alloc_chunk : 1),
                               reinterpret_cast<void**>(&scratch));
+    if (rc) return rc;
+  }
+  T* ga = nullptr;
+  if (MK) {
+    int rc = ctx->scratch_get(21, sizeof(T) * ga_stride * (alloc_chunk > 0 ? alloc_chunk : 1),
+                              reinterpret_cast<void**>(&ga));
     if (rc) return rc;
   }
   for (long s0 = 0; s0 < a0.n; s0 += chunk) {
@@ -405,31 +433,43 @@ int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
     a.outV = a0.outV + (size_t)s0 * a0.L;
     a.outR = a0.outR + (size_t)s0 * a0.L;
     a.outCount = a0.outCount + s0;
+    if (MK) {
+      a.mclass = a0.mclass + s0;
+      a.mask = a0.mask + (size_t)s0 * a0.ldmask;
+      a.ga = ga;
+      a.ga_stride = ga_stride;
+    }
     hipLaunchKernelGGL(kern, dim3((unsigned)cnt), dim3(64 * W), lds, st, a);
     SPX_HIP(hipGetLastError());
   }
   return SPX_OK;
 }
 
-template <typename T, int W>
-int ormp_dispatch_nq(Ctx* ctx, const OmpArgs<T>& a, int form) {
+template <typename T, int W, bool MK>
+int ormp_dispatch_nq_t(Ctx* ctx, const OmpArgs<T>& a, int form) {
   constexpr int VEC = Vec16o<T>::N;
   const int per = VEC * 64 * W;
   const int nq = (a.p + per - 1) / per;
   if (sizeof(T) * (size_t)a.L * a.L <= kUnLdsBytes) {
-    if (nq <= 1) return ormp_run<T, W, 1, true>(ctx, a, form);
-    if (nq <= 2) return ormp_run<T, W, 2, true>(ctx, a, form);
-    if (nq <= 4) return ormp_run<T, W, 4, true>(ctx, a, form);
-    if (nq <= 8) return ormp_run<T, W, 8, true>(ctx, a, form);
-    if (nq <= 16) return ormp_run<T, W, 16, true>(ctx, a, form);
+    if (nq <= 1) return ormp_run<T, W, 1, true, MK>(ctx, a, form);
+    if (nq <= 2) return ormp_run<T, W, 2, true, MK>(ctx, a, form);
+    if (nq <= 4) return ormp_run<T, W, 4, true, MK>(ctx, a, form);
+    if (nq <= 8) return ormp_run<T, W, 8, true, MK>(ctx, a, form);
+    if (nq <= 16) return ormp_run<T, W, 16, true, MK>(ctx, a, form);
     return -1;
   }
-  if (nq <= 1) return ormp_run<T, W, 1, false>(ctx, a, form);
-  if (nq <= 2) return ormp_run<T, W, 2, false>(ctx, a, form);
-  if (nq <= 4) return ormp_run<T, W, 4, false>(ctx, a, form);
-  if (nq <= 8) return ormp_run<T, W, 8, false>(ctx, a, form);
-  if (nq <= 16) return ormp_run<T, W, 16, false>(ctx, a, form);
+  if (nq <= 1) return ormp_run<T, W, 1, false, MK>(ctx, a, form);
+  if (nq <= 2) return ormp_run<T, W, 2, false, MK>(ctx, a, form);
+  if (nq <= 4) return ormp_run<T, W, 4, false, MK>(ctx, a, form);
+  if (nq <= 8) return ormp_run<T, W, 8, false, MK>(ctx, a, form);
+  if (nq <= 16) return ormp_run<T, W, 16, false, MK>(ctx, a, form);
   return -1;
+}
+
+template <typename T, int W>
+int ormp_dispatch_nq(Ctx* ctx, const OmpArgs<T>& a, int form) {
+  return a.mclass ? ormp_dispatch_nq_t<T, W, true>(ctx, a, form)
+                  : ormp_dispatch_nq_t<T, W, false>(ctx, a, form);
 }
 
 }  // namespace

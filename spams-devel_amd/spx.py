@@ -353,6 +353,41 @@ def lasso(X, D=None, Q=None, q=None, return_reg_path=False, L=-1,
     return (A, path) if return_reg_path else A
 
 
+def lassoMask(X, D, B, L=-1, lambda1=None, lambda2=0., mode=PENALTY, pos=False,
+              numThreads=-1, verbose=False, precision="fp32", form="batched",
+              waves_per_signal=0, ctx=None):
+    """LARS-Lasso over partly observed signals (spams.lassoMask,
+    swig/python/spams.py:144-153).  B: m x n, nonzero (True) = row observed.
+    A column with no observed row comes back empty.  ``numThreads`` is
+    accepted and ignored."""
+    if lambda1 is None:
+        raise ValueError("lassoMask : lambda1 must be defined")
+    dtype = np.asarray(X).dtype
+    sfx = _sfx(dtype)
+    X = np.asfortranarray(X, dtype=dtype)
+    D = np.asfortranarray(D, dtype=dtype)
+    m, n = X.shape
+    if D.shape[0] != m:
+        raise ValueError("lassoMask : incompatible matrix dimensions")
+    B = np.asfortranarray(np.asarray(B) != 0, dtype=np.uint8)
+    if B.shape != (m, n):
+        raise ValueError("lassoMask : Mask has non acceptable dimensions")
+    p = D.shape[1]
+    lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lambda1, dtype=dtype)))
+    if lam.size not in (1, n):
+        raise ValueError("lassoMask : lambda1 must be a scalar or have one entry per signal")
+    o = spx_lasso_opts(L, mode, int(pos), -1, PREC[precision], FORM[form],
+                       waves_per_signal, 0)
+    out = spx_csc()
+    ctx = ctx or default_context()
+    f = getattr(lib(), "spx_lasso_mask_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _P, _I, _ct(dtype),
+                  ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_csc)]
+    _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(B), _ptr(lam), lam.size, lambda2,
+             ctypes.byref(o), ctypes.byref(out)))
+    return _csc_to_scipy(out, dtype)
+
+
 def lassoWeighted(X, D, W, L=-1, lambda1=None, mode=PENALTY, pos=False,
                   numThreads=-1, verbose=False, precision="fp32", form="batched",
                   waves_per_signal=0, ctx=None):
@@ -420,6 +455,46 @@ def omp(X, D, L=None, eps=None, lambda1=None, return_reg_path=False,
     f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _I, _P, _I, _P, _I,
                   ctypes.POINTER(spx_omp_opts), ctypes.POINTER(spx_csc), _P]
     _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(Ls), nL, _ptr(E), nE,
+             _ptr(La), nLa, ctypes.byref(o), ctypes.byref(out), _ptr(path)))
+    A = _csc_to_scipy(out, dtype)
+    return (A, path) if return_reg_path else A
+
+
+def ompMask(X, D, B, L=None, eps=None, lambda1=None, return_reg_path=False,
+            numThreads=-1, precision="fp32", form="batched", waves_per_signal=0,
+            ctx=None):
+    """ORMP over partly observed signals (spams.ompMask,
+    swig/python/spams.py:201-231).  B: m x n, nonzero (True) = row observed.
+    ``return_reg_path`` also returns the first signal's path (p x min(m, p))."""
+    dtype = np.asarray(X).dtype
+    sfx = _sfx(dtype)
+    X = np.asfortranarray(X, dtype=dtype)
+    D = np.asfortranarray(D, dtype=dtype)
+    m, n = X.shape
+    if D.shape[0] != m:
+        raise ValueError("ompMask : incompatible matrix dimensions")
+    B = np.asfortranarray(np.asarray(B) != 0, dtype=np.uint8)
+    if B.shape != (m, n):
+        raise ValueError("ompMask : Mash has non acceptable dimensions")
+    p = D.shape[1]
+
+    def vec(a, dt):
+        if a is None:
+            return None, 0
+        a = np.ascontiguousarray(np.atleast_1d(np.asarray(a, dtype=dt)))
+        return a, a.size
+
+    Ls, nL = vec(L, np.int32)
+    E, nE = vec(eps, dtype)
+    La, nLa = vec(lambda1, dtype)
+    o = spx_omp_opts(PREC[precision], FORM[form], waves_per_signal)
+    out = spx_csc()
+    ctx = ctx or default_context()
+    path = np.zeros((p, min(m, p)), dtype=dtype, order="F") if return_reg_path else None
+    f = getattr(lib(), "spx_omp_mask_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I,
+                  ctypes.POINTER(spx_omp_opts), ctypes.POINTER(spx_csc), _P]
+    _check(f(ctx.h, _ptr(X), m, n, _ptr(D), p, _ptr(B), _ptr(Ls), nL, _ptr(E), nE,
              _ptr(La), nLa, ctypes.byref(o), ctypes.byref(out), _ptr(path)))
     A = _csc_to_scipy(out, dtype)
     return (A, path) if return_reg_path else A
@@ -525,6 +600,24 @@ def lasso_weighted_dev(ctx, d: Dictionary, X, W, lam, V, R, count, L=-1,
              lam.numel(), ctypes.byref(o), ctypes.byref(lists)))
 
 
+def lasso_mask_dev(ctx, d: Dictionary, X, B, lam, V, R, count, lambda2=0.0, L=-1,
+                   mode=PENALTY, pos=False, precision="fp32", form="batched",
+                   waves_per_signal=0, stats=None):
+    """Device masked LARS: X torch (n, m), B torch uint8 (n, m) (column-major
+    m x n mask, nonzero = observed), other arguments as lasso_dev."""
+    n = X.shape[0]
+    Lout = V.shape[1]
+    sfx = "f32" if d.dtype == np.float32 else "f64"
+    o = spx_lasso_opts(L, mode, int(pos), -1, PREC[precision], FORM[form],
+                       waves_per_signal)
+    lists = spx_lists_dev(_tptr(V), _tptr(R), _tptr(count), Lout, _tptr(stats))
+    f = getattr(lib(), "spx_lasso_mask_dev_" + sfx)
+    f.argtypes = [_P, _P, _P, _I, _I, _P, _I, _P, _I, _ct(d.dtype),
+                  ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_lists_dev)]
+    _check(f(ctx.h, d.h, _tptr(X), X.stride(0), n, _tptr(B), B.stride(0), _tptr(lam),
+             lam.numel(), lambda2, ctypes.byref(o), ctypes.byref(lists)))
+
+
 def lasso_dtx_dev(ctx, d: Dictionary, DtX, norms, lam, V, R, count, lambda2=0.0,
                   L=-1, mode=PENALTY, pos=False, max_length_path=-1,
                   form="batched", waves_per_signal=0, stats=None):
@@ -565,6 +658,24 @@ def omp_dev(ctx, d: Dictionary, X, V, R, count, L=None, eps=None, lambda1=None,
     nn = lambda t: 0 if t is None else t.numel()
     _check(f(ctx.h, d.h, _tptr(X), X.shape[1], n, _tptr(L), nn(L), _tptr(eps),
              nn(eps), _tptr(lambda1), nn(lambda1), ctypes.byref(o),
+             ctypes.byref(lists)))
+
+
+def omp_mask_dev(ctx, d: Dictionary, X, B, V, R, count, L=None, eps=None, lambda1=None,
+                 precision="fp32", form="batched", waves_per_signal=0):
+    """Device masked ORMP: B torch uint8 (n, m) (column-major m x n mask),
+    other arguments as omp_dev."""
+    n = X.shape[0]
+    Lout = V.shape[1]
+    sfx = "f32" if d.dtype == np.float32 else "f64"
+    o = spx_omp_opts(PREC[precision], FORM[form], waves_per_signal)
+    lists = spx_lists_dev(_tptr(V), _tptr(R), _tptr(count), Lout, None)
+    f = getattr(lib(), "spx_omp_mask_dev_" + sfx)
+    f.argtypes = [_P, _P, _P, _I, _I, _P, _I, _P, _I, _P, _I, _P, _I,
+                  ctypes.POINTER(spx_omp_opts), ctypes.POINTER(spx_lists_dev)]
+    nn = lambda t: 0 if t is None else t.numel()
+    _check(f(ctx.h, d.h, _tptr(X), X.stride(0), n, _tptr(B), B.stride(0), _tptr(L), nn(L),
+             _tptr(eps), nn(eps), _tptr(lambda1), nn(lambda1), ctypes.byref(o),
              ctypes.byref(lists)))
 
 

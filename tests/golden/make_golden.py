@@ -130,6 +130,32 @@ def main():
             cases.append((f"lassoweighted_{tag}_mode{mode}_lam{lam}_pos{pos}", dict(
                 kind="lasso", X=X, D=D, W=Wt, lam=np.asarray(lam, dt), mode=mode, pos=pos,
                 pB=pB, r=r, v=v)))
+        # partly observed signals (_lassoMask, _ompMask): 80 % of rows
+        # observed, one column fully observed, one with no observed row (the
+        # reference then stores L explicit zeros at row 0; compare densely)
+        X, D = small(21, 64, 256, 40, 5, 0.05, True, dt)
+        B = (np.random.default_rng(22).uniform(size=(64, 40)) < 0.8).astype(np.uint8)
+        B[:, 0] = 1
+        B[:, 1] = 0
+        B = np.asfortranarray(B)
+        for mode, lam, pos, l2 in ((2, 0.1, 0, 0.0), (2, 0.1, 1, 0.0), (2, 0.05, 0, 0.01),
+                                   (0, 0.8, 0, 0.0), (1, 0.3, 0, 0.0)):
+            A = oracle.ref_lasso_mask(X, D, B, lam, lambda2=l2, mode=mode, pos=bool(pos))
+            pB, r, v = csc_parts(A)
+            cases.append((f"lassomask_{tag}_mode{mode}_lam{lam}_pos{pos}_l2{l2}", dict(
+                kind="lasso", X=X, D=D, B=B, lam=np.asarray(lam, dt),
+                lambda2=np.asarray(l2, dt), mode=mode, pos=pos, pB=pB, r=r, v=v)))
+        X, D = small(23, 64, 256, 40, np.arange(40) % 12 + 1, 0.01, False, dt)
+        for name, L, eps, lam in (("L8", 8, None, None), ("eps", None, 1e-3, None),
+                                  ("L16eps", 16, 1e-2, None), ("lam", None, None, 0.01)):
+            A = oracle.ref_omp_mask(X, D, B, L=L, eps=eps, lambda1=lam)
+            pB, r, v = csc_parts(A)
+            cases.append((f"ompmask_{tag}_{name}", dict(
+                kind="omp", X=X, D=D, B=B,
+                L=np.asarray(-1 if L is None else L, np.int32),
+                eps=np.asarray(np.nan if eps is None else eps, dt),
+                lam=np.asarray(np.nan if lam is None else lam, dt),
+                pB=pB, r=r, v=v)))
     only_new = "--all" not in sys.argv
     wrote = 0
     for name, d in cases:

@@ -594,3 +594,111 @@ def test_lasso_weighted_device_entry_with_padded_weights(spx, orc):
         spx.lasso_weighted_dev(ctx, d, Xt, Wbad, lam, V, R, cnt)
     d.close()
     ctx.close()
+
+
+# ------------------------------------------------- partly observed signals
+
+def _golden_dense_dup(g):
+    import scipy.sparse as sp
+    # dense, so that the reference's explicit zeros (a column with no
+    # observed row) compare as the empty column they stand for
+    return np.asarray(sp.csc_matrix((g["v"], g["r"], g["pB"]),
+                                    shape=(g["D"].shape[1], g["X"].shape[1])).todense())
+
+
+@pytest.mark.parametrize("name", golden_files("lassomask_"))
+def test_lasso_mask_matches_golden(spx, name):
+    print("case", name, flush=True)
+    g = load_golden(name)
+    ref = _golden_dense_dup(g)
+    f64 = "f64" in name
+    for form, w in (("batched", 0), ("batched", 4), ("plain", 0)):
+        A = spx.lassoMask(g["X"], g["D"], g["B"], lambda1=g["lam"].item(),
+                          lambda2=g["lambda2"].item(), mode=int(g["mode"]), pos=bool(g["pos"]),
+                          form=form, waves_per_signal=w)
+        assert A[:, 1].nnz == 0  # no observed row: empty column
+        A = dense(A)
+        same = np.all((A != 0) == (ref != 0), axis=0)
+        if f64:
+            assert same.all(), (form, w)
+            np.testing.assert_allclose(A, ref, atol=1e-9)
+        else:
+            assert same.mean() >= 0.9, (form, w)
+            assert np.abs(A[:, same] - ref[:, same]).max(initial=0) <= 2e-4
+
+
+@pytest.mark.parametrize("name", golden_files("ompmask_"))
+def test_omp_mask_matches_golden(spx, name):
+    print("case", name, flush=True)
+    g = load_golden(name)
+    ref = _golden_dense_dup(g)
+    L = None if int(np.atleast_1d(g["L"])[0]) < 0 else g["L"]
+    eps = None if np.isnan(g["eps"]) else g["eps"].item()
+    lam = None if np.isnan(g["lam"]) else g["lam"].item()
+    f64 = "f64" in name
+    for form, w in (("batched", 0), ("batched", 4), ("plain", 0)):
+        A = dense(spx.ompMask(g["X"], g["D"], g["B"], L=L, eps=eps, lambda1=lam, form=form,
+                              waves_per_signal=w))
+        same = np.all((A != 0) == (ref != 0), axis=0)
+        if f64:
+            assert same.all(), (form, w)
+            np.testing.assert_allclose(A, ref, atol=1e-9)
+        else:
+            assert same.mean() >= 0.9, (form, w)
+            assert np.abs(A[:, same] - ref[:, same]).max(initial=0) <= 2e-4
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_lasso_mask_headline_shape_with_handover(spx, orc, dt):
+    """m=256, p=1024, 85 % of rows observed, small lambdas: the masked Gram
+    columns follow their atoms through drops and the hand-over."""
+    print("case lasso mask headline-shape", dt, flush=True)
+    X, D, lam = W.lars_headline(seed=82, n=300, dtype=dt)
+    B = (np.random.default_rng(83).uniform(size=X.shape) < 0.85).astype(np.uint8)
+    B[:, :3] = 1
+    lam0 = 0.03
+    A = dense(spx.lassoMask(X, D, B, lambda1=lam0))
+    Bo = dense(orc.lasso_mask(X, D, B, lam0))
+    assert (Bo != 0).sum(0).max() > 24  # the hand-over is exercised
+    same = np.all((A != 0) == (Bo != 0), axis=0)
+    if dt == np.float64:
+        assert same.all()
+        np.testing.assert_allclose(A, Bo, atol=1e-9)
+    else:
+        assert same.mean() >= 0.9
+        assert np.abs(A[:, same] - Bo[:, same]).max() <= 2e-4
+
+
+def test_omp_mask_path_and_device_entry(spx, orc):
+    import torch
+    print("case omp mask path/dev", flush=True)
+    X, D = rand_problem(84, 64, 256, 120, np.float64, k=8, noise=0.01, norm=False)
+    B = (np.random.default_rng(85).uniform(size=X.shape) < 0.75).astype(np.uint8)
+    A, path = spx.ompMask(X, D, B, L=10, return_reg_path=True)
+    Ao = dense(orc.omp_mask(X, D, B, L=10))
+    np.testing.assert_allclose(dense(A), Ao, atol=1e-9)
+    # path column 9 holds signal 0's final coefficients
+    np.testing.assert_allclose(path[:, 9], Ao[:, 0], atol=1e-9)
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to("cuda:0")
+    Bt = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda:0")
+    Lt = torch.full((1,), 10, dtype=torch.int32, device="cuda:0")
+    V = torch.zeros((120, 10), dtype=torch.float64, device="cuda:0")
+    R = torch.zeros((120, 10), dtype=torch.int32, device="cuda:0")
+    cnt = torch.zeros((120,), dtype=torch.int32, device="cuda:0")
+    spx.omp_mask_dev(ctx, d, Xt, Bt, V, R, cnt, L=Lt)
+    lam = torch.full((1,), 0.05, dtype=torch.float64, device="cuda:0")
+    Vl = torch.zeros((120, 64), dtype=torch.float64, device="cuda:0")
+    Rl = torch.zeros((120, 64), dtype=torch.int32, device="cuda:0")
+    cl = torch.zeros((120,), dtype=torch.int32, device="cuda:0")
+    spx.lasso_mask_dev(ctx, d, Xt, Bt, lam, Vl, Rl, cl)
+    torch.cuda.synchronize()
+    for (Vh, Rh, ch, ref) in ((V, R, cnt, Ao), (Vl, Rl, cl, dense(orc.lasso_mask(X, D, B, 0.05)))):
+        Vh, Rh, ch = Vh.cpu().numpy(), Rh.cpu().numpy(), ch.cpu().numpy()
+        A2 = np.zeros((256, 120))
+        for i in range(120):
+            A2[Rh[i, :ch[i]], i] = Vh[i, :ch[i]]
+        np.testing.assert_allclose(A2, ref, atol=1e-9)
+    d.close()
+    ctx.close()

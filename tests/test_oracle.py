@@ -403,3 +403,77 @@ def test_lasso_weighted_kkt_conditions(orc):
     # a transposed or dropped weight breaks the conditions
     B = dense(orc.lasso_weighted(X, D, np.ones_like(Wt), lam))
     assert _weighted_kkt_violation(X, D, Wt, B, lam) > 1e-3
+
+
+# ------------------------------------------------------ partly observed signals
+
+def _golden_csc_dense(g):
+    import scipy.sparse as sp
+    # dense: the reference stores L explicit zeros at row 0 for a column
+    # with no observed row
+    return np.asarray(sp.csc_matrix((g["v"], g["r"], g["pB"]),
+                                    shape=(g["D"].shape[1], g["X"].shape[1])).todense())
+
+
+@pytest.mark.parametrize("name", golden_files("lassomask_"))
+def test_lasso_mask_matches_golden(orc, name):
+    g = load_golden(name)
+    A = dense(orc.lasso_mask(g["X"], g["D"], g["B"], g["lam"].item(), lambda2=g["lambda2"].item(),
+                             mode=int(g["mode"]), pos=bool(g["pos"])))
+    assert np.array_equal(A, _golden_csc_dense(g))
+
+
+@pytest.mark.parametrize("name", golden_files("ompmask_"))
+def test_omp_mask_matches_golden(orc, name):
+    g = load_golden(name)
+    L = None if int(np.atleast_1d(g["L"])[0]) < 0 else g["L"]
+    eps = None if np.isnan(g["eps"]) else g["eps"].item()
+    lam = None if np.isnan(g["lam"]) else g["lam"].item()
+    A = dense(orc.omp_mask(g["X"], g["D"], g["B"], L=L, eps=eps, lambda1=lam))
+    assert np.array_equal(A, _golden_csc_dense(g))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_masks_match_reference_build(orc, dt):
+    if not orc.has_ref():
+        pytest.skip("reference build absent")
+    X, D = _rand(41, 24, 50, 30, dt)
+    B = (np.random.default_rng(42).uniform(size=X.shape) < 0.7).astype(np.uint8)
+    B[:, 3] = 0
+    B[:, 4] = 1
+    for mode, lam in ((2, 0.1), (0, 0.6), (1, 0.2)):
+        a = dense(orc.lasso_mask(X, D, B, lam, mode=mode))
+        b = dense(orc.ref_lasso_mask(X, D, B, lam, mode=mode))
+        assert np.array_equal(a, b), mode
+    for L, eps in ((5, None), (None, 0.02)):
+        a = dense(orc.omp_mask(X, D, B, L=L, eps=eps))
+        b = dense(orc.ref_omp_mask(X, D, B, L=L, eps=eps))
+        assert np.array_equal(a, b), (L, eps)
+
+
+def test_masks_reduce_to_the_plain_solvers(orc):
+    """All rows observed = the plain path; no row observed = empty; a masked
+    signal = the plain path over its observed rows with lambda (eps) scaled
+    by |M| / m and 1e-10 added to the Gram diagonal."""
+    X, D = _rand(43, 20, 40, 12, np.float64)
+    ones = np.ones(X.shape, np.uint8)
+    np.testing.assert_array_equal(dense(orc.lasso_mask(X, D, ones, 0.1)),
+                                  dense(orc.lasso(X, D, 0.1)))
+    np.testing.assert_array_equal(dense(orc.omp_mask(X, D, ones, L=4)),
+                                  dense(orc.omp(X, D, L=4)))
+    assert orc.lasso_mask(X, D, np.zeros_like(ones), 0.1).nnz == 0
+    assert orc.omp_mask(X, D, np.zeros_like(ones), L=4).nnz == 0
+    B = ones.copy()
+    B[::3, :] = 0
+    keep = B[:, 0] != 0
+    cnt = int(keep.sum())
+    A = dense(orc.lasso_mask(X, D, B, 0.1))
+    # the plain solver on the observed rows: lambda * cnt / m; its ridge
+    # 1e-10 is the masked path's default
+    Aref = dense(orc.lasso(np.asfortranarray(X[keep]), np.asfortranarray(D[keep]),
+                           0.1 * cnt / X.shape[0]))
+    np.testing.assert_allclose(A, Aref, atol=1e-12)
+    # OMP: 1e-10 on the masked Gram's diagonal changes nothing at 1e-9
+    Ao = dense(orc.omp_mask(X, D, B, L=4))
+    Aoref = dense(orc.omp(np.asfortranarray(X[keep]), np.asfortranarray(D[keep]), L=4))
+    np.testing.assert_allclose(Ao, Aoref, atol=1e-8)
