@@ -12,7 +12,7 @@ HIP events on the engine's stream; roofline figures use each kernel's
 algorithmic work.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-    python bench.py --workload omp|fista|served     (other configurations)
+    python bench.py --workload omp|fista|served|variants  (other configurations)
 """
 from __future__ import annotations
 
@@ -439,6 +439,94 @@ def run_omp(args):
             "mean_nnz": float(counts.mean()), "plain_gpu": plain, "cpu_baseline": cpu}
 
 
+# ------------------------------------------- weighted / masked variants
+
+def run_variants(args):
+    """Weighted LARS (lassoWeighted), masked LARS (lassoMask) and masked OMP
+    (ompMask) at the headline shapes, inputs resident in HBM, each beside the
+    reference's own entry point on the host cores (bounded sample)."""
+    import workloads as W
+    E = Engine()
+    spx, torch = E.spx, E.torch
+    n = args.n if args.n != 200000 else 100000
+    rng = np.random.default_rng(2024)
+    X, D, _ = W.lars_headline(n=n)
+    m, p = D.shape
+    lam = 0.1
+    Wt = rng.uniform(0.5, 2.0, (p, n)).astype(np.float32)
+    B = (rng.uniform(size=(m, n)) < 0.85).astype(np.uint8)
+    Xo, Do, L, eps = W.omp_batch(n=n)
+    Bo = (rng.uniform(size=Xo.shape) < 0.85).astype(np.uint8)
+    Xt, Wtt, Bt = E.to_dev(X.T), E.to_dev(Wt.T), E.to_dev(B.T)
+    Xot, Bot = E.to_dev(Xo.T), E.to_dev(Bo.T)
+    d = spx.Dictionary(E.ctx, D)
+    do = spx.Dictionary(E.ctx, Do)
+    lam_t = E.to_dev(np.array([lam], np.float32))
+    L_t = E.to_dev(np.array([L], np.int32))
+    e_t = E.to_dev(np.array([eps], np.float32))
+    Lw = spx.lasso_L(m, p, lambda1=lam)
+    Lo = spx.omp_L(m, p, L)
+    bufs = {}
+
+    def lists(key, Lout):
+        if key not in bufs:
+            bufs[key] = (torch.empty((n, Lout), dtype=torch.float32, device=E.dev),
+                         torch.empty((n, Lout), dtype=torch.int32, device=E.dev),
+                         torch.empty((n,), dtype=torch.int32, device=E.dev))
+        return bufs[key]
+
+    runs = {
+        "lasso_weighted": (lambda: spx.lasso_weighted_dev(E.ctx, d, Xt, Wtt, lam_t,
+                                                          *lists("w", Lw)),
+                           lambda ns: oracle.ref_lasso_weighted(
+                               np.asfortranarray(X[:, :ns]), D,
+                               np.asfortranarray(Wt[:, :ns]), lam, num_threads=th),
+                           "X 256x%d headline signals, D 256x1024, W U[0.5,2] per atom and "
+                           "signal, lambda 0.1" % n),
+        "lasso_mask": (lambda: spx.lasso_mask_dev(E.ctx, d, Xt, Bt, lam_t, *lists("m", Lw)),
+                       lambda ns: oracle.ref_lasso_mask(np.asfortranarray(X[:, :ns]), D,
+                                                        B[:, :ns], lam, num_threads=th),
+                       "X 256x%d headline signals, D 256x1024, 85%% of rows observed "
+                       "(Bernoulli), lambda 0.1" % n),
+        "omp_mask": (lambda: spx.omp_mask_dev(E.ctx, do, Xot, Bot, *lists("o", Lo), L=L_t,
+                                              eps=e_t),
+                     lambda ns: oracle.ref_omp_mask(np.asfortranarray(Xo[:, :ns]), Do,
+                                                    Bo[:, :ns], L=L, eps=eps, num_threads=th),
+                     "OMP batch X 256x%d, D 256x1024, L=32, eps=1e-3, 85%% of rows "
+                     "observed" % n),
+    }
+    import oracle
+    th = cpu_threads()
+    out = {}
+    for name, (gpu, cpu_run, desc) in runs.items():
+        log("bench: variant %s" % name)
+        for _ in range(max(args.warmup, 1)):
+            gpu()
+        torch.cuda.synchronize()
+        ev = E.events(2)
+        t0 = time.perf_counter()
+        ev[0].record(E.stream)
+        for _ in range(args.steps):
+            gpu()
+        ev[1].record(E.stream)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        cnt = bufs[{"lasso_weighted": "w", "lasso_mask": "m", "omp_mask": "o"}[name]][2]
+        rec = {"signals_per_s": n / wall, "ms_per_step": wall * 1e3,
+               "event_ms_per_step": ev[0].elapsed_time(ev[1]) / args.steps,
+               "mean_nnz": float(cnt.float().mean().item()), "workload": desc}
+        if not args.no_cpu:
+            ns, dt = timed_sample(cpu_run, n, args.cpu_seconds / 2)
+            rec["cpu_baseline"] = {"value": ns / dt, "unit": "signals/s", "cores": th,
+                                   "kind": "reference",
+                                   "sample": f"first {ns} signals, {dt:.1f} s, the reference "
+                                             "entry point built in place"}
+        out[name] = rec
+    return {"metric": "signals/sec of the weighted and masked decomposition variants",
+            "unit": "signals/s", "n_gpus": 1, "steps": args.steps, "dtype": "f32",
+            "data": "synthetic", "higher_is_better": True, "variants": out}
+
+
 # ---------------------------------------------------------------- FISTA
 
 def run_fista(args):
@@ -523,7 +611,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=200000)
-    ap.add_argument("--workload", default="lars", choices=["lars", "omp", "fista", "served"])
+    ap.add_argument("--workload", default="lars",
+                    choices=["lars", "omp", "fista", "served", "variants"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -545,6 +634,8 @@ def main():
         out = run_omp(args)
     elif args.workload == "fista":
         out = run_fista(args)
+    elif args.workload == "variants":
+        out = run_variants(args)
     else:
         out = {"metric": "served sparse coding p50/p99 latency", "served": served_report(n_jobs=300)}
     print(json.dumps(out), flush=True)

@@ -7,6 +7,8 @@
 //   _lassoD     cpp_library/spams.h:107-143   (lasso2 / coreLARS2 homotopy)
 //   _lassoQq    cpp_library/spams.h:145-188   (Gram matrix given)
 //   _lassoWeighted  cpp_library/spams.h:217-249  (lassoWeight / coreLARS2W)
+//   _lassoMask  cpp_library/spams.h:190-215   (lasso_mask)
+//   _ompMask    cpp_library/spams.h:295-343   (omp_mask)
 //   _omp        cpp_library/spams.h:250-293   (omp / coreORMP)
 //   _fistaFlat  cpp_library/spams.h:460-608   (square loss; l1, elastic-net)
 //   lasso2      decomp/decomp.h:126-135        omp  decomp/decomp.h:53-57
@@ -190,6 +192,33 @@ inline int lasso_weighted_call(const double* X, int m, int n, const double* D, i
                                spx_csc* out) {
   return spx_lasso_weighted_f64(context(), X, m, n, D, p, W, lam, 1, o, out);
 }
+inline int lasso_mask_call(const float* X, int m, int n, const float* D, int p,
+                           const unsigned char* B, const float* lam, float l2,
+                           const spx_lasso_opts* o, spx_csc* out) {
+  return spx_lasso_mask_f32(context(), X, m, n, D, p, B, lam, 1, l2, o, out);
+}
+inline int lasso_mask_call(const double* X, int m, int n, const double* D, int p,
+                           const unsigned char* B, const double* lam, double l2,
+                           const spx_lasso_opts* o, spx_csc* out) {
+  return spx_lasso_mask_f64(context(), X, m, n, D, p, B, lam, 1, l2, o, out);
+}
+inline int omp_mask_call(const float* X, int m, int n, const float* D, int p,
+                         const unsigned char* B, const int* L, int nL, const float* e, int ne,
+                         const float* la, int nla, spx_csc* out, float* path) {
+  return spx_omp_mask_f32(context(), X, m, n, D, p, B, L, nL, e, ne, la, nla, nullptr, out,
+                          path);
+}
+inline int omp_mask_call(const double* X, int m, int n, const double* D, int p,
+                         const unsigned char* B, const int* L, int nL, const double* e, int ne,
+                         const double* la, int nla, spx_csc* out, double* path) {
+  return spx_omp_mask_f64(context(), X, m, n, D, p, B, L, nL, e, ne, la, nla, nullptr, out,
+                          path);
+}
+// Matrix<bool> masks are passed as their bytes (0 / 1)
+static_assert(sizeof(bool) == 1, "masks are read as one byte per entry");
+inline const unsigned char* mask_bytes(const Matrix<bool>* B) {
+  return reinterpret_cast<const unsigned char*>(B->rawX());
+}
 inline int omp_call(const float* X, int m, int n, const float* D, int p, const int* L, int nL,
                     const float* e, int ne, const float* la, int nla, spx_csc* out,
                     float* path) {
@@ -325,6 +354,64 @@ SpMatrix<T>* _lassoWeighted(Matrix<T>* X, Matrix<T>* D, Matrix<T>* W, int L, con
     alpha->adopt(out);
   } catch (...) {
     delete alpha;
+    throw;
+  }
+  return alpha;
+}
+
+// _lassoMask (cpp_library/spams.h:190-215): B (m x n) marks the observed
+// rows of each signal.  A column with no observed row comes back empty.
+template <typename T>
+SpMatrix<T>* _lassoMask(Matrix<T>* X, Matrix<T>* D, Matrix<bool>* B, int L, const T constraint,
+                        const T lambda2, constraint_type mode, const bool pos,
+                        const int numThreads, bool verbose) {
+  (void)verbose;
+  (void)numThreads;
+  if (X->m() != D->m()) throw "lassoMask : incompatible matrix dimensions";
+  if (B->m() != X->m() || B->n() != X->n()) throw "lassoMask : Mask has non acceptable dimensions";
+  SpMatrix<T>* alpha = new SpMatrix<T>();
+  try {
+    spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, -1, SPX_PREC_FP32, SPX_FORM_BATCHED, 0, 0};
+    spx_csc out{};
+    spams_amd_detail::check(spams_amd_detail::lasso_mask_call(
+        X->rawX(), X->m(), X->n(), D->rawX(), D->n(), spams_amd_detail::mask_bytes(B),
+        &constraint, lambda2, &o, &out));
+    alpha->adopt(out);
+  } catch (...) {
+    delete alpha;
+    throw;
+  }
+  return alpha;
+}
+
+// _ompMask (cpp_library/spams.h:295-343)
+template <typename T>
+SpMatrix<T>* _ompMask(Matrix<T>* X, Matrix<T>* D, Matrix<bool>* B, Matrix<T>** path,
+                      bool return_reg_path, bool given_L, Vector<int>* L, bool given_eps,
+                      Vector<T>* eps, bool given_Lambda, Vector<T>* Lambda,
+                      const int numThreads) {
+  (void)numThreads;
+  if (X->m() != D->m()) throw "ompMask : incompatible matrix dimensions";
+  if (B->m() != X->m() || B->n() != X->n()) throw "ompMask : Mash has non acceptable dimensions";
+  if (!given_L && !given_eps && !given_Lambda)
+    throw "omp : You should either provide L, eps or lambda";
+  const int scalar_L = X->m() < D->n() ? X->m() : D->n();
+  *path = return_reg_path ? new Matrix<T>(D->n(), scalar_L) : nullptr;
+  SpMatrix<T>* alpha = new SpMatrix<T>();
+  try {
+    const int n = X->n();
+    spx_csc out{};
+    spams_amd_detail::check(spams_amd_detail::omp_mask_call(
+        X->rawX(), X->m(), n, D->rawX(), D->n(), spams_amd_detail::mask_bytes(B),
+        given_L ? L->rawX() : nullptr, given_L ? (L->n() > 1 ? n : 1) : 0,
+        given_eps ? eps->rawX() : nullptr, given_eps ? (eps->n() > 1 ? n : 1) : 0,
+        given_Lambda ? Lambda->rawX() : nullptr, given_Lambda ? (Lambda->n() > 1 ? n : 1) : 0,
+        &out, *path ? (*path)->rawX() : (T*)nullptr));
+    alpha->adopt(out);
+  } catch (...) {
+    delete alpha;
+    delete *path;
+    *path = nullptr;
     throw;
   }
   return alpha;

@@ -4,13 +4,21 @@
  * pointers and sizes (column-major, `int` dimensions, CSC with `int` indices),
  * returns an int status (0 on success) and leaves a message for
  * spx_last_error() on failure.  The C++ header spams_amd.h re-creates the
- * reference's spams.h templates (_lassoD, _omp, _fistaFlat) on top of it.
+ * reference's spams.h templates (_lassoD, _lassoQq, _lassoMask,
+ * _lassoWeighted, _omp, _ompMask, _fistaFlat) on top of it.
  *
  * Reference interfaces replaced (paths relative to the SPAMS tree):
  *   spx_lasso_*       _lassoD      cpp_library/spams.h:107-143
  *                     lasso2       decomp/decomp.h:1563-1646 (+coreLARS2 :1706)
+ *   spx_lasso_qq_*    _lassoQq     cpp_library/spams.h:145-188
+ *   spx_lasso_mask_*  _lassoMask   cpp_library/spams.h:190-215
+ *                     lasso_mask   decomp/decomp.h:1466-1561
+ *   spx_lasso_weighted_*  _lassoWeighted  cpp_library/spams.h:217-249
+ *                     lassoWeight  decomp/decomp.h:1315-1391 (+coreLARS2W :1917)
  *   spx_omp_*         _omp         cpp_library/spams.h:250-293
  *                     omp          decomp/decomp.h:322-404 (+coreORMP :538)
+ *   spx_omp_mask_*    _ompMask     cpp_library/spams.h:295-343
+ *                     omp_mask     decomp/decomp.h:406-519
  *   spx_fista_flat_*  _fistaFlat   cpp_library/spams.h:460-608
  *                     FISTA::solver prox/fista.h:3618-3752
  *   spx_csc           SpMatrix<T>  linalg/linalg.h:775-934 (pB/pE/r/v CSC)

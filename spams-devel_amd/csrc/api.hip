@@ -269,12 +269,11 @@ static int products(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
   return col_sqnorms<T>(ctx->stream, d->m, n, X, ldx, *norms);
 }
 
-// Masked-signal inputs of the solvers (device): class per signal (0 / 1 / 2
-// for no / all / some rows observed) and the m x n row mask.
+// Masked-signal inputs of the solvers (device): count of observed rows per
+// signal and the rows its masked Gram columns visit (m x n).
 struct MaskIn {
-  const unsigned char* cls;
-  const unsigned char* B;
-  int ldb;
+  const int* obs;
+  const int* rows;
 };
 
 // W (device, p x n, leading dimension ldw) selects the weighted penalty of
@@ -356,9 +355,8 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   }
   if (mi) {
     // coreLARS2's default step cap 4 min(L, p) (decomp/decomp.h:1724)
-    a.mclass = mi->cls;
-    a.mask = mi->B;
-    a.ldmask = mi->ldb;
+    a.mobs = mi->obs;
+    a.mrows = mi->rows;
     a.Dd = static_cast<const T*>(d->D);
     a.Dt = static_cast<const T*>(d->Dt);
     a.m = d->m;
@@ -437,16 +435,17 @@ static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
     const int m = d->m;
     void *bx = nullptr, *bc = nullptr, *be = nullptr;
     int rc = ctx->scratch_get(18, sizeof(T) * (size_t)m * n, &bx);
-    if (!rc) rc = ctx->scratch_get(19, (size_t)n, &bc);
+    if (!rc) rc = ctx->scratch_get(19, sizeof(int) * (size_t)n * (m + 1), &bc);
     if (!rc) rc = ctx->scratch_get(20, sizeof(T) * (size_t)n, &be);
     if (rc) return rc;
     T* Xm = static_cast<T*>(bx);
-    unsigned char* cls = static_cast<unsigned char*>(bc);
+    int* obs = static_cast<int*>(bc);
+    int* rows = obs + n;
     T* epsv = static_cast<T*>(be);
-    rc = mask_prep<T>(ctx->stream, X, ldx, m, n, B, ldb, Xm, cls, eps, nEps, epsv, 1);
+    rc = mask_prep<T>(ctx->stream, X, ldx, m, n, B, ldb, Xm, obs, rows, eps, nEps, epsv, 1);
     if (!rc) rc = products<T>(ctx, d, Xm, m, n, o->precision, &DtX, &norms);
     if (rc) return rc;
-    const MaskIn mi{cls, B, ldb};
+    const MaskIn mi{obs, rows};
     return omp_solve_dtx<T>(ctx, d, DtX, norms, n, L, nL, nEps ? epsv : nullptr, nEps ? n : 0,
                             lambda, nLambda, o, out, &mi);
   }
@@ -464,9 +463,8 @@ static int omp_solve_dtx(spx_ctx* ctx, const spx_dict* d, const T* DtX, const T*
                          const MaskIn* mi) {
   OmpArgs<T> a{};
   if (mi) {
-    a.mclass = mi->cls;
-    a.mask = mi->B;
-    a.ldmask = mi->ldb;
+    a.mobs = mi->obs;
+    a.mrows = mi->rows;
     a.Dd = static_cast<const T*>(d->D);
     a.Dt = static_cast<const T*>(d->Dt);
     a.m = d->m;
@@ -680,20 +678,21 @@ static int lasso_mask_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, 
   const int m = d->m;
   void *bx = nullptr, *bc = nullptr, *bl = nullptr;
   int rc = ctx->scratch_get(18, sizeof(T) * (size_t)m * n, &bx);
-  if (!rc) rc = ctx->scratch_get(19, (size_t)n, &bc);
+  if (!rc) rc = ctx->scratch_get(19, sizeof(int) * (size_t)n * (m + 1), &bc);
   if (!rc) rc = ctx->scratch_get(20, sizeof(T) * (size_t)n, &bl);
   if (rc) return rc;
   T* Xm = static_cast<T*>(bx);
-  unsigned char* cls = static_cast<unsigned char*>(bc);
+  int* obs = static_cast<int*>(bc);
+  int* rows = obs + n;
   T* lamv = static_cast<T*>(bl);
   const int scale = o->mode == SPX_PENALTY || o->mode == SPX_L2ERROR;
-  rc = mask_prep<T>(ctx->stream, X, ldx, m, n, B, ldb, Xm, cls, lambda, nlam, lamv, scale);
+  rc = mask_prep<T>(ctx->stream, X, ldx, m, n, B, ldb, Xm, obs, rows, lambda, nlam, lamv, scale);
   if (rc) return rc;
   T* DtX = nullptr;
   T* norms = nullptr;
   rc = products<T>(ctx, d, Xm, m, n, o->precision & SPX_PREC_BF16X3, &DtX, &norms);
   if (rc) return rc;
-  const MaskIn mi{cls, B, ldb};
+  const MaskIn mi{obs, rows};
   return lasso_dtx_dev<T>(ctx, d, DtX, d->ldg, norms, n, lamv, n, lambda2, o, out, nullptr, 0,
                           &mi);
 }

@@ -11,7 +11,7 @@ template <typename T>
 int lars_pass_w(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
                 int* counter, long max_blocks) {
   if (a.weights) return lars_weighted_pass<T>(ctx, w, fast, a, persistent, counter, max_blocks);
-  if (a.mclass) return lars_masked_pass<T>(ctx, w, fast, a, persistent, counter, max_blocks);
+  if (a.mobs) return lars_masked_pass<T>(ctx, w, fast, a, persistent, counter, max_blocks);
   return lars_pass_wt<T, 0>(ctx, w, fast, a, persistent, counter, max_blocks);
 }
 

@@ -281,7 +281,7 @@ __device__ __forceinline__ void inverse_drop(const T* Ms, int lds, T* Md, int ld
 // (128 VGPRs): its signals are latency-bound, so occupancy pays more than
 // registers.
 // VAR: 0 plain penalty, 1 weighted penalty (a.weights), 2 masked signals
-// (a.mclass).
+// (a.mobs).
 template <typename T, int W, int NQ, int CAP, int VAR>
 __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_kernel(LarsArgs<T> a) {
   constexpr int GT = 64 * W;
@@ -413,10 +413,13 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
     int s_active = 0;
     int iter = 0, admissions = 0;
     bool empty = false, overflow = false, reached = false;
-    // masked: class 2 signals use Gram columns over their observed rows
-    const int mcls = mk ? a.mclass[sig] : 1;
-    const bool own = mk && mcls == 2;
-    const unsigned char* msk = own ? a.mask + (size_t)sig * a.ldmask : nullptr;
+    // masked: partly observed signals use Gram columns over their observed
+    // rows; a signal with no observed row has an empty column
+    const int nobs = mk ? a.mobs[sig] : 0;
+    const bool own = mk && nobs > 0 && nobs < a.m;
+    const int* mrl = own ? a.mrows + (size_t)sig * a.m : nullptr;
+    const bool mcomp = own && masked_gram_comp(nobs, a.m);
+    const int mnsel = own ? masked_gram_nsel(nobs, a.m) : 0;
     if (mk)
       for (int j = tid; j < LC; j += GT) sSlot[j] = j;
     if (resume) {
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
       if (own) {
         // the Gram columns of the handed-over active set, slots 0..s0-1
         for (int k = 0; k < s0; ++k)
-          masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.m, p, ldg, msk, st.ind[k],
+          masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.G, a.m, p, ldg, mrl, mnsel, mcomp, st.ind[k],
                                      gaw + (size_t)k * ldg, T(0));
       }
       for (int j = tid; j < LC; j += GT) {
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         sInd[j] = -1;
       }
       normX = a.norms[sig];
-      if (mk && mcls == 0) empty = true;  // no observed row: empty column
+      if (mk && nobs == 0) empty = true;
       __syncthreads();
 
       // first atom: i?amax of DtR, or the plain max when pos
@@ -508,7 +511,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           const T* gcol = a.G + (size_t)c * ldg;
           if (own) {
             T* slot = gaw + (size_t)sSlot[i] * ldg;
-            masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.m, p, ldg, msk, c, slot, T(0));
+            masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.G, a.m, p, ldg, mrl, mnsel, mcomp, c, slot, T(0));
             gcol = slot;
           }
           for (int j = tid; j <= i; j += GT) {
@@ -1031,8 +1034,8 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
     if (a.nlam > 1) b.lambda = a.lambda + s0;
     if (a.weights) b.weights = a.weights + (size_t)s0 * a.ldw;
     if (VAR == 2) {
-      b.mclass = a.mclass + s0;
-      b.mask = a.mask + (size_t)s0 * a.ldmask;
+      b.mobs = a.mobs + s0;
+      b.mrows = a.mrows + (size_t)s0 * a.m;
       b.ga = ga;
       b.ga_stride = ga_stride;
     }

@@ -1,9 +1,11 @@
 // Set-up of the masked-signal solvers (lasso_mask, decomp/decomp.h:1466-1561;
-// omp_mask, :406-519): per signal the count of observed rows, its class
-// (0 none, 1 all, 2 some), the signal with its unobserved rows zeroed (so
-// that D'x and ||x||^2 are those of the masked copies, Vector::copyMask,
-// linalg/linalg.h:4080-4090) and the constraint scaled by the observed
-// fraction for class 2 (constraint * |M| / m, decomp/decomp.h:1535 and :489).
+// omp_mask, :406-519): per signal the count of observed rows (0 = none, m =
+// all, else some: the solvers' three cases), the signal with its unobserved
+// rows zeroed (so that D'x and ||x||^2 are those of the masked copies,
+// Vector::copyMask, linalg/linalg.h:4080-4090) and, for a partly observed
+// signal, the constraint scaled by the observed fraction (constraint * |M| /
+// m, decomp/decomp.h:1535 and :489).
+#include "mask_gram.h"
 #include "spx_common.h"
 #include "spx_internal.h"
 
@@ -15,8 +17,8 @@ namespace {
 template <typename T>
 __global__ __launch_bounds__(256) void mask_prep_kernel(
     const T* __restrict__ X, int ldx, int m, int n, const unsigned char* __restrict__ B, int ldb,
-    T* __restrict__ Xm, unsigned char* __restrict__ cls, const T* __restrict__ s_in, int ns,
-    T* __restrict__ s_out, int scale) {
+    T* __restrict__ Xm, int* __restrict__ obs, int* __restrict__ rows,
+    const T* __restrict__ s_in, int ns, T* __restrict__ s_out, int scale) {
   const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (col >= n) return;
@@ -30,12 +32,23 @@ __global__ __launch_bounds__(256) void mask_prep_kernel(
     cnt += o ? 1 : 0;
   }
   cnt = static_cast<int>(wave_sum(static_cast<float>(cnt)));  // exact below 2^24
+  // the rows the Gram columns visit, ascending: the observed ones, or the
+  // unobserved ones when those are fewer
+  const bool comp = masked_gram_comp(cnt, m);
+  int* rl = rows + (size_t)col * m;
+  int base = 0;
+  for (int r0 = 0; r0 < m; r0 += 64) {
+    const int r = r0 + lane;
+    const bool sel = r < m && ((b[r] != 0) != comp);
+    const unsigned long long bal = __ballot(sel);
+    if (sel) rl[base + __popcll(bal & ((1ull << lane) - 1))] = r;
+    base += __popcll(bal);
+  }
   if (lane == 0) {
-    const unsigned char c = cnt == 0 ? 0 : (cnt == m ? 1 : 2);
-    cls[col] = c;
+    obs[col] = cnt;
     if (s_out) {
       const T v = ns == 0 ? T(0) : s_in[ns > 1 ? col : 0];
-      s_out[col] = (scale && c == 2) ? v * T(cnt) / T(m) : v;
+      s_out[col] = (scale && cnt > 0 && cnt < m) ? v * T(cnt) / T(m) : v;
     }
   }
 }
@@ -44,17 +57,17 @@ __global__ __launch_bounds__(256) void mask_prep_kernel(
 
 template <typename T>
 int mask_prep(hipStream_t st, const T* X, int ldx, int m, int n, const unsigned char* B, int ldb,
-              T* Xm, unsigned char* cls, const T* s_in, int ns, T* s_out, int scale) {
+              T* Xm, int* obs, int* rows, const T* s_in, int ns, T* s_out, int scale) {
   if (n <= 0) return SPX_OK;
   hipLaunchKernelGGL(mask_prep_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, X, ldx, m, n, B,
-                     ldb, Xm, cls, s_in, ns, s_out, scale);
+                     ldb, Xm, obs, rows, s_in, ns, s_out, scale);
   SPX_HIP(hipGetLastError());
   return SPX_OK;
 }
 
 template int mask_prep<float>(hipStream_t, const float*, int, int, int, const unsigned char*, int,
-                              float*, unsigned char*, const float*, int, float*, int);
+                              float*, int*, int*, const float*, int, float*, int);
 template int mask_prep<double>(hipStream_t, const double*, int, int, int, const unsigned char*,
-                               int, double*, unsigned char*, const double*, int, double*, int);
+                               int, double*, int*, int*, const double*, int, double*, int);
 
 }  // namespace spx

@@ -48,7 +48,7 @@ constexpr size_t kUnLdsBytes = 32 * 1024;
 
 // Held to 4 wavefronts per SIMD up to 16 rows per lane: the signals are
 // latency-bound, so occupancy pays more than registers.  MK: masked signals
-// (omp_mask, decomp/decomp.h:406-519; a.mclass set).
+// (omp_mask, decomp/decomp.h:406-519; a.mobs set).
 template <typename T, int W, int NQ, bool ULDS, bool MK>
 __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T> a) {
   constexpr int GT = 64 * W;
@@ -115,12 +115,14 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
     const T lambda = a.nLambda == 0 ? T(0) : a.lambda[a.nLambda > 1 ? sig : 0];
     int Li = a.nL == 0 ? a.scalarL : a.Ls[a.nL > 1 ? sig : 0];
     if (Li > L) Li = L;
-    // masked: class 0 (no observed row) is empty; class 2 uses the Gram
-    // columns of its observed rows plus 1e-10 on the diagonal
-    const int mcls = MK ? a.mclass[sig] : 1;
-    const bool own = MK && mcls == 2;
-    const unsigned char* msk = own ? a.mask + (size_t)sig * a.ldmask : nullptr;
-    if (MK && mcls == 0) Li = 0;
+    // masked: no observed row -> empty; partly observed -> the Gram columns
+    // of its observed rows plus 1e-10 on the diagonal
+    const int nobs = MK ? a.mobs[sig] : 0;
+    const bool own = MK && nobs > 0 && nobs < a.m;
+    const int* mrl = own ? a.mrows + (size_t)sig * a.m : nullptr;
+    const bool mcomp = own && masked_gram_comp(nobs, a.m);
+    const int mnsel = own ? masked_gram_nsel(nobs, a.m) : 0;
+    if (MK && nobs == 0) Li = 0;
     __syncthreads();
 
     int j = 0;
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
         if (own) {
           // the new atom's Gram column, slot j (selection order)
           T* slot = gaw + (size_t)j * ldg;
-          masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.m, p, ldg, msk, c, slot, T(1e-10));
+          masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.G, a.m, p, ldg, mrl, mnsel, mcomp, c, slot, T(1e-10));
           gcol = slot;
         }
         for (int k = tid; k < j; k += GT) sGc[k] = gcol[sInd[k]];
@@ -434,8 +436,8 @@ int ormp_run(Ctx* ctx, const OmpArgs<T>& a0, int form) {
     a.outR = a0.outR + (size_t)s0 * a0.L;
     a.outCount = a0.outCount + s0;
     if (MK) {
-      a.mclass = a0.mclass + s0;
-      a.mask = a0.mask + (size_t)s0 * a0.ldmask;
+      a.mobs = a0.mobs + s0;
+      a.mrows = a0.mrows + (size_t)s0 * a0.m;
       a.ga = ga;
       a.ga_stride = ga_stride;
     }
@@ -468,7 +470,7 @@ int ormp_dispatch_nq_t(Ctx* ctx, const OmpArgs<T>& a, int form) {
 
 template <typename T, int W>
 int ormp_dispatch_nq(Ctx* ctx, const OmpArgs<T>& a, int form) {
-  return a.mclass ? ormp_dispatch_nq_t<T, W, true>(ctx, a, form)
+  return a.mobs ? ormp_dispatch_nq_t<T, W, true>(ctx, a, form)
                   : ormp_dispatch_nq_t<T, W, false>(ctx, a, form);
 }
 

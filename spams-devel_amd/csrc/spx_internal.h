@@ -61,14 +61,14 @@ struct LarsArgs {
   const T* weights;
   int ldw;
   T tol;            // stop tolerance of |step|, normX and the mode margins
-  // Masked signals (lasso_mask): per-signal class (0 no row observed, 1 all
-  // rows, 2 some rows), the m x n row mask (leading dimension ldmask) and
-  // the dictionary (D m x p, Dt p x m) from which a class-2 signal's Gram
-  // columns over its observed rows are formed into its workgroup's slot of
-  // `ga` (ga_stride elements per workgroup, columns of ldg).
-  const unsigned char* mclass;
-  const unsigned char* mask;
-  int ldmask;
+  // Masked signals (lasso_mask): per-signal count of observed rows (0: empty
+  // column; m: the plain path), per signal the rows its Gram columns visit
+  // (m x n, mask_gram.h) and the dictionary (D m x p, Dt p x m) from which a
+  // partly observed signal's Gram columns over its observed rows are formed
+  // into its workgroup's slot of `ga` (ga_stride elements per workgroup,
+  // columns of ldg).
+  const int* mobs;
+  const int* mrows;
   const T* Dd;
   const T* Dt;
   int m;
@@ -100,11 +100,11 @@ struct OmpArgs {
   int chunk;        // signals taken per atomic grab of the work counter
   T* path;          // signal 0's path (p x path_cols) or null
   int path_cols;
-  // Masked signals (omp_mask): as in LarsArgs; a class-2 signal's Gram
-  // columns (ridge 1e-10) fill its workgroup's slot of ga in selection order.
-  const unsigned char* mclass;
-  const unsigned char* mask;
-  int ldmask;
+  // Masked signals (omp_mask): as in LarsArgs; a partly observed signal's
+  // Gram columns (ridge 1e-10) fill its workgroup's slot of ga in selection
+  // order.
+  const int* mobs;
+  const int* mrows;
   const T* Dd;
   const T* Dt;
   int m;
@@ -125,7 +125,7 @@ int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves);
 template <typename T>
 int lars_weighted_pass(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
                        int* counter, long max_blocks);
-// The same for the masked-signal kernels (a.mclass set).
+// The same for the masked-signal kernels (a.mobs set).
 template <typename T>
 int lars_masked_pass(Ctx* ctx, int w, bool fast, const LarsArgs<T>& a, bool persistent,
                      int* counter, long max_blocks);
@@ -153,12 +153,13 @@ template <typename T>
 int sort_list(Ctx* ctx, const T* key, const int* list, const int* count_dev, int n, int* out);
 
 // Masked-signal set-up (one wavefront per signal): Xm (m x n) = X with the
-// unobserved rows (B == 0) zeroed, cls[i] = 0 / 1 / 2 for no / all / some
-// rows observed, and s_out[i] = s_in[i or 0] (0 when ns == 0), times |M| / m
-// for class 2 when `scale`.
+// unobserved rows (B == 0) zeroed, obs[i] = signal i's count of observed
+// rows, rows (m x n) = per signal the rows its masked Gram columns visit
+// (mask_gram.h), and s_out[i] = s_in[i or 0] (0 when ns == 0), times
+// obs[i] / m when `scale` and 0 < obs[i] < m.
 template <typename T>
 int mask_prep(hipStream_t st, const T* X, int ldx, int m, int n, const unsigned char* B, int ldb,
-              T* Xm, unsigned char* cls, const T* s_in, int ns, T* s_out, int scale);
+              T* Xm, int* obs, int* rows, const T* s_in, int ns, T* s_out, int scale);
 
 // Clears n ints with a kernel (capture-safe replacement for a memset node).
 int zero_ints(hipStream_t st, int* p, int n);

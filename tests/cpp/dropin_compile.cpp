@@ -41,6 +41,11 @@ void use_omp() {
   SpMatrix<double>* s2 = _omp(&X, &D, &path, true, true, &vL, false, &ve, false, &vl, -1);
   delete s2;
   delete path;
+  Matrix<bool> B(64, 10);
+  SpMatrix<double>* s3 = _ompMask(&X, &D, &B, &path, false, true, &vL, false, &ve, false, &vl, -1);
+  delete s3;
+  SpMatrix<double>* s4 = _lassoMask(&X, &D, &B, 10, 0.15, 0.0, PENALTY, false, -1, false);
+  delete s4;
 }
 
 void use_fista() {
