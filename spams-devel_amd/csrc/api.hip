@@ -430,7 +430,8 @@ static int omp_dev(spx_ctx* ctx, const spx_dict* d, const T* X, int ldx, int n,
   T* DtX = nullptr;
   T* norms = nullptr;
   if (B) {
-    // masked copies, classes and eps * |M| / m (decomp/decomp.h:489)
+    // masked copies, observed counts, row lists and eps * |M| / m
+    // (decomp/decomp.h:491)
     if (int rc = need_dictionary(d)) return rc;
     const int m = d->m;
     void *bx = nullptr, *bc = nullptr, *be = nullptr;

@@ -4,7 +4,7 @@
 // rows zeroed (so that D'x and ||x||^2 are those of the masked copies,
 // Vector::copyMask, linalg/linalg.h:4080-4090) and, for a partly observed
 // signal, the constraint scaled by the observed fraction (constraint * |M| /
-// m, decomp/decomp.h:1535 and :489).
+// m, decomp/decomp.h:1534 and :491).
 #include "mask_gram.h"
 #include "spx_common.h"
 #include "spx_internal.h"
