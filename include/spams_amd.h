@@ -13,6 +13,8 @@
 //   _fistaFlat  cpp_library/spams.h:460-608   (square loss; l1, elastic-net)
 //   lasso2      decomp/decomp.h:126-135        omp  decomp/decomp.h:53-57
 //   cppLasso / cppOMP  cpp_library/cppspams.h:45-51, :106-113
+//   _alltrainDL cpp_library/spams.h:1153-1340 (trainDL: Trainer::train,
+//               dictLearn/dicts.h:346-759; modeD = L2, no whitening)
 //
 // Types mirror the reference's layouts (linalg/linalg.h): Matrix<T> is
 // column-major with int dimensions and may wrap external memory; Vector<T>
@@ -30,6 +32,8 @@
 
 // constraint_type (decomp/decomp.h:95)
 enum constraint_type { L1COEFFS, L2ERROR, PENALTY, SPARSITY, L2ERROR2, PENALTY2, FISTAMODE };
+// constraint_type_D (dictLearn/dicts.h:39)
+enum constraint_type_D { L2, L1L2, L1L2FL, L1L2MU };
 
 template <typename T>
 class Vector {
@@ -103,6 +107,9 @@ class Matrix {
 // column-major matrix (the SpMatrix dictionary form is not provided).
 template <typename T>
 using AbstractMatrixB = Matrix<T>;
+// Data<T> stand-in for trainDL's signal argument: dense column-major signals.
+template <typename T>
+using Data = Matrix<T>;
 
 template <typename T>
 class SpMatrix {
@@ -238,6 +245,14 @@ T* path_buffer(Matrix<T>* path, int p, int L, int length_path) {
   const int cols = length_path < 0 ? 4 * L : length_path;
   if (path->m() != p || path->n() < cols) throw "lasso : the path matrix must be p x max_length_path";
   return path->rawX();
+}
+inline int train_dl_call(const float* X, int m, int n, float* D, int K, int D_given, float* A,
+                         float* B, int* it, const spx_train_dl_opts* o) {
+  return spx_train_dl_f32(context(), X, m, n, D, K, D_given, A, B, it, o);
+}
+inline int train_dl_call(const double* X, int m, int n, double* D, int K, int D_given,
+                         double* A, double* B, int* it, const spx_train_dl_opts* o) {
+  return spx_train_dl_f64(context(), X, m, n, D, K, D_given, A, B, it, o);
 }
 inline int fista_call(const float* X, int m, int n, const float* D, int p, const float* W0,
                       float* W, const spx_fista_opts* o, float* info) {
@@ -499,6 +514,80 @@ Matrix<T>* _fistaFlat(Matrix<T>* X, AbstractMatrixB<T>* D, Matrix<T>* alpha0, Ma
     throw spx_last_error();
   }
   return info;
+}
+
+// _alltrainDL (cpp_library/spams.h:1153-1340) for trainDL: online (or batch)
+// dictionary learning with LARS (L1COEFFS, L2ERROR, PENALTY) or ORMP
+// (SPARSITY, L2ERROR2, PENALTY2) codes, modeD = L2.  Returns the learned
+// dictionary (m x K); with return_model, *omA (K x K), *omB (m x K) and
+// *omiter (1) receive the model, else NULL.  A model (m_A non-empty) is
+// continued from iteration m_iter when D1 is given.  The structured
+// (FISTAMODE), in-memory, sparse-dictionary, whitening, stochastic and log
+// options are not provided and throw.
+template <typename T>
+Matrix<T>* _alltrainDL(Data<T>* X, bool in_memory, Matrix<T>** omA, Matrix<T>** omB,
+                       Vector<int>** omiter, bool return_model, Matrix<T>* m_A, Matrix<T>* m_B,
+                       int m_iter, Matrix<T>* D1, Vector<T>* eta_g, SpMatrix<bool>* groups,
+                       SpMatrix<bool>* groups_var, Vector<int>* own_variables,
+                       Vector<int>* N_own_variables, int num_threads, T tol, bool fixed_step,
+                       bool ista, int batch_size, int K, double lambda1, double lambda2,
+                       double lambda3, int iter, double t0, constraint_type mode,
+                       char* name_regul, bool posAlpha, bool posD, bool expand,
+                       constraint_type_D modeD, bool whiten, bool clean, bool verbose,
+                       double gamma1, double gamma2, T rho, int iter_updateD, bool stochastic,
+                       int modeParam, bool batch, bool log, char* logName) {
+  (void)eta_g; (void)groups; (void)groups_var; (void)own_variables; (void)N_own_variables;
+  (void)tol; (void)fixed_step; (void)ista; (void)lambda3; (void)name_regul; (void)verbose;
+  (void)gamma1; (void)gamma2; (void)logName;
+  if (in_memory || mode == FISTAMODE || modeD != L2 || whiten || expand || stochastic || log)
+    throw "trainDL: option not provided by spx (in_memory, structured, modeD, whiten, "
+          "expand, stochastic or log)";
+  const int n = X->m();
+  const int M = X->n();
+  const bool D_given = D1 && D1->n() > 0;
+  if (!D_given) {
+    if (K < 0) throw "trainDL : bad parameter K\n";
+  } else {
+    K = D1->n();
+    if (n != D1->m()) throw "trainDL : sizes of D are not consistent\n";
+  }
+  const bool model = D_given && m_A && m_A->n() > 0;
+  if (model && (m_A->m() != K || m_A->n() != K || !m_B || m_B->m() != n || m_B->n() != K))
+    throw "trainDL : sizes of the model are not consistent";
+  Matrix<T>* D = new Matrix<T>(n, K);
+  if (D_given) std::memcpy(D->rawX(), D1->rawX(), sizeof(T) * (size_t)n * K);
+  Matrix<T>* A = new Matrix<T>(K, K);
+  Matrix<T>* B = new Matrix<T>(n, K);
+  int it = 0;
+  if (model) {
+    std::memcpy(A->rawX(), m_A->rawX(), sizeof(T) * (size_t)K * K);
+    std::memcpy(B->rawX(), m_B->rawX(), sizeof(T) * (size_t)n * K);
+    it = m_iter;
+  }
+  spx_train_dl_opts o{batch_size, num_threads, lambda1, lambda2, iter, t0, (int)mode,
+                      posAlpha ? 1 : 0, posD ? 1 : 0, clean ? 1 : 0, (double)rho,
+                      iter_updateD, modeParam, batch ? 1 : 0, SPX_PREC_FP32, 0};
+  const int rc = spams_amd_detail::train_dl_call(X->rawX(), n, M, D->rawX(), K, D_given ? 1 : 0,
+                                                 A->rawX(), B->rawX(), &it, &o);
+  if (rc != SPX_OK) {
+    delete D;
+    delete A;
+    delete B;
+    throw spx_last_error();
+  }
+  if (return_model) {
+    *omA = A;
+    *omB = B;
+    *omiter = new Vector<int>(1);
+    (**omiter)[0] = it;
+  } else {
+    delete A;
+    delete B;
+    if (omA) *omA = nullptr;
+    if (omB) *omB = nullptr;
+    if (omiter) *omiter = nullptr;
+  }
+  return D;
 }
 
 // cppLasso / cppOMP (cpp_library/cppspams.h:45-51, :106-113).

@@ -334,6 +334,44 @@ int spx_fista_flat_f64(spx_ctx* ctx, const double* X, int m, int n,
                        const double* D, int p, const double* W0, double* W,
                        const spx_fista_opts* opts, double* optim_info);
 
+/* Options of spx_train_dl_*: the trainDL arguments of _alltrainDL
+ * (cpp_library/spams.h:1153-1190), with the meaning and defaults of
+ * swig/python/spams.py:485-500, for modeD = L2 without whitening. */
+typedef struct {
+  int batchsize;    /* signals per iteration; <= 0 -> 256 * (num_threads + 1) */
+  int num_threads;  /* only the batchsize default reads it; <= 0 -> host CPUs  */
+  double lambda1;   /* LARS constraint, SPARSITY's atom count, L2ERROR2's eps  */
+  double lambda2;   /* ridge max(lambda2, 1e-10) on the Gram diagonal          */
+  int iter;         /* iterations (> 0), or seconds of training when < 0      */
+  double t0;        /* online: initial model A = |t0| I, B = |t0| D            */
+  int mode;         /* SPX_L1COEFFS / SPX_L2ERROR / SPX_PENALTY (LARS),
+                     * 3 SPARSITY, 4 L2ERROR2, 5 PENALTY2 (ORMP)              */
+  int posAlpha;     /* nonnegative codes (LARS modes)                          */
+  int posD;         /* nonnegative dictionary                                  */
+  int clean;        /* replace duplicated / vanishing atoms                    */
+  double rho;       /* modeParam 1 and 2                                       */
+  int iter_updateD; /* dictionary-update sweeps per iteration (1 online, 5 batch
+                     * in spams.py)                                            */
+  int modeParam;    /* 0 AUTO, 1 PARAM1, 2 PARAM2 (the online weights)         */
+  int batch;        /* batch mode: every iteration codes all n signals         */
+  int precision;    /* SPX_PREC_BF16X3: f32 products on BF16x3 MFMA            */
+  int waves_per_signal; /* solver width (0 = size rule)                        */
+} spx_train_dl_opts;
+
+/* Dictionary learning: replaces _trainDL / _alltrainDL (cpp_library/spams.h:
+ * 1153-1340, Trainer::train dictLearn/dicts.h:346-759).  X: m x n signals.
+ * D: m x K, the learned dictionary (out); read as the initial dictionary when
+ * D_given (else K signals drawn like the reference's srandom(0)/random()).
+ * A (K x K), B (m x K) and *itercount: the model (return_model); a model is
+ * given when *itercount > 0 (then A and B are read), and the three are
+ * written on return (A and B may be NULL when no model is wanted). */
+int spx_train_dl_f32(spx_ctx* ctx, const float* X, int m, int n, float* D,
+                     int K, int D_given, float* A, float* B, int* itercount,
+                     const spx_train_dl_opts* opts);
+int spx_train_dl_f64(spx_ctx* ctx, const double* X, int m, int n, double* D,
+                     int K, int D_given, double* A, double* B, int* itercount,
+                     const spx_train_dl_opts* opts);
+
 void spx_csc_free(spx_csc* csc);
 
 /* ----------------------------------------------------- device entry points */
@@ -472,6 +510,16 @@ int spx_dict_lipschitz(spx_ctx* ctx, const spx_dict* dict, int iters, double* L)
  * (device pointer). */
 int spx_dict_gram_f32(spx_ctx* ctx, const spx_dict* dict, float* G);
 int spx_dict_gram_f64(spx_ctx* ctx, const spx_dict* dict, double* G);
+
+/* Device dictionary learning: X (device, m x n, leading dimension ldx >= m),
+ * D (m x K), A (K x K), B (m x K) device buffers, all read/written as in
+ * spx_train_dl_* (A and B required).  Synchronous. */
+int spx_train_dl_dev_f32(spx_ctx* ctx, const float* X, int ldx, int m, int n,
+                         float* D, int K, int D_given, float* A, float* B,
+                         int* itercount, const spx_train_dl_opts* opts);
+int spx_train_dl_dev_f64(spx_ctx* ctx, const double* X, int ldx, int m, int n,
+                         double* D, int K, int D_given, double* A, double* B,
+                         int* itercount, const spx_train_dl_opts* opts);
 
 #ifdef __cplusplus
 }

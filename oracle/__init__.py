@@ -549,3 +549,63 @@ def ref_fista_flat(X, D, W0, lambda1, regul="l1", lambda2=0.0, max_it=1000,
     if rc != 0:
         raise RuntimeError("reference fistaFlat failed")
     return W, info
+
+
+# ------------------------------------------------------- dictionary learning
+
+# constraint_type of trainDL (decomp/decomp.h:95): the LARS modes above plus
+# the ORMP modes of Trainer::train (dictLearn/dicts.h:584-591)
+SPARSITY, L2ERROR2, PENALTY2 = 3, 4, 5
+
+
+def _train_dl_call(fn_name, libh, X, D, K, A, B, itercount, batchsize, lambda1,
+                   lambda2, iter, t0, mode, posAlpha, posD, clean, rho,
+                   iter_updateD, modeParam, batch, num_threads=None):
+    dtype = X.dtype
+    X = _f(X, dtype)
+    m, M = X.shape
+    D_given = D is not None
+    if D_given:
+        K = D.shape[1]
+    Dw = _f(D, dtype).copy(order="F") if D_given else np.zeros((m, K), dtype, order="F")
+    Aw = _f(A, dtype).copy(order="F") if A is not None else np.zeros((K, K), dtype, order="F")
+    Bw = _f(B, dtype).copy(order="F") if B is not None else np.zeros((m, K), dtype, order="F")
+    it = ctypes.c_int(int(itercount) if A is not None else 0)
+    ct = _ct(dtype)
+    f = getattr(libh, fn_name + "_" + _sfx(dtype))
+    f.restype = _I
+    args = [_ptr(X), m, M, _ptr(Dw), K, int(D_given), _ptr(Aw), _ptr(Bw), ctypes.byref(it)]
+    types = [_P, _I, _I, _P, _I, _I, _P, _P, ctypes.POINTER(_I)]
+    if num_threads is not None:
+        args.append(num_threads)
+        types.append(_I)
+    args += [batchsize, lambda1, lambda2, iter, t0, mode, int(posAlpha), int(posD),
+             int(clean), rho, iter_updateD, modeParam, int(batch)]
+    types += [_I, ct, ct, _I, ct, _I, _I, _I, _I, ct, _I, _I, _I]
+    f.argtypes = types
+    rc = f(*args)
+    if rc != 0:
+        raise RuntimeError(fn_name + " failed (unsupported options?)")
+    return Dw, Aw, Bw, it.value
+
+
+def train_dl(X, D=None, K=-1, A=None, B=None, itercount=0, batchsize=256,
+             lambda1=0.1, lambda2=1e-9, iter=10, t0=1e-5, mode=PENALTY,
+             posAlpha=False, posD=False, clean=True, rho=1.0, iter_updateD=1,
+             modeParam=0, batch=False):
+    """C restatement of trainDL (Trainer::train, dictLearn/dicts.h:346-759),
+    one thread.  Returns (D, A, B, itercount)."""
+    return _train_dl_call("orc_train_dl", lib(), X, D, K, A, B, itercount, batchsize,
+                          lambda1, lambda2, iter, t0, mode, posAlpha, posD, clean, rho,
+                          iter_updateD, modeParam, batch)
+
+
+def ref_train_dl(X, D=None, K=-1, A=None, B=None, itercount=0, batchsize=256,
+                 lambda1=0.1, lambda2=1e-9, iter=10, t0=1e-5, mode=PENALTY,
+                 posAlpha=False, posD=False, clean=True, rho=1.0, iter_updateD=1,
+                 modeParam=0, batch=False, num_threads=1):
+    """The reference _alltrainDL (cpp_library/spams.h:1153-1340), compiled in
+    place.  Returns (D, A, B, itercount)."""
+    return _train_dl_call("ref_train_dl", ref(), X, D, K, A, B, itercount, batchsize,
+                          lambda1, lambda2, iter, t0, mode, posAlpha, posD, clean, rho,
+                          iter_updateD, modeParam, batch, num_threads=num_threads)

@@ -14,6 +14,7 @@
 //   _lassoQq    cpp_library/spams.h:145-188
 //   _lassoMask / _lassoWeighted   cpp_library/spams.h:190-249
 //   _ompMask    cpp_library/spams.h:295-343
+//   _alltrainDL cpp_library/spams.h:1153-1340 (trainDL, dictLearn/dicts.h:346-759)
 // plus one per-signal-lambda LARS loop (ref_lasso_pervec_*) that mirrors the
 // OpenMP loop of lasso2 (decomp/decomp.h:1574-1646) with the constraint taken
 // per signal, calling the reference coreLARS2 (decomp/decomp.h:1706-1915).
@@ -272,6 +273,55 @@ int fista_impl(const T* X, int m, int n, const T* D, int p, const T* W0, T* W,
   return 0;
 }
 
+// _alltrainDL (cpp_library/spams.h:1153-1340) with the trainDL arguments of
+// spams.py (swig/python/spams.py:485-500): no structured regulariser, no log.
+// D: m x K in/out (initial dictionary when D_given); A, B, itercount: the
+// model in/out (given when itercount_io > 0, like m_A/m_B/m_iter).
+template <typename T>
+int train_dl_impl(const T* X, int m, int M, T* D, int K, int D_given, T* A, T* B,
+                  int* itercount_io, int num_threads, int batchsize, T lambda1,
+                  T lambda2, int iter, T t0, int mode, int posAlpha, int posD,
+                  int clean, T rho, int iter_updateD, int modeParam, int batch) {
+  Matrix<T> Xm(const_cast<T*>(X), m, M);
+  Matrix<T> D1;
+  if (D_given) D1.setData(D, m, K);
+  Matrix<T> mA, mB;
+  if (*itercount_io > 0) {
+    mA.setData(A, K, K);
+    mB.setData(B, m, K);
+  }
+  Vector<T> eta_g;
+  SpMatrix<bool> groups, groups_var;
+  Vector<int> own_variables, N_own_variables;
+  Matrix<T>* omA = nullptr;
+  Matrix<T>* omB = nullptr;
+  Vector<int>* omiter = nullptr;
+  char regul[] = "none";
+  char logname[] = "";
+  Matrix<T>* Dout = nullptr;
+  try {
+    Dout = _alltrainDL<T>(&Xm, false, &omA, &omB, &omiter, true, &mA, &mB,
+                          *itercount_io, &D1, &eta_g, &groups, &groups_var,
+                          &own_variables, &N_own_variables, num_threads, T(1e-6),
+                          true, false, batchsize, K, lambda1, lambda2, 0.0, iter, t0,
+                          static_cast<constraint_type>(mode), regul, posAlpha != 0,
+                          posD != 0, false, L2, false, clean != 0, false, 0.0, 0.0, rho,
+                          iter_updateD, false, modeParam, batch != 0, false, logname);
+  } catch (const char* msg) {
+    fprintf(stderr, "reference trainDL: %s\n", msg);
+    return -1;
+  }
+  std::memcpy(D, Dout->rawX(), sizeof(T) * (size_t)m * K);
+  std::memcpy(A, omA->rawX(), sizeof(T) * (size_t)K * K);
+  std::memcpy(B, omB->rawX(), sizeof(T) * (size_t)m * K);
+  *itercount_io = (*omiter)[0];
+  delete Dout;
+  delete omA;
+  delete omB;
+  delete omiter;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -324,6 +374,16 @@ extern "C" {
                            int nLambda, int num_threads, int return_path) {    \
     return omp_mask_impl<T>(X, m, n, D, p, B, L, nL, eps, nEps, lambda,        \
                             nLambda, num_threads, return_path);                \
+  }                                                                            \
+  int ref_train_dl_##SUF(const T* X, int m, int M, T* D, int K, int D_given,  \
+                         T* A, T* B, int* itercount_io, int num_threads,       \
+                         int batchsize, T lambda1, T lambda2, int iter, T t0,  \
+                         int mode, int posAlpha, int posD, int clean, T rho,   \
+                         int iter_updateD, int modeParam, int batch) {         \
+    return train_dl_impl<T>(X, m, M, D, K, D_given, A, B, itercount_io,        \
+                            num_threads, batchsize, lambda1, lambda2, iter,    \
+                            t0, mode, posAlpha, posD, clean, rho,              \
+                            iter_updateD, modeParam, batch);                   \
   }                                                                            \
   int ref_sp_nnz_##SUF(void* h) { return sp_nnz<T>(h); }                       \
   int ref_sp_n_##SUF(void* h) {                                                \

@@ -18,11 +18,15 @@
  *   orc_fista_flat         FISTA::solver etc.     prox/fista.h:2905-3007,
  *                                                 3530-3573, 3618-3752
  *   orc_convert            SpMatrix::convert      linalg/linalg.h:4840-4862
+ *   orc_train_dl           _trainDL + Trainer::train  cpp_library/spams.h:1153-1340,
+ *                                                 dictLearn/dicts.h:296-334, 346-759
  *
  * The f32/f64 twins are generated from spx_oracle_body.inc.
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <omp.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -46,10 +50,30 @@
   void P##symv_(char*, int*, T*, T*, int*, T*, int*, T*, T*, int*);            \
   void P##syr_(char*, int*, T*, T*, int*, T*, int*);                           \
   void P##trmv_(char*, char*, char*, int*, T*, int*, T*, int*);                \
+  T P##nrm2_(int*, T*, int*);                                                  \
   void P##syrk_(char*, char*, int*, int*, T*, T*, int*, T*, T*, int*);         \
   int i##P##amax_(int*, T*, int*);
 BLAS_DECL(s, float)
 BLAS_DECL(d, double)
+
+/* The reference draws from the C library's random() after srandom(0)
+ * (dictLearn/dicts.h:400, :459).  The same additive-feedback generator with a
+ * private 128-byte state (the default state size) gives the same stream. */
+typedef struct {
+  struct random_data d;
+  char state[128];
+} orc_rng;
+
+static void orc_rng_seed(orc_rng* g) {
+  memset(g, 0, sizeof *g);
+  initstate_r(0, g->state, sizeof g->state, &g->d);
+}
+
+static long orc_rng_next(orc_rng* g) {
+  int32_t r;
+  random_r(&g->d, &r);
+  return (long)r;
+}
 
 #define T float
 #define FN(x) x##_f32

@@ -1,0 +1,834 @@
+// Online dictionary learning (trainDL): Trainer::train, dictLearn/dicts.h:346-759,
+// behind the facade _alltrainDL (cpp_library/spams.h:1153-1340).
+//
+// One iteration on the device:
+//   G = D'D                      MFMA GEMM (Matrix::XtX, dicts.h:553)
+//   cleaning                     scan for the first offending atom pair, replace,
+//                                continue the scan (Trainer::cleanDict, :296-334)
+//   G += max(lambda2, 1e-10) I   (dicts.h:557)
+//   code a minibatch             gather columns perm[(j + i bs) mod M], D'X on MFMA,
+//                                the engine's LARS (coreLARS2) or ORMP (coreORMPB)
+//   A_t = sum a a', B_t = sum x a'   dense codes, two MFMA GEMMs (rank1Update, :601-602)
+//   A, B statistics              online scaling with the even/odd epoch buffers
+//                                (:668-715) or batch mode (:604-609)
+//   dictionary update            block-coordinate descent over the atoms (:717-741)
+//
+// The dictionary update is sequential over atoms: d_k depends on every d_c
+// updated before it.  It runs in blocks of `b` atoms: a panel kernel forms
+// P = D A(:, block) with the dictionary as it stands at the block's start
+// (many workgroups), then one workgroup walks the block's atoms in order,
+// adding the changes of the atoms it already updated in this block
+// (sum_c (d_c^new - d_c^old) A(c, k)), so that every d_k sees exactly
+// D A(:, k) of the reference's gemv.
+#include <sys/time.h>
+#include <unistd.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "spx.h"
+#include "spx_common.h"
+#include "spx_internal.h"
+
+namespace spx {
+
+namespace {
+
+constexpr int kDlMaxRowsPerThread = 8;  // block kernel: m <= 256 * 8
+constexpr int kDlBlockThreads = 256;
+constexpr int kDlPanelCols = 16;        // panel kernel: atoms per workgroup
+// panel kernel: waves splitting the inner sum (their partials fill 32 KiB of LDS)
+template <typename T>
+constexpr int dl_panel_waves() { return sizeof(T) == 4 ? 8 : 4; }
+constexpr int kDlChunk = 16384;         // signals coded per solver launch
+
+// The C library's additive-feedback generator (random_r) with a private
+// 128-byte state seeded like srandom(0): the stream the reference draws its
+// initial atoms, permutation and replacement signals from (dicts.h:400, :459).
+struct Rng {
+  random_data d;
+  char state[128];
+  Rng() { seed(); }
+  Rng(const Rng&) = delete;
+  Rng& operator=(const Rng&) = delete;
+  void seed() {
+    std::memset(&d, 0, sizeof d);
+    std::memset(state, 0, sizeof state);
+    initstate_r(0, state, sizeof state, &d);
+  }
+  long next() {
+    int32_t r;
+    random_r(&d, &r);
+    return (long)r;
+  }
+};
+
+// Makes ctx's device current for the lifetime of the guard.
+struct DlDevice {
+  int prev = -1;
+  explicit DlDevice(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DlDevice() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+double wall_seconds() {
+  timeval tv;
+  gettimeofday(&tv, nullptr);
+  return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Xb(:, j) = X(:, perm[(base + j) mod M]) for j < nb, and its transpose XbT
+// (ldt x m, rows nb..ldt-1 zero); 64 x 64 tiles through LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void dl_gather_kernel(const T* __restrict__ X, int ldx,
+                                                        int m, int M,
+                                                        const int* __restrict__ perm,
+                                                        long long base, int nb, T* Xb,
+                                                        T* XbT, int ldt) {
+  __shared__ T tile[64][65];
+  const int j0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int jj = ty; jj < 64; jj += 4) {
+    const int j = j0 + jj;
+    const int r = r0 + tx;
+    T v = 0;
+    if (j < nb && r < m) {
+      const int col = perm[(int)((base + j) % M)];
+      v = X[(size_t)col * ldx + r];
+      Xb[(size_t)j * m + r] = v;
+    }
+    tile[jj][tx] = v;
+  }
+  __syncthreads();
+  for (int rr = ty; rr < 64; rr += 4) {
+    const int r = r0 + rr;
+    const int j = j0 + tx;
+    if (r < m && j < ldt) XbT[(size_t)r * ldt + j] = tile[tx][rr];
+  }
+}
+
+// aT(j, r[t, j]) = v[t, j] for t < count[j] (aT zeroed beforehand).
+template <typename T>
+__global__ void dl_scatter_kernel(const T* __restrict__ v, const int* __restrict__ r,
+                                  const int* __restrict__ count, int L, int nb, T* aT,
+                                  int ldt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  const int c = count[j];
+  for (int t = 0; t < c; ++t) {
+    const int k = r[(size_t)j * L + t];
+    aT[(size_t)k * ldt + j] = v[(size_t)j * L + t];
+  }
+}
+
+// lower triangle <- upper triangle (K x K, leading dimension K): the batch
+// statistics A_t = sum a a' exactly symmetric, as the reference's sparse
+// rank-one updates make it.
+template <typename T>
+__global__ void dl_mirror_kernel(T* A, int K) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)K * K) return;
+  const int row = (int)(e % K), col = (int)(e / K);
+  if (row > col) A[e] = A[(size_t)row * K + col];
+}
+
+template <typename T>
+__global__ void dl_ridge_kernel(T* G, int ldg, int K, T diag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < K) G[(size_t)k * ldg + k] += diag;
+}
+
+// Online statistics (dicts.h:670-715), element-wise over n entries:
+//   swap:     odd <- even, even <- 0 (a new epoch parity)
+//   even, odd *= scal; orig *= scal when use_orig
+//   even += scal2 * bat;  S = (use_orig ? orig : 0) + odd + even
+template <typename T>
+__global__ void dl_combine_kernel(long long n, T* S, T* orig, T* odd, T* even,
+                                  const T* __restrict__ bat, int swap, T scal, int use_orig,
+                                  T scal2) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  T ev = even[e], od = odd[e];
+  if (swap) {
+    od = ev;
+    ev = 0;
+  }
+  ev = ev * scal;
+  od = od * scal;
+  T s = 0;
+  if (use_orig) {
+    const T o = orig[e] * scal;
+    orig[e] = o;
+    s = o;
+  }
+  ev = ev + scal2 * bat[e];
+  s = s + od;
+  s = s + ev;
+  even[e] = ev;
+  odd[e] = od;
+  S[e] = s;
+}
+
+// A = t0 I, B = t0 D (the fresh online model, dicts.h:434-441); zero both in
+// batch mode.
+template <typename T>
+__global__ void dl_init_model_kernel(T* A, T* B, const T* __restrict__ D, int m, int K,
+                                     T t0, int online) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nA = (long long)K * K, nB = (long long)m * K;
+  if (e < nA) A[e] = (online && e % (K + 1) == 0) ? t0 : T(0);
+  if (e < nB) B[e] = online ? t0 * D[e] : T(0);
+}
+
+// D(:, k) = X(:, idx[k]) (initial atoms, dicts.h:403-410).
+template <typename T>
+__global__ void dl_copy_cols_kernel(const T* __restrict__ X, int ldx, int m,
+                                    const int* __restrict__ idx, int K, T* D) {
+  const int k = blockIdx.y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < K && r < m) D[(size_t)k * m + r] = X[(size_t)idx[k] * ldx + r];
+}
+
+// Column-wise thrsPos (when pos) and normalisation, one wavefront per column:
+// scale to unit norm when the norm exceeds `thr` (Matrix::normalize with
+// thr = 1e-10, linalg/linalg.h:1297-1312; Vector::normalize with EPSILON =
+// 10e-10, :3155-3158).  cols: column indices (null = 0..ncols-1).
+template <typename T>
+__global__ __launch_bounds__(256) void dl_normalize_kernel(T* D, int m, int ncols,
+                                                           const int* __restrict__ cols,
+                                                           int pos, T thr) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= ncols) return;
+  T* d = D + (size_t)(cols ? cols[w] : w) * m;
+  T s = 0;
+  for (int r = lane; r < m; r += 64) {
+    T v = d[r];
+    if (pos && v < T(0)) {
+      v = 0;
+      d[r] = v;
+    }
+    s += v * v;
+  }
+  s = wave_sum(s);
+  const T nrm = sqrt(s);
+  if (nrm > thr) {
+    const T inv = T(1) / nrm;
+    for (int r = lane; r < m; r += 64) d[r] = d[r] * inv;
+  }
+}
+
+// Cleaning scan: the first pair (i, j >= i), in the reference's row-major
+// order and at linear position i K + j >= start, whose atoms are correlated
+// beyond 0.999999 (j > i) or whose atom has a squared norm below 1e-4
+// (j == i); atomicMin of its position into *out (INT_MAX when none).
+template <typename T>
+__global__ void dl_clean_find_kernel(const T* __restrict__ G, int ldg, int K, int start,
+                                     int* out) {
+  const long long total = (long long)K * K;
+  for (long long e = start + (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / K), j = (int)(e % K);
+    if (j < i) continue;
+    const T gij = G[(size_t)i * ldg + j];
+    bool hit;
+    if (j == i) {
+      hit = fabs(gij) < T(1e-4);
+    } else {
+      hit = fabs(gij) / sqrt(G[(size_t)i * ldg + i] * G[(size_t)j * ldg + j]) > T(0.999999);
+    }
+    if (hit) atomicMin(out, (int)e);
+  }
+}
+
+// Row and column j of G from the replaced atom d_j: G(k, j) = G(j, k) = d_k'd_j
+// (the gemv and the copy loop of dicts.h:326-329); one wavefront per k.
+template <typename T>
+__global__ __launch_bounds__(256) void dl_gram_col_kernel(const T* __restrict__ D, int m,
+                                                          int K, int j, T* G, int ldg) {
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= K) return;
+  const T* dk = D + (size_t)k * m;
+  const T* dj = D + (size_t)j * m;
+  T s = 0;
+  for (int r = lane; r < m; r += 64) s += dk[r] * dj[r];
+  s = wave_sum(s);
+  if (lane == 0) {
+    G[(size_t)j * ldg + k] = s;
+    G[(size_t)k * ldg + j] = s;
+  }
+}
+
+// Panel of the dictionary update: P(r, kk) = sum_c D(r, c) A(c, k0 + kk) for
+// kk < nk (<= kDlPanelCols), reading A's symmetric row k0 + kk (contiguous
+// in c).  Workgroup = 64 rows x kDlPanelCols atoms; its waves split c and
+// are summed through LDS.
+template <typename T>
+__global__ __launch_bounds__(64 * dl_panel_waves<T>()) void dl_panel_kernel(
+    const T* __restrict__ D, int m, int K, const T* __restrict__ A, int k0, int nk, T* P) {
+  constexpr int kDlPanelWaves = dl_panel_waves<T>();
+  __shared__ T part[kDlPanelWaves][kDlPanelCols][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  const int kb = k0 + blockIdx.y * kDlPanelCols;
+  int nc = k0 + nk - kb;
+  if (nc > kDlPanelCols) nc = kDlPanelCols;
+  T acc[kDlPanelCols];
+#pragma unroll
+  for (int q = 0; q < kDlPanelCols; ++q) acc[q] = 0;
+  const int per = (K + kDlPanelWaves - 1) / kDlPanelWaves;
+  const int c0 = w * per;
+  const int c1 = c0 + per < K ? c0 + per : K;
+  if (r < m) {
+    for (int c = c0; c < c1; ++c) {
+      const T d = D[(size_t)c * m + r];
+      const T* a = A + (size_t)c * K + kb;  // A(kb.., c) = A(c, kb..)
+#pragma unroll
+      for (int q = 0; q < kDlPanelCols; ++q)
+        if (q < nc) acc[q] += d * a[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kDlPanelCols; ++q) part[w][q][lane] = acc[q];
+  __syncthreads();
+  // waves fold the partial sums: wave w takes atoms q = w, w + waves, ...
+  for (int q = w; q < nc; q += kDlPanelWaves) {
+    T s = 0;
+#pragma unroll
+    for (int v = 0; v < kDlPanelWaves; ++v) s += part[v][q][lane];
+    if (r < m) P[(size_t)(kb - k0 + q) * m + r] = s;
+  }
+}
+
+// The atoms k0 .. k0 + nk - 1 of the dictionary update in order (one
+// workgroup; thread t owns rows t, t + 256, ...):
+//   t = P(:, kk) + sum_{c < kk} dD(:, c) A(k0 + c, k)
+//   A_kk > 1e-6:  d = (b_k - t) / A_kk + d_k, thrsPos when posD, then scaled to
+//                 norm 1 when its norm exceeds 1 (Vector::normalize2)
+//   otherwise:    d = 0 when zero_unused, else unchanged
+// dD(:, kk) = d - d_k^old is kept in LDS for the atoms after it.
+template <typename T>
+__global__ __launch_bounds__(kDlBlockThreads) void dl_block_kernel(
+    T* D, int m, int K, const T* __restrict__ A, const T* __restrict__ B, int k0, int nk,
+    const T* __restrict__ P, int posD, int zero_unused) {
+  extern __shared__ unsigned char dl_lds[];
+  T* dD = reinterpret_cast<T*>(dl_lds);         // nk x m (atom-major)
+  T* Ab = dD + (size_t)nk * m;                  // nk x nk block of A
+  T* red = Ab + (size_t)nk * nk;                // 2 x 4 wave partials
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int e = tid; e < nk * nk; e += kDlBlockThreads) {
+    const int c = e % nk, q = e / nk;
+    Ab[e] = A[(size_t)(k0 + q) * K + k0 + c];  // Ab[q * nk + c] = A(k0 + c, k0 + q)
+  }
+  __syncthreads();
+  const int R = (m + kDlBlockThreads - 1) / kDlBlockThreads;
+  int phase = 0;  // reduction slot parity, flipped on every step with a barrier
+  for (int kk = 0; kk < nk; ++kk) {
+    const int k = k0 + kk;
+    const T akk = Ab[kk * nk + kk];
+    T* dk = D + (size_t)k * m;
+    T nd[kDlMaxRowsPerThread];
+    if (akk > T(1e-6)) {
+      const T inv = T(1) / akk;
+      T s = 0;
+#pragma unroll
+      for (int q = 0; q < kDlMaxRowsPerThread; ++q) {
+        nd[q] = 0;
+        const int r = tid + q * kDlBlockThreads;
+        if (q < R && r < m) {
+          T t = P[(size_t)kk * m + r];
+          for (int c = 0; c < kk; ++c) t += dD[(size_t)c * m + r] * Ab[kk * nk + c];
+          T v = (B[(size_t)k * m + r] - t) * inv + dk[r];
+          if (posD && v < T(0)) v = 0;
+          nd[q] = v;
+          s += v * v;
+        }
+      }
+      s = wave_sum(s);
+      T* slot = red + phase * 4;
+      phase ^= 1;
+      if (lane == 0) slot[w] = s;
+      __syncthreads();
+      const T nrm = sqrt((slot[0] + slot[1]) + (slot[2] + slot[3]));
+      const T sc = nrm > T(1) ? T(1) / nrm : T(1);
+#pragma unroll
+      for (int q = 0; q < kDlMaxRowsPerThread; ++q) {
+        const int r = tid + q * kDlBlockThreads;
+        if (q < R && r < m) {
+          const T v = nrm > T(1) ? nd[q] * sc : nd[q];
+          dD[(size_t)kk * m + r] = v - dk[r];
+          dk[r] = v;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kDlMaxRowsPerThread; ++q) {
+        const int r = tid + q * kDlBlockThreads;
+        if (q < R && r < m) {
+          if (zero_unused) {
+            dD[(size_t)kk * m + r] = -dk[r];
+            dk[r] = 0;
+          } else {
+            dD[(size_t)kk * m + r] = 0;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host
+
+inline unsigned blocks_for(long long n, int threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+// Atoms per dictionary-update block: the block's changes (m x b) and its A
+// block (b x b) in at most 48 KiB of LDS, at most 64 atoms.
+template <typename T>
+int dl_block_atoms(int m) {
+  const size_t budget = 48 * 1024;
+  int b = 64;
+  while (b > 1 && ((size_t)b * m + (size_t)b * b + 8) * sizeof(T) > budget) --b;
+  return b;
+}
+
+template <typename T>
+int dl_update_dict(hipStream_t st, T* D, int m, int K, const T* A, const T* B, T* P, int b,
+                   int posD, int zero_unused) {
+  for (int k0 = 0; k0 < K; k0 += b) {
+    const int nk = K - k0 < b ? K - k0 : b;
+    dim3 pg((m + 63) / 64, (nk + kDlPanelCols - 1) / kDlPanelCols);
+    hipLaunchKernelGGL(dl_panel_kernel<T>, pg, dim3(64 * dl_panel_waves<T>()), 0, st,
+                       static_cast<const T*>(D), m, K, A, k0, nk, P);
+    SPX_HIP(hipGetLastError());
+    const size_t lds = ((size_t)nk * m + (size_t)nk * nk + 8) * sizeof(T);
+    hipLaunchKernelGGL(dl_block_kernel<T>, dim3(1), dim3(kDlBlockThreads), lds, st, D, m, K,
+                       A, B, k0, nk, static_cast<const T*>(P), posD, zero_unused);
+    SPX_HIP(hipGetLastError());
+  }
+  return SPX_OK;
+}
+
+struct DlMem {
+  std::vector<void*> ptrs;
+  ~DlMem() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename U>
+  int get(U** out, size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, (count ? count : 1) * sizeof(U)) != hipSuccess) {
+      set_error("trainDL: device allocation of %zu bytes failed", count * sizeof(U));
+      return SPX_ERR_NOMEM;
+    }
+    ptrs.push_back(p);
+    *out = static_cast<U*>(p);
+    return SPX_OK;
+  }
+};
+
+#define DL_TRY(expr)          \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_) return rc_;      \
+  } while (0)
+
+// The trainer on device buffers: X (m x M, leading dimension ldx), D (m x K),
+// A (K x K), B (m x K), all device memory; D, A, B and *itercount in/out.
+template <typename T>
+int train_dl_dev(spx_ctx* ctx, const T* X, int ldx, int m, int M, T* D, int K, int D_given,
+                 T* A, T* B, int* itercount, const spx_train_dl_opts* o) {
+  if (!ctx || !X || !D || !A || !B || !itercount || !o || m <= 0 || M <= 0 || K <= 0 ||
+      ldx < m) {
+    set_error("trainDL: invalid arguments (m=%d, n=%d, K=%d)", m, M, K);
+    return SPX_ERR_ARG;
+  }
+  const int mode = o->mode;
+  const T lambda1 = static_cast<T>(o->lambda1);
+  const T lambda2 = static_cast<T>(o->lambda2);
+  if (mode < 0 || mode > 5) {
+    set_error("trainDL: unsupported mode %d", mode);
+    return SPX_ERR_UNSUP;
+  }
+  if (mode == 2 && lambda1 == T(0) && lambda2 > T(0) && !o->posAlpha) {
+    set_error("trainDL: the ridge-only problem (lambda1 = 0, lambda2 > 0) is not supported");
+    return SPX_ERR_UNSUP;
+  }
+  if (o->modeParam < 0 || o->modeParam > 2 || o->iter_updateD < 0 || *itercount < 0) {
+    set_error("trainDL: invalid modeParam / iter_updateD / iteration count");
+    return SPX_ERR_ARG;
+  }
+  if (m > kDlBlockThreads * kDlMaxRowsPerThread) {
+    set_error("trainDL: signal length m=%d exceeds %d", m, kDlBlockThreads * kDlMaxRowsPerThread);
+    return SPX_ERR_UNSUP;
+  }
+  if ((long long)K * K > INT_MAX) {
+    set_error("trainDL: K=%d too large", K);
+    return SPX_ERR_UNSUP;
+  }
+  // L of Trainer::train (dicts.h:386-387); SPARSITY takes lambda1 atoms,
+  // at most min(m, K) (a further atom's residual norm is zero)
+  const int Lmk = m < K ? m : K;
+  int L = mode == 3 ? (int)o->lambda1 : Lmk;
+  if (L > Lmk) L = Lmk;
+  if (L <= 0) {
+    set_error("trainDL: SPARSITY mode needs lambda1 >= 1 atoms");
+    return SPX_ERR_ARG;
+  }
+  int threads = o->num_threads > 0 ? o->num_threads : (int)sysconf(_SC_NPROCESSORS_ONLN);
+  if (threads < 1) threads = 1;
+  const int batchsize = o->batchsize > 0 ? o->batchsize : 256 * (threads + 1);
+  const int bs = o->batch ? M : (batchsize < M ? batchsize : M);
+  const int itc = *itercount;
+  const bool online = !o->batch;
+  const int prec = o->precision & SPX_PREC_BF16X3;
+  DlDevice g(ctx->device);
+  hipStream_t st = ctx->stream;
+
+  Rng rng;
+  DlMem mem;
+  const size_t nA = (size_t)K * K, nB = (size_t)m * K;
+  const int ldg = pad_ld(K);
+  const int CH = bs < kDlChunk ? bs : kDlChunk;
+  const int ldt = (CH + 3) & ~3;
+  T *G, *Aorig, *Aodd, *Aeven, *Borig, *Bodd, *Beven, *Xb, *XbT, *aT, *DtX, *norms, *P, *lv,
+      *scal_dev;
+  T* Abuf[2] = {nullptr, nullptr};  // batch statistics, ping-pong over the chunks
+  T* Bbuf[2] = {nullptr, nullptr};
+  int *perm, *lr, *lc, *idx, *hit, *Ldev;
+  DL_TRY(mem.get(&G, (size_t)ldg * K));
+  DL_TRY(mem.get(&Aorig, nA));
+  DL_TRY(mem.get(&Aodd, nA));
+  DL_TRY(mem.get(&Aeven, nA));
+  DL_TRY(mem.get(&Abuf[0], nA));
+  DL_TRY(mem.get(&Borig, nB));
+  DL_TRY(mem.get(&Bodd, nB));
+  DL_TRY(mem.get(&Beven, nB));
+  DL_TRY(mem.get(&Bbuf[0], nB));
+  if (bs > CH) {
+    DL_TRY(mem.get(&Abuf[1], nA));
+    DL_TRY(mem.get(&Bbuf[1], nB));
+  }
+  DL_TRY(mem.get(&Xb, (size_t)m * CH));
+  DL_TRY(mem.get(&XbT, (size_t)ldt * m));
+  DL_TRY(mem.get(&aT, (size_t)ldt * K));
+  DL_TRY(mem.get(&DtX, (size_t)ldg * CH));
+  DL_TRY(mem.get(&norms, (size_t)CH));
+  DL_TRY(mem.get(&lv, (size_t)L * CH));
+  DL_TRY(mem.get(&lr, (size_t)L * CH));
+  DL_TRY(mem.get(&lc, (size_t)CH));
+  DL_TRY(mem.get(&perm, (size_t)M));
+  DL_TRY(mem.get(&idx, (size_t)K));
+  DL_TRY(mem.get(&hit, 1));
+  DL_TRY(mem.get(&scal_dev, 3));
+  DL_TRY(mem.get(&Ldev, 1));
+  const int bA = dl_block_atoms<T>(m);
+  DL_TRY(mem.get(&P, (size_t)m * bA));
+  SPX_HIP(hipMemsetAsync(G, 0, sizeof(T) * (size_t)ldg * K, st));
+  SPX_HIP(hipMemsetAsync(DtX, 0, sizeof(T) * (size_t)ldg * CH, st));
+  // solver scalars: lambda (LARS constraint), eps and lambda (ORMP), L; the
+  // host copies stay alive until the stream synchronises below
+  const T sc[3] = {lambda1, mode == 4 ? lambda1 : T(0), mode == 5 ? lambda1 : T(0)};
+  SPX_HIP(hipMemcpyAsync(scal_dev, sc, sizeof sc, hipMemcpyHostToDevice, st));
+  SPX_HIP(hipMemcpyAsync(Ldev, &L, sizeof(int), hipMemcpyHostToDevice, st));
+
+  // initial dictionary: K random signals (dicts.h:400-411)
+  if (!D_given) {
+    std::vector<int> h(K);
+    for (int k = 0; k < K; ++k) h[k] = (int)(rng.next() % M);
+    SPX_HIP(hipMemcpyAsync(idx, h.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(dl_copy_cols_kernel<T>, dim3((m + 255) / 256, K), dim3(256), 0, st, X,
+                       ldx, m, static_cast<const int*>(idx), K, D);
+    SPX_HIP(hipGetLastError());
+    SPX_HIP(hipStreamSynchronize(st));  // h leaves scope
+  }
+  // thrsPos when posD, then Matrix::normalize (dicts.h:424-427)
+  hipLaunchKernelGGL(dl_normalize_kernel<T>, dim3((K + 3) / 4), dim3(256), 0, st, D, m, K,
+                     static_cast<const int*>(nullptr), o->posD, T(1e-10));
+  SPX_HIP(hipGetLastError());
+  if (itc == 0) {
+    const long long ne = (long long)(nA > nB ? nA : nB);
+    hipLaunchKernelGGL(dl_init_model_kernel<T>, dim3(blocks_for(ne, 256)), dim3(256), 0, st, A,
+                       B, static_cast<const T*>(D), m, K, static_cast<T>(std::fabs(o->t0)),
+                       online ? 1 : 0);
+    SPX_HIP(hipGetLastError());
+  }
+  // srandom(0); perm.randperm(M) (dicts.h:459-461, linalg/linalg.h:2906-2918)
+  {
+    rng.seed();
+    std::vector<int> table(M), h(M);
+    for (int i = 0; i < M; ++i) table[i] = i;
+    int size = M;
+    for (int i = 0; i < M; ++i) {
+      const int pick = (int)(rng.next() % size);
+      h[i] = table[pick];
+      table[pick] = table[size - 1];
+      --size;
+    }
+    SPX_HIP(hipMemcpyAsync(perm, h.data(), sizeof(int) * M, hipMemcpyHostToDevice, st));
+    SPX_HIP(hipStreamSynchronize(st));
+  }
+  SPX_HIP(hipMemsetAsync(Aodd, 0, sizeof(T) * nA, st));
+  SPX_HIP(hipMemsetAsync(Aeven, 0, sizeof(T) * nA, st));
+  SPX_HIP(hipMemsetAsync(Bodd, 0, sizeof(T) * nB, st));
+  SPX_HIP(hipMemsetAsync(Beven, 0, sizeof(T) * nB, st));
+  SPX_HIP(hipMemcpyAsync(Aorig, A, sizeof(T) * nA, hipMemcpyDeviceToDevice, st));
+  SPX_HIP(hipMemcpyAsync(Borig, B, sizeof(T) * nB, hipMemcpyDeviceToDevice, st));
+
+  // solver set-up over the trainer's Gram (ridge stored in G, so no shift)
+  spx_dict dict;  // view for the width rule; owns nothing
+  dict.m = m;
+  dict.p = K;
+  dict.ldg = ldg;
+  dict.elem = sizeof(T);
+  dict.G = G;
+  const int waves = o->waves_per_signal > 0 ? o->waves_per_signal : tuned_waves(&dict, mode < 3 ? 0 : 1);
+
+  const long long JJ = o->iter < 0 ? 100000000LL : o->iter;
+  const double t_start = wall_seconds();
+  const T ridge = static_cast<T>(o->lambda2 > 1e-10 ? o->lambda2 : 1e-10);
+  bool even = true;
+  int count = 0, countPrev = 0;
+  long long i;
+  for (i = 0; i < JJ; ++i) {
+    if (o->iter < 0) {
+      SPX_HIP(hipStreamSynchronize(st));
+      if (wall_seconds() - t_start > -(double)o->iter) break;
+    }
+    // G = D'D
+    DL_TRY(gemm_tn<T>(st, K, K, m, D, m, D, m, G, ldg, T(1), T(0), nullptr, 0, prec));
+    if (o->clean) {
+      int start = 0;
+      for (;;) {
+        const int none = INT_MAX;
+        SPX_HIP(hipMemcpyAsync(hit, &none, sizeof(int), hipMemcpyHostToDevice, st));
+        const long long rem = (long long)K * K - start;
+        unsigned nbk = blocks_for(rem, 256);
+        if (nbk > 4096) nbk = 4096;
+        hipLaunchKernelGGL(dl_clean_find_kernel<T>, dim3(nbk), dim3(256), 0, st,
+                           static_cast<const T*>(G), ldg, K, start, hit);
+        SPX_HIP(hipGetLastError());
+        int h = INT_MAX;
+        SPX_HIP(hipMemcpyAsync(&h, hit, sizeof(int), hipMemcpyDeviceToHost, st));
+        SPX_HIP(hipStreamSynchronize(st));
+        if (h == INT_MAX) break;
+        const int j = h % K;
+        const int ind = (int)(rng.next() % M);
+        SPX_HIP(hipMemcpyAsync(idx, &ind, sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(dl_copy_cols_kernel<T>, dim3((m + 255) / 256, 1), dim3(256), 0, st,
+                           X, ldx, m, static_cast<const int*>(idx), 1, D + (size_t)j * m);
+        SPX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(dl_normalize_kernel<T>, dim3(1), dim3(256), 0, st, D + (size_t)j * m,
+                           m, 1, static_cast<const int*>(nullptr), o->posD, T(10e-10));
+        SPX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(dl_gram_col_kernel<T>, dim3((K + 3) / 4), dim3(256), 0, st,
+                           static_cast<const T*>(D), m, K, j, G, ldg);
+        SPX_HIP(hipGetLastError());
+        SPX_HIP(hipStreamSynchronize(st));  // ind leaves scope
+        start = h + 1;
+        if (start >= K * K) break;
+      }
+    }
+    hipLaunchKernelGGL(dl_ridge_kernel<T>, dim3((K + 255) / 256), dim3(256), 0, st, G, ldg, K,
+                       ridge);
+    SPX_HIP(hipGetLastError());
+    // code the minibatch in chunks; A_t, B_t accumulate over the chunks
+    int cur = 0;
+    for (int c0 = 0; c0 < bs; c0 += CH) {
+      const int nb = bs - c0 < CH ? bs - c0 : CH;
+      const int ld = (nb + 3) & ~3;
+      const long long base = i * (long long)bs + c0;
+      hipLaunchKernelGGL(dl_gather_kernel<T>, dim3((ld + 63) / 64, (m + 63) / 64), dim3(256), 0,
+                         st, X, ldx, m, M, static_cast<const int*>(perm), base, nb, Xb, XbT,
+                         ld);
+      SPX_HIP(hipGetLastError());
+      DL_TRY(gemm_tn<T>(st, K, nb, m, D, m, Xb, m, DtX, ldg, T(1), T(0), nullptr, 0, prec));
+      DL_TRY(col_sqnorms<T>(st, m, nb, Xb, m, norms));
+      if (mode < 3) {
+        LarsArgs<T> a{};
+        a.tol = T(1e-15);
+        a.G = G;
+        a.ldg = ldg;
+        a.diag = T(0);
+        a.DtX = DtX;
+        a.ldd = ldg;
+        a.norms = norms;
+        a.lambda = scal_dev;
+        a.nlam = 1;
+        a.p = K;
+        a.n = nb;
+        a.L = L;
+        a.mode = mode;
+        a.pos = o->posAlpha ? 1 : 0;
+        a.length_path = 4 * L;  // coreLARS2's default cap (decomp/decomp.h:1724)
+        a.outV = lv;
+        a.outR = lr;
+        a.outCount = lc;
+        DL_TRY(lars_solve<T>(ctx, a, SPX_FORM_BATCHED, waves));
+      } else {
+        OmpArgs<T> a{};
+        a.G = G;
+        a.ldg = ldg;
+        a.DtX = DtX;
+        a.ldd = ldg;
+        a.norms = norms;
+        a.Ls = Ldev;
+        a.nL = 1;
+        a.eps = scal_dev + 1;
+        a.nEps = 1;
+        a.lambda = scal_dev + 2;
+        a.nLambda = 1;
+        a.scalarL = Lmk;
+        a.p = K;
+        a.n = nb;
+        a.L = L;
+        a.outV = lv;
+        a.outR = lr;
+        a.outCount = lc;
+        DL_TRY(omp_solve<T>(ctx, a, SPX_FORM_BATCHED, waves));
+      }
+      SPX_HIP(hipMemsetAsync(aT, 0, sizeof(T) * (size_t)ld * K, st));
+      hipLaunchKernelGGL(dl_scatter_kernel<T>, dim3((nb + 255) / 256), dim3(256), 0, st,
+                         static_cast<const T*>(lv), static_cast<const int*>(lr),
+                         static_cast<const int*>(lc), L, nb, aT, ld);
+      SPX_HIP(hipGetLastError());
+      const T beta = c0 ? T(1) : T(0);
+      const int nxt = c0 ? 1 - cur : 0;
+      DL_TRY(gemm_tn<T>(st, K, K, ld, aT, ld, aT, ld, Abuf[nxt], K, T(1), beta,
+                        c0 ? Abuf[cur] : nullptr, K, prec));
+      DL_TRY(gemm_tn<T>(st, m, K, ld, XbT, ld, aT, ld, Bbuf[nxt], m, T(1), beta,
+                        c0 ? Bbuf[cur] : nullptr, m, prec));
+      cur = nxt;
+    }
+    T* Abat = Abuf[cur];
+    T* Bbat = Bbuf[cur];
+    hipLaunchKernelGGL(dl_mirror_kernel<T>, dim3(blocks_for((long long)nA, 256)), dim3(256), 0,
+                       st, Abat, K);
+    SPX_HIP(hipGetLastError());
+    int zero_unused;
+    if (!online) {
+      SPX_HIP(hipMemcpyAsync(A, Abat, sizeof(T) * nA, hipMemcpyDeviceToDevice, st));
+      SPX_HIP(hipMemcpyAsync(B, Bbat, sizeof(T) * nB, hipMemcpyDeviceToDevice, st));
+      zero_unused = o->clean ? 1 : 0;
+    } else {
+      const int epoch = (int)((((i + 1) % M) * (long long)bs) / M);
+      int swap = 0;
+      if ((even && (epoch % 2) == 1) || (!even && (epoch % 2) == 0)) {
+        swap = 1;
+        count = countPrev;
+        countPrev = 0;
+        even = !even;
+      }
+      const long long ii = itc + i;
+      long long num_elem = ii < bs ? ii * bs : (long long)bs * bs + ii - bs;
+      if (num_elem > 2LL * M) num_elem = 2LL * M;
+      const T scal2 = T(T(1.0) / bs);
+      const int totaliter = itc + count;
+      T scal;
+      if (o->modeParam == 2) {
+        scal = static_cast<T>(o->rho);
+      } else if (o->modeParam == 1) {
+        const T pw = static_cast<T>(
+            std::pow((double)(T(totaliter) / T(totaliter + 1)), -(double)static_cast<T>(o->rho)));
+        scal = static_cast<T>(0.95 > (double)pw ? 0.95 : (double)pw);
+      } else {
+        scal = T(itc + num_elem + 1 - bs) / T(itc + num_elem + 1);
+      }
+      const int use_orig = (itc > 0 && i * bs < M) || (itc == 0 && o->t0 != 0 && i * bs < 10000);
+      hipLaunchKernelGGL(dl_combine_kernel<T>, dim3(blocks_for((long long)nA, 256)), dim3(256), 0,
+                         st, (long long)nA, A, Aorig, Aodd, Aeven, static_cast<const T*>(Abat),
+                         swap, scal, use_orig, scal2);
+      SPX_HIP(hipGetLastError());
+      hipLaunchKernelGGL(dl_combine_kernel<T>, dim3(blocks_for((long long)nB, 256)), dim3(256), 0,
+                         st, (long long)nB, B, Borig, Bodd, Beven, static_cast<const T*>(Bbat),
+                         swap, scal, use_orig, scal2);
+      SPX_HIP(hipGetLastError());
+      ++count;
+      ++countPrev;
+      zero_unused = (o->clean && (itc + i) * (long long)bs > 10000) ? 1 : 0;
+    }
+    for (int s = 0; s < o->iter_updateD; ++s)
+      DL_TRY(dl_update_dict<T>(st, D, m, K, A, B, P, bA, o->posD ? 1 : 0, zero_unused));
+  }
+  SPX_HIP(hipStreamSynchronize(st));
+  *itercount = (int)(itc + i);
+  return SPX_OK;
+}
+
+// Host buffers: copies in, trains on the device, copies D (and A, B) out.
+template <typename T>
+int train_dl_host(spx_ctx* ctx, const T* X, int m, int M, T* D, int K, int D_given, T* A,
+                  T* B, int* itercount, const spx_train_dl_opts* o) {
+  if (!ctx || !X || !D || !itercount || !o || m <= 0 || M <= 0 || K <= 0) {
+    set_error("trainDL: invalid arguments (m=%d, n=%d, K=%d)", m, M, K);
+    return SPX_ERR_ARG;
+  }
+  if (*itercount > 0 && (!A || !B)) {
+    set_error("trainDL: a model (itercount > 0) needs A and B");
+    return SPX_ERR_ARG;
+  }
+  DlDevice g(ctx->device);
+  hipStream_t st = ctx->stream;
+  DlMem mem;
+  T *Xd, *Dd, *Ad, *Bd;
+  const size_t nA = (size_t)K * K, nB = (size_t)m * K;
+  DL_TRY(mem.get(&Xd, (size_t)m * M));
+  DL_TRY(mem.get(&Dd, nB));
+  DL_TRY(mem.get(&Ad, nA));
+  DL_TRY(mem.get(&Bd, nB));
+  SPX_HIP(hipMemcpyAsync(Xd, X, sizeof(T) * (size_t)m * M, hipMemcpyHostToDevice, st));
+  if (D_given) SPX_HIP(hipMemcpyAsync(Dd, D, sizeof(T) * nB, hipMemcpyHostToDevice, st));
+  if (*itercount > 0) {
+    SPX_HIP(hipMemcpyAsync(Ad, A, sizeof(T) * nA, hipMemcpyHostToDevice, st));
+    SPX_HIP(hipMemcpyAsync(Bd, B, sizeof(T) * nB, hipMemcpyHostToDevice, st));
+  }
+  DL_TRY(train_dl_dev<T>(ctx, Xd, m, m, M, Dd, K, D_given, Ad, Bd, itercount, o));
+  SPX_HIP(hipMemcpyAsync(D, Dd, sizeof(T) * nB, hipMemcpyDeviceToHost, st));
+  if (A) SPX_HIP(hipMemcpyAsync(A, Ad, sizeof(T) * nA, hipMemcpyDeviceToHost, st));
+  if (B) SPX_HIP(hipMemcpyAsync(B, Bd, sizeof(T) * nB, hipMemcpyDeviceToHost, st));
+  SPX_HIP(hipStreamSynchronize(st));
+  return SPX_OK;
+}
+
+}  // namespace
+
+}  // namespace spx
+
+extern "C" {
+
+int spx_train_dl_f32(spx_ctx* ctx, const float* X, int m, int n, float* D, int K, int D_given,
+                     float* A, float* B, int* itercount, const spx_train_dl_opts* opts) {
+  return spx::train_dl_host<float>(ctx, X, m, n, D, K, D_given, A, B, itercount, opts);
+}
+
+int spx_train_dl_f64(spx_ctx* ctx, const double* X, int m, int n, double* D, int K,
+                     int D_given, double* A, double* B, int* itercount,
+                     const spx_train_dl_opts* opts) {
+  return spx::train_dl_host<double>(ctx, X, m, n, D, K, D_given, A, B, itercount, opts);
+}
+
+int spx_train_dl_dev_f32(spx_ctx* ctx, const float* X, int ldx, int m, int n, float* D, int K,
+                         int D_given, float* A, float* B, int* itercount,
+                         const spx_train_dl_opts* opts) {
+  return spx::train_dl_dev<float>(ctx, X, ldx, m, n, D, K, D_given, A, B, itercount, opts);
+}
+
+int spx_train_dl_dev_f64(spx_ctx* ctx, const double* X, int ldx, int m, int n, double* D,
+                         int K, int D_given, double* A, double* B, int* itercount,
+                         const spx_train_dl_opts* opts) {
+  return spx::train_dl_dev<double>(ctx, X, ldx, m, n, D, K, D_given, A, B, itercount, opts);
+}
+
+}  // extern "C"

@@ -22,7 +22,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspx.so")
 
 # constraint_type (decomp/decomp.h:95)
-L1COEFFS, L2ERROR, PENALTY = 0, 1, 2
+L1COEFFS, L2ERROR, PENALTY, SPARSITY, L2ERROR2, PENALTY2 = 0, 1, 2, 3, 4, 5
+# constraint_type_D (dictLearn/dicts.h:39)
+L2 = 0
 PREC = {"fp32": 0, "bf16x3": 1, "fp32+refit64": 2, "bf16x3+refit64": 3}
 FORM = {"batched": 0, "plain": 1}
 SOLVERS = {"lasso": 0, "omp": 1}
@@ -54,6 +56,14 @@ class spx_fista_opts(ctypes.Structure):
                 ("lambda2", ctypes.c_double), ("tol", ctypes.c_double),
                 ("it0", _I), ("max_iter_backtracking", _I), ("regul", _I),
                 ("pos", _I), ("ista", _I), ("precision", _I), ("graph", _I)]
+
+
+class spx_train_dl_opts(ctypes.Structure):
+    _fields_ = [("batchsize", _I), ("num_threads", _I), ("lambda1", ctypes.c_double),
+                ("lambda2", ctypes.c_double), ("iter", _I), ("t0", ctypes.c_double),
+                ("mode", _I), ("posAlpha", _I), ("posD", _I), ("clean", _I),
+                ("rho", ctypes.c_double), ("iter_updateD", _I), ("modeParam", _I),
+                ("batch", _I), ("precision", _I), ("waves_per_signal", _I)]
 
 
 class spx_lists_dev(ctypes.Structure):
@@ -545,6 +555,95 @@ def fistaFlat(Y, X, W0, return_optim_info=False, numThreads=-1, max_it=1000,
     if return_optim_info:
         return W, info
     return W
+
+
+def trainDL(X, return_model=False, model=None, D=None, numThreads=-1, batchsize=-1,
+            K=-1, lambda1=None, lambda2=10e-10, iter=-1, t0=1e-5, mode=PENALTY,
+            posAlpha=False, posD=False, expand=False, modeD=L2, whiten=False,
+            clean=True, verbose=True, gamma1=0., gamma2=0., rho=1.0,
+            iter_updateD=None, stochastic_deprecated=False, modeParam=0,
+            batch=False, log_deprecated=False, logName='', precision="fp32",
+            waves_per_signal=0, ctx=None):
+    """Dictionary learning (spams.trainDL, swig/python/spams.py:485-500 and
+    :412-484) for modeD=L2 without whitening.  Returns D (m x K, Fortran
+    order), or (D, model) with model = {'A', 'B', 'iter'} when return_model.
+    A model is used (warm restart) when D is given too, as in
+    cpp_library/spams.h:1201-1210.  `iter` < 0 trains for -iter seconds."""
+    if lambda1 is None:
+        raise ValueError("trainDL : lambda1 must be defined")
+    if modeD != L2 or whiten or expand or stochastic_deprecated or log_deprecated:
+        raise NotImplementedError("trainDL: option not provided by spx (modeD=L2, no "
+                                  "whitening / expand / stochastic / log)")
+    if iter_updateD is None:
+        iter_updateD = 5 if batch else 1
+    dtype = np.asarray(X).dtype
+    sfx = _sfx(dtype)
+    Xf = np.asfortranarray(X, dtype=dtype)
+    m, n = Xf.shape
+    D_given = D is not None and np.asarray(D).size > 0
+    if D_given:
+        Dw = np.array(D, dtype=dtype, order="F", copy=True)
+        if Dw.shape[0] != m:
+            raise RuntimeError("trainDL : sizes of D are not consistent\n")
+        K = Dw.shape[1]
+    else:
+        if K < 0:
+            raise RuntimeError("trainDL : bad parameter K\n")
+        Dw = np.zeros((m, K), dtype=dtype, order="F")
+    it = ctypes.c_int(0)
+    A = np.zeros((K, K), dtype=dtype, order="F")
+    B = np.zeros((m, K), dtype=dtype, order="F")
+    if model is not None and D_given:
+        A = np.array(model['A'], dtype=dtype, order="F", copy=True)
+        B = np.array(model['B'], dtype=dtype, order="F", copy=True)
+        it = ctypes.c_int(int(model['iter']))
+    o = spx_train_dl_opts(int(batchsize), int(numThreads), float(lambda1), float(lambda2),
+                          int(iter), float(t0), int(mode), int(posAlpha), int(posD),
+                          int(clean), float(rho), int(iter_updateD), int(modeParam),
+                          int(batch), PREC[precision], int(waves_per_signal))
+    ctx = ctx or default_context()
+    f = getattr(lib(), "spx_train_dl_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _P, _I, _I, _P, _P, ctypes.POINTER(_I),
+                  ctypes.POINTER(spx_train_dl_opts)]
+    _check(f(ctx.h, _ptr(Xf), m, n, _ptr(Dw), K, int(D_given), _ptr(A), _ptr(B),
+             ctypes.byref(it), ctypes.byref(o)))
+    if return_model:
+        return Dw, {'A': A, 'B': B, 'iter': it.value}
+    return Dw
+
+
+def train_dl_dev(ctx, X, D, A, B, itercount=0, D_given=True, **opts):
+    """Device dictionary learning (spx_train_dl_dev_*) on torch tensors:
+    X (m x n), D (m x K), A (K x K), B (m x K), column-major as the
+    transposed views of row-major tensors (pass X.T.contiguous().T etc. or
+    tensors created as (n, m) and viewed transposed).  Returns the new
+    iteration count; D, A, B are updated in place."""
+    dtype = np.float32 if X.element_size() == 4 else np.float64
+    sfx = _sfx(dtype)
+    m, n = X.shape
+    K = D.shape[1]
+    for name, t, shape, ld in (("D", D, (m, K), m), ("A", A, (K, K), K), ("B", B, (m, K), m)):
+        if tuple(t.shape) != shape or t.stride(0) != 1 or t.stride(1) != ld or t.dtype != X.dtype:
+            raise ValueError("train_dl_dev: %s must be a column-major %dx%d tensor of X's dtype"
+                             % (name, shape[0], shape[1]))
+    if X.stride(0) != 1 or X.stride(1) < m:
+        raise ValueError("train_dl_dev: X must be column-major (a transposed row-major tensor)")
+    o = spx_train_dl_opts(opts.get("batchsize", -1), opts.get("numThreads", -1),
+                          float(opts["lambda1"]), float(opts.get("lambda2", 10e-10)),
+                          int(opts.get("iter", 1)), float(opts.get("t0", 1e-5)),
+                          int(opts.get("mode", PENALTY)), int(opts.get("posAlpha", False)),
+                          int(opts.get("posD", False)), int(opts.get("clean", True)),
+                          float(opts.get("rho", 1.0)), int(opts.get("iter_updateD", 1)),
+                          int(opts.get("modeParam", 0)), int(opts.get("batch", False)),
+                          PREC[opts.get("precision", "fp32")],
+                          int(opts.get("waves_per_signal", 0)))
+    it = ctypes.c_int(int(itercount))
+    f = getattr(lib(), "spx_train_dl_dev_" + sfx)
+    f.argtypes = [_P, _P, _I, _I, _I, _P, _I, _I, _P, _P, ctypes.POINTER(_I),
+                  ctypes.POINTER(spx_train_dl_opts)]
+    _check(f(ctx.h, _tptr(X), X.stride(1), m, n, _tptr(D), K, int(D_given), _tptr(A),
+             _tptr(B), ctypes.byref(it), ctypes.byref(o)))
+    return it.value
 
 
 # ------------------------------------------------------------- device API

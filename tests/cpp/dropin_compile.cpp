@@ -59,3 +59,25 @@ void use_fista() {
                                     false, false, logn, false, &iw, 1, true, false, 0);
   delete info;
 }
+
+void use_train_dl() {
+  Matrix<float> X(64, 1000), D0;
+  Matrix<float> mA, mB;
+  Matrix<float>* A = nullptr;
+  Matrix<float>* B = nullptr;
+  Vector<int>* it = nullptr;
+  Vector<float> eta;
+  SpMatrix<bool> g, gv;
+  Vector<int> own, nown;
+  char regul[] = "none";
+  char logname[] = "";
+  Matrix<float>* D = _alltrainDL<float>(&X, false, &A, &B, &it, true, &mA, &mB, 0, &D0, &eta, &g,
+                                        &gv, &own, &nown, -1, 1e-6f, true, false, 512, 128, 0.15,
+                                        1e-9, 0., 10, 1e-5, PENALTY, regul, false, false, false,
+                                        L2, false, true, false, 0., 0., 1.0f, 1, false, 0, false,
+                                        false, logname);
+  delete D;
+  delete A;
+  delete B;
+  delete it;
+}

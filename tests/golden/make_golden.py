@@ -156,6 +156,41 @@ def main():
                 eps=np.asarray(np.nan if eps is None else eps, dt),
                 lam=np.asarray(np.nan if lam is None else lam, dt),
                 pB=pB, r=r, v=v)))
+    # dictionary learning (trainDL): the reference trainer with one thread
+    for dt, tag in ((np.float32, "f32"), (np.float64, "f64")):
+        X, _ = small(31, 16, 32, 400, 3, 0.05, True, dt)
+        # an initial dictionary with duplicated (and opposite) atoms: cleaning
+        # replaces atoms 7, 20 and 25 in the first iteration
+        Dc = W.unit_dict(np.random.default_rng(34), 16, 32, dt)
+        Dc[:, 7] = Dc[:, 3]
+        Dc[:, 20] = Dc[:, 12]
+        Dc[:, 25] = -Dc[:, 3]
+        base = dict(batchsize=64, lambda1=0.15, lambda2=1e-9, iter=20, t0=1e-5, mode=2,
+                    posAlpha=0, posD=0, clean=1, rho=1.0, iter_updateD=1, modeParam=0, batch=0)
+        for name, Xi, over in (("penalty", X, {}),
+                               ("l2error", X, dict(mode=1, lambda1=0.3)),
+                               ("sparsity", X, dict(mode=3, lambda1=5)),
+                               ("l2error2", X, dict(mode=4, lambda1=0.1)),
+                               ("pos", X, dict(posAlpha=1, posD=1, lambda1=0.1)),
+                               ("batch", X, dict(batch=1, iter=4, iter_updateD=5)),
+                               ("param2", X, dict(modeParam=2, rho=0.9)),
+                               ("clean", X, dict(iter=30))):
+            prm = dict(base, **over)
+            Dinit = Dc if name == "clean" else None
+            Dd, Ad, Bd, it = oracle.ref_train_dl(Xi, D=Dinit, K=32, **prm)
+            extra = {} if Dinit is None else dict(Dinit=Dinit)
+            cases.append((f"traindl_{tag}_{name}", dict(
+                kind="traindl", X=Xi, K=32, D=Dd, A=Ad, B=Bd, itercount=it, **extra,
+                **{k: np.asarray(v) for k, v in prm.items()})))
+        # warm restart: 7 iterations from a given D, then 5 more from the model
+        D0 = W.unit_dict(np.random.default_rng(33), 16, 32, dt)
+        prm = dict(base, iter=7)
+        D1, A1, B1, it1 = oracle.ref_train_dl(X, D=D0, **prm)
+        prm2 = dict(base, iter=5)
+        D2, A2, B2, it2 = oracle.ref_train_dl(X, D=D1, A=A1, B=B1, itercount=it1, **prm2)
+        cases.append((f"traindl_{tag}_restart", dict(
+            kind="traindl", X=X, K=32, D0=D0, D1=D1, A1=A1, B1=B1, it1=it1,
+            D=D2, A=A2, B=B2, itercount=it2, **{k: np.asarray(v) for k, v in prm2.items()})))
     only_new = "--all" not in sys.argv
     wrote = 0
     for name, d in cases:
