@@ -566,7 +566,7 @@ Matrix<T>* _alltrainDL(Data<T>* X, bool in_memory, Matrix<T>** omA, Matrix<T>** 
   }
   spx_train_dl_opts o{batch_size, num_threads, lambda1, lambda2, iter, t0, (int)mode,
                       posAlpha ? 1 : 0, posD ? 1 : 0, clean ? 1 : 0, (double)rho,
-                      iter_updateD, modeParam, batch ? 1 : 0, SPX_PREC_FP32, 0};
+                      iter_updateD, modeParam, batch ? 1 : 0, SPX_PREC_FP32, 0, 0};
   const int rc = spams_amd_detail::train_dl_call(X->rawX(), n, M, D->rawX(), K, D_given ? 1 : 0,
                                                  A->rawX(), B->rawX(), &it, &o);
   if (rc != SPX_OK) {

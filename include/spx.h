@@ -356,6 +356,8 @@ typedef struct {
   int batch;        /* batch mode: every iteration codes all n signals         */
   int precision;    /* SPX_PREC_BF16X3: f32 products on BF16x3 MFMA            */
   int waves_per_signal; /* solver width (0 = size rule)                        */
+  int eager_update; /* 1: launch the dictionary-update kernels one by one
+                     * instead of replaying them as a captured hipGraph        */
 } spx_train_dl_opts;
 
 /* Dictionary learning: replaces _trainDL / _alltrainDL (cpp_library/spams.h:

@@ -39,10 +39,11 @@ namespace {
 
 constexpr int kDlMaxRowsPerThread = 8;  // block kernel: m <= 256 * 8
 constexpr int kDlBlockThreads = 256;
-constexpr int kDlPanelCols = 16;        // panel kernel: atoms per workgroup
-// panel kernel: waves splitting the inner sum (their partials fill 32 KiB of LDS)
+constexpr int kDlMaxSplit = 16;         // panel kernel: at most this many inner chunks
+// panel kernel: inner atoms per LDS slice (the D and A slices fill 33 KiB)
 template <typename T>
-constexpr int dl_panel_waves() { return sizeof(T) == 4 ? 8 : 4; }
+constexpr int dl_panel_inner() { return sizeof(T) == 4 ? 64 : 32; }
+constexpr int kDlPanelInner = 64;       // chunk granularity (a multiple of both slices)
 constexpr int kDlChunk = 16384;         // signals coded per solver launch
 
 // The C library's additive-feedback generator (random_r) with a private
@@ -86,6 +87,54 @@ double wall_seconds() {
 }
 
 // ---------------------------------------------------------------- kernels
+
+// d0[e] = s0[e], d1[e] = s1[e], d2[e] = s2[e] for e < n: the three copies'
+// loads issued together, U per array and thread, before their LDS stores.
+template <typename T, int U, int NT>
+__device__ __forceinline__ void dl_fill3(T* d0, const T* __restrict__ s0, T* d1,
+                                         const T* __restrict__ s1, T* d2,
+                                         const T* __restrict__ s2, int n) {
+  for (int base = threadIdx.x; base < n; base += U * NT) {
+    T v0[U], v1[U], v2[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = base + u * NT;
+      const bool in = e < n;
+      v0[u] = in ? s0[e] : T(0);
+      v1[u] = in ? s1[e] : T(0);
+      v2[u] = in ? s2[e] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = base + u * NT;
+      if (e < n) {
+        d0[e] = v0[u];
+        d1[e] = v1[u];
+        d2[e] = v2[u];
+      }
+    }
+  }
+}
+
+// dst[e] = src[idx(e)] for e < n, threads striding by NT, with U loads in
+// flight per thread before their LDS stores (a load-store loop with a
+// runtime trip count would wait for every load in turn).
+template <typename T, int U, int NT, typename Idx>
+__device__ __forceinline__ void dl_fill(T* dst, const T* __restrict__ src, int n, Idx idx) {
+  for (int base = threadIdx.x; base < n; base += U * NT) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = base + u * NT;
+      v[u] = e < n ? src[idx(e)] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = base + u * NT;
+      if (e < n) dst[e] = v[u];
+    }
+  }
+}
 
 // Xb(:, j) = X(:, perm[(base + j) mod M]) for j < nb, and its transpose XbT
 // (ldt x m, rows nb..ldt-1 zero); 64 x 64 tiles through LDS.
@@ -270,45 +319,82 @@ __global__ __launch_bounds__(256) void dl_gram_col_kernel(const T* __restrict__ 
   }
 }
 
-// Panel of the dictionary update: P(r, kk) = sum_c D(r, c) A(c, k0 + kk) for
-// kk < nk (<= kDlPanelCols), reading A's symmetric row k0 + kk (contiguous
-// in c).  Workgroup = 64 rows x kDlPanelCols atoms; its waves split c and
-// are summed through LDS.
+// Partial panels of the dictionary update: Pp(s, kk, r) = sum over the
+// inner chunk s (atoms s kc .. s kc + kc - 1) of D(r, c) A(c, k0 + kk), for
+// kk < nk (<= 64).  Workgroup = 64 rows x one chunk; its 4 waves take the
+// atoms kk = w, w + 4, ...; the slices of D (64 rows) and of A (columns
+// k0 .. k0 + nk - 1) go through LDS, IN inner atoms at a time.
 template <typename T>
-__global__ __launch_bounds__(64 * dl_panel_waves<T>()) void dl_panel_kernel(
-    const T* __restrict__ D, int m, int K, const T* __restrict__ A, int k0, int nk, T* P) {
-  constexpr int kDlPanelWaves = dl_panel_waves<T>();
-  __shared__ T part[kDlPanelWaves][kDlPanelCols][64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int r = blockIdx.x * 64 + lane;
-  const int kb = k0 + blockIdx.y * kDlPanelCols;
-  int nc = k0 + nk - kb;
-  if (nc > kDlPanelCols) nc = kDlPanelCols;
-  T acc[kDlPanelCols];
+__global__ __launch_bounds__(256) void dl_panel_kernel(const T* __restrict__ D, int m, int K,
+                                                       const T* __restrict__ A, int k0, int nk,
+                                                       int kc, T* __restrict__ Pp) {
+  constexpr int IN = dl_panel_inner<T>();
+  __shared__ T As[64][IN + 1];
+  __shared__ T Ds[IN][64 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r0 = blockIdx.x * 64;
+  const int r = r0 + lane;
+  const int s = blockIdx.y;
+  const int cb = s * kc;
+  const int ce = cb + kc < K ? cb + kc : K;
+  T acc[16];
 #pragma unroll
-  for (int q = 0; q < kDlPanelCols; ++q) acc[q] = 0;
-  const int per = (K + kDlPanelWaves - 1) / kDlPanelWaves;
-  const int c0 = w * per;
-  const int c1 = c0 + per < K ? c0 + per : K;
-  if (r < m) {
-    for (int c = c0; c < c1; ++c) {
-      const T d = D[(size_t)c * m + r];
-      const T* a = A + (size_t)c * K + kb;  // A(kb.., c) = A(c, kb..)
+  for (int j = 0; j < 16; ++j) acc[j] = 0;
+  for (int c0 = cb; c0 < ce; c0 += IN) {
+    const int nc = ce - c0 < IN ? ce - c0 : IN;
+    __syncthreads();  // the previous slices are consumed
+    {
+      T va[64 * IN / 256], vd[IN * 64 / 256];
 #pragma unroll
-      for (int q = 0; q < kDlPanelCols; ++q)
-        if (q < nc) acc[q] += d * a[q];
+      for (int u = 0; u < 64 * IN / 256; ++u) {
+        const int e = tid + u * 256;
+        const int q = e / IN, c = e % IN;
+        va[u] = (q < nk && c < nc) ? A[(size_t)(k0 + q) * K + c0 + c] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < IN * 64 / 256; ++u) {
+        const int e = tid + u * 256;
+        const int c = e / 64, rr = e % 64;
+        vd[u] = (c < nc && r0 + rr < m) ? D[(size_t)(c0 + c) * m + r0 + rr] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 64 * IN / 256; ++u) {
+        const int e = tid + u * 256;
+        As[e / IN][e % IN] = va[u];
+      }
+#pragma unroll
+      for (int u = 0; u < IN * 64 / 256; ++u) {
+        const int e = tid + u * 256;
+        Ds[e / 64][e % 64] = vd[u];
+      }
+    }
+    __syncthreads();
+    // rows q >= nk of As are zero: no guard, so the LDS reads batch
+#pragma unroll 4
+    for (int c = 0; c < nc; ++c) {
+      const T d = Ds[c][lane];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc[j] += d * As[w + 4 * j][c];
     }
   }
+  if (r < m) {
 #pragma unroll
-  for (int q = 0; q < kDlPanelCols; ++q) part[w][q][lane] = acc[q];
-  __syncthreads();
-  // waves fold the partial sums: wave w takes atoms q = w, w + waves, ...
-  for (int q = w; q < nc; q += kDlPanelWaves) {
-    T s = 0;
-#pragma unroll
-    for (int v = 0; v < kDlPanelWaves; ++v) s += part[v][q][lane];
-    if (r < m) P[(size_t)(kb - k0 + q) * m + r] = s;
+    for (int j = 0; j < 16; ++j) {
+      const int q = w + 4 * j;
+      if (q < nk) Pp[((size_t)s * nk + q) * m + r] = acc[j];
+    }
   }
+}
+
+// P(r, kk) = sum of the S partial panels, in chunk order.
+template <typename T>
+__global__ __launch_bounds__(256) void dl_panel_sum_kernel(const T* __restrict__ Pp, int S,
+                                                           int n, T* __restrict__ P) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  T t = 0;
+  for (int s = 0; s < S; ++s) t += Pp[(size_t)s * n + e];
+  P[e] = t;
 }
 
 // The atoms k0 .. k0 + nk - 1 of the dictionary update in order (one
@@ -317,39 +403,58 @@ __global__ __launch_bounds__(64 * dl_panel_waves<T>()) void dl_panel_kernel(
 //   A_kk > 1e-6:  d = (b_k - t) / A_kk + d_k, thrsPos when posD, then scaled to
 //                 norm 1 when its norm exceeds 1 (Vector::normalize2)
 //   otherwise:    d = 0 when zero_unused, else unchanged
-// dD(:, kk) = d - d_k^old is kept in LDS for the atoms after it.
+// The block's P, B and D columns are staged in LDS first; dD(:, kk) holds
+// P(:, kk), then d - d_k^old for the atoms after it; the new columns go back
+// to D at the end, so the steps touch LDS only.
 template <typename T>
 __global__ __launch_bounds__(kDlBlockThreads) void dl_block_kernel(
     T* D, int m, int K, const T* __restrict__ A, const T* __restrict__ B, int k0, int nk,
     const T* __restrict__ P, int posD, int zero_unused) {
   extern __shared__ unsigned char dl_lds[];
-  T* dD = reinterpret_cast<T*>(dl_lds);         // nk x m (atom-major)
-  T* Ab = dD + (size_t)nk * m;                  // nk x nk block of A
-  T* red = Ab + (size_t)nk * nk;                // 2 x 4 wave partials
+  const int nm = nk * m;
+  T* dD = reinterpret_cast<T*>(dl_lds);  // nk x m (atom-major)
+  T* Bs = dD + nm;                       // B(:, k0 ..) nk x m
+  T* Db = Bs + nm;                       // D(:, k0 ..) nk x m, updated in place
+  T* Ab = Db + nm;                       // nk x nk block of A
+  T* red = Ab + (size_t)nk * nk;         // 2 x 4 wave partials
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  for (int e = tid; e < nk * nk; e += kDlBlockThreads) {
-    const int c = e % nk, q = e / nk;
-    Ab[e] = A[(size_t)(k0 + q) * K + k0 + c];  // Ab[q * nk + c] = A(k0 + c, k0 + q)
-  }
+  const T* Bb = B + (size_t)k0 * m;
+  T* Dg = D + (size_t)k0 * m;
+  dl_fill3<T, 16, kDlBlockThreads>(dD, P, Bs, Bb, Db, Dg, nm);
+  // Ab[q * nk + c] = A(k0 + c, k0 + q)
+  dl_fill<T, 16, kDlBlockThreads>(Ab, A, nk * nk, [=](int e) {
+    return (size_t)(k0 + e / nk) * K + k0 + e % nk;
+  });
   __syncthreads();
   const int R = (m + kDlBlockThreads - 1) / kDlBlockThreads;
   int phase = 0;  // reduction slot parity, flipped on every step with a barrier
   for (int kk = 0; kk < nk; ++kk) {
-    const int k = k0 + kk;
     const T akk = Ab[kk * nk + kk];
-    T* dk = D + (size_t)k * m;
-    T nd[kDlMaxRowsPerThread];
+    T* dcol = dD + (size_t)kk * m;
+    T* Dk = Db + (size_t)kk * m;
+    const T* bk = Bs + (size_t)kk * m;
     if (akk > T(1e-6)) {
       const T inv = T(1) / akk;
+      T nd[kDlMaxRowsPerThread];
       T s = 0;
 #pragma unroll
       for (int q = 0; q < kDlMaxRowsPerThread; ++q) {
         nd[q] = 0;
         const int r = tid + q * kDlBlockThreads;
         if (q < R && r < m) {
-          T t = P[(size_t)kk * m + r];
-          for (int c = 0; c < kk; ++c) t += dD[(size_t)c * m + r] * Ab[kk * nk + c];
-          T v = (B[(size_t)k * m + r] - t) * inv + dk[r];
+          // four partial sums, so that the LDS reads of the chain overlap
+          const T* ab = Ab + kk * nk;
+          T t0 = dcol[r], t1 = 0, t2 = 0, t3 = 0;
+          int c = 0;
+          for (; c + 4 <= kk; c += 4) {
+            t0 += dD[(size_t)c * m + r] * ab[c];
+            t1 += dD[(size_t)(c + 1) * m + r] * ab[c + 1];
+            t2 += dD[(size_t)(c + 2) * m + r] * ab[c + 2];
+            t3 += dD[(size_t)(c + 3) * m + r] * ab[c + 3];
+          }
+          for (; c < kk; ++c) t0 += dD[(size_t)c * m + r] * ab[c];
+          const T t = (t0 + t1) + (t2 + t3);
+          T v = (bk[r] - t) * inv + Dk[r];
           if (posD && v < T(0)) v = 0;
           nd[q] = v;
           s += v * v;
@@ -367,8 +472,8 @@ __global__ __launch_bounds__(kDlBlockThreads) void dl_block_kernel(
         const int r = tid + q * kDlBlockThreads;
         if (q < R && r < m) {
           const T v = nrm > T(1) ? nd[q] * sc : nd[q];
-          dD[(size_t)kk * m + r] = v - dk[r];
-          dk[r] = v;
+          dcol[r] = v - Dk[r];
+          Dk[r] = v;
         }
       }
     } else {
@@ -377,15 +482,17 @@ __global__ __launch_bounds__(kDlBlockThreads) void dl_block_kernel(
         const int r = tid + q * kDlBlockThreads;
         if (q < R && r < m) {
           if (zero_unused) {
-            dD[(size_t)kk * m + r] = -dk[r];
-            dk[r] = 0;
+            dcol[r] = -Dk[r];
+            Dk[r] = 0;
           } else {
-            dD[(size_t)kk * m + r] = 0;
+            dcol[r] = 0;
           }
         }
       }
     }
   }
+  __syncthreads();
+  for (int e = tid; e < nm; e += kDlBlockThreads) Dg[e] = Db[e];
 }
 
 // ------------------------------------------------------------------ host
@@ -394,32 +501,62 @@ inline unsigned blocks_for(long long n, int threads) {
   return (unsigned)((n + threads - 1) / threads);
 }
 
-// Atoms per dictionary-update block: the block's changes (m x b) and its A
-// block (b x b) in at most 48 KiB of LDS, at most 64 atoms.
+constexpr size_t kDlBlockLds = 150 * 1024;  // of the 160 KiB of a gfx950 CU
+
+size_t dl_block_lds(int nk, int m, size_t elem) {
+  return (3 * (size_t)nk * m + (size_t)nk * nk + 8) * elem;
+}
+
+// Atoms per dictionary-update block: the block's P, B and D columns (3 of
+// m x b) and its A block (b x b) in kDlBlockLds of LDS, at most 64 atoms.
 template <typename T>
 int dl_block_atoms(int m) {
-  const size_t budget = 48 * 1024;
   int b = 64;
-  while (b > 1 && ((size_t)b * m + (size_t)b * b + 8) * sizeof(T) > budget) --b;
+  while (b > 1 && dl_block_lds(b, m, sizeof(T)) > kDlBlockLds) --b;
   return b;
 }
 
+// Inner chunk of the partial panels: a multiple of kDlPanelInner giving at
+// most kDlMaxSplit chunks.
+inline int dl_panel_chunk(int K) {
+  int kc = (K + kDlMaxSplit - 1) / kDlMaxSplit;
+  kc = (kc + kDlPanelInner - 1) / kDlPanelInner * kDlPanelInner;
+  return kc;
+}
+
+// One sweep of the dictionary update.  Pp: S x b x m partial panels, then
+// (from offset S b m) the summed panel b x m.
 template <typename T>
-int dl_update_dict(hipStream_t st, T* D, int m, int K, const T* A, const T* B, T* P, int b,
+int dl_update_dict(hipStream_t st, T* D, int m, int K, const T* A, const T* B, T* Pp, int b,
                    int posD, int zero_unused) {
+  const int kc = dl_panel_chunk(K);
+  const int S = (K + kc - 1) / kc;
+  T* P = Pp + (size_t)S * b * m;
   for (int k0 = 0; k0 < K; k0 += b) {
     const int nk = K - k0 < b ? K - k0 : b;
-    dim3 pg((m + 63) / 64, (nk + kDlPanelCols - 1) / kDlPanelCols);
-    hipLaunchKernelGGL(dl_panel_kernel<T>, pg, dim3(64 * dl_panel_waves<T>()), 0, st,
-                       static_cast<const T*>(D), m, K, A, k0, nk, P);
+    hipLaunchKernelGGL(dl_panel_kernel<T>, dim3((m + 63) / 64, S), dim3(256), 0, st,
+                       static_cast<const T*>(D), m, K, A, k0, nk, kc, Pp);
     SPX_HIP(hipGetLastError());
-    const size_t lds = ((size_t)nk * m + (size_t)nk * nk + 8) * sizeof(T);
+    const int np = nk * m;
+    hipLaunchKernelGGL(dl_panel_sum_kernel<T>, dim3((np + 255) / 256), dim3(256), 0, st,
+                       static_cast<const T*>(Pp), S, np, P);
+    SPX_HIP(hipGetLastError());
+    const size_t lds = dl_block_lds(nk, m, sizeof(T));
     hipLaunchKernelGGL(dl_block_kernel<T>, dim3(1), dim3(kDlBlockThreads), lds, st, D, m, K,
                        A, B, k0, nk, static_cast<const T*>(P), posD, zero_unused);
     SPX_HIP(hipGetLastError());
   }
   return SPX_OK;
 }
+
+// The instantiated update sweeps of one trainer call (per zero_unused).
+struct DlGraphs {
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  ~DlGraphs() {
+    for (hipGraphExec_t e : exec)
+      if (e) (void)hipGraphExecDestroy(e);
+  }
+};
 
 struct DlMem {
   std::vector<void*> ptrs;
@@ -535,7 +672,9 @@ int train_dl_dev(spx_ctx* ctx, const T* X, int ldx, int m, int M, T* D, int K, i
   DL_TRY(mem.get(&scal_dev, 3));
   DL_TRY(mem.get(&Ldev, 1));
   const int bA = dl_block_atoms<T>(m);
-  DL_TRY(mem.get(&P, (size_t)m * bA));
+  SPX_HIP(hipFuncSetAttribute((const void*)dl_block_kernel<T>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDlBlockLds));
+  DL_TRY(mem.get(&P, (size_t)m * bA * ((K + dl_panel_chunk(K) - 1) / dl_panel_chunk(K) + 1)));
   SPX_HIP(hipMemsetAsync(G, 0, sizeof(T) * (size_t)ldg * K, st));
   SPX_HIP(hipMemsetAsync(DtX, 0, sizeof(T) * (size_t)ldg * CH, st));
   // solver scalars: lambda (LARS constraint), eps and lambda (ORMP), L; the
@@ -595,6 +734,44 @@ int train_dl_dev(spx_ctx* ctx, const T* X, int ldx, int m, int M, T* D, int K, i
   dict.elem = sizeof(T);
   dict.G = G;
   const int waves = o->waves_per_signal > 0 ? o->waves_per_signal : tuned_waves(&dict, mode < 3 ? 0 : 1);
+
+  // iter_updateD sweeps of the dictionary update, captured once per
+  // zero_unused flag as a hipGraph and replayed (or launched one by one
+  // when eager_update)
+  DlGraphs graphs;
+  auto update = [&](int zu) -> int {
+    if (o->iter_updateD <= 0) return SPX_OK;
+    if (o->eager_update) {
+      for (int s = 0; s < o->iter_updateD; ++s)
+        DL_TRY(dl_update_dict<T>(st, D, m, K, A, B, P, bA, o->posD ? 1 : 0, zu));
+      return SPX_OK;
+    }
+    if (!graphs.exec[zu]) {
+      if (!ctx->capture_stream)
+        SPX_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+      hipStream_t cs = ctx->capture_stream;
+      SPX_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      int rc = SPX_OK;
+      for (int s = 0; s < o->iter_updateD && !rc; ++s)
+        rc = dl_update_dict<T>(cs, D, m, K, A, B, P, bA, o->posD ? 1 : 0, zu);
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(cs, &graph);
+      if (rc || ec != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!rc) set_error("trainDL: graph capture failed: %s", hipGetErrorString(ec));
+        return rc ? rc : SPX_ERR_HIP;
+      }
+      const hipError_t ei = hipGraphInstantiate(&graphs.exec[zu], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        graphs.exec[zu] = nullptr;
+        set_error("trainDL: graph instantiation failed: %s", hipGetErrorString(ei));
+        return SPX_ERR_HIP;
+      }
+    }
+    SPX_HIP(hipGraphLaunch(graphs.exec[zu], st));
+    return SPX_OK;
+  };
 
   const long long JJ = o->iter < 0 ? 100000000LL : o->iter;
   const double t_start = wall_seconds();
@@ -759,8 +936,7 @@ int train_dl_dev(spx_ctx* ctx, const T* X, int ldx, int m, int M, T* D, int K, i
       ++countPrev;
       zero_unused = (o->clean && (itc + i) * (long long)bs > 10000) ? 1 : 0;
     }
-    for (int s = 0; s < o->iter_updateD; ++s)
-      DL_TRY(dl_update_dict<T>(st, D, m, K, A, B, P, bA, o->posD ? 1 : 0, zero_unused));
+    DL_TRY(update(zero_unused));
   }
   SPX_HIP(hipStreamSynchronize(st));
   *itercount = (int)(itc + i);

@@ -63,7 +63,8 @@ class spx_train_dl_opts(ctypes.Structure):
                 ("lambda2", ctypes.c_double), ("iter", _I), ("t0", ctypes.c_double),
                 ("mode", _I), ("posAlpha", _I), ("posD", _I), ("clean", _I),
                 ("rho", ctypes.c_double), ("iter_updateD", _I), ("modeParam", _I),
-                ("batch", _I), ("precision", _I), ("waves_per_signal", _I)]
+                ("batch", _I), ("precision", _I), ("waves_per_signal", _I),
+                ("eager_update", _I)]
 
 
 class spx_lists_dev(ctypes.Structure):
@@ -563,7 +564,7 @@ def trainDL(X, return_model=False, model=None, D=None, numThreads=-1, batchsize=
             clean=True, verbose=True, gamma1=0., gamma2=0., rho=1.0,
             iter_updateD=None, stochastic_deprecated=False, modeParam=0,
             batch=False, log_deprecated=False, logName='', precision="fp32",
-            waves_per_signal=0, ctx=None):
+            waves_per_signal=0, eager_update=False, ctx=None):
     """Dictionary learning (spams.trainDL, swig/python/spams.py:485-500 and
     :412-484) for modeD=L2 without whitening.  Returns D (m x K, Fortran
     order), or (D, model) with model = {'A', 'B', 'iter'} when return_model.
@@ -600,7 +601,8 @@ def trainDL(X, return_model=False, model=None, D=None, numThreads=-1, batchsize=
     o = spx_train_dl_opts(int(batchsize), int(numThreads), float(lambda1), float(lambda2),
                           int(iter), float(t0), int(mode), int(posAlpha), int(posD),
                           int(clean), float(rho), int(iter_updateD), int(modeParam),
-                          int(batch), PREC[precision], int(waves_per_signal))
+                          int(batch), PREC[precision], int(waves_per_signal),
+                          int(eager_update))
     ctx = ctx or default_context()
     f = getattr(lib(), "spx_train_dl_" + sfx)
     f.argtypes = [_P, _P, _I, _I, _P, _I, _I, _P, _P, ctypes.POINTER(_I),
@@ -636,7 +638,8 @@ def train_dl_dev(ctx, X, D, A, B, itercount=0, D_given=True, **opts):
                           float(opts.get("rho", 1.0)), int(opts.get("iter_updateD", 1)),
                           int(opts.get("modeParam", 0)), int(opts.get("batch", False)),
                           PREC[opts.get("precision", "fp32")],
-                          int(opts.get("waves_per_signal", 0)))
+                          int(opts.get("waves_per_signal", 0)),
+                          int(opts.get("eager_update", False)))
     it = ctypes.c_int(int(itercount))
     f = getattr(lib(), "spx_train_dl_dev_" + sfx)
     f.argtypes = [_P, _P, _I, _I, _I, _P, _I, _I, _P, _P, ctypes.POINTER(_I),

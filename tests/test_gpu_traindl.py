@@ -138,3 +138,19 @@ def test_train_dl_time_limit_and_errors(spx):
     # the ridge-only problem (conjugate-gradient branch of the reference) is refused
     with pytest.raises(RuntimeError):
         spx.trainDL(X, K=32, lambda1=0.0, lambda2=0.1, iter=2)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_train_dl_graph_replays_equal_eager_launches(spx, dt):
+    """The dictionary-update sweeps replayed as a captured hipGraph (many
+    replays in one call, with the solver's eager work between them) give
+    the eager launches' dictionary bit for bit, in batch mode (5 sweeps per
+    replay) and online."""
+    rng = np.random.default_rng(54)
+    D0 = W.unit_dict(rng, 64, 200, dt)
+    X = W.sparse_signals(rng, D0, 1500, 5, 0.05, True, dt)
+    for kw in (dict(batchsize=256, iter=12), dict(batch=True, iter=3, iter_updateD=5)):
+        Dg, mg = spx.trainDL(X, D=D0, lambda1=0.12, return_model=True, **kw)
+        De, me = spx.trainDL(X, D=D0, lambda1=0.12, return_model=True, eager_update=True, **kw)
+        assert np.array_equal(Dg, De)
+        assert np.array_equal(mg["A"], me["A"]) and np.array_equal(mg["B"], me["B"])
