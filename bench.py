@@ -12,7 +12,7 @@ HIP events on the engine's stream; roofline figures use each kernel's
 algorithmic work.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-    python bench.py --workload omp|fista|served|variants  (other configurations)
+    python bench.py --workload omp|fista|served|variants|traindl  (other configurations)
 """
 from __future__ import annotations
 
@@ -527,6 +527,78 @@ def run_variants(args):
             "data": "synthetic", "higher_is_better": True, "variants": out}
 
 
+# ------------------------------------------------------- dictionary learning
+
+def run_traindl(args):
+    """Online dictionary learning (trainDL) at the headline decomposition
+    size: D 256x1024 from a given unit-norm N(0,1) start, signals as in the
+    LARS headline (n = 100000 resident in HBM), lambda 0.15, minibatches of
+    1024, FP32 (and BF16x3 products).  One step = one trainer iteration:
+    G = D'D, cleaning scan, minibatch D'X + LARS, statistics, dictionary
+    update.  The per-iteration cost is the difference of two device calls
+    (a long and a short run from the same start), so set-up is excluded."""
+    import workloads as W
+    E = Engine()
+    spx, torch = E.spx, E.torch
+    n = args.n if args.n != 200000 else 100000
+    X, _, _ = W.lars_headline(n=n)
+    m = X.shape[0]
+    K, bs, lam = 1024, 1024, 0.15
+    D0 = W.unit_dict(np.random.default_rng(1004), m, K, np.float32)
+    Xt = E.to_dev(X.T).T
+
+    def dev_run(iters, prec):
+        Dt = E.to_dev(D0.T).T  # column-major views of row-major tensors
+        At = E.to_dev(np.zeros((K, K), np.float32)).T
+        Bt = E.to_dev(np.zeros((K, m), np.float32)).T
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        spx.train_dl_dev(E.ctx, Xt, Dt, At, Bt, itercount=0, D_given=True, lambda1=lam,
+                         batchsize=bs, iter=iters, precision=prec)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    out = {"metric": "signals/sec of online dictionary learning (trainDL) m256 K1024",
+           "unit": "signals/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic", "scaling": "weak",
+           "config": {"workload": "trainDL online, D0 256x1024 unit-norm N(0,1), X 256x%d "
+                                  "headline signals, lambda 0.15, batchsize 1024, clean, "
+                                  "iter_updateD 1" % n, "m": m, "K": K, "n": n,
+                                  "batchsize": bs}}
+    short, long_ = 10, 10 + 10 * max(args.steps, 1)
+    variants = {}
+    for prec in ("fp32", "bf16x3"):
+        log("bench: trainDL %s" % prec)
+        dev_run(short, prec)  # warm-up (code objects, graph capture path)
+        ts = dev_run(short, prec)
+        tl = dev_run(long_, prec)
+        per_it = (tl - ts) / (long_ - short)
+        variants[prec] = {"ms_per_iteration": per_it * 1e3, "signals_per_s": bs / per_it,
+                          "call_s": {str(short): ts, str(long_): tl}}
+    out["variants"] = variants
+    out["value"] = variants["fp32"]["signals_per_s"]
+    out["ms_per_step"] = variants["fp32"]["ms_per_iteration"]
+    out["steps"] = long_ - short
+    if not args.no_cpu:
+        import oracle
+        th = cpu_threads()
+        log("bench: trainDL CPU baseline")
+        Xs = np.asfortranarray(X[:, :20000])
+        t = time.perf_counter()
+        oracle.ref_train_dl(Xs, D=D0, lambda1=lam, batchsize=bs, iter=1, num_threads=th)
+        t1 = time.perf_counter() - t
+        t = time.perf_counter()
+        oracle.ref_train_dl(Xs, D=D0, lambda1=lam, batchsize=bs, iter=4, num_threads=th)
+        t4 = time.perf_counter() - t
+        per = (t4 - t1) / 3
+        out["cpu_baseline"] = {"value": bs / per, "unit": "signals/s", "cores": th,
+                               "kind": "reference", "ms_per_iteration": per * 1e3,
+                               "sample": "the reference trainer built in place (_alltrainDL), "
+                                         "iterations 2-4 of the same workload on 20000 of "
+                                         "its signals, %d OpenMP threads" % th}
+    return out
+
+
 # ---------------------------------------------------------------- FISTA
 
 def run_fista(args):
@@ -612,7 +684,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=200000)
     ap.add_argument("--workload", default="lars",
-                    choices=["lars", "omp", "fista", "served", "variants"])
+                    choices=["lars", "omp", "fista", "served", "variants", "traindl"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -636,6 +708,8 @@ def main():
         out = run_fista(args)
     elif args.workload == "variants":
         out = run_variants(args)
+    elif args.workload == "traindl":
+        out = run_traindl(args)
     else:
         out = {"metric": "served sparse coding p50/p99 latency", "served": served_report(n_jobs=300)}
     print(json.dumps(out), flush=True)

@@ -154,3 +154,19 @@ def test_train_dl_graph_replays_equal_eager_launches(spx, dt):
         De, me = spx.trainDL(X, D=D0, lambda1=0.12, return_model=True, eager_update=True, **kw)
         assert np.array_equal(Dg, De)
         assert np.array_equal(mg["A"], me["A"]) and np.array_equal(mg["B"], me["B"])
+
+
+def test_train_dl_long_signals_fp64_matches_oracle(spx):
+    """m = 520: each thread of the dictionary update's block walk owns three
+    rows (every other test has one or two); same bar as the FP64 fixtures."""
+    import oracle
+    rng = np.random.default_rng(55)
+    Dtrue = W.unit_dict(rng, 520, 48, np.float64)
+    X = W.sparse_signals(rng, Dtrue, 400, 4, 0.05, True, np.float64)
+    D0 = W.unit_dict(np.random.default_rng(56), 520, 48, np.float64)
+    kw = dict(lambda1=0.12, batchsize=64, iter=6)
+    D, mdl = spx.trainDL(X, D=D0, return_model=True, **kw)
+    Do, Ao, Bo, ito = oracle.train_dl(X, D=D0, **kw)
+    assert mdl["iter"] == ito == 6
+    assert max(np.abs(D - Do).max(), np.abs(mdl["A"] - Ao).max(),
+               np.abs(mdl["B"] - Bo).max()) < 1e-10
