@@ -76,6 +76,19 @@ def measured_traffic(names):
     return float(sum(d[k] for k in names))
 
 
+def valu_issue():
+    """VALU issue fraction of the two LARS passes (VALU instructions x 4
+    cycles / (SIMDs x launch cycles)) from the committed rocprofv3 SQ counter
+    summary of the headline workload; null when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "lars_valu_counters.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    return {k: round(v["valu_issue_frac"], 3) for k, v in d.items()}
+
+
 def lars_kernel_names(p, w, elem=4):
     """Template names of the two LARS launches at group width w: the LDS pass
     (capacity 24 in FP32, 16 in FP64) and the full-capacity pass (width >= 4);
@@ -305,6 +318,8 @@ def run_lars(args):
                "sample": f"first {ns} signals of the same workload, {dt:.1f} s, per-signal-lambda "
                          f"lasso2/coreLARS2 loop of the reference built in place, {th} OpenMP threads"}
 
+    a64 = admissions.astype(np.float64)
+    restream_bytes = float(4.0 * p * np.sum(a64 * (a64 + 1) / 2))
     return {
         "metric": METRIC,
         "value": value,
@@ -330,7 +345,16 @@ def run_lars(args):
                      "frac": solver_bytes / (kern_ms * 1e-3) / HBM_PEAK,
                      "traffic": measured_traffic(lars_kernel_names(p, waves_used)),
                      "algorithmic_bytes": solver_bytes,
-                     "launch_ms": {k: passes[k] for k in ("lars_fast", "lars_full") if k in passes}},
+                     "launch_ms": {k: passes[k] for k in ("lars_fast", "lars_full") if k in passes},
+                     # diagnostic: the Gram columns every homotopy step re-reads (4 p bytes
+                     # per active atom per step; a(a+1)/2 column reads for a path of a
+                     # admissions without drops), mostly served from L2/MALL
+                     "gram_restream": {
+                         "bytes": restream_bytes,
+                         "TB_per_s": restream_bytes / (kern_ms * 1e-3) / 1e12},
+                     # diagnostic: vector-ALU issue of the two passes from the committed
+                     # rocprofv3 SQ counters of this workload (the binding limit)
+                     "valu_issue": valu_issue()},
         "roofline_gemm": {"bound": "mfma", "kernel": "gemm_tn_kernel<float>",
                           "achieved": gemm_flops / (gemm_ms * 1e-3) / 1e12,
                           "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
