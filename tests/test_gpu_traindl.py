@@ -170,3 +170,44 @@ def test_train_dl_long_signals_fp64_matches_oracle(spx):
     assert mdl["iter"] == ito == 6
     assert max(np.abs(D - Do).max(), np.abs(mdl["A"] - Ao).max(),
                np.abs(mdl["B"] - Bo).max()) < 1e-10
+
+
+def test_train_dl_batch_mode_over_several_chunks_fp64(spx):
+    """Batch mode with 20000 signals: each iteration codes them in two
+    solver chunks (16384 + 3616) whose statistics accumulate through the
+    ping-pong buffers; FP64 against the oracle."""
+    import oracle
+    rng = np.random.default_rng(57)
+    Dtrue = W.unit_dict(rng, 32, 64, np.float64)
+    X = W.sparse_signals(rng, Dtrue, 20000, 3, 0.05, True, np.float64)
+    D0 = W.unit_dict(np.random.default_rng(58), 32, 64, np.float64)
+    kw = dict(lambda1=0.1, batch=True, iter=2, iter_updateD=2)
+    D, mdl = spx.trainDL(X, D=D0, return_model=True, **kw)
+    Do, Ao, Bo, ito = oracle.train_dl(X, D=D0, **kw)
+    assert mdl["iter"] == ito == 2
+    # A and B sum 20000 codes: compare relative to their scale
+    assert np.abs(mdl["A"] - Ao).max() < 1e-10 * max(1.0, np.abs(Ao).max())
+    assert np.abs(mdl["B"] - Bo).max() < 1e-10 * max(1.0, np.abs(Bo).max())
+    assert np.abs(D - Do).max() < 1e-10
+
+
+@pytest.mark.parametrize("case", ["iter0", "small_n"])
+def test_train_dl_edge_cases_fp64(spx, case):
+    """iter = 0 returns the normalised start and the fresh online model
+    (A = t0 I, B = t0 D); fewer signals than the batch size codes all of
+    them each iteration (the permutation wraps)."""
+    import oracle
+    rng = np.random.default_rng(59)
+    Dtrue = W.unit_dict(rng, 12, 20, np.float64)
+    n = 300 if case == "iter0" else 50
+    X = W.sparse_signals(rng, Dtrue, n, 3, 0.05, True, np.float64)
+    D0 = 3.0 * W.unit_dict(np.random.default_rng(60), 12, 20, np.float64)  # not unit norm
+    kw = dict(lambda1=0.12, batchsize=64, iter=0 if case == "iter0" else 5)
+    D, mdl = spx.trainDL(X, D=D0, return_model=True, **kw)
+    Do, Ao, Bo, ito = oracle.train_dl(X, D=D0, **kw)
+    assert mdl["iter"] == ito
+    assert max(np.abs(D - Do).max(), np.abs(mdl["A"] - Ao).max(),
+               np.abs(mdl["B"] - Bo).max()) < 1e-10
+    if case == "iter0":
+        assert np.allclose(np.linalg.norm(D, axis=0), 1.0)
+        assert np.allclose(mdl["A"], 1e-5 * np.eye(20))
