@@ -557,7 +557,9 @@ def run_traindl(args):
     """Online dictionary learning (trainDL) at the headline decomposition
     size: D 256x1024 from a given unit-norm N(0,1) start, signals as in the
     LARS headline (n = 100000 resident in HBM), lambda 0.15, minibatches of
-    1024, FP32 (and BF16x3 products).  One step = one trainer iteration:
+    4352 = 256 (16 + 1), the reference's default batch size at the 16 host
+    threads its baseline uses (cpp_library/spams.h:1193), FP32 (and BF16x3
+    products).  One step = one trainer iteration:
     G = D'D, cleaning scan, minibatch D'X + LARS, statistics, dictionary
     update.  The per-iteration cost is the difference of two device calls
     (a long and a short run from the same start), so set-up is excluded."""
@@ -567,7 +569,7 @@ def run_traindl(args):
     n = args.n if args.n != 200000 else 100000
     X, _, _ = W.lars_headline(n=n)
     m = X.shape[0]
-    K, bs, lam = 1024, 1024, 0.15
+    K, bs, lam = 1024, 256 * (cpu_threads() + 1), 0.15
     D0 = W.unit_dict(np.random.default_rng(1004), m, K, np.float32)
     Xt = E.to_dev(X.T).T
 
@@ -586,8 +588,9 @@ def run_traindl(args):
            "unit": "signals/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
            "data": "synthetic", "scaling": "weak",
            "config": {"workload": "trainDL online, D0 256x1024 unit-norm N(0,1), X 256x%d "
-                                  "headline signals, lambda 0.15, batchsize 1024, clean, "
-                                  "iter_updateD 1" % n, "m": m, "K": K, "n": n,
+                                  "headline signals, lambda 0.15, batchsize %d (the "
+                                  "reference default at %d threads), clean, iter_updateD 1"
+                                  % (n, bs, cpu_threads()), "m": m, "K": K, "n": n,
                                   "batchsize": bs}}
     short, long_ = 10, 10 + 10 * max(args.steps, 1)
     variants = {}
@@ -612,13 +615,13 @@ def run_traindl(args):
         oracle.ref_train_dl(Xs, D=D0, lambda1=lam, batchsize=bs, iter=1, num_threads=th)
         t1 = time.perf_counter() - t
         t = time.perf_counter()
-        oracle.ref_train_dl(Xs, D=D0, lambda1=lam, batchsize=bs, iter=4, num_threads=th)
-        t4 = time.perf_counter() - t
-        per = (t4 - t1) / 3
+        oracle.ref_train_dl(Xs, D=D0, lambda1=lam, batchsize=bs, iter=3, num_threads=th)
+        t3 = time.perf_counter() - t
+        per = (t3 - t1) / 2
         out["cpu_baseline"] = {"value": bs / per, "unit": "signals/s", "cores": th,
                                "kind": "reference", "ms_per_iteration": per * 1e3,
                                "sample": "the reference trainer built in place (_alltrainDL), "
-                                         "iterations 2-4 of the same workload on 20000 of "
+                                         "iterations 2-3 of the same workload on 20000 of "
                                          "its signals, %d OpenMP threads" % th}
     return out
 

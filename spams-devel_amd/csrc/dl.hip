@@ -642,22 +642,22 @@ int train_dl_dev(spx_ctx* ctx, const T* X, int ldx, int m, int M, T* D, int K, i
   const int ldt = (CH + 3) & ~3;
   T *G, *Aorig, *Aodd, *Aeven, *Borig, *Bodd, *Beven, *Xb, *XbT, *aT, *DtX, *norms, *P, *lv,
       *scal_dev;
-  T* Abuf[2] = {nullptr, nullptr};  // batch statistics, ping-pong over the chunks
-  T* Bbuf[2] = {nullptr, nullptr};
+  T *Abat, *Bbat, *Apart, *Bpart;  // batch statistics and their split-K partial products
   int *perm, *lr, *lc, *idx, *hit, *Ldev;
   DL_TRY(mem.get(&G, (size_t)ldg * K));
   DL_TRY(mem.get(&Aorig, nA));
   DL_TRY(mem.get(&Aodd, nA));
   DL_TRY(mem.get(&Aeven, nA));
-  DL_TRY(mem.get(&Abuf[0], nA));
+  DL_TRY(mem.get(&Abat, nA));
   DL_TRY(mem.get(&Borig, nB));
   DL_TRY(mem.get(&Bodd, nB));
   DL_TRY(mem.get(&Beven, nB));
-  DL_TRY(mem.get(&Bbuf[0], nB));
-  if (bs > CH) {
-    DL_TRY(mem.get(&Abuf[1], nA));
-    DL_TRY(mem.get(&Bbuf[1], nB));
-  }
+  DL_TRY(mem.get(&Bbat, nB));
+  // A_t (K x K) and B_t (m x K) have inner dimension = the chunk's signals;
+  // their few output tiles are split over it to fill the chip
+  const int SA = gemm_split_count(K, K, ldt), SB = gemm_split_count(m, K, ldt);
+  DL_TRY(mem.get(&Apart, nA * SA));
+  DL_TRY(mem.get(&Bpart, nB * SB));
   DL_TRY(mem.get(&Xb, (size_t)m * CH));
   DL_TRY(mem.get(&XbT, (size_t)ldt * m));
   DL_TRY(mem.get(&aT, (size_t)ldt * K));
@@ -822,7 +822,6 @@ int train_dl_dev(spx_ctx* ctx, const T* X, int ldx, int m, int M, T* D, int K, i
                        ridge);
     SPX_HIP(hipGetLastError());
     // code the minibatch in chunks; A_t, B_t accumulate over the chunks
-    int cur = 0;
     for (int c0 = 0; c0 < bs; c0 += CH) {
       const int nb = bs - c0 < CH ? bs - c0 : CH;
       const int ld = (nb + 3) & ~3;
@@ -881,16 +880,12 @@ int train_dl_dev(spx_ctx* ctx, const T* X, int ldx, int m, int M, T* D, int K, i
                          static_cast<const T*>(lv), static_cast<const int*>(lr),
                          static_cast<const int*>(lc), L, nb, aT, ld);
       SPX_HIP(hipGetLastError());
-      const T beta = c0 ? T(1) : T(0);
-      const int nxt = c0 ? 1 - cur : 0;
-      DL_TRY(gemm_tn<T>(st, K, K, ld, aT, ld, aT, ld, Abuf[nxt], K, T(1), beta,
-                        c0 ? Abuf[cur] : nullptr, K, prec));
-      DL_TRY(gemm_tn<T>(st, m, K, ld, XbT, ld, aT, ld, Bbuf[nxt], m, T(1), beta,
-                        c0 ? Bbuf[cur] : nullptr, m, prec));
-      cur = nxt;
+      const int acc = c0 ? 1 : 0;
+      DL_TRY(gemm_tn_split<T>(st, K, K, ld, aT, ld, aT, ld, Abat, acc, Apart,
+                              gemm_split_count(K, K, ld), prec));
+      DL_TRY(gemm_tn_split<T>(st, m, K, ld, XbT, ld, aT, ld, Bbat, acc, Bpart,
+                              gemm_split_count(m, K, ld), prec));
     }
-    T* Abat = Abuf[cur];
-    T* Bbat = Bbuf[cur];
     hipLaunchKernelGGL(dl_mirror_kernel<T>, dim3(blocks_for((long long)nA, 256)), dim3(256), 0,
                        st, Abat, K);
     SPX_HIP(hipGetLastError());

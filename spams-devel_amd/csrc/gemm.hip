@@ -40,6 +40,18 @@ __device__ __forceinline__ void tile_origin(int& bm, int& bn) {
   bn = (b / mt) * BN;
 }
 
+// Split over the inner dimension (grid z > 1): slice z takes the inner
+// indices [z kz, z kz + kz) and writes its own partial product, zstride
+// elements after the previous slice's.
+// Returns the slice's first inner index and shortens K to the slice.
+__device__ __forceinline__ int split_k_begin(int& K, int kz) {
+  if (gridDim.z <= 1) return 0;
+  const int kb = (int)blockIdx.z * kz;
+  const int rest = K - kb;
+  K = rest < kz ? (rest > 0 ? rest : 0) : kz;
+  return kb;
+}
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -133,7 +145,13 @@ template <typename T, bool VECLOAD>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void gemm_tn_kernel(
     int M, int N, int K, const T* __restrict__ A, int lda,
     const T* __restrict__ B, int ldb, T* __restrict__ C, int ldc, T alpha,
-    T beta, const T* __restrict__ E, int lde) {
+    T beta, const T* __restrict__ E, int lde, int kz, size_t zstride) {
+  {
+    const int kb = split_k_begin(K, kz);
+    A += kb;
+    B += kb;
+    C += (size_t)blockIdx.z * zstride;
+  }
   // K chunk per barrier pair: 32 in FP32 (16 KiB of operands per chunk),
   // 16 in FP64 (same bytes)
   constexpr int BK = sizeof(T) == 4 ? 32 : 16;
@@ -233,7 +251,13 @@ template <bool VECLOAD>
 __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
     int M, int N, int K, const float* __restrict__ A, int lda,
     const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
-    float alpha, float beta, const float* __restrict__ E, int lde) {
+    float alpha, float beta, const float* __restrict__ E, int lde, int kz, size_t zstride) {
+  {
+    const int kb = split_k_begin(K, kz);
+    A += kb;
+    B += kb;
+    C += (size_t)blockIdx.z * zstride;
+  }
   constexpr int BK = 32;
   constexpr int LD = BK + 8;  // [row][k] layout (k contiguous), padded
   typedef float f4 __attribute__((ext_vector_type(4)));
@@ -382,10 +406,10 @@ int gemm_tn(hipStream_t st, int M, int N, int K, const T* A, int lda,
                        (reinterpret_cast<uintptr_t>(B) % 16 == 0);
       if (vec)
         hipLaunchKernelGGL(gemm_tn_bf16x3_kernel<true>, grid, dim3(256), 0, st, M, N,
-                           K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde);
+                           K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde, 0, (size_t)0);
       else
         hipLaunchKernelGGL(gemm_tn_bf16x3_kernel<false>, grid, dim3(256), 0, st, M, N,
-                           K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde);
+                           K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde, 0, (size_t)0);
       SPX_HIP(hipGetLastError());
       return SPX_OK;
     }
@@ -396,12 +420,91 @@ int gemm_tn(hipStream_t st, int M, int N, int K, const T* A, int lda,
                    (reinterpret_cast<uintptr_t>(B) % (4 * sizeof(T)) == 0);
   if (vec)
     hipLaunchKernelGGL((gemm_tn_kernel<T, true>), grid, dim3(256), 0, st, M, N, K, A,
-                       lda, B, ldb, C, ldc, alpha, beta, E, lde);
+                       lda, B, ldb, C, ldc, alpha, beta, E, lde, 0, (size_t)0);
   else
     hipLaunchKernelGGL((gemm_tn_kernel<T, false>), grid, dim3(256), 0, st, M, N, K, A,
-                       lda, B, ldb, C, ldc, alpha, beta, E, lde);
+                       lda, B, ldb, C, ldc, alpha, beta, E, lde, 0, (size_t)0);
   SPX_HIP(hipGetLastError());
   return SPX_OK;
+}
+
+namespace {
+
+// C = (accumulate ? C : 0) + sum over z of part[z] (n elements each, zstride
+// apart), summed in slice order.
+template <typename T>
+__global__ void split_sum_kernel(const T* __restrict__ part, int S, size_t zstride, long long n,
+                                 T* C, int accumulate) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  T s = accumulate ? C[e] : T(0);
+  for (int z = 0; z < S; ++z) s += part[(size_t)z * zstride + e];
+  C[e] = s;
+}
+
+}  // namespace
+
+template <typename T>
+int gemm_tn_split(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
+                  T* C, int accumulate, T* part, int S, int precision) {
+  if (M <= 0 || N <= 0) return SPX_OK;
+  const int mt = (M + BM - 1) / BM, nt = (N + BN - 1) / BN;
+  if (S < 1) S = 1;
+  int kz = (K + S - 1) / S;
+  kz = (kz + 31) / 32 * 32;  // whole 32-deep chunks; 16-byte aligned slice starts
+  S = (K + kz - 1) / kz;
+  if (S < 1) S = 1;
+  if (nt > 65535) {
+    set_error("gemm_tn_split: N=%d too wide", N);
+    return SPX_ERR_UNSUP;
+  }
+  const size_t zstride = (size_t)M * N;
+  dim3 grid(mt, nt, S);
+  const bool vec = (lda % 4 == 0) && (ldb % 4 == 0) &&
+                   (reinterpret_cast<uintptr_t>(A) % (4 * sizeof(T)) == 0) &&
+                   (reinterpret_cast<uintptr_t>(B) % (4 * sizeof(T)) == 0);
+  bool launched = false;
+  if constexpr (sizeof(T) == 4) {
+    if (precision & SPX_PREC_BF16X3) {
+      if (vec)
+        hipLaunchKernelGGL(gemm_tn_bf16x3_kernel<true>, grid, dim3(256), 0, st, M, N, K, A, lda,
+                           B, ldb, part, M, 1.f, 0.f, static_cast<const float*>(nullptr), 0, kz,
+                           zstride);
+      else
+        hipLaunchKernelGGL(gemm_tn_bf16x3_kernel<false>, grid, dim3(256), 0, st, M, N, K, A,
+                           lda, B, ldb, part, M, 1.f, 0.f, static_cast<const float*>(nullptr),
+                           0, kz, zstride);
+      launched = true;
+    }
+  }
+  if (!launched) {
+    if (vec)
+      hipLaunchKernelGGL((gemm_tn_kernel<T, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B,
+                         ldb, part, M, T(1), T(0), static_cast<const T*>(nullptr), 0, kz,
+                         zstride);
+    else
+      hipLaunchKernelGGL((gemm_tn_kernel<T, false>), grid, dim3(256), 0, st, M, N, K, A, lda,
+                         B, ldb, part, M, T(1), T(0), static_cast<const T*>(nullptr), 0, kz,
+                         zstride);
+  }
+  SPX_HIP(hipGetLastError());
+  {
+    const long long n = (long long)M * N;
+    hipLaunchKernelGGL(split_sum_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       static_cast<const T*>(part), S, zstride, n, C, accumulate);
+    SPX_HIP(hipGetLastError());
+  }
+  return SPX_OK;
+}
+
+// Slices for gemm_tn_split: enough workgroups to fill the chip (about 512
+// output-tile slices), each slice at least 256 deep.
+int gemm_split_count(int M, int N, int K) {
+  const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  int S = (512 + tiles - 1) / tiles;
+  const int maxS = K / 256 > 1 ? K / 256 : 1;
+  if (S > maxS) S = maxS;
+  return S < 1 ? 1 : S;
 }
 
 template <typename T>
@@ -421,6 +524,10 @@ template int gemm_tn<double>(hipStream_t, int, int, int, const double*, int,
                              const double*, int, int);
 template int col_sqnorms<float>(hipStream_t, int, int, const float*, int,
                                 float*);
+template int gemm_tn_split<float>(hipStream_t, int, int, int, const float*, int, const float*,
+                                  int, float*, int, float*, int, int);
+template int gemm_tn_split<double>(hipStream_t, int, int, int, const double*, int,
+                                   const double*, int, double*, int, double*, int, int);
 template int col_sqnorms<double>(hipStream_t, int, int, const double*, int,
                                  double*);
 

@@ -116,6 +116,14 @@ template <typename T>
 int gemm_tn(hipStream_t st, int M, int N, int K, const T* A, int lda,
             const T* B, int ldb, T* C, int ldc, T alpha, T beta, const T* E,
             int lde, int precision);
+// C (M x N, leading dimension M) = (accumulate ? C : 0) + A'B, with the
+// inner dimension split into about S slices computed in one launch (grid z)
+// into `part` (S x M x N elements) and summed in slice order.
+template <typename T>
+int gemm_tn_split(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
+                  T* C, int accumulate, T* part, int S, int precision);
+// Slices that fill the chip for an M x N product with inner dimension K.
+int gemm_split_count(int M, int N, int K);
 template <typename T>
 int col_sqnorms(hipStream_t st, int m, int n, const T* X, int ldx, T* out);
 template <typename T>
