@@ -366,7 +366,8 @@ typedef struct {
  * D_given (else K signals drawn like the reference's srandom(0)/random()).
  * A (K x K), B (m x K) and *itercount: the model (return_model); a model is
  * given when *itercount > 0 (then A and B are read), and the three are
- * written on return (A and B may be NULL when no model is wanted). */
+ * written on return (A and B may be NULL when no model is wanted).  A model
+ * needs its dictionary (D_given), as in the facade. */
 int spx_train_dl_f32(spx_ctx* ctx, const float* X, int m, int n, float* D,
                      int K, int D_given, float* A, float* B, int* itercount,
                      const spx_train_dl_opts* opts);

@@ -607,6 +607,11 @@ int train_dl_dev(spx_ctx* ctx, const T* X, int ldx, int m, int M, T* D, int K, i
     set_error("trainDL: invalid modeParam / iter_updateD / iteration count");
     return SPX_ERR_ARG;
   }
+  if (*itercount > 0 && !D_given) {
+    // the facade continues a model only with its dictionary (cpp_library/spams.h:1201-1210)
+    set_error("trainDL: a model (itercount > 0) needs its dictionary (D_given)");
+    return SPX_ERR_ARG;
+  }
   if (m > kDlBlockThreads * kDlMaxRowsPerThread) {
     set_error("trainDL: signal length m=%d exceeds %d", m, kDlBlockThreads * kDlMaxRowsPerThread);
     return SPX_ERR_UNSUP;

@@ -614,6 +614,42 @@ def trainDL(X, return_model=False, model=None, D=None, numThreads=-1, batchsize=
     return Dw
 
 
+def nmf(X, return_lasso=False, model=None, numThreads=-1, batchsize=-1, K=-1, iter=-1,
+        t0=1e-5, clean=True, rho=1.0, modeParam=0, batch=False, ctx=None):
+    """Non-negative matrix factorisation (spams.nmf, swig/python/spams.py:
+    530-548): trainDL with lambda1 = 0 and nonnegative codes and dictionary;
+    with return_lasso, also the nonnegative codes V = lasso(X, U, 0, pos)."""
+    lambda1 = 0
+    U = trainDL(X, model=model, numThreads=numThreads, batchsize=batchsize, K=K, iter=iter,
+                t0=t0, clean=clean, rho=rho, verbose=False, modeParam=modeParam, batch=batch,
+                lambda1=lambda1, mode=PENALTY, posAlpha=True, posD=True, whiten=False, ctx=ctx)
+    if not return_lasso:
+        return U
+    if ssp.issparse(X):
+        raise ValueError("sparse matrix for lasso not yet implemented")
+    V = lasso(X, D=U, return_reg_path=False, numThreads=numThreads, lambda1=lambda1,
+              mode=PENALTY, pos=True, ctx=ctx)
+    return (U, V)
+
+
+def nnsc(X, return_lasso=False, model=None, lambda1=None, numThreads=-1, batchsize=-1,
+         K=-1, iter=-1, t0=1e-5, clean=True, rho=1.0, modeParam=0, batch=False, ctx=None):
+    """Non-negative sparse coding (spams.nnsc, swig/python/spams.py:550-562):
+    trainDL with nonnegative codes and dictionary; with return_lasso, also
+    V = lasso(X, U, lambda1) (without the positivity constraint, as the
+    reference calls it)."""
+    if lambda1 is None:
+        raise ValueError("nnsc : lambda1 must be defined")
+    U = trainDL(X, model=model, numThreads=numThreads, batchsize=batchsize, K=K, iter=iter,
+                t0=t0, clean=clean, rho=rho, verbose=False, modeParam=modeParam, batch=batch,
+                lambda1=lambda1, mode=PENALTY, posAlpha=True, posD=True, whiten=False, ctx=ctx)
+    if not return_lasso:
+        return U
+    V = lasso(X, D=U, return_reg_path=False, numThreads=numThreads, lambda1=lambda1,
+              mode=PENALTY, ctx=ctx)
+    return (U, V)
+
+
 def train_dl_dev(ctx, X, D, A, B, itercount=0, D_given=True, **opts):
     """Device dictionary learning (spx_train_dl_dev_*) on torch tensors:
     X (m x n), D (m x K), A (K x K), B (m x K), column-major as the

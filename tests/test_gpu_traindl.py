@@ -138,6 +138,21 @@ def test_train_dl_time_limit_and_errors(spx):
     # the ridge-only problem (conjugate-gradient branch of the reference) is refused
     with pytest.raises(RuntimeError):
         spx.trainDL(X, K=32, lambda1=0.0, lambda2=0.1, iter=2)
+    # a model without its dictionary is refused by the C entry point
+    import ctypes
+    o = spx.spx_train_dl_opts(64, 1, 0.15, 1e-9, 2, 1e-5, 2, 0, 0, 1, 1.0, 1, 0, 0, 0, 0, 0)
+    Dn = np.zeros((16, 32), np.float32, order="F")
+    An = np.zeros((32, 32), np.float32, order="F")
+    Bn = np.zeros((16, 32), np.float32, order="F")
+    it = ctypes.c_int(3)
+    f = spx.lib().spx_train_dl_f32
+    f.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int] * 2 + [ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                  ctypes.POINTER(spx.spx_train_dl_opts)]
+    Xf = np.asfortranarray(X)
+    rc = f(spx.default_context().h, Xf.ctypes.data, 16, 300, Dn.ctypes.data, 32, 0,
+           An.ctypes.data, Bn.ctypes.data, ctypes.byref(it), ctypes.byref(o))
+    assert rc != 0 and b"dictionary" in spx.lib().spx_last_error()
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
@@ -211,3 +226,29 @@ def test_train_dl_edge_cases_fp64(spx, case):
     if case == "iter0":
         assert np.allclose(np.linalg.norm(D, axis=0), 1.0)
         assert np.allclose(mdl["A"], 1e-5 * np.eye(20))
+
+
+@pytest.mark.parametrize("which", ["nmf", "nnsc"])
+def test_nmf_nnsc_match_oracle_fp64(spx, which):
+    """nmf and nnsc (swig/python/spams.py:530-562): trainDL with nonnegative
+    codes and dictionary (lambda1 = 0 for nmf), then the codes by lasso
+    (nonnegative for nmf, unconstrained for nnsc, as the reference calls
+    it); FP64 against the oracle's trainer and lasso."""
+    import oracle
+    rng = np.random.default_rng(61)
+    U0 = np.abs(rng.standard_normal((16, 24)))
+    V0 = np.abs(rng.standard_normal((24, 300))) * (rng.random((24, 300)) < 0.2)
+    X = np.asfortranarray(U0 @ V0 + 0.01 * np.abs(rng.standard_normal((16, 300))))
+    lam = 0.0 if which == "nmf" else 0.05
+    kw = dict(K=24, batchsize=64, iter=8)
+    if which == "nmf":
+        U, V = spx.nmf(X, return_lasso=True, **kw)
+    else:
+        U, V = spx.nnsc(X, return_lasso=True, lambda1=lam, **kw)
+    Uo, _, _, _ = oracle.train_dl(X, lambda1=lam, lambda2=10e-10, posAlpha=True, posD=True, **kw)
+    assert np.abs(U - Uo).max() < 1e-10
+    assert U.min() >= 0
+    Vo = oracle.lasso(X, Uo, lam, pos=(which == "nmf")).toarray()
+    assert np.abs(V.toarray() - Vo).max() < 1e-9
+    if which == "nmf":
+        assert V.toarray().min() >= 0
