@@ -295,19 +295,22 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int L = a.L;
   const int LC = CAP > 0 ? (CAP < L ? CAP : L) : L;  // active-set capacity
+  // LDS layout capacity: CAP in the LDS pass (a compile-time layout, so the
+  // arrays below sit at constant offsets), L in the full-capacity pass
+  const int LCL = CAP > 0 ? CAP : L;
   const int p = a.p;
   const int ldg = a.ldg;
   T* sDtR = reinterpret_cast<T*>(smem);  // PT
   T* sPart = sDtR + PT;                   // GT partial sums
   T* sCoef = sPart + GT;                  // LC
-  T* sU = sCoef + LC;                     // LC
-  T* sW = sU + LC;                        // LC
-  T* sGs = sW + LC;                       // LC
-  T* sV = sGs + LC;                       // LC
-  int* sInd = reinterpret_cast<int*>(sV + LC);  // LC
-  int* sTmpI = sInd + LC;                       // LC
+  T* sU = sCoef + LCL;                    // LC
+  T* sW = sU + LCL;                       // LC
+  T* sGs = sW + LCL;                      // LC
+  T* sV = sGs + LCL;                      // LC
+  int* sInd = reinterpret_cast<int*>(sV + LCL);  // LC
+  int* sTmpI = sInd + LCL;                       // LC
   KeyIdx<T>* sRed = reinterpret_cast<KeyIdx<T>*>(
-      (reinterpret_cast<uintptr_t>(sTmpI + LC) + 15) & ~uintptr_t(15));
+      (reinterpret_cast<uintptr_t>(sTmpI + LCL) + 15) & ~uintptr_t(15));
   T* sRed3 = reinterpret_cast<T*>(sRed + W);  // 3W
   int* sSig = reinterpret_cast<int*>(sRed3 + 3 * W);
   T* sM0 = reinterpret_cast<T*>(
@@ -315,12 +318,12 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
   T* sM1 = sM0 + (CAP > 0 ? CAP * CAP : 0);
   double* sAlpha = reinterpret_cast<double*>(
       (reinterpret_cast<uintptr_t>(sM1 + (CAP > 0 ? CAP * CAP : 0)) + 15) & ~uintptr_t(15));
-  double* sRes = sAlpha + LC;  // FP64 refit vectors; sRes has max(LC, 64)
+  double* sRes = sAlpha + LCL;  // FP64 refit vectors; sRes has max(LC, 64)
   // full-capacity pass: LDS copy of the inverse while the active set has at
   // most CL atoms (CL x CL, leading dimension CL)
   const int CL = CAP == 0 ? a.lds_cap : 0;
   T* sMl = reinterpret_cast<T*>(
-      (reinterpret_cast<uintptr_t>(sRes + (LC > 64 ? LC : 64)) + 15) & ~uintptr_t(15));
+      (reinterpret_cast<uintptr_t>(sRes + (LCL > 64 ? LCL : 64)) + 15) & ~uintptr_t(15));
   // weighted penalty: the signal's weights of the group's rows (1 past p)
   T* sWt = reinterpret_cast<T*>(
       (reinterpret_cast<uintptr_t>(sMl + CL * CL) + 15) & ~uintptr_t(15));
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
   // masked: active position -> Gram-column slot, and its shift buffer
   int* sSlot = reinterpret_cast<int*>(
       (reinterpret_cast<uintptr_t>(sMl + CL * CL) + 15) & ~uintptr_t(15));
-  int* sSlotT = sSlot + LC;
+  int* sSlotT = sSlot + LCL;
   T* const gaw = mk ? a.ga + (size_t)blockIdx.x * a.ga_stride : nullptr;
 
   const int tid = threadIdx.x;
@@ -950,7 +953,8 @@ template <typename T, int W, int NQ, int CAP, int VAR>
 int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
                 long max_blocks) {
   const int LC = CAP > 0 ? (CAP < a.L ? CAP : a.L) : a.L;
-  size_t lds = LarsLds<T, W, NQ, CAP>::bytes(LC, 0, VAR);
+  const int LCL = CAP > 0 ? CAP : a.L;  // the kernel's LDS layout capacity
+  size_t lds = LarsLds<T, W, NQ, CAP>::bytes(LCL, 0, VAR);
   if (lds > 160 * 1024) {
     set_error("lasso: L=%d, p=%d need %zu bytes of LDS per signal", a.L, a.p, lds);
     return SPX_ERR_UNSUP;
@@ -969,11 +973,11 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
     if (per_cu < 1) per_cu = 1;
     const size_t budget = (size_t)156 * 1024 / per_cu;
     int cl = 0;
-    while (cl + 4 <= a.L && LarsLds<T, W, NQ, CAP>::bytes(LC, cl + 4, VAR) <= budget) cl += 4;
+    while (cl + 4 <= a.L && LarsLds<T, W, NQ, CAP>::bytes(LCL, cl + 4, VAR) <= budget) cl += 4;
     if (cl >= a.L) cl = a.L;
     if (cl >= FastCap<T>::C) {
       b.lds_cap = cl;
-      lds = LarsLds<T, W, NQ, CAP>::bytes(LC, cl, VAR);
+      lds = LarsLds<T, W, NQ, CAP>::bytes(LCL, cl, VAR);
     }
   }
   if (persistent) {
