@@ -252,3 +252,20 @@ def test_nmf_nnsc_match_oracle_fp64(spx, which):
     assert np.abs(V.toarray() - Vo).max() < 1e-9
     if which == "nmf":
         assert V.toarray().min() >= 0
+
+
+@pytest.mark.parametrize("mode,lam", [(3, 5), (4, 0.05)])
+def test_train_dl_ormp_modes_mid_size_fp64(spx, mode, lam):
+    """SPARSITY and L2ERROR2 coding (coreORMPB over the trainer's ridged Gram)
+    at m=64, K=256 from a given dictionary; FP64 against the oracle."""
+    import oracle
+    rng = np.random.default_rng(62)
+    Dtrue = W.unit_dict(rng, 64, 256, np.float64)
+    X = W.sparse_signals(rng, Dtrue, 2000, 5, 0.05, True, np.float64)
+    D0 = W.unit_dict(np.random.default_rng(63), 64, 256, np.float64)
+    kw = dict(lambda1=lam, mode=mode, batchsize=512, iter=3)
+    D, mdl = spx.trainDL(X, D=D0, return_model=True, **kw)
+    Do, Ao, Bo, ito = oracle.train_dl(X, D=D0, **kw)
+    assert mdl["iter"] == ito == 3
+    assert max(np.abs(D - Do).max(), np.abs(mdl["A"] - Ao).max(),
+               np.abs(mdl["B"] - Bo).max()) < 1e-9
