@@ -12,6 +12,7 @@
 //   _omp        cpp_library/spams.h:250-293   (omp / coreORMP)
 //   _fistaFlat  cpp_library/spams.h:460-608   (square loss; l1, elastic-net)
 //   lasso2      decomp/decomp.h:126-135        omp  decomp/decomp.h:53-57
+//   lasso       decomp/decomp.h:111-123 (the Cholesky variant; D or G/DtX given)
 //   cppLasso / cppOMP  cpp_library/cppspams.h:45-51, :106-113
 //   _alltrainDL cpp_library/spams.h:1153-1340 (trainDL: Trainer::train,
 //               dictLearn/dicts.h:346-759; modeD = L2, no whitening)
@@ -110,6 +111,9 @@ using AbstractMatrixB = Matrix<T>;
 // Data<T> stand-in for trainDL's signal argument: dense column-major signals.
 template <typename T>
 using Data = Matrix<T>;
+// AbstractMatrix<T> stand-in for the Gram-given forms' G and DtX: dense.
+template <typename T>
+using AbstractMatrix = Matrix<T>;
 
 template <typename T>
 class SpMatrix {
@@ -277,6 +281,62 @@ void lasso2(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, int L,
   T* pp = spams_amd_detail::path_buffer(path, D.n(), L, length_path);
   spams_amd_detail::check(spams_amd_detail::lasso_call(X.rawX(), X.m(), X.n(), D.rawX(), D.n(),
                                                        &constraint, 1, lambda2, &o, &out, pp));
+  spalpha.adopt(out);
+}
+
+// lasso (decomp/decomp.h:111-116): the Cholesky-based variant of the
+// reference; spx runs its inverse-Gram homotopy (same solution) and, with
+// ols, refits each final active set by least squares.  L already normalised.
+template <typename T>
+void lasso(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, int L,
+           const T constraint, const T lambda2 = 0, constraint_type mode = PENALTY,
+           const bool pos = false, const bool ols = false, const int numThreads = -1,
+           Matrix<T>* path = nullptr, const int length_path = -1) {
+  (void)numThreads;
+  spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, length_path, SPX_PREC_FP32, SPX_FORM_BATCHED, 0,
+                   ols ? 1 : 0};
+  spx_csc out{};
+  T* pp = spams_amd_detail::path_buffer(path, D.n(), L, length_path);
+  spams_amd_detail::check(spams_amd_detail::lasso_call(X.rawX(), X.m(), X.n(), D.rawX(), D.n(),
+                                                       &constraint, 1, lambda2, &o, &out, pp));
+  spalpha.adopt(out);
+}
+
+// lasso2 / lasso with the Gram matrix G (p x p, used as given) and the
+// correlations DtX (p x n) given (decomp/decomp.h:118-123, :131-135); X only
+// gives the signals' norms.  L already normalised.
+template <typename T>
+void lasso2(const Data<T>& X, const AbstractMatrix<T>& G, const AbstractMatrix<T>& DtX,
+            SpMatrix<T>& spalpha, int L, const T constraint, constraint_type mode = PENALTY,
+            const bool pos = false, const int numThreads = -1, Matrix<T>* path = nullptr,
+            const int length_path = -1) {
+  (void)numThreads;
+  if (G.m() != G.n() || DtX.m() != G.n() || DtX.n() != X.n())
+    throw "lasso2 : incompatible matrix dimensions";
+  spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, length_path, SPX_PREC_FP32, SPX_FORM_BATCHED, 0};
+  spx_csc out{};
+  T* pp = spams_amd_detail::path_buffer(path, G.n(), L, length_path);
+  spams_amd_detail::check(spams_amd_detail::lasso_qq_call(X.rawX(), X.m(), X.n(), G.rawX(),
+                                                          DtX.rawX(), G.n(), &constraint, T(0),
+                                                          &o, &out, pp));
+  spalpha.adopt(out);
+}
+
+template <typename T>
+void lasso(const Data<T>& X, const AbstractMatrix<T>& G, const AbstractMatrix<T>& DtX,
+           SpMatrix<T>& spalpha, int L, const T constraint, constraint_type mode = PENALTY,
+           const bool pos = false, const bool ols = false, const int numThreads = -1,
+           Matrix<T>* path = nullptr, const int length_path = -1) {
+  (void)numThreads;
+  if (G.m() != G.n() || DtX.m() != G.n() || DtX.n() != X.n())
+    throw "lasso : incompatible matrix dimensions";
+  spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, length_path, SPX_PREC_FP32, SPX_FORM_BATCHED, 0,
+                   ols ? 1 : 0};
+  spx_csc out{};
+  T* pp = spams_amd_detail::path_buffer(path, G.n(), L, length_path);
+  spams_amd_detail::check(spams_amd_detail::lasso_qq_call(X.rawX(), X.m(), X.n(), G.rawX(),
+                                                          DtX.rawX(), G.n(), &constraint, T(0),
+                                                          &o, &out, pp));
   spalpha.adopt(out);
 }
 

@@ -81,3 +81,12 @@ void use_train_dl() {
   delete B;
   delete it;
 }
+
+void use_decomp_level() {
+  Matrix<float> X(64, 20), D(64, 256), G(256, 256), DtX(256, 20);
+  SpMatrix<float> a1, a2, a3, a4;
+  lasso(X, D, a1, 64, 0.15f);                         // decomp/decomp.h:111-116
+  lasso(X, D, a2, 64, 0.15f, 0.f, PENALTY, false, true);
+  lasso2(X, G, DtX, a3, 64, 0.15f);                   // decomp/decomp.h:131-135
+  lasso(X, G, DtX, a4, 64, 0.15f, L1COEFFS, true, false);
+}

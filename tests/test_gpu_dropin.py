@@ -1,0 +1,78 @@
+"""The spams.h-compatible C++ templates (include/spams_amd.h) run on the GPU
+as a reference C++ caller would use them: tests/cpp/dropin_run.cpp is built
+against libspx.so and run in a child process, and its results are checked
+against the oracle (FP64; within 1e-9), the least-squares refit against the
+least-squares fit on its support."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as ssp
+
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def read_csc(path, p):
+    b = open(path, "rb").read()
+    n = int(np.frombuffer(b, np.int32, 1, 0)[0])
+    nnz = int(np.frombuffer(b, np.int64, 1, 4)[0])
+    o = 12
+    pB = np.frombuffer(b, np.int32, n + 1, o); o += 4 * (n + 1)
+    r = np.frombuffer(b, np.int32, nnz, o); o += 4 * nnz
+    v = np.frombuffer(b, np.float64, nnz, o)
+    return ssp.csc_matrix((v, r, pB), shape=(p, n)).toarray()
+
+
+def test_cpp_dropin_runs_and_matches_oracle(tmp_path):
+    import oracle
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ absent")
+    lib_dir = os.path.join(ROOT, "spams-devel_amd")
+    exe = str(tmp_path / "dropin_run")
+    r = subprocess.run([gxx, "-std=c++14", "-O1", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "dropin_run.cpp"), "-o", exe,
+                        "-L", lib_dir, "-lspx", "-Wl,-rpath," + lib_dir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    rng = np.random.default_rng(71)
+    m, p, n = 32, 64, 120
+    D = W.unit_dict(rng, m, p, np.float64)
+    X = W.sparse_signals(rng, D, n, 4, 0.05, True, np.float64)
+    (tmp_path / "dims.txt").write_text("%d %d %d" % (m, n, p))
+    np.asfortranarray(X).T.tofile(str(tmp_path / "X.bin"))   # column-major bytes
+    np.asfortranarray(D).T.tofile(str(tmp_path / "D.bin"))
+    print("dropin_run: m=%d n=%d p=%d" % (m, n, p), flush=True)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # _lassoD and the Gram-given lasso2 against the lasso restatement
+    ref = oracle.lasso(X, D, 0.15).toarray()
+    assert np.abs(read_csc(str(tmp_path / "lassoD.bin"), p) - ref).max() < 1e-9
+    G = D.T @ D + 1e-10 * np.eye(p)
+    refq = oracle.lasso_qq(X, G, D.T @ X, 0.15).toarray()
+    assert np.abs(read_csc(str(tmp_path / "lasso2_gram.bin"), p) - refq).max() < 1e-9
+    # lasso with ols: the same supports, least squares on each
+    A = read_csc(str(tmp_path / "lasso_ols.bin"), p)
+    for j in range(n):
+        s = np.flatnonzero(ref[:, j])
+        assert set(np.flatnonzero(A[:, j])) <= set(s)
+        if len(s):
+            ls = np.linalg.lstsq(D[:, s], X[:, j], rcond=None)[0]
+            assert np.abs(A[s, j] - ls).max() < 1e-8
+    # _omp (L = 8)
+    refo = oracle.omp(X, D, L=8).toarray()
+    assert np.abs(read_csc(str(tmp_path / "omp.bin"), p) - refo).max() < 1e-9
+    # _alltrainDL: 5 iterations of 64 signals from the given D, one thread's order
+    Do, Ao, _, ito = oracle.train_dl(X, D=D, lambda1=0.15, lambda2=1e-9, batchsize=64, iter=5)
+    Dl = np.fromfile(str(tmp_path / "traindl_D.bin"), np.float64).reshape(p, m).T
+    Al = np.fromfile(str(tmp_path / "traindl_A.bin"), np.float64).reshape(p, p).T
+    it = int(np.fromfile(str(tmp_path / "traindl_iter.bin"), np.int32)[0])
+    assert it == ito == 5
+    assert np.abs(Dl - Do).max() < 1e-10
+    assert np.abs(Al - Ao).max() < 1e-10
