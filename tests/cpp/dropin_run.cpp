@@ -116,6 +116,54 @@ int main(int argc, char** argv) {
     delete A;
     delete B;
     delete it;
+    // _lassoWeighted / _lassoMask / _ompMask (cpp_library/spams.h:190-249, :295-343)
+    Matrix<double> Wt(p, n);
+    std::vector<double> braw((size_t)m * n);
+    if (!read_doubles("W.bin", Wt.rawX(), (size_t)p * n) ||
+        !read_doubles("B.bin", braw.data(), braw.size()))
+      return 6;
+    Matrix<bool> Bm(m, n);
+    for (size_t e = 0; e < braw.size(); ++e) Bm.rawX()[e] = braw[e] != 0.0;
+    SpMatrix<double>* w = _lassoWeighted<double>(&X, &D, &Wt, -1, 0.1, PENALTY, false, -1, false);
+    write_csc("lasso_weighted", *w);
+    delete w;
+    SpMatrix<double>* mk = _lassoMask<double>(&X, &D, &Bm, -1, 0.1, 0.0, PENALTY, false, -1, false);
+    write_csc("lasso_mask", *mk);
+    delete mk;
+    Matrix<double>* mpath = nullptr;
+    SpMatrix<double>* om = _ompMask<double>(&X, &D, &Bm, &mpath, false, true, &vL, false, &ve,
+                                            false, &vl, -1);
+    write_csc("omp_mask", *om);
+    delete om;
+    // _fistaFlat, l1, fixed step (cpp_library/spams.h:460-608); W written in place
+    double L0 = 0;
+    if (!read_doubles("L0.bin", &L0, 1)) return 7;
+    Matrix<double> W0(p, n), Wf(p, n);
+    W0.setZeros();
+    Vector<int> groups;
+    char rl1[] = "l1";
+    char lsq[] = "square";
+    char lname[] = "";
+    Matrix<double>* info = _fistaFlat<double>(
+        &X, &D, &W0, &Wf, &groups, 1, 60, L0, true, 1.5, 0.05, 1.0, 0.0, 0.0, 1.0, 0.0, 1.0, 0.0,
+        100, 1000, false, false, false, false, false, rl1, lsq, false, false, false, false,
+        false, false, lname, false, nullptr, 1, true, false, 0);
+    write_raw("fista_W", Wf.rawX(), sizeof(double) * (size_t)p * n);
+    delete info;
+    // cppLasso / cppOMP (cpp_library/cppspams.h:45-51, :106-113)
+    Matrix<double>* cpath = nullptr;
+    SpMatrix<double>* cl = cppLasso(X, D, &cpath, false, 10, 0.15);
+    write_csc("cpp_lasso", *cl);
+    delete cl;
+    const int L8 = 8;
+    double eps0 = 0, lam0 = 0;
+    SpMatrix<double> co;
+    cppOMP(X, D, co, &L8, &eps0, &lam0);
+    write_csc("cpp_omp", co);
+    // lasso2 with D given (decomp/decomp.h:126-129), L = 10
+    SpMatrix<double> l2;
+    lasso2(X, D, l2, 10, 0.15);
+    write_csc("lasso2_d", l2);
   } catch (const char* msg) {
     std::fprintf(stderr, "dropin_run: %s\n", msg);
     return 5;

@@ -1,6 +1,8 @@
 """The spams.h-compatible C++ templates (include/spams_amd.h) run on the GPU
 as a reference C++ caller would use them: tests/cpp/dropin_run.cpp is built
-against libspx.so and run in a child process, and its results are checked
+against libspx.so and run in a child process (_lassoD, lasso with ols,
+lasso2 with D and with G/DtX, _omp, _lassoWeighted, _lassoMask, _ompMask,
+_fistaFlat, cppLasso, cppOMP, _alltrainDL), and its results are checked
 against the oracle (FP64; within 1e-9), the least-squares refit against the
 least-squares fit on its support."""
 import os
@@ -45,6 +47,13 @@ def test_cpp_dropin_runs_and_matches_oracle(tmp_path):
     m, p, n = 32, 64, 120
     D = W.unit_dict(rng, m, p, np.float64)
     X = W.sparse_signals(rng, D, n, 4, 0.05, True, np.float64)
+    Wt = rng.uniform(0.5, 2.0, (p, n))
+    B = (rng.uniform(size=(m, n)) < 0.8).astype(np.float64)
+    B[:, 3] = 0.0                                    # a signal with no observed row
+    L0 = float(np.linalg.eigvalsh(D.T @ D).max())
+    np.asfortranarray(Wt).T.tofile(str(tmp_path / "W.bin"))
+    np.asfortranarray(B).T.tofile(str(tmp_path / "B.bin"))
+    np.array([L0]).tofile(str(tmp_path / "L0.bin"))
     (tmp_path / "dims.txt").write_text("%d %d %d" % (m, n, p))
     np.asfortranarray(X).T.tofile(str(tmp_path / "X.bin"))   # column-major bytes
     np.asfortranarray(D).T.tofile(str(tmp_path / "D.bin"))
@@ -76,3 +85,22 @@ def test_cpp_dropin_runs_and_matches_oracle(tmp_path):
     assert it == ito == 5
     assert np.abs(Dl - Do).max() < 1e-10
     assert np.abs(Al - Ao).max() < 1e-10
+
+    # weighted and masked decompositions
+    refw = oracle.lasso_weighted(X, D, Wt, 0.1).toarray()
+    assert np.abs(read_csc(str(tmp_path / "lasso_weighted.bin"), p) - refw).max() < 1e-9
+    Bu = B.astype(np.uint8)
+    refm = oracle.lasso_mask(X, D, Bu, 0.1).toarray()
+    assert np.abs(read_csc(str(tmp_path / "lasso_mask.bin"), p) - refm).max() < 1e-9
+    refom = oracle.omp_mask(X, D, Bu, L=8).toarray()
+    assert np.abs(read_csc(str(tmp_path / "omp_mask.bin"), p) - refom).max() < 1e-9
+    # FISTA (l1, fixed step L0 = ||D'D||, 60 iterations)
+    Wf = np.fromfile(str(tmp_path / "fista_W.bin"), np.float64).reshape(n, p).T
+    Wo, _ = oracle.fista_flat(X, D, np.zeros((p, n)), 0.05, regul="l1", max_it=60, L0=L0,
+                              fixed_step=True, tol=0.0, it0=100)
+    assert np.abs(Wf - Wo).max() < 1e-9
+    # cppLasso (L = 10), cppOMP (L = 8), lasso2 with D (L = 10)
+    refl = oracle.lasso(X, D, 0.15, L=10).toarray()
+    assert np.abs(read_csc(str(tmp_path / "cpp_lasso.bin"), p) - refl).max() < 1e-9
+    assert np.abs(read_csc(str(tmp_path / "lasso2_d.bin"), p) - refl).max() < 1e-9
+    assert np.abs(read_csc(str(tmp_path / "cpp_omp.bin"), p) - refo).max() < 1e-9
