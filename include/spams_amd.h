@@ -285,8 +285,9 @@ void lasso2(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, int L,
 }
 
 // lasso (decomp/decomp.h:111-116): the Cholesky-based variant of the
-// reference; spx runs its inverse-Gram homotopy (same solution) and, with
-// ols, refits each final active set by least squares.  L already normalised.
+// reference.  spx runs its inverse-Gram homotopy with coreLARS's step and
+// stop rules (decomp/decomp.h:844-1093) and, with ols, refits each final
+// active set by least squares.  L already normalised.
 template <typename T>
 void lasso(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, int L,
            const T constraint, const T lambda2 = 0, constraint_type mode = PENALTY,
@@ -294,7 +295,7 @@ void lasso(const Matrix<T>& X, const Matrix<T>& D, SpMatrix<T>& spalpha, int L,
            Matrix<T>* path = nullptr, const int length_path = -1) {
   (void)numThreads;
   spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, length_path, SPX_PREC_FP32, SPX_FORM_BATCHED, 0,
-                   ols ? 1 : 0};
+                   ols ? 1 : 0, 1};
   spx_csc out{};
   T* pp = spams_amd_detail::path_buffer(path, D.n(), L, length_path);
   spams_amd_detail::check(spams_amd_detail::lasso_call(X.rawX(), X.m(), X.n(), D.rawX(), D.n(),
@@ -331,7 +332,7 @@ void lasso(const Data<T>& X, const AbstractMatrix<T>& G, const AbstractMatrix<T>
   if (G.m() != G.n() || DtX.m() != G.n() || DtX.n() != X.n())
     throw "lasso : incompatible matrix dimensions";
   spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, length_path, SPX_PREC_FP32, SPX_FORM_BATCHED, 0,
-                   ols ? 1 : 0};
+                   ols ? 1 : 0, 1};
   spx_csc out{};
   T* pp = spams_amd_detail::path_buffer(path, G.n(), L, length_path);
   spams_amd_detail::check(spams_amd_detail::lasso_qq_call(X.rawX(), X.m(), X.n(), G.rawX(),
@@ -340,17 +341,16 @@ void lasso(const Data<T>& X, const AbstractMatrix<T>& G, const AbstractMatrix<T>
   spalpha.adopt(out);
 }
 
-// _lassoD (cpp_library/spams.h:107-143).  cholevsky selects the Cholesky
-// variant in the reference; both variants solve the same problem and spx
-// runs its single inverse-Gram homotopy for either.  ols returns the
-// least-squares fit on each final active set.
+// _lassoD (cpp_library/spams.h:107-143).  cholevsky selects the step and
+// stop rules of the Cholesky variant coreLARS (they differ from coreLARS2's in
+// L2ERROR mode); spx runs its single inverse-Gram homotopy for either.  ols
+// returns the least-squares fit on each final active set.
 template <typename T>
 SpMatrix<T>* _lassoD(Matrix<T>* X, Matrix<T>* D, Matrix<T>** path, bool return_reg_path, int L,
                      const T constraint, const T lambda2, constraint_type mode, const bool pos,
                      const bool ols, const int numThreads, int max_length_path,
                      const bool verbose, bool cholevsky) {
   (void)verbose;
-  (void)cholevsky;
   (void)numThreads;
   if (X->m() != D->m()) throw "lasso : incompatible matrix dimensions";
   if (L < 0) L = D->n();
@@ -359,7 +359,7 @@ SpMatrix<T>* _lassoD(Matrix<T>* X, Matrix<T>* D, Matrix<T>** path, bool return_r
   SpMatrix<T>* alpha = new SpMatrix<T>();
   try {
     spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, max_length_path, SPX_PREC_FP32,
-                     SPX_FORM_BATCHED, 0, ols ? 1 : 0};
+                     SPX_FORM_BATCHED, 0, ols ? 1 : 0, cholevsky ? 1 : 0};
     spx_csc out{};
     spams_amd_detail::check(spams_amd_detail::lasso_call(
         X->rawX(), X->m(), X->n(), D->rawX(), D->n(), &constraint, 1, lambda2, &o, &out,
@@ -383,7 +383,6 @@ SpMatrix<T>* _lassoQq(Matrix<T>* X, Matrix<T>* Q, Matrix<T>* q, Matrix<T>** path
                       const int numThreads, int max_length_path, const bool verbose,
                       bool cholevsky) {
   (void)verbose;
-  (void)cholevsky;
   (void)numThreads;
   if (Q->m() != Q->n()) throw "lasso : Q must be square";
   const int K = Q->m();
@@ -394,7 +393,7 @@ SpMatrix<T>* _lassoQq(Matrix<T>* X, Matrix<T>* Q, Matrix<T>* q, Matrix<T>** path
   SpMatrix<T>* alpha = new SpMatrix<T>();
   try {
     spx_lasso_opts o{L, (int)mode, pos ? 1 : 0, max_length_path, SPX_PREC_FP32,
-                     SPX_FORM_BATCHED, 0, ols ? 1 : 0};
+                     SPX_FORM_BATCHED, 0, ols ? 1 : 0, cholevsky ? 1 : 0};
     spx_csc out{};
     spams_amd_detail::check(spams_amd_detail::lasso_qq_call(
         X->rawX(), X->m(), X->n(), Q->rawX(), q->rawX(), K, &constraint, lambda2, &o, &out,

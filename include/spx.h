@@ -92,7 +92,12 @@ typedef struct {
   int waves_per_signal;/* 0 = autotuned; else 1, 2, 4 or 16                  */
   int ols;             /* 1: least-squares refit on the final active set (the
                         * facade's ols, decomp/decomp.h:1087-1091); the path
-                        * itself is the same homotopy */
+                        * itself is the same homotopy (not with L2ERROR) */
+  int cholesky;        /* 1: the path of the Cholesky variant coreLARS
+                        * (decomp/decomp.h:844-1093, _lassoD's cholesky): in
+                        * L2ERROR mode the last step is not cut at the
+                        * residual target; a step that would increase the
+                        * residual ends the path */
 } spx_lasso_opts;
 
 /* Options of spx_omp_* (the _omp facade's vector arguments are passed

@@ -98,11 +98,14 @@ def _csc_from_lists(vM, rM, L, n, p, dtype):
 # ------------------------------------------------------------------ restatement
 
 def lasso(X, D, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
-          max_length_path=-1, num_threads=-1, return_stats=False, return_reg_path=False):
+          max_length_path=-1, num_threads=-1, return_stats=False, return_reg_path=False,
+          cholesky=False):
     """_lassoD + lasso2 + coreLARS2 (restated).  ``lambda1`` may be a scalar
     or a length-n vector (one constraint per signal).  ``return_reg_path``:
     also the first signal's path (p x max_length_path), NaN where coreLARS2
-    never writes (the reference leaves those entries uninitialised)."""
+    never writes (the reference leaves those entries uninitialised).
+    ``cholesky``: the path of the Cholesky variant coreLARS (its L2ERROR step
+    and stop, its exit on a residual increase)."""
     dtype = X.dtype
     X = _f(X, dtype)
     D = _f(D, dtype)
@@ -126,9 +129,10 @@ def lasso(X, D, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
         path = np.full((p, mlp), np.nan, dtype, order="F")
     f = getattr(lib(), "orc_lasso_" + sfx)
     f.restype = _I
-    f.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, ct, _I, _I, _I, _I, _P, _P, _P, _P]
+    f.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, ct, _I, _I, _I, _I, _P, _P, _P, _P, _I]
     f(_ptr(X), m, n, _ptr(D), p, L, _ptr(lam), lam.size, lambda2, mode,
-      int(pos), max_length_path, num_threads, _ptr(vM), _ptr(rM), _ptr(stats), _ptr(path))
+      int(pos), max_length_path, num_threads, _ptr(vM), _ptr(rM), _ptr(stats), _ptr(path),
+      int(cholesky))
     out = _csc_from_lists(vM, rM, Le, n, p, dtype)
     res = (out,)
     if return_stats:
@@ -139,7 +143,7 @@ def lasso(X, D, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
 
 
 def lasso_qq(X, Q, q, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
-             max_length_path=-1, num_threads=-1):
+             max_length_path=-1, num_threads=-1, cholesky=False):
     """_lassoQq + lasso2 + coreLARS2 (restated): Gram Q used as given."""
     dtype = X.dtype
     X = _f(X, dtype)
@@ -160,9 +164,9 @@ def lasso_qq(X, Q, q, lambda1, lambda2=0.0, L=-1, mode=PENALTY, pos=False,
     rM = np.full((Le, n), -1, np.int32, order="F")
     f = getattr(lib(), "orc_lasso_qq_" + sfx)
     f.restype = _I
-    f.argtypes = [_P, _I, _I, _P, _P, _I, _I, _P, _I, ct, _I, _I, _I, _I, _P, _P, _P]
+    f.argtypes = [_P, _I, _I, _P, _P, _I, _I, _P, _I, ct, _I, _I, _I, _I, _P, _P, _P, _I]
     f(_ptr(X), m, n, _ptr(Q), _ptr(q), p, L, _ptr(lam), lam.size, lambda2, mode,
-      int(pos), max_length_path, num_threads, _ptr(vM), _ptr(rM), None)
+      int(pos), max_length_path, num_threads, _ptr(vM), _ptr(rM), None, int(cholesky))
     return _csc_from_lists(vM, rM, Le, n, p, dtype)
 
 

@@ -339,6 +339,17 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   a.stats = out->stats;
   a.refit = (o->precision & SPX_PREC_REFIT64) ? 1 : 0;
   a.ols = o->ols ? 1 : 0;
+  // cholesky selects the path of coreLARS (decomp/decomp.h:844-1093); with
+  // ols the reference tracks the least-squares residual instead, which only
+  // the L2ERROR stop reads
+  if (o->ols && o->mode == SPX_L2ERROR) {
+    set_error("lasso: ols with mode L2ERROR (the least-squares residual stop of the "
+              "Cholesky variant) is not provided");
+    return SPX_ERR_UNSUP;
+  }
+  // the weighted and masked solvers have no Cholesky variant (coreLARS2W,
+  // coreLARS2 with mask): the option is read by the plain solver only
+  a.chol = (o->cholesky && !o->ols && !W && !mi) ? 1 : 0;
   if (out->path && out->path_cols > 0) {
     a.path = static_cast<T*>(out->path);
     a.path_cols = out->path_cols;

@@ -751,6 +751,9 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           const T delta = coeff2 * coeff2 - coeff1 * coeff3;
           step_max2 = delta < 0 ? INF : (coeff2 - sqrt(delta)) / coeff1;
           if (!wt) step_max2 = step_max2 < cc ? step_max2 : cc;
+          // the Cholesky variant (coreLARS) takes the whole step past the
+          // residual target (decomp/decomp.h:997-1033)
+          if (a.chol) step_max2 = INF;
         } else {
           step_max2 = coeff1 < 0 ? INF : (lam - thrs) / coeff1;
           if (!wt) step_max2 = step_max2 < cc ? step_max2 : cc;
@@ -758,6 +761,16 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         step = step < step_max2 ? step : step_max2;
         step = step < step_max ? step : step_max;
         if (step == INF) break;
+        if (a.chol) {
+          // Cholesky variant: a step that would increase the residual ends
+          // the path and drops the entry at the current position
+          // (decomp/decomp.h:1036-1043)
+          const T t = coeff1 * step * step - 2 * coeff2 * step;
+          if (t > T(0) || !(fabs(t) < INF)) {
+            s_active = i;
+            break;
+          }
+        }
         reached = (a.mode == SPX_PENALTY) && (step == step_max2);
 
         // -------------------------------------------------- updates
@@ -847,7 +860,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         const T tol = a.tol;
         if ((iter >= a.length_path - 1) || (fabs(step) < tol && !tie) ||
             step == step_max2 || (normX < tol) || (i == (Lm - 1)) ||
-            (a.mode == SPX_L2ERROR && normX - lam < tol) ||
+            (a.mode == SPX_L2ERROR && (newAtom || !a.chol) && normX - lam < tol) ||
             (a.mode == SPX_L1COEFFS && (lam - thrs < tol)))
           break;
       }

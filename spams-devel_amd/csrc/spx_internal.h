@@ -61,6 +61,10 @@ struct LarsArgs {
   const T* weights;
   int ldw;
   T tol;            // stop tolerance of |step|, normX and the mode margins
+  // Cholesky variant (coreLARS, decomp/decomp.h:844-1093): no residual cap
+  // on the L2ERROR step, its stop tested after admissions only, and an exit
+  // on a residual increase
+  int chol;
   // Masked signals (lasso_mask): per-signal count of observed rows (0: empty
   // column; m: the plain path), per signal the rows its Gram columns visit
   // (m x n, mask_gram.h) and the dictionary (D m x p, Dt p x m) from which a

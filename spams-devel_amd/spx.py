@@ -43,7 +43,8 @@ class spx_csc(ctypes.Structure):
 
 class spx_lasso_opts(ctypes.Structure):
     _fields_ = [("L", _I), ("mode", _I), ("pos", _I), ("max_length_path", _I),
-                ("precision", _I), ("form", _I), ("waves_per_signal", _I), ("ols", _I)]
+                ("precision", _I), ("form", _I), ("waves_per_signal", _I), ("ols", _I),
+                ("cholesky", _I)]
 
 
 class spx_omp_opts(ctypes.Structure):
@@ -307,10 +308,13 @@ def lasso(X, D=None, Q=None, q=None, return_reg_path=False, L=-1,
     """LARS-Lasso (spams.lasso, swig/python/spams.py:109-142).
 
     ``lambda1`` is the reference's scalar constraint, or a length-n vector
-    (one constraint per signal).  ``cholesky=True`` solves the same problem
-    with the same homotopy (the engine keeps an explicit inverse of the active
-    Gram block); ``ols=True`` returns the least-squares fit on each final
-    active set.  With ``Q``/``q`` (Gram matrix and correlations) the Gram is
+    (one constraint per signal).  ``cholesky=True`` follows the path of the
+    reference's Cholesky variant coreLARS (decomp/decomp.h:844-1093): in
+    L2ERROR mode its last step is not cut at the residual target, and a step
+    that would increase the residual ends the path (the engine keeps an
+    explicit inverse of the active Gram block either way); ``ols=True``
+    returns the least-squares fit on each final active set (not in L2ERROR
+    mode).  With ``Q``/``q`` (Gram matrix and correlations) the Gram is
     used as given.  ``return_reg_path`` also returns the first signal's path
     (p x max_length_path).  ``numThreads`` is accepted and ignored."""
     if Q is not None:
@@ -341,7 +345,8 @@ def lasso(X, D=None, Q=None, q=None, return_reg_path=False, L=-1,
     if lam.size not in (1, n):
         raise ValueError("lasso : lambda1 must be a scalar or have one entry per signal")
     o = spx_lasso_opts(L, mode, int(pos), max_length_path, PREC[precision],
-                       FORM[form], waves_per_signal, int(bool(ols)))
+                       FORM[form], waves_per_signal, int(bool(ols)),
+                       int(bool(cholesky)))
     path = None
     if return_reg_path:
         mlp = max_length_path if max_length_path >= 0 else 4 * (p if L < 0 else L)

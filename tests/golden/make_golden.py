@@ -53,6 +53,15 @@ def main():
                 cases.append((name, dict(kind="lasso", X=X, D=D, lam=np.asarray(lam, dt),
                                          lambda2=np.asarray(l2, dt), mode=mode, pos=pos,
                                          pB=pB, r=r, v=v)))
+        # the Cholesky variant (cholesky=True: coreLARS), whose L2ERROR step
+        # and stop differ from coreLARS2's
+        X, D = small(41, 64, 256, 40, 5, 0.05, True, dt)
+        for mode, lam, pos in ((1, 0.3, 0), (1, 0.05, 0), (1, 0.2, 1), (2, 0.1, 0), (0, 0.8, 0)):
+            A = oracle.ref_lasso(X, D, lam, mode=mode, pos=bool(pos), cholesky=True)
+            pB, r, v = csc_parts(A)
+            cases.append((f"lassochol_{tag}_mode{mode}_lam{lam}_pos{pos}", dict(
+                kind="lasso", X=X, D=D, lam=np.asarray(lam, dt), lambda2=np.asarray(0.0, dt),
+                mode=mode, pos=pos, cholesky=1, pB=pB, r=r, v=v)))
         # per-signal lambda
         X, D = small(13, 64, 256, 40, 5, 0.05, True, dt)
         lam = np.exp(np.random.default_rng(14).uniform(np.log(0.05), np.log(0.5), 40)).astype(dt)

@@ -437,6 +437,35 @@ def test_lasso_qq_matches_golden(spx, name):
     assert np.abs(A - ref).max() <= tol
 
 
+@pytest.mark.parametrize("name", golden_files("lassochol_"))
+def test_lasso_cholesky_matches_golden(spx, name):
+    """cholesky=True: the step and stop rules of coreLARS (FP64 1e-9, FP32
+    2e-4 on the signals whose supports agree, 90 % of them at least)."""
+    print("case", name, flush=True)
+    g = load_golden(name)
+    A = dense(spx.lasso(g["X"], g["D"], lambda1=g["lam"].item(), mode=int(g["mode"]),
+                        pos=bool(g["pos"]), cholesky=True))
+    ref = np.zeros_like(A)
+    for j in range(ref.shape[1]):
+        ref[g["r"][g["pB"][j]:g["pB"][j + 1]], j] = g["v"][g["pB"][j]:g["pB"][j + 1]]
+    if g["X"].dtype == np.float64:
+        assert np.array_equal(A != 0, ref != 0)
+        np.testing.assert_allclose(A, ref, atol=1e-9)
+    else:
+        same = np.all((A != 0) == (ref != 0), axis=0)
+        assert same.mean() >= 0.9
+        assert np.abs(A[:, same] - ref[:, same]).max(initial=0) <= 2e-4
+
+
+def test_lasso_ols_with_l2error_is_refused(spx):
+    """ols in L2ERROR mode stops on the least-squares residual in the
+    reference; the engine refuses it rather than return another path."""
+    print("case lasso ols l2error refused", flush=True)
+    g = load_golden("lassools_f64.npz")
+    with pytest.raises(RuntimeError):
+        spx.lasso(g["X"], g["D"], lambda1=0.3, mode=1, ols=True)
+
+
 def test_lasso_ols_matches_golden_f64(spx):
     print("case lasso ols f64", flush=True)
     g = load_golden("lassools_f64.npz")

@@ -320,6 +320,62 @@ def test_reference_ols_is_least_squares_on_the_lasso_support(orc):
     assert np.abs(base - ols).max() > 1e-3
 
 
+@pytest.mark.parametrize("name", golden_files("lassochol_"))
+def test_lasso_cholesky_matches_golden(orc, name):
+    """cholesky=True (coreLARS, decomp/decomp.h:844-1093): the restatement
+    keeps an explicit inverse where the reference updates a Cholesky factor,
+    so agreement is to rounding: FP64 same supports and 1e-12; FP32 (where
+    an atom at rounding distance from a tie may enter or not) same supports
+    on 95 % of the signals and 2e-5 on those."""
+    g = load_golden(name)
+    A = orc.lasso(g["X"], g["D"], g["lam"].item(), lambda2=g["lambda2"].item(),
+                  mode=int(g["mode"]), pos=bool(g["pos"]), cholesky=True).toarray()
+    import scipy.sparse as sp
+    ref = sp.csc_matrix((g["v"], g["r"], g["pB"]),
+                        shape=(g["D"].shape[1], g["X"].shape[1])).toarray()
+    if g["X"].dtype == np.float64:
+        assert np.array_equal(A != 0, ref != 0)
+        np.testing.assert_allclose(A, ref, atol=1e-12)
+    else:
+        same = np.all((A != 0) == (ref != 0), axis=0)
+        assert same.mean() >= 0.95
+        assert np.abs(A[:, same] - ref[:, same]).max() <= 2e-5
+
+
+def _breakpoint_gaps(X, D, A, pos):
+    """(c_active - max inactive |correlation|) / c_active per signal."""
+    gaps = []
+    for j in range(X.shape[1]):
+        c = D.T @ (X[:, j] - D @ A[:, j])
+        if pos:
+            c = np.maximum(c, 0)
+        S = A[:, j] != 0
+        ca = np.abs(c[S]).mean()
+        gaps.append((ca - np.abs(c[~S]).max()) / ca)
+    return np.asarray(gaps)
+
+
+@pytest.mark.parametrize("name", [f for f in golden_files("lassochol_") if "f64_mode1" in f])
+def test_cholesky_l2error_ends_at_a_breakpoint_below_the_target(orc, name):
+    """In L2ERROR mode coreLARS takes whole homotopy steps (no cut at the
+    residual target, decomp/decomp.h:997-1033) and tests the target after
+    admissions (:1076): every signal ends at a breakpoint of the path (the
+    next atom's correlation ties the active ones) with its residual below
+    lambda.  coreLARS2's cut step (the planted mistake: the rule of the other
+    variant) lands on the target between breakpoints and fails the check."""
+    g = load_golden(name)
+    X, D, lam, pos = g["X"], g["D"], g["lam"].item(), bool(g["pos"])
+    import scipy.sparse as sp
+    ref = sp.csc_matrix((g["v"], g["r"], g["pB"]), shape=(D.shape[1], X.shape[1])).toarray()
+    for A in (orc.lasso(X, D, lam, mode=1, pos=pos, cholesky=True).toarray(), ref):
+        res = ((X - D @ A) ** 2).sum(0)
+        assert (res <= lam + 1e-12).all()
+        assert np.abs(_breakpoint_gaps(X, D, A, pos)).max() < 1e-8
+    B = orc.lasso(X, D, lam, mode=1, pos=pos).toarray()
+    np.testing.assert_allclose(((X - D @ B) ** 2).sum(0), lam, atol=1e-9)
+    assert _breakpoint_gaps(X, D, B, pos).min() > 1e-5
+
+
 @pytest.mark.parametrize("name", golden_files("omppath_"))
 def test_omp_path_fixture_is_consistent(orc, name):
     """The reference's OMP path (decomp/decomp.h:605-613, :634-639): the
