@@ -305,7 +305,14 @@ def test_autotuner_stores_a_width_that_batched_calls_use(spx, orc):
     assert d.waves("lasso") == 1  # size rule before tuning (p=1024 FP32)
     w, trial = d.autotune_lasso(Xt, lt)
     assert w in (1, 2, 4, 16) and set(trial) == {1, 2, 4, 16} and w in trial
-    assert trial[w] == min(trial.values())
+    # the rule: walk the widths in ascending order, a wider one replaces the
+    # current choice only when more than 3 % faster
+    exp = None
+    for k in sorted(trial):
+        if exp is None or trial[k] < 0.97 * trial[exp]:
+            exp = k
+    assert w == exp
+    assert trial[w] <= min(trial.values()) / 0.97 ** 3
     assert d.waves("lasso") == w
     n, Lout = X.shape[1], spx.lasso_L(256, 1024, -1, spx.PENALTY, float(lam.max()))
 
