@@ -281,7 +281,7 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 }
 
 template <bool VECLOAD>
-__global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
+__global__ __launch_bounds__(256, 2) void gemm_tn_bf16x3_kernel(
     int M, int N, int K, const float* __restrict__ A, int lda,
     const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
     float alpha, float beta, const float* __restrict__ E, int lde, int kz, size_t zstride) {
@@ -294,8 +294,9 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
   constexpr int BK = 32;
   constexpr int LD = BK + 8;  // [row][k] layout (k contiguous), padded
   typedef float f4 __attribute__((ext_vector_type(4)));
-  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM * LD], Al[BM * LD];
-  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * LD], Bl[BN * LD];
+  // two buffers (the registers fill the idle one after the multiply)
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[2][BM * LD], Al[2][BM * LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[2][BN * LD], Bl[2][BN * LD];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -315,15 +316,20 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
   // loader: thread t loads 16 consecutive k of row (t >> 1)
   const int lr = tid >> 1, lk = (tid & 1) * 16;
   f4 ra[4], rb[4];
+  // whole chunks take the vector loads (one uniform branch per chunk)
   auto load = [&](int k0) {
     const int i = bm + lr, j = bn + lr;
+    if (VECLOAD && k0 + BK <= K) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int k = k0 + lk + 4 * v;
-      if (VECLOAD && k + 3 < K) {
+      for (int v = 0; v < 4; ++v) {
+        const int k = k0 + lk + 4 * v;
         ra[v] = i < M ? *reinterpret_cast<const f4*>(A + (size_t)i * lda + k) : f4(0);
         rb[v] = j < N ? *reinterpret_cast<const f4*>(B + (size_t)j * ldb + k) : f4(0);
-      } else {
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k = k0 + lk + 4 * v;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           ra[v][e] = (i < M && k + e < K) ? A[(size_t)i * lda + k + e] : 0.f;
@@ -332,7 +338,7 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
       }
     }
   };
-  auto store = [&]() {
+  auto store = [&](int sb) {
     bf16x8 h[2], l[2];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
@@ -343,10 +349,10 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
         h[v >> 1][(v & 1) * 4 + e] = hh;
         l[v >> 1][(v & 1) * 4 + e] = ll;
       }
-    *reinterpret_cast<bf16x8*>(&Ah[lr * LD + lk]) = h[0];
-    *reinterpret_cast<bf16x8*>(&Ah[lr * LD + lk + 8]) = h[1];
-    *reinterpret_cast<bf16x8*>(&Al[lr * LD + lk]) = l[0];
-    *reinterpret_cast<bf16x8*>(&Al[lr * LD + lk + 8]) = l[1];
+    *reinterpret_cast<bf16x8*>(&Ah[sb][lr * LD + lk]) = h[0];
+    *reinterpret_cast<bf16x8*>(&Ah[sb][lr * LD + lk + 8]) = h[1];
+    *reinterpret_cast<bf16x8*>(&Al[sb][lr * LD + lk]) = l[0];
+    *reinterpret_cast<bf16x8*>(&Al[sb][lr * LD + lk + 8]) = l[1];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
@@ -356,30 +362,32 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
         h[v >> 1][(v & 1) * 4 + e] = hh;
         l[v >> 1][(v & 1) * 4 + e] = ll;
       }
-    *reinterpret_cast<bf16x8*>(&Bh[lr * LD + lk]) = h[0];
-    *reinterpret_cast<bf16x8*>(&Bh[lr * LD + lk + 8]) = h[1];
-    *reinterpret_cast<bf16x8*>(&Bl[lr * LD + lk]) = l[0];
-    *reinterpret_cast<bf16x8*>(&Bl[lr * LD + lk + 8]) = l[1];
+    *reinterpret_cast<bf16x8*>(&Bh[sb][lr * LD + lk]) = h[0];
+    *reinterpret_cast<bf16x8*>(&Bh[sb][lr * LD + lk + 8]) = h[1];
+    *reinterpret_cast<bf16x8*>(&Bl[sb][lr * LD + lk]) = l[0];
+    *reinterpret_cast<bf16x8*>(&Bl[sb][lr * LD + lk + 8]) = l[1];
   };
 
-  load(0);
   // 16x16x32 operand layout: lane l holds row (l % 16), k = 8*(l/16)+0..7
   const int ko = 8 * (lane >> 4);
+  load(0);
+  store(0);
+  __syncthreads();
+  int cur = 0;
   for (int k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
-    store();
-    __syncthreads();
-    if (k0 + BK < K) load(k0 + BK);
+    const bool more = k0 + BK < K;
+    if (more) load(k0 + BK);
     bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
       const int ra_ = (wm + 16 * x + (lane & 15)) * LD + ko;
       const int rb_ = (wn + 16 * x + (lane & 15)) * LD + ko;
-      ah[x] = *reinterpret_cast<const bf16x8*>(&Ah[ra_]);
-      al[x] = *reinterpret_cast<const bf16x8*>(&Al[ra_]);
-      bh[x] = *reinterpret_cast<const bf16x8*>(&Bh[rb_]);
-      bl[x] = *reinterpret_cast<const bf16x8*>(&Bl[rb_]);
+      ah[x] = *reinterpret_cast<const bf16x8*>(&Ah[cur][ra_]);
+      al[x] = *reinterpret_cast<const bf16x8*>(&Al[cur][ra_]);
+      bh[x] = *reinterpret_cast<const bf16x8*>(&Bh[cur][rb_]);
+      bl[x] = *reinterpret_cast<const bf16x8*>(&Bl[cur][rb_]);
     }
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -388,6 +396,11 @@ __global__ __launch_bounds__(256) void gemm_tn_bf16x3_kernel(
         acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
         acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
       }
+    __builtin_amdgcn_s_setprio(0);
+    // buffer cur ^ 1 was last read before the previous barrier
+    if (more) store(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
   }
 #pragma unroll
   for (int x = 0; x < 4; ++x)
