@@ -304,18 +304,26 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
   const int prec = o->precision;
 
   // workspace: y, grad (p x n); R (m x n); z, diff (p x n, backtracking only);
-  // per-signal L, active, pending, best dual, stop sums
+  // per-signal L, active, pending, best dual, stop sums; the slices of the
+  // two products when a small batch leaves most of the chip idle (the inner
+  // dimension is then split, gemm_tn_product)
   const bool bt = !o->fixed_step;
   const size_t pn = (size_t)p * n, mn = (size_t)m * n;
-  const size_t need = sizeof(T) * (2 * pn + mn + (bt ? 2 * pn : 0) + 4 * (size_t)n) +
-                      sizeof(int) * (2 * (size_t)n + 4) + 256;
+  const int sR = gemm_small_split(m, n, p), sG = gemm_small_split(p, n, m);
+  const size_t npart = std::max(sR > 1 ? (size_t)sR * mn : 0, sG > 1 ? (size_t)sG * pn : 0);
+  // every piece starts 256-byte aligned (take below)
+  auto padded = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+  const size_t need = padded(sizeof(T) * pn) * (bt ? 4 : 2) + padded(sizeof(T) * mn) +
+                      padded(sizeof(T) * n) * 2 + padded(sizeof(T) * 2 * n) +
+                      padded(sizeof(int) * n) * 2 + padded(sizeof(int) * 4) +
+                      padded(sizeof(T) * npart);
   void* ws = nullptr;
   int rc = ctx->scratch_get(7, need, &ws);
   if (rc) return rc;
   char* cur = static_cast<char*>(ws);
   auto take = [&](size_t bytes) {
     char* r = cur;
-    cur += (bytes + 255) & ~size_t(255);
+    cur += padded(bytes);
     return r;
   };
   T* y = reinterpret_cast<T*>(take(sizeof(T) * pn));
@@ -329,6 +337,7 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
   int* active = reinterpret_cast<int*>(take(sizeof(int) * n));
   int* pending = reinterpret_cast<int*>(take(sizeof(int) * n));
   int* npend = reinterpret_cast<int*>(take(sizeof(int) * 4));
+  T* part = npart ? reinterpret_cast<T*>(take(sizeof(T) * npart)) : nullptr;
   (void)cur;
 
   ProxParams<T> pp;
@@ -351,16 +360,20 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
 
     // gradient of the square loss at `at`: R = D*at - X, grad = D'R
     auto gradient = [&](const T* at) -> int {
-      int r = gemm_tn<T>(qs, m, n, p, Dt, p, at, p, R, m, T(1), T(-1), X, m, prec);
+      int r = gemm_tn_product<T>(qs, m, n, p, Dt, p, at, p, R, m, T(1), T(-1), X, m, part, sR,
+                                 prec);
       if (r) return r;
-      return gemm_tn<T>(qs, p, n, m, D, m, R, m, grad, p, T(1), T(0), nullptr, 0, prec);
+      return gemm_tn_product<T>(qs, p, n, m, D, m, R, m, grad, p, T(1), T(0), nullptr, 0, part,
+                                sG, prec);
     };
     // objective / gap evaluation at W
     auto evaluate = [&](int mode, int it) -> int {
-      int r = gemm_tn<T>(qs, m, n, p, Dt, p, W, p, R, m, T(1), T(-1), X, m, prec);
+      int r = gemm_tn_product<T>(qs, m, n, p, Dt, p, W, p, R, m, T(1), T(-1), X, m, part, sR,
+                                 prec);
       if (r) return r;
       if (duality) {
-        r = gemm_tn<T>(qs, p, n, m, D, m, R, m, grad, p, T(1), T(0), nullptr, 0, prec);
+        r = gemm_tn_product<T>(qs, p, n, m, D, m, R, m, grad, p, T(1), T(0), nullptr, 0, part,
+                               sG, prec);
         if (r) return r;
       }
       hipLaunchKernelGGL(eval_kernel<T>, cols4(n), dim3(256), 0, qs, m, p, n, X,
@@ -396,7 +409,8 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
           hipLaunchKernelGGL(prox_trial_kernel<T>, cols4(n), dim3(256), 0, qs, p,
                              n, base, grad, Lsig, pending, pp, zb, diff, stopsum);
           SPX_HIP(hipGetLastError());
-          rc2 = gemm_tn<T>(qs, m, n, p, Dt, p, diff, p, R, m, T(1), T(0), nullptr, 0, prec);
+          rc2 = gemm_tn_product<T>(qs, m, n, p, Dt, p, diff, p, R, m, T(1), T(0), nullptr, 0,
+                                   part, sR, prec);
           if (rc2) return rc2;
           hipLaunchKernelGGL(zero_one_kernel, dim3(1), dim3(1), 0, qs, npend);
           hipLaunchKernelGGL(backtrack_check_kernel<T>, cols4(n), dim3(256), 0, qs,

@@ -488,22 +488,75 @@ __global__ void split_sum_kernel(const T* __restrict__ part, int S, size_t zstri
   C[e] = s;
 }
 
+// C(i, j) = alpha * sum over z of part[z](i, j) + beta * E(i, j), the slices
+// summed in order (part: M x N with leading dimension M per slice).
+template <typename T>
+__global__ void split_sum_epi_kernel(const T* __restrict__ part, int S, size_t zstride, int M,
+                                     long long n, T* __restrict__ C, int ldc, T alpha, T beta,
+                                     const T* __restrict__ E, int lde) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const long long j = e / M;
+  const int i = (int)(e - j * M);
+  T s = T(0);
+  for (int z = 0; z < S; ++z) s += part[(size_t)z * zstride + e];
+  T v = alpha * s;
+  if (E) v += beta * E[(size_t)i + (size_t)j * lde];
+  C[(size_t)i + (size_t)j * ldc] = v;
+}
+
 }  // namespace
+
+// Launches the slices of a split product into `part`; returns the slice
+// count used (whole 32-deep chunks per slice), 0 on a launch error.
+template <typename T>
+static int split_partials(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B,
+                          int ldb, T* part, int S, int precision);
 
 template <typename T>
 int gemm_tn_split(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
                   T* C, int accumulate, T* part, int S, int precision) {
   if (M <= 0 || N <= 0) return SPX_OK;
+  if ((N + BN - 1) / BN > 65535) {
+    set_error("gemm_tn_split: N=%d too wide", N);
+    return SPX_ERR_UNSUP;
+  }
+  S = split_partials<T>(st, M, N, K, A, lda, B, ldb, part, S, precision);
+  if (S == 0) return SPX_ERR_HIP;
+  const size_t zstride = (size_t)M * N;
+  const long long n = (long long)M * N;
+  hipLaunchKernelGGL(split_sum_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     static_cast<const T*>(part), S, zstride, n, C, accumulate);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+template <typename T>
+int gemm_tn_product(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B,
+                    int ldb, T* C, int ldc, T alpha, T beta, const T* E, int lde, T* part,
+                    int S, int precision) {
+  if (S <= 1 || !part || (N + BN - 1) / BN > 65535)
+    return gemm_tn<T>(st, M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, E, lde, precision);
+  if (M <= 0 || N <= 0) return SPX_OK;
+  S = split_partials<T>(st, M, N, K, A, lda, B, ldb, part, S, precision);
+  if (S == 0) return SPX_ERR_HIP;
+  const long long n = (long long)M * N;
+  hipLaunchKernelGGL(split_sum_epi_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     static_cast<const T*>(part), S, (size_t)M * N, M, n, C, ldc, alpha, beta, E,
+                     lde);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+template <typename T>
+static int split_partials(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B,
+                          int ldb, T* part, int S, int precision) {
   const int mt = (M + BM - 1) / BM, nt = (N + BN - 1) / BN;
   if (S < 1) S = 1;
   int kz = (K + S - 1) / S;
   kz = (kz + 31) / 32 * 32;  // whole 32-deep chunks; 16-byte aligned slice starts
   S = (K + kz - 1) / kz;
   if (S < 1) S = 1;
-  if (nt > 65535) {
-    set_error("gemm_tn_split: N=%d too wide", N);
-    return SPX_ERR_UNSUP;
-  }
   const size_t zstride = (size_t)M * N;
   dim3 grid(mt, nt, S);
   const bool vec = (lda % 4 == 0) && (ldb % 4 == 0) &&
@@ -533,14 +586,12 @@ int gemm_tn_split(hipStream_t st, int M, int N, int K, const T* A, int lda, cons
                          B, ldb, part, M, T(1), T(0), static_cast<const T*>(nullptr), 0, kz,
                          zstride);
   }
-  SPX_HIP(hipGetLastError());
-  {
-    const long long n = (long long)M * N;
-    hipLaunchKernelGGL(split_sum_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       static_cast<const T*>(part), S, zstride, n, C, accumulate);
-    SPX_HIP(hipGetLastError());
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("gemm split launch failed: %s", hipGetErrorString(e));
+    return 0;
   }
-  return SPX_OK;
+  return S;
 }
 
 // Slices for gemm_tn_split: enough workgroups to fill the chip (about 512
@@ -551,6 +602,13 @@ int gemm_split_count(int M, int N, int K) {
   const int maxS = K / 256 > 1 ? K / 256 : 1;
   if (S > maxS) S = maxS;
   return S < 1 ? 1 : S;
+}
+
+// Slices for a product that leaves most of the chip idle (fewer than 128
+// output tiles, e.g. a small batch of signals); 1 otherwise.
+int gemm_small_split(int M, int N, int K) {
+  const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  return tiles < 128 ? gemm_split_count(M, N, K) : 1;
 }
 
 template <typename T>
@@ -572,6 +630,12 @@ template int col_sqnorms<float>(hipStream_t, int, int, const float*, int,
                                 float*);
 template int gemm_tn_split<float>(hipStream_t, int, int, int, const float*, int, const float*,
                                   int, float*, int, float*, int, int);
+template int gemm_tn_product<float>(hipStream_t, int, int, int, const float*, int, const float*,
+                                    int, float*, int, float, float, const float*, int, float*,
+                                    int, int);
+template int gemm_tn_product<double>(hipStream_t, int, int, int, const double*, int,
+                                     const double*, int, double*, int, double, double,
+                                     const double*, int, double*, int, int);
 template int gemm_tn_split<double>(hipStream_t, int, int, int, const double*, int,
                                    const double*, int, double*, int, double*, int, int);
 template int col_sqnorms<double>(hipStream_t, int, int, const double*, int,

@@ -126,8 +126,17 @@ int gemm_tn(hipStream_t st, int M, int N, int K, const T* A, int lda,
 template <typename T>
 int gemm_tn_split(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
                   T* C, int accumulate, T* part, int S, int precision);
+// gemm_tn's C = alpha A'B + beta E (C leading dimension ldc), split over the
+// inner dimension into S slices when S > 1 (`part`: S x M x N elements), the
+// slices summed in order; S <= 1 is gemm_tn itself.
+template <typename T>
+int gemm_tn_product(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B,
+                    int ldb, T* C, int ldc, T alpha, T beta, const T* E, int lde, T* part,
+                    int S, int precision);
 // Slices that fill the chip for an M x N product with inner dimension K.
 int gemm_split_count(int M, int N, int K);
+// gemm_split_count for products with fewer than 128 output tiles, else 1.
+int gemm_small_split(int M, int N, int K);
 template <typename T>
 int col_sqnorms(hipStream_t st, int m, int n, const T* X, int ldx, T* out);
 template <typename T>

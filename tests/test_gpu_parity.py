@@ -247,6 +247,26 @@ def test_fista_f32_against_oracle(spx, orc):
         np.testing.assert_allclose(ig[0], io[0], rtol=1e-5)
 
 
+@pytest.mark.parametrize("fixed", [True, False])
+def test_fista_small_batch_split_products_match_oracle(spx, orc, fixed):
+    """A small batch leaves most of the chip idle, so the products split
+    their inner dimension (m=128, p=1024, n=20: D alpha - X in 4 slices,
+    summed in order); FP64 results equal the oracle's to 1e-9, fixed step
+    and backtracking (whose trial product splits too)."""
+    print("case fista split products fixed=%d" % fixed, flush=True)
+    X, D = rand_problem(10, 128, 1024, 20, np.float64, k=8, noise=0.02, norm=False)
+    L0 = float(np.linalg.eigvalsh(D.T @ D)[-1]) if fixed else 0.1
+    W0 = np.zeros((1024, 20), np.float64, order="F")
+    for regul in ("l1", "elastic-net"):
+        kw = dict(regul=regul, lambda2=0.01, max_it=60, L0=L0, fixed_step=fixed, tol=0.0,
+                  it0=1000)
+        Wg, ig = spx.fistaFlat(X, D, W0, return_optim_info=True, loss="square",
+                               lambda1=0.05, **kw)
+        Wo, io = orc.fista_flat(X, D, W0, 0.05, **kw)
+        np.testing.assert_allclose(Wg, Wo, atol=1e-9)
+        np.testing.assert_allclose(ig[0], io[0], rtol=1e-9)
+
+
 # -------------------------------------------------------------- precision
 
 def test_refit64_moves_fp32_results_to_the_fp64_solution(spx, orc):
