@@ -394,14 +394,17 @@ def test_launch_timers_count_solver_launches(spx):
     d.close()
 
 
-def test_fista_graph_replays_equal_eager(spx, orc):
+@pytest.mark.parametrize("shape", [(64, 256, 48), (128, 1024, 24)])
+def test_fista_graph_replays_equal_eager(spx, orc, shape):
     """The fixed-step loop captured as a hipGraph: every replay (with eager
     engine and torch work between the replays, on changed inputs) equals the
-    eager launches bit for bit, and the eager result matches the oracle."""
+    eager launches bit for bit, and the eager result matches the oracle.  At
+    m=128, p=1024 the small batch splits the products' inner dimension, so
+    the graph holds the slice launches and their ordered sums."""
     import torch
-    print("case fista graph", flush=True)
+    print("case fista graph", shape, flush=True)
     ctx = gpu_ctx(spx)
-    X, D = rand_problem(21, 64, 256, 48, np.float32, k=8, noise=0.02, norm=False)
+    X, D = rand_problem(21, *shape, np.float32, k=8, noise=0.02, norm=False)
     L0 = float(np.linalg.eigvalsh(D.astype(np.float64).T @ D)[-1])
     d = spx.Dictionary(ctx, D)
     n, p = X.shape[1], D.shape[1]
