@@ -125,12 +125,11 @@ __device__ __forceinline__ void store_block(const V& acc, int i0, int j0,
 // 16-byte (FP32) / two 16-byte (FP64) loads; the four steps together cover the
 // 16 k of the group.  LDS tiles are [row][k]
 // with a row stride of BK + 4 elements.  The next chunk is loaded into
-// registers while the current one is multiplied.  FP32 keeps two LDS buffers:
-// the registers go to the idle buffer after the multiply, one barrier per
-// chunk, and the MFMA block runs at raised wave priority so that the other
+// registers while the current one is multiplied.  Two LDS buffers: the
+// registers go to the idle buffer after the multiply, one barrier per chunk,
+// and the MFMA block runs at raised wave priority so that the other
 // workgroup's loads and LDS stores issue between its MFMAs (1.2-1.5x the
-// single-buffer form on m=512..2048, n=8192..200000 products).  FP64 keeps
-// one LDS buffer and two barriers per chunk.
+// single-buffer, two-barrier form on m=512..2048, n=8192..200000 products).
 
 template <typename T>
 struct Vec4;
@@ -143,9 +142,10 @@ struct Vec4<double> {
   typedef double type __attribute__((ext_vector_type(4)));
 };
 
-// FP32: two workgroups per CU (two 36 KiB LDS buffer pairs each).
+// Two workgroups per CU (two LDS buffer pairs of 36 KiB in FP32, 40 KiB in
+// FP64, each).
 template <typename T, bool VECLOAD>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void gemm_tn_kernel(
+__global__ __launch_bounds__(256, 2) void gemm_tn_kernel(
     int M, int N, int K, const T* __restrict__ A, int lda,
     const T* __restrict__ B, int ldb, T* __restrict__ C, int ldc, T alpha,
     T beta, const T* __restrict__ E, int lde, int kz, size_t zstride) {
@@ -160,10 +160,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void gemm_tn_kernel(
   constexpr int BK = sizeof(T) == 4 ? 32 : 16;
   constexpr int LD = BK + 4;  // LDS row stride (elements)
   constexpr int NV = BK / 16;  // float4 (double4) loads per row half and operand
-  constexpr int ST = sizeof(T) == 4 ? 2 : 1;  // LDS buffers
   typedef typename Vec4<T>::type V4;
-  __shared__ __attribute__((aligned(16))) T As[ST][BM * LD];
-  __shared__ __attribute__((aligned(16))) T Bs[ST][BN * LD];
+  __shared__ __attribute__((aligned(16))) T As[2][BM * LD];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BN * LD];
   typedef typename Acc<T>::type V;
 
   const int tid = threadIdx.x;
@@ -230,40 +229,30 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void gemm_tn_kernel(
         a4[x] = *reinterpret_cast<const V4*>(&As[sb][(wm + 16 * x + (lane & 15)) * LD + kq + 16 * v]);
         b4[x] = *reinterpret_cast<const V4*>(&Bs[sb][(wn + 16 * x + (lane & 15)) * LD + kq + 16 * v]);
       }
-      if constexpr (ST == 2) __builtin_amdgcn_s_setprio(1);
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
           for (int y = 0; y < 4; ++y) acc[x][y] = Acc<T>::mfma(a4[x][t], b4[y][t], acc[x][y]);
-      if constexpr (ST == 2) __builtin_amdgcn_s_setprio(0);
+      __builtin_amdgcn_s_setprio(0);
     }
   };
 
   load(0);
-  if constexpr (ST == 2) {
-    // buffer cur holds chunk k0; the registers fill buffer cur ^ 1, which
-    // every wave finished reading before the previous barrier
-    store(0);
+  // buffer cur holds chunk k0; the registers fill buffer cur ^ 1, which
+  // every wave finished reading before the previous barrier
+  store(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    const bool more = k0 + BK < K;
+    if (more) load(k0 + BK);
+    multiply(cur);
+    if (more) store(cur ^ 1);
     __syncthreads();
-    int cur = 0;
-    for (int k0 = 0; k0 < K; k0 += BK) {
-      const bool more = k0 + BK < K;
-      if (more) load(k0 + BK);
-      multiply(cur);
-      if (more) store(cur ^ 1);
-      __syncthreads();
-      cur ^= 1;
-    }
-  } else {
-    for (int k0 = 0; k0 < K; k0 += BK) {
-      __syncthreads();
-      store(0);
-      __syncthreads();
-      if (k0 + BK < K) load(k0 + BK);
-      multiply(0);
-    }
+    cur ^= 1;
   }
 #pragma unroll
   for (int x = 0; x < 4; ++x)
