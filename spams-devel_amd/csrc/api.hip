@@ -64,17 +64,39 @@ __global__ void transpose_kernel(const T* __restrict__ A, int m, int n,
   }
 }
 
-// y = scale * G x for a p x p column-major G (leading dimension ld),
-// one thread per row.
+// part[s][i] = sum over the columns j of slice s of G(i, j) x(j), for a p x p
+// column-major G (leading dimension ld): block (b, s) takes rows 64b..64b+63
+// and the slice's columns, its four waves every fourth column, combined in
+// wave order (deterministic).
 template <typename T>
-__global__ __launch_bounds__(256) void gram_gemv_kernel(int p, const T* __restrict__ G,
-                                                        int ld, const T* __restrict__ x,
-                                                        const double* __restrict__ nrm,
-                                                        T* __restrict__ y) {
+__global__ __launch_bounds__(256) void gram_gemv_part_kernel(int p, int cs,
+                                                             const T* __restrict__ G, int ld,
+                                                             const T* __restrict__ x,
+                                                             double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + r;
+  const int c0 = blockIdx.y * cs;
+  const int c1 = c0 + cs < p ? c0 + cs : p;
+  double acc = 0;
+  if (i < p)
+    for (int j = c0 + q; j < c1; j += 4) acc += static_cast<double>(G[i + (size_t)j * ld]) * x[j];
+  red[q][r] = acc;
+  __syncthreads();
+  if (q == 0 && i < p)
+    part[(size_t)blockIdx.y * p + i] = ((red[0][r] + red[1][r]) + red[2][r]) + red[3][r];
+}
+
+// y = scale * (sum over the S slices of part, in slice order)
+template <typename T>
+__global__ __launch_bounds__(256) void gram_gemv_sum_kernel(int p, int S,
+                                                            const double* __restrict__ part,
+                                                            const double* __restrict__ nrm,
+                                                            T* __restrict__ y) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= p) return;
   double acc = 0;
-  for (int j = 0; j < p; ++j) acc += static_cast<double>(G[i + (size_t)j * ld]) * x[j];
+  for (int z = 0; z < S; ++z) acc += part[(size_t)z * p + i];
   const double sc = nrm ? 1.0 / *nrm : 1.0;
   y[i] = static_cast<T>(acc * sc);
 }
@@ -105,21 +127,31 @@ __global__ void fill_kernel(T* v, int n, T val) {
 template <typename T>
 static int lipschitz(spx_ctx* ctx, const spx_dict* d, int iters, double* L) {
   const int p = d->p;
+  // G x in column slices of at least 128 columns, at most 16 (the chip then
+  // holds (p / 64) x 16 workgroups)
+  int S = (p + 127) / 128;
+  if (S > 16) S = 16;
+  if (S < 1) S = 1;
+  const int cs = (p + S - 1) / S;
   void* b = nullptr;
-  int rc = ctx->scratch_get(12, 2 * sizeof(T) * (size_t)p + 64, &b);
+  int rc = ctx->scratch_get(12, 2 * sizeof(T) * (size_t)p + sizeof(double) * ((size_t)S * p + 1) + 64,
+                            &b);
   if (rc) return rc;
   T* x = static_cast<T*>(b);
   T* y = x + p;
   double* nrm = reinterpret_cast<double*>(
       (reinterpret_cast<uintptr_t>(y + p) + 15) & ~uintptr_t(15));
+  double* part = nrm + 1;
   hipStream_t st = ctx->stream;
   const dim3 g((p + 255) / 256);
   hipLaunchKernelGGL(fill_kernel<T>, g, dim3(256), 0, st, x, p, T(1));
   hipLaunchKernelGGL(norm2_kernel<T>, dim3(1), dim3(1024), 0, st, p, x, nrm);
   for (int it = 0; it < iters; ++it) {
     // x <- G (x / ||x||), then ||x|| -> the eigenvalue estimate
-    hipLaunchKernelGGL(gram_gemv_kernel<T>, g, dim3(256), 0, st, p,
-                       static_cast<const T*>(d->G), d->ldg, x, nrm, y);
+    hipLaunchKernelGGL(gram_gemv_part_kernel<T>, dim3((p + 63) / 64, S), dim3(256), 0, st, p,
+                       cs, static_cast<const T*>(d->G), d->ldg, x, part);
+    hipLaunchKernelGGL(gram_gemv_sum_kernel<T>, g, dim3(256), 0, st, p, S,
+                       static_cast<const double*>(part), nrm, y);
     hipLaunchKernelGGL(norm2_kernel<T>, dim3(1), dim3(1024), 0, st, p, y, nrm);
     T* t = x;
     x = y;

@@ -297,10 +297,13 @@ def test_bf16x3_lasso_within_stated_tolerance(spx, orc):
     assert np.all(np.abs(oa - ob) <= 1e-4 * np.abs(ob))
 
 
-def test_device_power_iteration_lipschitz(spx):
-    print("case lipschitz", flush=True)
+@pytest.mark.parametrize("mp", [(64, 200), (256, 2048)])
+def test_device_power_iteration_lipschitz(spx, mp):
+    """Largest eigenvalue of D'D by the device power iteration (G x in
+    column slices summed in order): within 1e-4 of numpy's."""
+    print("case lipschitz", mp, flush=True)
     rng = np.random.default_rng(81)
-    D = W.unit_dict(rng, 64, 200, np.float32)
+    D = W.unit_dict(rng, *mp, np.float32)
     ctx = gpu_ctx(spx)
     d = spx.Dictionary(ctx, D)
     L = d.lipschitz(300)
