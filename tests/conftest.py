@@ -33,6 +33,33 @@ def load_golden(name):
     return dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))
 
 
+def assert_csc_layout(A, pB, r, skip_cols=()):
+    """The engine's CSC arrays equal the reference's SpMatrix arrays
+    (linalg/linalg.h:4840-4862: column pointers pB, rows sorted ascending
+    inside each column) bit for bit.  ``skip_cols`` lists columns whose
+    reference entries are an artefact the engine does not reproduce (a
+    masked signal with no observed row); the arrays are then compared with
+    those columns taken out of both."""
+    A = A.tocsc() if not hasattr(A, "indptr") else A
+    ip = np.asarray(A.indptr, np.int64)
+    ix = np.asarray(A.indices, np.int64)
+    pB = np.asarray(pB, np.int64)
+    r = np.asarray(r, np.int64)
+    if skip_cols:
+        keep = np.ones(len(pB) - 1, bool)
+        keep[list(skip_cols)] = False
+
+        def strip(ptr, idx):
+            cnt = np.diff(ptr)[keep]
+            rows = np.concatenate([idx[ptr[j]:ptr[j + 1]] for j in np.flatnonzero(keep)]
+                                  or [np.zeros(0, np.int64)])
+            return np.concatenate([[0], np.cumsum(cnt)]), rows
+        ip, ix = strip(ip, ix)
+        pB, r = strip(pB, r)
+    assert ip.shape == pB.shape and np.array_equal(ip, pB), "column pointers differ"
+    assert ix.shape == r.shape and np.array_equal(ix, r), "row indices differ"
+
+
 def csc_equal(a, b):
     a = a.tocsc()
     b = b.tocsc()

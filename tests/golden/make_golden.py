@@ -103,6 +103,28 @@ def main():
                 lambda1=np.asarray(0.05, dt), lambda2=np.asarray(0.01, dt),
                 L0=np.asarray(L0, dt), fixed=fixed, max_it=200, it0=10,
                 tol=np.asarray(1e-10, dt))))
+        # FISTA exits and prox variants: an l1 run whose relative duality gap
+        # falls below tol at an it0 multiple (prox/fista.h:2981-2988), ISTA
+        # (no momentum), and the positivity-constrained prox (thrsPos)
+        L0fix = float(np.linalg.eigvalsh(D.astype(np.float64).T @ D)[-1])
+        variants = [("fistagap", "l1", fixed, dict(tol=1e-3, it0=10, max_it=500))
+                    for fixed in (1, 0)]
+        variants += [("fistaista", regul, fixed, dict(tol=1e-10, it0=10, max_it=200, ista=1))
+                     for regul in ("l1", "elastic-net") for fixed in (1, 0)]
+        variants += [("fistapos", regul, fixed, dict(tol=1e-10, it0=10, max_it=200, pos=1))
+                     for regul in ("l1", "elastic-net") for fixed in (1, 0)]
+        for fam, regul, fixed, kw in variants:
+            L0 = L0fix if fixed else 0.1
+            ista, pos = kw.get("ista", 0), kw.get("pos", 0)
+            Wt, info = oracle.ref_fista_flat(X, D, W0, 0.05, regul=regul, lambda2=0.01,
+                                             max_it=kw["max_it"], L0=L0,
+                                             fixed_step=bool(fixed), tol=kw["tol"],
+                                             it0=kw["it0"], ista=bool(ista), pos=bool(pos))
+            cases.append((f"{fam}_{tag}_{regul}_fixed{fixed}", dict(
+                kind="fista", X=X, D=D, W0=W0, W=Wt, info=info, regul=regul,
+                lambda1=np.asarray(0.05, dt), lambda2=np.asarray(0.01, dt),
+                L0=np.asarray(L0, dt), fixed=fixed, max_it=kw["max_it"], it0=kw["it0"],
+                tol=np.asarray(kw["tol"], dt), ista=ista, pos=pos)))
         # regularisation paths, Gram-given form, least-squares refit
         X, D = small(17, 64, 256, 10, 5, 0.05, True, dt)
         A, path = oracle.ref_lasso(X, D, 0.05, return_reg_path=True)

@@ -14,13 +14,16 @@ import pytest
 import scipy.sparse as ssp
 
 import workloads as W
+from conftest import assert_csc_layout
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def read_csc(path, p):
+def read_csc_raw(path, p):
+    """The SpMatrix<double> a C++ caller received, as written by
+    dropin_run.cpp: n, nnz, pB[n+1], r[nnz], v[nnz]."""
     b = open(path, "rb").read()
     n = int(np.frombuffer(b, np.int32, 1, 0)[0])
     nnz = int(np.frombuffer(b, np.int64, 1, 4)[0])
@@ -28,7 +31,11 @@ def read_csc(path, p):
     pB = np.frombuffer(b, np.int32, n + 1, o); o += 4 * (n + 1)
     r = np.frombuffer(b, np.int32, nnz, o); o += 4 * nnz
     v = np.frombuffer(b, np.float64, nnz, o)
-    return ssp.csc_matrix((v, r, pB), shape=(p, n)).toarray()
+    return ssp.csc_matrix((v, r, pB), shape=(p, n))
+
+
+def read_csc(path, p):
+    return read_csc_raw(path, p).toarray()
 
 
 def test_cpp_dropin_runs_and_matches_oracle(tmp_path):
@@ -61,8 +68,12 @@ def test_cpp_dropin_runs_and_matches_oracle(tmp_path):
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     # _lassoD and the Gram-given lasso2 against the lasso restatement
-    ref = oracle.lasso(X, D, 0.15).toarray()
-    assert np.abs(read_csc(str(tmp_path / "lassoD.bin"), p) - ref).max() < 1e-9
+    refs = oracle.lasso(X, D, 0.15)
+    ref = refs.toarray()
+    got = read_csc_raw(str(tmp_path / "lassoD.bin"), p)
+    assert np.abs(got.toarray() - ref).max() < 1e-9
+    # the SpMatrix arrays themselves: pB and the sorted rows, bit for bit
+    assert_csc_layout(got, refs.indptr, refs.indices)
     G = D.T @ D + 1e-10 * np.eye(p)
     refq = oracle.lasso_qq(X, G, D.T @ X, 0.15).toarray()
     assert np.abs(read_csc(str(tmp_path / "lasso2_gram.bin"), p) - refq).max() < 1e-9
@@ -75,8 +86,11 @@ def test_cpp_dropin_runs_and_matches_oracle(tmp_path):
             ls = np.linalg.lstsq(D[:, s], X[:, j], rcond=None)[0]
             assert np.abs(A[s, j] - ls).max() < 1e-8
     # _omp (L = 8)
-    refo = oracle.omp(X, D, L=8).toarray()
-    assert np.abs(read_csc(str(tmp_path / "omp.bin"), p) - refo).max() < 1e-9
+    refos = oracle.omp(X, D, L=8)
+    refo = refos.toarray()
+    got = read_csc_raw(str(tmp_path / "omp.bin"), p)
+    assert np.abs(got.toarray() - refo).max() < 1e-9
+    assert_csc_layout(got, refos.indptr, refos.indices)
     # _alltrainDL: 5 iterations of 64 signals from the given D, one thread's order
     Do, Ao, _, ito = oracle.train_dl(X, D=D, lambda1=0.15, lambda2=1e-9, batchsize=64, iter=5)
     Dl = np.fromfile(str(tmp_path / "traindl_D.bin"), np.float64).reshape(p, m).T

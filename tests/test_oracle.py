@@ -47,16 +47,27 @@ def test_omp_matches_golden(orc, name):
     assert csc_equal(A, ref)
 
 
-@pytest.mark.parametrize("name", golden_files("fista_"))
+@pytest.mark.parametrize("name", golden_files("fista_") + golden_files("fistagap_")
+                         + golden_files("fistaista_") + golden_files("fistapos_"))
 def test_fista_matches_golden(orc, name):
+    """All four optim_info rows (loss, best dual, relative gap, iterations)
+    and the iterates, bit for bit; the fistagap fixtures stop on the
+    relative duality gap before max_it, the fistaista ones run without
+    momentum, the fistapos ones through the positivity prox."""
     g = load_golden(name)
     Wt, info = orc.fista_flat(g["X"], g["D"], g["W0"], g["lambda1"].item(),
                               regul=str(g["regul"]), lambda2=g["lambda2"].item(),
                               max_it=int(g["max_it"]), L0=g["L0"].item(),
                               fixed_step=bool(g["fixed"]), tol=g["tol"].item(),
-                              it0=int(g["it0"]))
+                              it0=int(g["it0"]), ista=bool(g.get("ista", 0)),
+                              pos=bool(g.get("pos", 0)))
     assert np.array_equal(Wt, g["W"])
     assert np.array_equal(info, g["info"], equal_nan=True)
+    if name.startswith("fistagap_"):
+        assert info[3].max() < int(g["max_it"]) and np.all(info[3] % int(g["it0"]) == 0)
+        assert np.all(info[2] < g["tol"])
+    if name.startswith("fistapos_"):
+        assert Wt.min() >= 0 and Wt.max() > 0
 
 
 # ------------------------------------------------ reference build, random
@@ -161,6 +172,133 @@ def test_kkt_check_detects_a_planted_error(orc):
     ok = np.all(np.abs(C) <= lam * (1 + 1e-9)) and np.allclose(
         C[on], lam * np.sign(A[on]), rtol=1e-9)
     assert not ok
+
+
+def test_lasso_l1coeffs_reaches_the_l1_budget_at_a_kkt_point(orc):
+    """L1COEFFS (mode 0): the path stops where ||alpha||_1 reaches lambda1,
+    and alpha is a lasso solution at the path's final correlation level c:
+    |d_j'r| <= c for every atom, with equality and matching sign on the
+    support (r the residual, the Gram's 1e-10 ridge included)."""
+    X, D = _rand(31, 40, 120, 25, np.float64)
+    t = 0.6
+    A = dense(orc.lasso(X, D, t, mode=0))
+    np.testing.assert_allclose(np.abs(A).sum(0), t, rtol=1e-9)
+    C = D.T @ (X - D @ A) - 1e-10 * A
+    for j in range(X.shape[1]):
+        on = A[:, j] != 0
+        c = np.abs(C[on, j])
+        np.testing.assert_allclose(c, c.mean(), rtol=1e-8)
+        assert np.all(np.abs(C[:, j]) <= c.mean() * (1 + 1e-8))
+        assert np.all(np.sign(C[on, j]) == np.sign(A[on, j]))
+    # a planted error (budget off by 1e-4) fails the first check
+    assert not np.allclose(np.abs(A).sum(0) * (1 + 1e-4), t, rtol=1e-9, atol=0)
+
+
+def test_lasso_lambda2_kkt_conditions(orc):
+    """lambda2 > 0 (elastic-net lasso): the Gram carries +lambda2 on its
+    diagonal (decomp/decomp.h:1566-1569), so the solution satisfies
+    |d_j'r - lambda2 alpha_j| <= lambda1, with equality and sign on the
+    support; the same check without the lambda2 term fails."""
+    X, D = _rand(32, 40, 120, 25, np.float64)
+    lam, lam2 = 0.08, 0.05
+    A = dense(orc.lasso(X, D, lam, lambda2=lam2))
+    C = D.T @ (X - D @ A) - lam2 * A
+    assert np.all(np.abs(C) <= lam * (1 + 1e-9))
+    on = A != 0
+    np.testing.assert_allclose(C[on], lam * np.sign(A[on]), rtol=1e-9)
+    C0 = D.T @ (X - D @ A)
+    assert not np.allclose(C0[on], lam * np.sign(A[on]), rtol=1e-6)
+
+
+def _ls_resid2(X, D, A):
+    out = np.empty(X.shape[1])
+    for j in range(X.shape[1]):
+        S = np.flatnonzero(A[:, j])
+        r = X[:, j] - (D[:, S] @ np.linalg.lstsq(D[:, S], X[:, j], rcond=None)[0]
+                       if len(S) else 0)
+        out[j] = r @ r
+    return out
+
+
+def test_omp_eps_stops_at_the_first_residual_below_eps(orc):
+    """eps stop (decomp/decomp.h:599-600): after the last selected atom the
+    squared residual is <= eps; with that atom left out (the greedy prefix,
+    i.e. the same solver at L = count - 1) it was still > eps."""
+    rng = np.random.default_rng(33)
+    D = W.unit_dict(rng, 30, 80, np.float64)
+    X = W.sparse_signals(rng, D, 40, np.arange(40) % 10 + 1, 0.003, False, np.float64)
+    eps = 0.01
+    A = dense(orc.omp(X, D, eps=eps))
+    k = (A != 0).sum(0)
+    r_end = _ls_resid2(X, D, A)
+    stopped = (k < 30) & (k > 0)
+    assert stopped.sum() >= 30
+    assert np.all(r_end[stopped] <= eps)
+    Lprev = np.maximum(k - 1, 0).astype(np.int32)
+    Ap = dense(orc.omp(X, D, L=Lprev))
+    r_prev = _ls_resid2(X, D, Ap)
+    assert np.all(r_prev[stopped] > eps)
+
+
+def test_omp_lambda_stop_rule(orc):
+    """lambda stop (decomp/decomp.h:566-573): an atom is admitted only if it
+    lowers the squared residual by at least 2 lambda, and the path stops at
+    the first best atom that would lower it by less: the last admitted atom
+    lowered it by >= 2 lambda, and no further atom lowers it by 2 lambda
+    (the largest decrease over candidates a is (d_a'r)^2 / ||P d_a||^2, P the
+    projector off the support)."""
+    X, D = _rand(34, 30, 80, 30, np.float64, k=6, noise=0.02, norm=False)
+    lam = 0.01
+    A = dense(orc.omp(X, D, lambda1=lam))
+    k = (A != 0).sum(0)
+    assert k.min() >= 1 and k.max() < 30
+    r_end = _ls_resid2(X, D, A)
+    r_prev = _ls_resid2(X, D, dense(orc.omp(X, D, L=(k - 1).astype(np.int32))))
+    assert np.all(r_prev - r_end >= 2 * lam * (1 - 1e-9))
+    for j in range(X.shape[1]):
+        S = np.flatnonzero(A[:, j])
+        Q, _ = np.linalg.qr(D[:, S])
+        r = X[:, j] - Q @ (Q.T @ X[:, j])
+        Pd = D - Q @ (Q.T @ D)
+        nrm = np.sum(Pd * Pd, axis=0)
+        ok = nrm > 1e-8
+        dec = np.zeros(D.shape[1])
+        dec[ok] = (D[:, ok].T @ r) ** 2 / nrm[ok]
+        dec[S] = 0
+        assert dec.max() < 2 * lam
+
+
+def test_fista_backtracking_is_beck_teboulle(orc):
+    """FISTA with backtracking (prox/fista.h:2950-2960, test :400-413): L
+    grows by gamma until ||D(y - z)||^2 <= L ||y - z||^2 (for the square
+    loss this is the quadratic upper bound of Beck & Teboulle, SIAM J.
+    Imaging Sci. 2009, section 4), L persists across iterations, and the
+    iterates follow the published FISTA momentum.  Restated here from the
+    paper in numpy (FP64) for 6 iterations and compared with the oracle."""
+    X, D = _rand(35, 20, 50, 5, np.float64, k=3, noise=0.05, norm=False)
+    lam, gamma, L0, its = 0.05, 1.5, 0.1, 6
+    p = D.shape[1]
+    Wt, info = orc.fista_flat(X, D, np.zeros((p, 5), order="F"), lam, regul="l1",
+                              max_it=its, L0=L0, fixed_step=False, gamma=gamma, tol=0.0,
+                              it0=1000)
+    for j in range(5):
+        x = X[:, j]
+        xk = np.zeros(p)
+        y = xk.copy()
+        t, L = 1.0, L0
+        for _ in range(its):
+            g = D.T @ (D @ y - x)
+            while True:
+                z = soft(y - g / L, lam / L)
+                dz = y - z
+                if np.sum((D @ dz) ** 2) <= L * np.sum(dz * dz):
+                    break
+                L *= gamma
+            tn = (1 + np.sqrt(1 + 4 * t * t)) / 2
+            y = z + ((t - 1) / tn) * (z - xk)
+            xk, t = z, tn
+        np.testing.assert_allclose(Wt[:, j], xk, atol=1e-12)
+    assert np.all(info[3] == its + 1)
 
 
 def test_omp_least_squares_on_support(orc):
