@@ -2,14 +2,19 @@
 
 Default (headline): LARS-Lasso, m=256, p=1024, n=200000 signals, lambda
 log-uniform in [0.05, 0.5] per signal, FP32.  One step = one pass of the path
-over the whole batch with inputs resident in HBM: D'X on MFMA, ||x||^2, then
-the LARS solver kernels producing the per-signal (value, index) lists.
-`value` is the batched form's signals/s (persistent work-queue kernels); the
-plain form (one workgroup per signal) is measured beside it, with the CPU
-reference, the error of each precision variant against the FP64 oracle, and
-p50/p99 latency of a served mixed-request load.  Kernel durations come from
-HIP events on the engine's stream; roofline figures use each kernel's
-algorithmic work.
+over the whole batch with X resident in HBM: D'X on MFMA, ||x||^2, the LARS
+solver kernels producing the per-signal (value, index) lists, and the CSC
+assembly (SpMatrix::convert) into HBM.  `value` is the batched form's
+signals/s (persistent work-queue kernels).  Beside it: the host boundary
+(host X in, host CSC out, PCIe both ways), the plain GPU form (a grid of
+workgroups, one signal each at a time), the reference's CPU path (built in
+place, 16 threads = this GPU's host-core share, and 1 thread), the error of
+each precision variant against the FP64 oracle, p50/p99 latency of a served
+mixed-request load, and sub-objects for the other configurations: the
+reference CPU case (D 64x256, n=10000), batch OMP (m256 p1024 n100000) and
+FISTA-flat (m512 p2048 n8192), each with its rooflines and CPU baseline.
+Kernel durations come from HIP events on the engine's stream; roofline
+figures use each kernel's algorithmic work.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
     python bench.py --workload omp|fista|served|variants|traindl  (other configurations)
@@ -38,13 +43,51 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# Host threads of the CPU baselines: the reference's numThreads = -1 would
+# take min(64, nproc) (linalg/misc.h:288-299); a one-GPU job here is allotted
+# 16 host cores (OMP_NUM_THREADS is 16 on the GPU host), so the baselines run
+# 16 threads, and one thread for the per-core figure.
+CPU_THREADS = 16
+
+
 def cpu_threads():
-    return min(os.cpu_count() or 1, 16)
+    return min(os.cpu_count() or 1, CPU_THREADS)
 
 
-def timed_sample(run, n_total, seconds):
-    """Times `run(ns)` on a prefix sized to take about `seconds`."""
-    ns, dt = min(256, n_total), 0.0
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "threads": cpu_threads(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+            "blas": "oracle/blas_ref.c: the reference BLAS loop orders of netlib, "
+                    "gcc -O3 -ffp-contract=off, single-threaded per call (SPAMS links an "
+                    "unspecified external BLAS)",
+            "reference_build": "the reference's own headers compiled in place "
+                               "(oracle/Makefile: g++ -O3 -fopenmp -std=c++14 -DUSE_BLAS_LIB)"}
+
+
+def median_time(run, reps=3):
+    """Median wall time of `run()` over `reps` runs after one warm-up."""
+    run()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def timed_sample(run, n_total, seconds, start=256):
+    """Times `run(ns)` on a prefix sized to take about `seconds` (the first
+    probe runs `start` signals)."""
+    ns, dt = min(start, n_total), 0.0
     while True:
         t = time.perf_counter()
         run(ns)
@@ -76,6 +119,19 @@ def measured_traffic(names):
     return float(sum(d[k] for k in names))
 
 
+def measured_traffic_ormp():
+    """HBM bytes per ormp_kernel launch of the OMP workload from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary; null when absent."""
+    path = os.path.join(ROOT, "profiles", "ormp_counters.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)["bytes_per_dispatch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    k = [v for n_, v in d.items() if n_.startswith("ormp_kernel")]
+    return float(k[0]) if k else None
+
+
 def valu_issue():
     """VALU issue fraction of the two LARS passes (VALU instructions x 4
     cycles / (SIMDs x launch cycles)) from the committed rocprofv3 SQ counter
@@ -100,7 +156,7 @@ def lars_kernel_names(p, w, elem=4):
         c = -(-p // ((16 // elem) * 64 * w_))
         return next(q for q in (1, 2, 4, 8, 16) if q >= c)
     w2 = max(4, w)
-    return [f"lars_kernel<{t}, {w}, {nq(w)}, {cap}>", f"lars_kernel<{t}, {w2}, {nq(w2)}, 0>"]
+    return [f"lars_kernel<{t}, {w}, {nq(w)}, {cap}, 0>", f"lars_kernel<{t}, {w2}, {nq(w2)}, 0, 0>"]
 
 
 class Engine:
@@ -204,6 +260,10 @@ def run_lars(args):
     R = torch.empty((n, Lout), dtype=torch.int32, device=E.dev)
     cnt = torch.empty((n,), dtype=torch.int32, device=E.dev)
     stats = torch.empty((n, 2), dtype=torch.int32, device=E.dev)
+    # the CSC output in HBM (SpMatrix: pB, r, v; capacity n x Lout)
+    cpB = torch.empty((n + 1,), dtype=torch.int32, device=E.dev)
+    cr = torch.empty((n * Lout,), dtype=torch.int32, device=E.dev)
+    cv = torch.empty((n * Lout,), dtype=torch.float32, device=E.dev)
 
     def step(form, ev=None):
         if ev:
@@ -216,6 +276,10 @@ def run_lars(args):
                           waves_per_signal=args.waves, stats=stats)
         if ev:
             ev[2].record(E.stream)
+        nnz = spx.lists_to_csc_dev(E.ctx, V, R, cnt, cpB, cr, cv)
+        if ev:
+            ev[3].record(E.stream)
+        return nnz
 
     tune = None
     if args.waves <= 0:
@@ -234,19 +298,21 @@ def run_lars(args):
     for _ in range(args.warmup):
         step("batched")
     torch.cuda.synchronize()
-    evs = [E.events(3) for _ in range(args.steps)]
+    evs = [E.events(4) for _ in range(args.steps)]
     log("bench: timing %d steps" % args.steps)
     E.ctx.reset_timing()
     E.ctx.set_timing(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
-        step("batched", evs[k])
+        nnz = step("batched", evs[k])
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
     E.ctx.set_timing(False)
     passes = {k: v[0] / max(v[1], 1) for k, v in E.ctx.read_timing().items() if v[1]}
+    csc_launch_ms = E.ctx.read_timing()["csc"][0] / max(args.steps, 1)
     gemm_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     solve_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    csc_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in evs]))
     kern_ms = passes.get("lars_fast", 0.0) + passes.get("lars_full", 0.0)
     waves_used = args.waves if args.waves > 0 else d.waves("lasso")
     value = n / wall
@@ -259,18 +325,30 @@ def run_lars(args):
     solver_bytes = float(np.sum(4 * p * (1 + admissions) + 8 * counts))
     gemm_flops = 2.0 * m * p * n
 
+    # the host boundary: spx_lasso_f32 with host X in and a host CSC out
+    # (H2D of X, dictionary set-up with G = D'D, D'X, solver, CSC, D2H)
+    log("bench: host boundary")
+    t_host = median_time(lambda: spx.lasso(X, D, lambda1=lam, ctx=E.ctx), reps=2)
+    host = {"value": n / t_host, "unit": "signals/s", "ms": t_host * 1e3,
+            "what": "spx_lasso_f32 (the _lassoD boundary): host X (%.0f MB) in, host CSC out, "
+                    "dictionary set-up (D, D', G on MFMA) included; PCIe both ways"
+                    % (X.nbytes / 1e6)}
+
     plain = None
     if not args.no_plain:
         log("bench: plain form")
         step("plain")
         torch.cuda.synchronize()
-        ev = E.events(3)
+        ev = E.events(4)
         tp = time.perf_counter()
         step("plain", ev)
         torch.cuda.synchronize()
         tp = time.perf_counter() - tp
         plain = {"value": n / tp, "unit": "signals/s", "solver_ms": ev[1].elapsed_time(ev[2]),
-                 "batched_over_plain": value / (n / tp)}
+                 "ms_per_step": tp * 1e3, "batched_over_plain": value / (n / tp),
+                 "form": "one grid of resident workgroups (4 wavefronts each), signals "
+                         "assigned grid-stride, one at a time per workgroup, full-capacity "
+                         "kernel; no work queue, no ordering, no LDS pass"}
 
     prec = None
     if not args.no_precision:
@@ -291,6 +369,7 @@ def run_lars(args):
             spx.lasso_dtx_dev(E.ctx, db, DtX, norms, lam_t, V, R, cnt, form="batched",
                               waves_per_signal=waves_used, stats=stats)
             ev[2].record(E.stream)
+            spx.lists_to_csc_dev(E.ctx, V, R, cnt, cpB, cr, cv)
         torch.cuda.synchronize()
         wb = (time.perf_counter() - t0) / args.steps
         prec["bf16x3_batched_step"] = {"signals_per_s": n / wb, "ms_per_step": wb * 1e3,
@@ -308,15 +387,29 @@ def run_lars(args):
         import oracle
         th = cpu_threads()
         kind = "reference" if oracle.has_ref() else "port"
-        if kind == "reference":
-            run = lambda ns: oracle.ref_lasso_pervec(np.asfortranarray(X[:, :ns]), D, lam[:ns],
-                                                     num_threads=th)
-        else:
-            run = lambda ns: oracle.lasso(np.asfortranarray(X[:, :ns]), D, lam[:ns], num_threads=th)
-        ns, dt = timed_sample(run, n, args.cpu_seconds)
+
+        def cpu_run(threads):
+            if kind == "reference":
+                return lambda ns: oracle.ref_lasso_pervec(np.asfortranarray(X[:, :ns]), D,
+                                                          lam[:ns], num_threads=threads)
+            return lambda ns: oracle.lasso(np.asfortranarray(X[:, :ns]), D, lam[:ns],
+                                           num_threads=threads)
+        ns, dt = timed_sample(cpu_run(th), n, args.cpu_seconds)
+        ns1, dt1 = timed_sample(cpu_run(1), n, args.cpu_seconds / 2)
+        what = ("per-signal-lambda lasso2/coreLARS2 loop of the reference built in place "
+                "(G = D'D, D'X, the OpenMP loop over signals, SpMatrix::convert)")
         cpu = {"value": ns / dt, "unit": "signals/s", "cores": th, "kind": kind,
-               "sample": f"first {ns} signals of the same workload, {dt:.1f} s, per-signal-lambda "
-                         f"lasso2/coreLARS2 loop of the reference built in place, {th} OpenMP threads"}
+               "sample": f"first {ns} signals of the same workload, {dt:.1f} s, {what}, "
+                         f"{th} OpenMP threads",
+               "one_thread": {"value": ns1 / dt1, "unit": "signals/s",
+                              "sample": f"first {ns1} signals, {dt1:.1f} s, 1 thread"},
+               "host": host_info()}
+
+    configs = {}
+    if not args.no_configs:
+        configs["c1_plain_lars_cpu_case"] = c1_report(E, args)
+        configs["c2_batch_omp"] = c2_report(E, args)
+        configs["c4_fista_flat"] = c4_report(E, args)
 
     a64 = admissions.astype(np.float64)
     restream_bytes = float(4.0 * p * np.sum(a64 * (a64 + 1) / 2))
@@ -335,8 +428,8 @@ def run_lars(args):
         "data": "synthetic",
         "config": {"workload": "LARS-Lasso penalty, D 256x1024 unit-norm N(0,1), X 256x%d "
                                "5-sparse + 0.05 noise normalised, lambda log-U[0.05,0.5] "
-                               "per signal, FP32 MFMA D'X" % n,
-                   "m": m, "p": p, "n": n, "form": "batched"},
+                               "per signal, FP32 MFMA D'X; step = X in HBM -> CSC in HBM" % n,
+                   "m": m, "p": p, "n": n, "form": "batched", "nnz": int(nnz)},
         "roofline": {"bound": "hbm",
                      "kernel": "lars_kernel: LDS pass over all signals + full-capacity pass "
                                "over the overflow list (one launch each per step)",
@@ -361,7 +454,9 @@ def run_lars(args):
                           "frac": gemm_flops / (gemm_ms * 1e-3) / FP32_MFMA_PEAK},
         "kernel_ms": {"dtx_and_norms": gemm_ms, "lars_solver": solve_ms,
                       "lars_order_sort": passes.get("lars_order", 0.0),
-                      "lars_overflow_sort": passes.get("lars_list", 0.0)},
+                      "lars_overflow_sort": passes.get("lars_list", 0.0),
+                      "csc_assembly": csc_ms, "csc_assembly_kernels": csc_launch_ms},
+        "host_boundary": host,
         "autotune": tune,
         "solver_stats": {"mean_admissions": float(admissions.mean()),
                          "max_admissions": int(admissions.max()),
@@ -372,7 +467,261 @@ def run_lars(args):
         "served": served,
         "latency_ms": None if served is None else {"p50": served["p50_ms"], "p99": served["p99_ms"]},
         "cpu_baseline": cpu,
+        "configs": configs,
     }
+
+
+# ------------------------------------------------------ other configurations
+
+def c1_report(E, args):
+    """The reference's CPU case: D 64x256, X 64x10000 (5-sparse + 0.05 noise,
+    normalised), lambda 0.15, FP32 _lassoD (lasso2 / coreLARS2).  The
+    reference built in place at 16 threads and 1 thread (median of 3 after a
+    warm-up), and the engine: the host boundary call and the device step."""
+    import oracle
+    import workloads as W
+    spx, torch = E.spx, E.torch
+    log("bench: c1 reference CPU case")
+    X, D, lam = W.lars_small()
+    m, n = X.shape
+    p = D.shape[1]
+    out = {"workload": "D 64x256 unit-norm N(0,1), X 64x%d 5-sparse + 0.05 noise normalised, "
+                       "lambda 0.15, PENALTY, FP32 (cpp_library/test_decomp.cpp:36-65 shape)" % n}
+    if not args.no_cpu and oracle.has_ref():
+        th = cpu_threads()
+        t16 = median_time(lambda: oracle.ref_lasso(X, D, lam, num_threads=th))
+        t1 = median_time(lambda: oracle.ref_lasso(X, D, lam, num_threads=1))
+        out["reference_cpu"] = {"kind": "reference", "cores": th, "ms": t16 * 1e3,
+                                "value": n / t16, "unit": "signals/s",
+                                "one_thread": {"ms": t1 * 1e3, "value": n / t1},
+                                "what": "_lassoD of the reference built in place (G, D'X, "
+                                        "OpenMP loop, convert), median of 3 after a warm-up"}
+    t_host = median_time(lambda: spx.lasso(X, D, lambda1=lam, ctx=E.ctx))
+    Xt = E.to_dev(X.T)
+    lam_t = E.to_dev(np.array([lam], np.float32))
+    d = spx.Dictionary(E.ctx, D)
+    Lout = spx.lasso_L(m, p, -1, spx.PENALTY, lam)
+    V = torch.empty((n, Lout), dtype=torch.float32, device=E.dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=E.dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=E.dev)
+    cpB = torch.empty((n + 1,), dtype=torch.int32, device=E.dev)
+    cr = torch.empty((n * Lout,), dtype=torch.int32, device=E.dev)
+    cv = torch.empty((n * Lout,), dtype=torch.float32, device=E.dev)
+
+    def dev_step():
+        spx.lasso_dev(E.ctx, d, Xt, lam_t, V, R, cnt)
+        spx.lists_to_csc_dev(E.ctx, V, R, cnt, cpB, cr, cv)
+        torch.cuda.synchronize()
+    t_dev = median_time(dev_step, reps=5)
+    d.close()
+    out["engine"] = {"host_boundary": {"ms": t_host * 1e3, "value": n / t_host,
+                                       "unit": "signals/s"},
+                     "device_step": {"ms": t_dev * 1e3, "value": n / t_dev,
+                                     "unit": "signals/s",
+                                     "what": "X in HBM -> D'X, norms, batched LARS, CSC in HBM"}}
+    if "reference_cpu" in out:
+        out["engine_over_reference_cpu"] = {
+            "host_boundary": out["reference_cpu"]["ms"] / (t_host * 1e3),
+            "device_step": out["reference_cpu"]["ms"] / (t_dev * 1e3)}
+    return out
+
+
+def c2_report(E, args, n=100000):
+    """Batch OMP (D 256x1024, X 256x100000 k ~ U{1..32} + 0.01 noise, L=32,
+    eps=1e-3, FP32): the device step D'X -> ORMP -> CSC with its HBM (ORMP)
+    and MFMA (D'X) rooflines, the plain form, and the reference _omp on the
+    host cores."""
+    import oracle
+    import workloads as W
+    spx, torch = E.spx, E.torch
+    log("bench: c2 batch OMP")
+    X, D, L, eps = W.omp_batch(n=n)
+    m, p = D.shape
+    Xt = E.to_dev(X.T)
+    d = spx.Dictionary(E.ctx, D)
+    Lout = spx.omp_L(m, p, L)
+    L_t = E.to_dev(np.array([L], np.int32))
+    e_t = E.to_dev(np.array([eps], np.float32))
+    V = torch.empty((n, Lout), dtype=torch.float32, device=E.dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=E.dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=E.dev)
+    cpB = torch.empty((n + 1,), dtype=torch.int32, device=E.dev)
+    cr = torch.empty((n * Lout,), dtype=torch.int32, device=E.dev)
+    cv = torch.empty((n * Lout,), dtype=torch.float32, device=E.dev)
+    ldg = (p + 15) // 16 * 16
+    DtX = torch.empty((n, ldg), dtype=torch.float32, device=E.dev)
+    ns_t = min(n, 16384)
+    w, trial = d.autotune_omp(Xt[:ns_t], L=L_t, eps=e_t)
+
+    def step(form="batched"):
+        spx.omp_dev(E.ctx, d, Xt, V, R, cnt, L=L_t, eps=e_t, form=form)
+        return spx.lists_to_csc_dev(E.ctx, V, R, cnt, cpB, cr, cv)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    steps = max(args.steps // 2, 3)
+    E.ctx.reset_timing()
+    E.ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    E.ctx.set_timing(False)
+    tm = E.ctx.read_timing()
+    ormp_ms = tm["ormp"][0] / max(tm["ormp"][1], 1)
+    ev = E.events(2)
+    ev[0].record(E.stream)
+    spx.dtx_dev(E.ctx, d, Xt, DtX)
+    ev[1].record(E.stream)
+    torch.cuda.synchronize()
+    gemm_ms = ev[0].elapsed_time(ev[1])
+    counts = cnt.cpu().numpy().astype(np.int64)
+    # algorithmic bytes per signal: the D'X column in, one Gram column per
+    # selected atom, (value, index) per coefficient out
+    solver_bytes = float(np.sum(4 * p * (1 + counts) + 8 * counts))
+    step("plain")
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    step("plain")
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - tp
+    out = {"workload": "OMP D 256x1024 unit-norm, X 256x%d k~U{1..32} + 0.01 noise (not "
+                       "normalised), L=32, eps=1e-3, FP32; step = X in HBM -> CSC in HBM" % n,
+           "value": n / wall, "unit": "signals/s", "ms_per_step": wall * 1e3, "steps": steps,
+           "waves_per_signal": w, "autotune_trial_ms": trial,
+           "mean_nnz": float(counts.mean()),
+           "roofline": {"bound": "hbm", "kernel": "ormp_kernel",
+                        "achieved": solver_bytes / (ormp_ms * 1e-3) / 1e9,
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": solver_bytes / (ormp_ms * 1e-3) / HBM_PEAK,
+                        "traffic": measured_traffic_ormp(),
+                        "algorithmic_bytes": solver_bytes, "launch_ms": ormp_ms,
+                        "gram_restream_TB_per_s":
+                            float(4.0 * p * np.sum(counts * (counts + 1) / 2))
+                            / (ormp_ms * 1e-3) / 1e12},
+           "roofline_gemm": {"bound": "mfma", "kernel": "gemm_tn_kernel<float>",
+                             "achieved": 2.0 * m * p * n / (gemm_ms * 1e-3) / 1e12,
+                             "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
+                             "frac": 2.0 * m * p * n / (gemm_ms * 1e-3) / FP32_MFMA_PEAK},
+           "plain_gpu": {"value": n / tp, "batched_over_plain": (n / wall) / (n / tp)}}
+    d.close()
+    if not args.no_cpu and oracle.has_ref():
+        th = cpu_threads()
+
+        def run(threads):
+            return lambda ns: oracle.ref_omp(np.asfortranarray(X[:, :ns]), D, L=L, eps=eps,
+                                             num_threads=threads)
+        ns, dt = timed_sample(run(th), n, args.cpu_seconds / 2)
+        ns1, dt1 = timed_sample(run(1), n, args.cpu_seconds / 4)
+        out["cpu_baseline"] = {"value": ns / dt, "unit": "signals/s", "cores": th,
+                               "kind": "reference",
+                               "sample": f"first {ns} signals, {dt:.1f} s, the reference _omp "
+                                         f"(omp/coreORMP, G, D'X, convert) built in place, "
+                                         f"{th} threads",
+                               "one_thread": {"value": ns1 / dt1,
+                                              "sample": f"first {ns1} signals, {dt1:.1f} s"}}
+    return out
+
+
+def c4_report(E, args, n=8192):
+    """FISTA-flat (D 512x2048, X 512x8192 from 20-sparse codes + 0.02 noise,
+    lambda 0.05, l1 and elastic-net lambda2 0.01, 200 iterations at the fixed
+    step 1/L, L from the device power iteration), FP32 MFMA and BF16x3
+    products.  The whole solve is timed as the replayed graph; an eager run
+    with launch timers splits it into the products (MFMA roofline) and the
+    fused prox pass (HBM roofline).  The objective gap against the FP64
+    oracle is taken on 32 signals; the reference _fistaFlat runs on a
+    bounded sample on the host cores."""
+    import oracle
+    import workloads as W
+    spx, torch = E.spx, E.torch
+    log("bench: c4 FISTA-flat")
+    X, D = W.fista_flat(n=n)
+    m, p = D.shape
+    Xt = E.to_dev(X.T)
+    ns = 32
+    Xs = np.asfortranarray(X[:, :ns])
+    its = 200
+    out = {"workload": "FISTA-flat square loss, D 512x2048 unit-norm, X 512x%d 20-sparse + "
+                       "0.02 noise (not normalised), lambda1 0.05 (l1; elastic-net lambda2 "
+                       "0.01), %d iterations, fixed step 1/L, L = lambda_max(D'D) by the "
+                       "device power iteration" % (n, its), "variants": {}}
+    L0 = None
+    for prec in ("fp32", "bf16x3"):
+        d = spx.Dictionary(E.ctx, D, precision=prec)
+        L0 = d.lipschitz(200)
+        for regul in ("l1", "elastic-net"):
+            W0 = torch.zeros((n, p), dtype=torch.float32, device=E.dev)
+            Wt = torch.zeros((n, p), dtype=torch.float32, device=E.dev)
+            info = torch.zeros((n, 4), dtype=torch.float32, device=E.dev)
+            kw = dict(regul=regul, lambda2=0.01, max_it=its, L0=L0, fixed_step=True,
+                      tol=0.0, it0=1000, precision=prec)
+            spx.fista_flat_dev(E.ctx, d, Xt, W0, Wt, info, 0.05, graph=True, **kw)
+            torch.cuda.synchronize()
+            ev = E.events(2)
+            ev[0].record(E.stream)
+            spx.fista_flat_dev(E.ctx, d, Xt, W0, Wt, info, 0.05, graph=True, **kw)
+            ev[1].record(E.stream)
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1])
+            # eager launches with timers: products vs prox pass
+            E.ctx.reset_timing()
+            E.ctx.set_timing(True)
+            spx.fista_flat_dev(E.ctx, d, Xt, W0, Wt, info, 0.05, graph=False, **kw)
+            torch.cuda.synchronize()
+            E.ctx.set_timing(False)
+            tm = E.ctx.read_timing()
+            prod_ms, prox_ms = tm["fista_products"][0], tm["fista_prox"][0]
+            flops = its * 4.0 * m * p * n
+            prox_bytes = its * 5.0 * 4 * p * n
+            obj = info.cpu().numpy()[:ns, 0].astype(np.float64)
+            _, io = oracle.fista_flat(Xs.astype(np.float64), D.astype(np.float64),
+                                      np.zeros((p, ns)), 0.05, regul=regul, lambda2=0.01,
+                                      max_it=its, L0=L0, fixed_step=True, tol=0.0, it0=1000,
+                                      num_threads=cpu_threads())
+            gap = (obj - io[0]) / np.abs(io[0])
+            peak = FP32_MFMA_PEAK if prec == "fp32" else BF16_MFMA_PEAK / 3
+            out["variants"][f"{prec}/{regul}"] = {
+                "ms": ms, "signals_per_s": n / (ms * 1e-3),
+                "roofline_mfma": {"achieved": flops / (ms * 1e-3) / 1e12,
+                                  "peak": peak / 1e12, "unit": "TFLOP/s",
+                                  "frac": flops / (ms * 1e-3) / peak,
+                                  "over": "the whole solve (graph replay)"},
+                "products_ms": prod_ms,
+                "products_frac": flops / (prod_ms * 1e-3) / peak,
+                "roofline_prox": {"bound": "hbm", "kernel": "prox_step_kernel",
+                                  "achieved": prox_bytes / (prox_ms * 1e-3) / 1e9,
+                                  "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                  "frac": prox_bytes / (prox_ms * 1e-3) / HBM_PEAK,
+                                  "launch_ms": prox_ms / its,
+                                  "algorithmic_bytes_per_launch": 5.0 * 4 * p * n},
+                "eager_ms": tm["fista_products"][0] + tm["fista_prox"][0],
+                "objective_rel_gap_vs_fp64_max": float(np.max(np.abs(gap))),
+                "lipschitz": L0}
+        d.close()
+    out["value"] = out["variants"]["fp32/l1"]["signals_per_s"]
+    out["unit"] = "signals/s"
+    if not args.no_cpu and oracle.has_ref():
+        th = cpu_threads()
+        W0h = np.zeros((p, n), np.float32, order="F")
+
+        def run(threads):
+            return lambda k: oracle.ref_fista_flat(np.asfortranarray(X[:, :k]), D,
+                                                   np.asfortranarray(W0h[:, :k]), 0.05,
+                                                   regul="l1", max_it=its, L0=L0,
+                                                   fixed_step=True, tol=0.0, it0=1000,
+                                                   num_threads=threads)
+        k16, dt = timed_sample(run(th), n, args.cpu_seconds / 2, start=th)
+        k1, dt1 = timed_sample(run(1), n, args.cpu_seconds / 4, start=1)
+        out["cpu_baseline"] = {"value": k16 / dt, "unit": "signals/s", "cores": th,
+                               "kind": "reference",
+                               "sample": f"first {k16} signals, {dt:.1f} s, the reference "
+                                         f"_fistaFlat (l1, {its} iterations) built in place, "
+                                         f"{th} threads",
+                               "one_thread": {"value": k1 / dt1,
+                                              "sample": f"first {k1} signals, {dt1:.1f} s"}}
+    return out
 
 
 # ------------------------------------------------------------------ OMP
@@ -717,6 +1066,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the sub-objects of the other configurations (c1, c2, c4)")
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB read + 256 MiB write copy kernel first "

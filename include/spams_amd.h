@@ -344,7 +344,9 @@ void lasso(const Data<T>& X, const AbstractMatrix<T>& G, const AbstractMatrix<T>
 // _lassoD (cpp_library/spams.h:107-143).  cholevsky selects the step and
 // stop rules of the Cholesky variant coreLARS (they differ from coreLARS2's in
 // L2ERROR mode); spx runs its single inverse-Gram homotopy for either.  ols
-// returns the least-squares fit on each final active set.
+// implies cholevsky (cpp_library/spams.h:136), tracks the least-squares
+// residual of the active set and returns the least-squares fit on each final
+// active set.
 template <typename T>
 SpMatrix<T>* _lassoD(Matrix<T>* X, Matrix<T>* D, Matrix<T>** path, bool return_reg_path, int L,
                      const T constraint, const T lambda2, constraint_type mode, const bool pos,

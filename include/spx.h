@@ -90,9 +90,12 @@ typedef struct {
   int precision;       /* SPX_PREC_* (f32 entry points only)                 */
   int form;            /* SPX_FORM_*                                         */
   int waves_per_signal;/* 0 = autotuned; else 1, 2, 4 or 16                  */
-  int ols;             /* 1: least-squares refit on the final active set (the
-                        * facade's ols, decomp/decomp.h:1087-1091); the path
-                        * itself is the same homotopy (not with L2ERROR) */
+  int ols;             /* 1: least-squares fit on the final active set (the
+                        * facade's ols, decomp/decomp.h:1087-1091); implies
+                        * cholesky, whose path then tracks the least-squares
+                        * residual of the active set (the L2ERROR stop reads
+                        * it, decomp/decomp.h:1044-1050) and has no exit on a
+                        * residual increase */
   int cholesky;        /* 1: the path of the Cholesky variant coreLARS
                         * (decomp/decomp.h:844-1093, _lassoD's cholesky): in
                         * L2ERROR mode the last step is not cut at the
@@ -143,14 +146,19 @@ int spx_ctx_set_stream(spx_ctx* ctx, void* hip_stream);
 int spx_ctx_own_stream(spx_ctx* ctx);
 int spx_ctx_synchronize(spx_ctx* ctx);
 
-/* Launch timing.  While on, the batched solvers record HIP events on the
- * context's stream around each of their launches, under these timers: */
+/* Launch timing.  While on, the batched solvers, FISTA and the CSC assembly
+ * record HIP events on the context's stream around their launches, under
+ * these timers: */
 enum {
   SPX_TIMER_LARS_ORDER = 0, /* work-order sort by lambda (longest path first) */
   SPX_TIMER_LARS_FAST = 1,  /* LARS pass with the active set in LDS */
   SPX_TIMER_LARS_FULL = 2,  /* LARS pass over the signals that outgrew LDS */
   SPX_TIMER_ORMP = 3,       /* ORMP solver */
   SPX_TIMER_LARS_LIST = 4,  /* ordering of the overflow list (longest first) */
+  SPX_TIMER_FISTA_PROX = 5, /* FISTA fused prox + momentum pass (fixed step,
+                             * eager launches; a replayed graph is not timed) */
+  SPX_TIMER_FISTA_PRODUCTS = 6, /* FISTA's two products per iteration (same) */
+  SPX_TIMER_CSC = 7,        /* device CSC assembly: count scan and column fill */
   SPX_TIMER_COUNT = 8
 };
 int spx_ctx_set_timing(spx_ctx* ctx, int on);

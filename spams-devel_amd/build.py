@@ -1,6 +1,6 @@
 """Builds the engine library libspx.so (HIP, gfx950) in-tree.
 
-    python spams-devel_amd/build.py [--jobs N]
+    python spams-devel_amd/build.py [--jobs N] [--incremental]
 
 Every csrc/*.hip file is compiled with hipcc for gfx950 into _build/ and
 linked into spams-devel_amd/libspx.so.  The C ABI is declared in
@@ -37,8 +37,16 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
+def _headers_mtime() -> float:
+    hdrs = glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
+    return max((os.path.getmtime(h) for h in hdrs), default=0.0)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(
+            os.path.getmtime(src), _headers_mtime()):
+        return obj
     cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -47,11 +55,13 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 8) -> str:
+def build(jobs: int = 8, incremental: bool = False) -> str:
+    """Compiles every source (or, with `incremental`, those whose object is
+    older than the source or any header) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s_: _compile(s_, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.run(cmd, check=True)
@@ -62,5 +72,7 @@ def build(jobs: int = 8) -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources older than their object's inputs")
     args = ap.parse_args()
-    print(build(args.jobs))
+    print(build(args.jobs, args.incremental))

@@ -182,7 +182,7 @@ int tuned_waves(const Dict* d, int solver) {
 // ------------------------------------------------------------------ timers
 
 ScopedTimer::ScopedTimer(Ctx* ctx, int id) : ctx_(ctx), id_(id) {
-  if (!ctx_->timing) return;
+  if (!ctx_->timing || id_ < 0) return;
   a_ = ctx_->event_get();
   if (a_ && hipEventRecord(a_, ctx_->stream) != hipSuccess) a_ = nullptr;
 }
@@ -371,17 +371,13 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   a.stats = out->stats;
   a.refit = (o->precision & SPX_PREC_REFIT64) ? 1 : 0;
   a.ols = o->ols ? 1 : 0;
-  // cholesky selects the path of coreLARS (decomp/decomp.h:844-1093); with
-  // ols the reference tracks the least-squares residual instead, which only
-  // the L2ERROR stop reads
-  if (o->ols && o->mode == SPX_L2ERROR) {
-    set_error("lasso: ols with mode L2ERROR (the least-squares residual stop of the "
-              "Cholesky variant) is not provided");
-    return SPX_ERR_UNSUP;
-  }
-  // the weighted and masked solvers have no Cholesky variant (coreLARS2W,
-  // coreLARS2 with mask): the option is read by the plain solver only
-  a.chol = (o->cholesky && !o->ols && !W && !mi) ? 1 : 0;
+  // cholesky selects the path of coreLARS (decomp/decomp.h:844-1093), and
+  // ols implies it (cpp_library/spams.h:136-137); with ols that path tracks
+  // the least-squares residual of the active set (read by the L2ERROR and
+  // normX stops) and has no exit on a residual increase.  The weighted and
+  // masked solvers have no Cholesky variant (coreLARS2W, coreLARS2 with a
+  // mask): the option is read by the plain solver only.
+  a.chol = ((o->cholesky || o->ols) && !W && !mi) ? 1 : 0;
   if (out->path && out->path_cols > 0) {
     a.path = static_cast<T*>(out->path);
     a.path_cols = out->path_cols;

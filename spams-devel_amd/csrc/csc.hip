@@ -107,6 +107,8 @@ int csc_count_scan(Ctx* ctx, const int* count, int n, int* pB, long long* nnz) {
   long long* incl = static_cast<long long*>(tmp);
   long long* sums = incl + (size_t)(nb > 0 ? nb : 1) * kScanBlock;
   long long* total = sums + nb + 1;
+  {
+  ScopedTimer tm(ctx, SPX_TIMER_CSC);
   if (n > 0) {
     hipLaunchKernelGGL(scan_blocks, dim3(nb), dim3(kScanBlock), 0, st, count,
                        n, incl, sums);
@@ -117,6 +119,7 @@ int csc_count_scan(Ctx* ctx, const int* count, int n, int* pB, long long* nnz) {
   hipLaunchKernelGGL(scan_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st,
                      incl, sums, n, pB, total);
   SPX_HIP(hipGetLastError());
+  }
   long long h = 0;
   SPX_HIP(hipMemcpyAsync(&h, total, sizeof(long long), hipMemcpyDeviceToHost, st));
   SPX_HIP(hipStreamSynchronize(st));
@@ -128,6 +131,7 @@ template <typename T>
 int csc_fill(Ctx* ctx, const T* v, const int* r, int L, int n, const int* pB,
              int* rOut, T* vOut) {
   if (n <= 0) return SPX_OK;
+  ScopedTimer tm(ctx, SPX_TIMER_CSC);
   hipLaunchKernelGGL(fill_columns<T>, dim3((n + 3) / 4), dim3(256), 0,
                      ctx->stream, v, r, L, n, pB, rOut, vOut);
   SPX_HIP(hipGetLastError());

@@ -358,8 +358,12 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
                        static_cast<T>(o->L0), Lsig, active, best, info);
     SPX_HIP(hipGetLastError());
 
+    // launch timers only for eager launches on the context's stream (a
+    // captured loop records no events)
+    const bool timed = qs == st;
     // gradient of the square loss at `at`: R = D*at - X, grad = D'R
     auto gradient = [&](const T* at) -> int {
+      ScopedTimer tm(ctx, timed ? SPX_TIMER_FISTA_PRODUCTS : -1);
       int r = gemm_tn_product<T>(qs, m, n, p, Dt, p, at, p, R, m, T(1), T(-1), X, m, part, sR,
                                  prec);
       if (r) return r;
@@ -397,6 +401,7 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
         t = tn;
       }
       if (!bt) {
+        ScopedTimer tm(ctx, timed ? SPX_TIMER_FISTA_PROX : -1);
         hipLaunchKernelGGL(prox_step_kernel<T>, cols4(n), dim3(256), 0, qs, p, n,
                            W, y, grad, Lsig, active, pp, beta, ista, stopsum);
         SPX_HIP(hipGetLastError());

@@ -26,7 +26,8 @@
 //
 // Batched form: persistent grids pulling signal indices from atomic counters
 // (fast pass over 0..n-1, then the full pass over the overflow list, on the
-// same stream).  Plain form: one workgroup per signal, full capacity.
+// same stream).  Plain form: a grid of resident workgroups, each taking
+// signals blockIdx.x + k * gridDim.x in turn, full capacity.
 #include <climits>
 #include <type_traits>
 
@@ -277,6 +278,27 @@ __device__ __forceinline__ void inverse_drop(const T* Ms, int lds, T* Md, int ld
   }
 }
 
+// Squared residual of the least-squares fit on the active set,
+// normX0 - b' M b with b = D'x over the s active atoms and M the inverse of
+// their Gram block (G + diag I).  coreLARS with ols tracks this quantity in
+// place of the homotopy residual (decomp/decomp.h:1044-1050 on admission,
+// :1191-1201 after a drop); FP64 accumulation.  Every thread returns it.
+template <typename T, int W>
+__device__ __forceinline__ T ls_residual(const T* Mf, int ldf, int s, const T* dtx,
+                                         const int* sInd, T normX0, double* sRes) {
+  constexpr int GT = 64 * W;
+  double acc = 0;
+  for (int r = threadIdx.x; r < s; r += GT) {
+    double row = 0;
+    for (int q = 0; q < s; ++q)
+      row += static_cast<double>(Mf[r + q * ldf]) * static_cast<double>(dtx[sInd[q]]);
+    acc += row * static_cast<double>(dtx[sInd[r]]);
+  }
+  double z1 = 0, z2 = 0;
+  group_sum3<double, W>(acc, z1, z2, sRes);
+  return static_cast<T>(static_cast<double>(normX0) - acc);
+}
+
 // The full-capacity pass at 4 wavefronts is held to 4 workgroups per CU
 // (128 VGPRs): its signals are latency-bound, so occupancy pays more than
 // registers.
@@ -365,7 +387,9 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         if (PREFETCH && tid == 0 && w < nwork) pre = atomicAdd(a.queue, chunk);
       }
     } else {
-      w = round == 0 ? (int)blockIdx.x : nwork;
+      // plain form: a static grid-stride assignment (no work queue, no order)
+      const long wl = (long)blockIdx.x + (long)round * gridDim.x;
+      w = wl < nwork ? (int)wl : nwork;
       wend = w + 1;
     }
     if (w >= nwork) break;
@@ -540,6 +564,11 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
             if (inl) inverse_admit<T, W, true, true>(sMl, CL, i, sGs, sV, sPart, sRed3, sRes);
             else inverse_admit<T, W, CAP == 0, CAP == 0>(Mc, ldm, i, sGs, sV, sPart, sRed3, sRes);
           }
+          // ols: normX is the least-squares residual of the active set
+          // (decomp/decomp.h:1044-1050)
+          if (a.ols)
+            normX = ls_residual<T, W>(inl ? sMl : Mc, inl ? CL : ldm, i + 1, dtx, sInd,
+                                      a.norms[sig], sRes);
         }
 
         const int s = i + 1;
@@ -761,10 +790,11 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         step = step < step_max2 ? step : step_max2;
         step = step < step_max ? step : step_max;
         if (step == INF) break;
-        if (a.chol) {
+        if (a.chol && !a.ols) {
           // Cholesky variant: a step that would increase the residual ends
           // the path and drops the entry at the current position
-          // (decomp/decomp.h:1036-1043)
+          // (decomp/decomp.h:1036-1043; not with ols, whose residual is the
+          // least-squares one)
           const T t = coeff1 * step * step - 2 * coeff2 * step;
           if (t > T(0) || !(fabs(t) < INF)) {
             s_active = i;
@@ -785,7 +815,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
 #pragma unroll
           for (int e = 0; e < VEC; ++e) sDtR[j0 + e] -= step * Areg[q][e];
         }
-        normX += coeff1 * step * step - 2 * coeff2 * step;
+        if (!a.ols) normX += coeff1 * step * step - 2 * coeff2 * step;
         thrs += step * coeff1;
         __syncthreads();
         if (a.path && sig == 0 && iter < a.path_cols) {
@@ -842,6 +872,11 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           }
           --s_active;
           newAtom = false;
+          // ols: the least-squares residual of the reduced active set
+          // (downDateLasso, decomp/decomp.h:1191-1201)
+          if (a.ols && s_active > 0)
+            normX = ls_residual<T, W>(inl ? sMl : Mc, inl ? CL : ldm, s_active, dtx, sInd,
+                                      a.norms[sig], sRes);
           i = i - 2;
           // an emptied active set has no direction: end the path (the
           // reference would read ind[-1] here)
@@ -961,7 +996,8 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
 }
 
 // Launches one instantiation: persistent grid over a counter (batched) or
-// one workgroup per signal (plain; CAP must be 0).
+// a grid-stride grid with one signal per workgroup at a time (plain; CAP
+// must be 0).
 template <typename T, int W, int NQ, int CAP, int VAR>
 int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
                 long max_blocks) {
@@ -1019,50 +1055,30 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
     SPX_HIP(hipGetLastError());
     return SPX_OK;
   }
-  // plain form: one workgroup per signal, chunks bounded by scratch size
+  // plain form: one grid of as many workgroups as fit the CUs at once (at
+  // most one per signal), each walking the signals blockIdx.x, blockIdx.x +
+  // gridDim.x, ... with its own inverse slot: no work queue, no ordering
+  int per_cu = 0;
+  SPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds));
+  if (per_cu < 1) per_cu = 1;
+  long blocks = (long)per_cu * ctx->num_cus;
+  if (blocks > a.n) blocks = a.n;
+  if (blocks < 1) blocks = 1;
   const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
-  const size_t ga_stride = VAR == 2 ? (size_t)a.ldg * a.L : 0;  // masked Gram columns
-  const size_t per_block = slot + sizeof(T) * ga_stride;
-  long chunk = (long)(512ull * 1024 * 1024 / (per_block ? per_block : 1));
-  if (chunk > 65535) chunk = 65535;
-  if (chunk < 1) chunk = 1;
-  const long alloc_chunk = chunk < a.n ? chunk : a.n;
   T* scratch = nullptr;
-  int rc = ctx->scratch_get(0, slot * (alloc_chunk > 0 ? alloc_chunk : 1),
-                            reinterpret_cast<void**>(&scratch));
+  int rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
   if (rc) return rc;
-  T* ga = nullptr;
+  b.queue = nullptr;
+  b.list = nullptr;
+  b.scratch = scratch;
   if (VAR == 2) {
-    rc = ctx->scratch_get(17, sizeof(T) * ga_stride * (alloc_chunk > 0 ? alloc_chunk : 1),
-                          reinterpret_cast<void**>(&ga));
+    // masked signals: L Gram columns per workgroup
+    b.ga_stride = (size_t)a.ldg * a.L;
+    rc = ctx->scratch_get(17, sizeof(T) * b.ga_stride * blocks, reinterpret_cast<void**>(&b.ga));
     if (rc) return rc;
   }
-  const int lds_cap = b.lds_cap;
-  for (long s0 = 0; s0 < a.n; s0 += chunk) {
-    const long cnt = (a.n - s0) < chunk ? (a.n - s0) : chunk;
-    b = a;
-    b.lds_cap = lds_cap;
-    b.queue = nullptr;
-    b.list = nullptr;
-    b.scratch = scratch;
-    b.n = (int)cnt;
-    b.DtX = a.DtX + (size_t)s0 * a.ldd;
-    b.norms = a.norms + s0;
-    if (a.nlam > 1) b.lambda = a.lambda + s0;
-    if (a.weights) b.weights = a.weights + (size_t)s0 * a.ldw;
-    if (VAR == 2) {
-      b.mobs = a.mobs + s0;
-      b.mrows = a.mrows + (size_t)s0 * a.m;
-      b.ga = ga;
-      b.ga_stride = ga_stride;
-    }
-    b.outV = a.outV + (size_t)s0 * a.L;
-    b.outR = a.outR + (size_t)s0 * a.L;
-    b.outCount = a.outCount + s0;
-    if (a.stats) b.stats = a.stats + 2 * s0;
-    hipLaunchKernelGGL(kern, dim3((unsigned)cnt), dim3(64 * W), lds, st, b);
-    SPX_HIP(hipGetLastError());
-  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, b);
+  SPX_HIP(hipGetLastError());
   return SPX_OK;
 }
 

@@ -198,7 +198,8 @@ template <typename T>
 bool ormp_fits(int w, int p);
 
 // Records the duration of the enclosed launches on the context's stream
-// under timer `id` (SPX_TIMER_*) while timing is switched on; no-op else.
+// under timer `id` (SPX_TIMER_*) while timing is switched on; no-op else,
+// and for a negative id.
 class ScopedTimer {
  public:
   ScopedTimer(Ctx* ctx, int id);

@@ -29,7 +29,8 @@ PREC = {"fp32": 0, "bf16x3": 1, "fp32+refit64": 2, "bf16x3+refit64": 3}
 FORM = {"batched": 0, "plain": 1}
 SOLVERS = {"lasso": 0, "omp": 1}
 # launch timers of spx_ctx_read_timing, in SPX_TIMER_* order
-TIMERS = ("lars_order", "lars_fast", "lars_full", "ormp", "lars_list")
+TIMERS = ("lars_order", "lars_fast", "lars_full", "ormp", "lars_list", "fista_prox",
+          "fista_products", "csc")
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -313,8 +314,9 @@ def lasso(X, D=None, Q=None, q=None, return_reg_path=False, L=-1,
     L2ERROR mode its last step is not cut at the residual target, and a step
     that would increase the residual ends the path (the engine keeps an
     explicit inverse of the active Gram block either way); ``ols=True``
-    returns the least-squares fit on each final active set (not in L2ERROR
-    mode).  With ``Q``/``q`` (Gram matrix and correlations) the Gram is
+    implies ``cholesky=True`` (as in the reference), tracks the
+    least-squares residual of the active set (the L2ERROR stop reads it) and
+    returns the least-squares fit on each final active set.  With ``Q``/``q`` (Gram matrix and correlations) the Gram is
     used as given.  ``return_reg_path`` also returns the first signal's path
     (p x max_length_path).  ``numThreads`` is accepted and ignored."""
     if Q is not None:

@@ -146,6 +146,15 @@ def main():
         pB, r, v = csc_parts(A)
         cases.append((f"lassools_{tag}", dict(kind="lasso", X=X, D=D, lam=np.asarray(0.05, dt),
                                               pB=pB, r=r, v=v)))
+        # ols in each mode: the Cholesky variant tracking the least-squares
+        # residual (its L2ERROR stop reads it, decomp/decomp.h:1044-1050)
+        Xo, Do = small(42, 64, 256, 40, 5, 0.05, True, dt)
+        for mode, lam in ((1, 0.3), (1, 0.05), (0, 0.8), (2, 0.1)):
+            A = oracle.ref_lasso(Xo, Do, lam, mode=mode, ols=True)
+            pB, r, v = csc_parts(A)
+            cases.append((f"lassools_{tag}_mode{mode}_lam{lam}", dict(
+                kind="lasso", X=Xo, D=Do, lam=np.asarray(lam, dt), mode=mode, ols=1,
+                pB=pB, r=r, v=v)))
         X, D = small(18, 64, 256, 10, 8, 0.01, False, dt)
         A, path = oracle.ref_omp(X, D, L=12, return_reg_path=True)
         pB, r, v = csc_parts(A)

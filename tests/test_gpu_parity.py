@@ -645,26 +645,25 @@ def test_lasso_cholesky_matches_golden(spx, name):
         assert np.abs(A[:, same] - ref[:, same]).max(initial=0) <= 2e-4
 
 
-def test_lasso_ols_with_l2error_is_refused(spx):
-    """ols in L2ERROR mode stops on the least-squares residual in the
-    reference; the engine refuses it rather than return another path."""
-    print("case lasso ols l2error refused", flush=True)
-    g = load_golden("lassools_f64.npz")
-    with pytest.raises(RuntimeError):
-        spx.lasso(g["X"], g["D"], lambda1=0.3, mode=1, ols=True)
-
-
-def test_lasso_ols_matches_golden_f64(spx):
-    print("case lasso ols f64", flush=True)
-    g = load_golden("lassools_f64.npz")
-    As = spx.lasso(g["X"], g["D"], lambda1=g["lam"].item(), ols=True)
-    assert_csc_layout(As, g["pB"], g["r"])
-    A = dense(As)
-    ref = np.zeros(A.shape)
-    for j in range(ref.shape[1]):
-        ref[g["r"][g["pB"][j]:g["pB"][j + 1]], j] = g["v"][g["pB"][j]:g["pB"][j + 1]]
-    assert np.array_equal(A != 0, ref != 0)
-    np.testing.assert_allclose(A, ref, atol=1e-9)
+@pytest.mark.parametrize("name", [f for f in golden_files("lassools_") if "f64" in f])
+def test_lasso_ols_matches_golden_f64(spx, name):
+    """ols in every mode against the reference's ols run (FP64): the path of
+    the Cholesky variant with the least-squares residual tracked (the
+    L2ERROR stop reads it), then the least-squares fit on the final active
+    set.  Same supports and CSC arrays, coefficients within 1e-9, batched
+    and plain forms."""
+    print("case", name, flush=True)
+    g = load_golden(name)
+    mode = int(g["mode"]) if "mode" in g else 2
+    for form in ("batched", "plain"):
+        As = spx.lasso(g["X"], g["D"], lambda1=g["lam"].item(), mode=mode, ols=True, form=form)
+        assert_csc_layout(As, g["pB"], g["r"])
+        A = dense(As)
+        ref = np.zeros(A.shape)
+        for j in range(ref.shape[1]):
+            ref[g["r"][g["pB"][j]:g["pB"][j + 1]], j] = g["v"][g["pB"][j]:g["pB"][j + 1]]
+        assert np.array_equal(A != 0, ref != 0), form
+        np.testing.assert_allclose(A, ref, atol=1e-9)
 
 
 def test_lasso_ols_f32_is_least_squares_on_its_support(spx, orc):

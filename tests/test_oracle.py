@@ -458,6 +458,34 @@ def test_reference_ols_is_least_squares_on_the_lasso_support(orc):
     assert np.abs(base - ols).max() > 1e-3
 
 
+@pytest.mark.parametrize("name", [f for f in golden_files("lassools_") if "f64_mode" in f])
+def test_reference_ols_fixtures_are_least_squares_fits(orc, name):
+    """The ols fixtures (reference _lassoD with ols in modes 0-2, FP64):
+    every column is the least-squares fit (D_A'D_A + 1e-10 I)^-1 D_A'x on its
+    support; in L2ERROR mode the path stops on the least-squares residual,
+    so the fit's squared residual is within the target lambda (the homotopy
+    residual of the plain path at that support would not be)."""
+    g = load_golden(name)
+    import scipy.sparse as sp
+    A = sp.csc_matrix((g["v"], g["r"], g["pB"]),
+                      shape=(g["D"].shape[1], g["X"].shape[1])).toarray()
+    X, D, lam = g["X"], g["D"], g["lam"].item()
+    for j in range(X.shape[1]):
+        S = np.nonzero(A[:, j])[0]
+        if not len(S):
+            continue
+        Ds = D[:, S]
+        c = np.linalg.solve(Ds.T @ Ds + 1e-10 * np.eye(len(S)), Ds.T @ X[:, j])
+        np.testing.assert_allclose(A[S, j], c, rtol=1e-8, atol=1e-11)
+        if int(g["mode"]) == 1:
+            r = X[:, j] - Ds @ c
+            assert r @ r <= lam + 1e-12
+    if int(g["mode"]) == 1:
+        # the least-squares stop ends earlier than coreLARS2's homotopy stop
+        B = orc.lasso(X, D, lam, mode=1).toarray()
+        assert (A != 0).sum() < (B != 0).sum()
+
+
 @pytest.mark.parametrize("name", golden_files("lassochol_"))
 def test_lasso_cholesky_matches_golden(orc, name):
     """cholesky=True (coreLARS, decomp/decomp.h:844-1093): the restatement
