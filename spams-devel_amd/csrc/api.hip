@@ -116,6 +116,22 @@ __global__ __launch_bounds__(1024) void norm2_kernel(int p, const T* __restrict_
   if (threadIdx.x == 0) *out = sqrt(red[0]);
 }
 
+// G(j, j) += v for j < p (leading dimension ldg)
+template <typename T>
+__global__ void add_diag_kernel(T* G, int ldg, int p, T v) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < p) G[(size_t)j * ldg + j] += v;
+}
+
+// dst = src + v I (p x p, leading dimension ldg, padding rows included)
+template <typename T>
+static int gram_with_ridge(hipStream_t st, const T* src, T* dst, int ldg, int p, T v) {
+  SPX_HIP(hipMemcpyAsync(dst, src, sizeof(T) * (size_t)ldg * p, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(add_diag_kernel<T>, dim3((p + 255) / 256), dim3(256), 0, st, dst, ldg, p, v);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
 template <typename T>
 __global__ void fill_kernel(T* v, int n, T val) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -215,7 +231,8 @@ static int dict_create(spx_ctx* ctx, const T* D, int m, int p, int on_device,
   const size_t nD = (size_t)m * p;
   if (hipMalloc(&d->D, nD * sizeof(T)) != hipSuccess ||
       hipMalloc(&d->Dt, nD * sizeof(T)) != hipSuccess ||
-      hipMalloc(&d->G, (size_t)d->ldg * p * sizeof(T)) != hipSuccess) {
+      hipMalloc(&d->G, (size_t)d->ldg * p * sizeof(T)) != hipSuccess ||
+      hipMalloc(&d->Gr, (size_t)d->ldg * p * sizeof(T)) != hipSuccess) {
     spx_dict_destroy(d);
     set_error("dict_create: device allocation failed (p=%d)", p);
     return SPX_ERR_NOMEM;
@@ -231,6 +248,9 @@ static int dict_create(spx_ctx* ctx, const T* D, int m, int p, int on_device,
   int rc = gemm_tn<T>(st, p, p, m, static_cast<const T*>(d->D), m,
                       static_cast<const T*>(d->D), m, static_cast<T*>(d->G),
                       d->ldg, T(1), T(0), nullptr, 0, precision);
+  if (!rc)
+    rc = gram_with_ridge<T>(st, static_cast<const T*>(d->G), static_cast<T*>(d->Gr), d->ldg, p,
+                            static_cast<T>(1e-10));
   if (rc) {
     spx_dict_destroy(d);
     return rc;
@@ -346,11 +366,24 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   DeviceGuard g(ctx->device);
   LarsArgs<T> a{};
   a.tol = T(1e-15);
-  a.G = static_cast<const T*>(d->G);
   a.ldg = d->ldg;
-  // G + max(lambda2, 1e-10) I (decomp/decomp.h:1566); a Gram given as such
-  // is used unchanged (decomp/decomp.h:1574)
+  // G + max(lambda2, 1e-10) I (decomp/decomp.h:1566, ProdMatrix::addDiag);
+  // a Gram given as such is used unchanged (decomp/decomp.h:1574).  The
+  // dictionary holds G + 1e-10 I; another ridge is formed in scratch.
   a.diag = d->gram_only ? T(0) : static_cast<T>(lambda2 > T(1e-10) ? double(lambda2) : 1e-10);
+  if (d->gram_only || W) {
+    // lassoWeight's ridge is 1e-10 (decomp/decomp.h:1337), set below
+    a.G = static_cast<const T*>(d->gram_only ? d->G : d->Gr);
+  } else if (!(lambda2 > T(1e-10))) {
+    a.G = static_cast<const T*>(d->Gr);
+  } else {
+    T* Gl = nullptr;
+    int rc = ctx->scratch_get(22, sizeof(T) * (size_t)d->ldg * d->p, reinterpret_cast<void**>(&Gl));
+    if (rc) return rc;
+    rc = gram_with_ridge<T>(ctx->stream, static_cast<const T*>(d->G), Gl, d->ldg, d->p, a.diag);
+    if (rc) return rc;
+    a.G = Gl;
+  }
   a.DtX = DtX;
   a.ldd = ldd;
   a.norms = norms;
@@ -1263,6 +1296,7 @@ void spx_dict_destroy(spx_dict* d) {
   if (d->D) (void)hipFree(d->D);
   if (d->Dt) (void)hipFree(d->Dt);
   if (d->G) (void)hipFree(d->G);
+  if (d->Gr) (void)hipFree(d->Gr);
   delete d;
 }
 
@@ -1275,7 +1309,8 @@ int spx_dict_dims(const spx_dict* d, int* m, int* p) {
 
 size_t spx_dict_bytes(const spx_dict* d) {
   if (!d) return 0;
-  return (size_t)d->elem * ((size_t)2 * d->m * d->p + (size_t)d->ldg * d->p);
+  return (size_t)d->elem * ((size_t)2 * d->m * d->p +
+                            (size_t)d->ldg * d->p * (d->Gr ? 2 : 1));
 }
 
 int spx_dict_autotune_lasso_f32(spx_ctx* ctx, spx_dict* d, const float* X, int ldx, int n,

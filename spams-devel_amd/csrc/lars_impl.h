@@ -4,7 +4,7 @@
 // One group of W wavefronts (one workgroup) solves one signal at a time:
 //   - the correlation vector DtR (p) lives in LDS, the active-atom flags in a
 //     bit mask of the thread that owns the rows;
-//   - the correlation change A = (G + diag I)(:, active) * u is accumulated
+//   - the correlation change A = G(:, active) * u is accumulated
 //     in registers (each thread owns NQ chunks of VEC = 16 bytes of rows),
 //     streaming the active Gram columns from the resident G (L2 / MALL),
 //     several columns in flight per round;
@@ -155,7 +155,7 @@ __device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
 
 // Admits the i-th active atom (i >= 1) into the explicit inverse M of the
 // active Gram block, i x i -> (i+1) x (i+1), by its Schur complement
-// (decomp/decomp.h:1756-1770); sGs(0:i+1) = (G + diag I)(active, new atom).
+// (decomp/decomp.h:1756-1770); sGs(0:i+1) = G(active, new atom).
 // v = M gs lands in sV.  An FP32 pivot below 1e-3 of the diagonal (or not
 // positive) is recomputed with FP64 accumulation.  ELEM: the rank-one update
 // runs one element per thread with eight loads in flight (M in HBM, or a
@@ -188,23 +188,39 @@ __device__ __forceinline__ void inverse_admit(T* M, int ldm, int i, const T* sGs
     schur = static_cast<T>(1.0 / (static_cast<double>(sGs[i]) - pd));
     __syncthreads();
   }
-  if (ELEM) {
+  if (ELEM && i > 0) {
+    // element e = r + q i of the i x i block; (r, q) advance by the uniform
+    // (GT % i, GT / i) per GT elements, so no per-element division
     const int nn = i * i;
+    const int dqG = GT / i, drG = GT - dqG * i;
+    int r0 = tid % i, q0 = tid / i;
     for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
       T mv[8];
+      int rr[8], qq[8];
+      int r = r0, q = q0;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
+        rr[u] = r;
+        qq[u] = q;
         const int e = e0 + u * GT;
-        mv[u] = e < nn ? M[e % i + (e / i) * ldm] : T(0);
+        mv[u] = e < nn ? M[r + q * ldm] : T(0);
+        r += drG;
+        q += dqG;
+        if (r >= i) {
+          r -= i;
+          ++q;
+        }
       }
+      r0 = r;
+      q0 = q;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = e0 + u * GT;
         if (e < nn) {
-          const int r = e % i, q = e / i;
-          const T vr = sV[r], vq = sV[q];
-          const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-          M[r + q * ldm] = mv[u] + upd;
+          const int ru = rr[u], qu = qq[u];
+          const T vr = sV[ru], vq = sV[qu];
+          const T upd = ru <= qu ? vr * (schur * vq) : vq * (schur * vr);
+          M[ru + qu * ldm] = mv[u] + upd;
         }
       }
     }
@@ -243,24 +259,38 @@ __device__ __forceinline__ void inverse_drop(const T* Ms, int lds, T* Md, int ld
   }
   __syncthreads();
   const T alpha = T(-1) / schur;
-  if (ELEM) {
+  if (ELEM && i > 0) {
+    // (r, q) advanced incrementally as in inverse_admit
     const int nn = i * i;
+    const int dqG = GT / i, drG = GT - dqG * i;
+    int r0 = tid % i, q0 = tid / i;
     for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
       T mv[8];
+      int rr[8], qq[8];
+      int r = r0, q = q0;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
+        rr[u] = r;
+        qq[u] = q;
         const int e = e0 + u * GT;
-        const int r = e % i, q = e / i;
         mv[u] = e < nn ? Ms[r + (r >= fz) + (q + (q >= fz)) * lds] : T(0);
+        r += drG;
+        q += dqG;
+        if (r >= i) {
+          r -= i;
+          ++q;
+        }
       }
+      r0 = r;
+      q0 = q;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = e0 + u * GT;
         if (e < nn) {
-          const int r = e % i, q = e / i;
-          const T ur = sV[r], uq = sV[q];
-          const T upd = r <= q ? ur * (alpha * uq) : uq * (alpha * ur);
-          Md[r + q * ldd] = mv[u] + upd;
+          const int ru = rr[u], qu = qq[u];
+          const T ur = sV[ru], uq = sV[qu];
+          const T upd = ru <= qu ? ur * (alpha * uq) : uq * (alpha * ur);
+          Md[ru + qu * ldd] = mv[u] + upd;
         }
       }
     }
@@ -280,7 +310,7 @@ __device__ __forceinline__ void inverse_drop(const T* Ms, int lds, T* Md, int ld
 
 // Squared residual of the least-squares fit on the active set,
 // normX0 - b' M b with b = D'x over the s active atoms and M the inverse of
-// their Gram block (G + diag I).  coreLARS with ols tracks this quantity in
+// their Gram block.  coreLARS with ols tracks this quantity in
 // place of the homotopy residual (decomp/decomp.h:1044-1050 on admission,
 // :1191-1201 after a drop); FP64 accumulation.  Every thread returns it.
 template <typename T, int W>
@@ -301,11 +331,13 @@ __device__ __forceinline__ T ls_residual(const T* Mf, int ldf, int s, const T* d
 
 // The full-capacity pass at 4 wavefronts is held to 4 workgroups per CU
 // (128 VGPRs): its signals are latency-bound, so occupancy pays more than
-// registers.
+// registers.  The LDS pass at 1 or 2 wavefronts with up to 4 chunks per
+// lane is held to 4 waves per SIMD (128 VGPRs, a few 64-bit values spilled
+// outside the step loop) for the same reason.
 // VAR: 0 plain penalty, 1 weighted penalty (a.weights), 2 masked signals
 // (a.mobs).
 template <typename T, int W, int NQ, int CAP, int VAR>
-__global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_kernel(LarsArgs<T> a) {
+__global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 && NQ <= 4) ? 4 : 1) void lars_kernel(LarsArgs<T> a) {
   constexpr int GT = 64 * W;
   constexpr int VEC = Vec16<T>::N;
   constexpr int PT = VEC * GT * NQ;  // rows covered by the group
@@ -362,7 +394,6 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
   T* const M1 = CAP > 0 ? sM1 : M0 + (size_t)L * L;
   const int ldm = CAP > 0 ? CAP : L;
   const int Lm = L < p ? L : p;
-  const T diag = a.diag;
   const int nwork = a.list ? *a.list_count : a.n;
 
   // work distribution: chunks of `chunk` consecutive work items per atomic
@@ -428,6 +459,11 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           if (!resume) v[e] = v[e] / we;
         }
       }
+      // rows past p hold +INF: both step candidates of such a row are +INF
+      // (or NaN, which never wins), so the search needs no row bound
+#pragma unroll
+      for (int e = 0; e < VEC; ++e)
+        if (j0 + e >= p) v[e] = INF;
       *reinterpret_cast<V*>(&sDtR[j0]) = v;
     }
     // active flags of this thread's rows (bit q * VEC + e)
@@ -446,6 +482,9 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
     const bool own = mk && nobs > 0 && nobs < a.m;
     const int* mrl = own ? a.mrows + (size_t)sig * a.m : nullptr;
     const bool mcomp = own && masked_gram_comp(nobs, a.m);
+    // the ridge of a masked Gram column summed over the observed rows (the
+    // complement form starts from G, which carries it)
+    const T mdiag = mcomp ? T(0) : a.diag;
     const int mnsel = own ? masked_gram_nsel(nobs, a.m) : 0;
     if (mk)
       for (int j = tid; j < LC; j += GT) sSlot[j] = j;
@@ -455,7 +494,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         // the Gram columns of the handed-over active set, slots 0..s0-1
         for (int k = 0; k < s0; ++k)
           masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.G, a.m, p, ldg, mrl, mnsel, mcomp, st.ind[k],
-                                     gaw + (size_t)k * ldg, T(0));
+                                     gaw + (size_t)k * ldg, mdiag);
       }
       for (int j = tid; j < LC; j += GT) {
         sCoef[j] = j < s0 ? st.coef[j] : T(0);
@@ -538,13 +577,11 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           const T* gcol = a.G + (size_t)c * ldg;
           if (own) {
             T* slot = gaw + (size_t)sSlot[i] * ldg;
-            masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.G, a.m, p, ldg, mrl, mnsel, mcomp, c, slot, T(0));
+            masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.G, a.m, p, ldg, mrl, mnsel, mcomp, c, slot,
+                                       mdiag);
             gcol = slot;
           }
-          for (int j = tid; j <= i; j += GT) {
-            const int r = sInd[j];
-            sGs[j] = gcol[r] + (r == c ? diag : T(0));
-          }
+          for (int j = tid; j <= i; j += GT) sGs[j] = gcol[sInd[j]];
           __syncthreads();
           if (i == 0) {
             if (tid == 0) {
@@ -613,46 +650,46 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
         }
         const T cc = fabs(sDtR[sInd[0]]);
 
-        // ------------------------------- A = (G + diag I)(:,active) * u
+        // ------------------------------------- A = G(:, active) * u
         V Areg[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) Areg[q] = V(0);
         // UNR Gram columns per round so their loads are in flight together;
-        // the sum over k keeps its sequential order
-        for (int k0 = 0; k0 < s; k0 += UNR) {
-          T uk[UNR];
-          const T* col[UNR];
+        // the sum over k keeps its sequential order.  The loads are
+        // unconditional, so no branch or register reuse serialises them:
+        // when the group's rows fit the column (PT <= ldg) they address it
+        // directly (immediate offsets); else rows past ldg read the column's
+        // first chunk, which only rows past p, never candidates, accumulate.
+        auto stream = [&](auto offset) {
+          for (int k0 = 0; k0 < s; k0 += UNR) {
+            T uk[UNR];
+            const T* col[UNR];
 #pragma unroll
-          for (int e = 0; e < UNR; ++e) {
-            const int k = k0 + e < s ? k0 + e : k0;
-            uk[e] = k0 + e < s ? sU[k] : T(0);
-            col[e] = own ? gaw + (size_t)sSlot[k] * ldg : a.G + (size_t)sInd[k] * ldg;
-          }
-          V g[UNR][NQ];
-#pragma unroll
-          for (int e = 0; e < UNR; ++e)
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const int j0 = VEC * (tid + GT * q);
-              g[e][q] = j0 < ldg ? *reinterpret_cast<const V*>(col[e] + j0) : V(0);
+            for (int e = 0; e < UNR; ++e) {
+              const int k = k0 + e < s ? k0 + e : s - 1;
+              const T ur = sU[k];
+              uk[e] = k0 + e < s ? ur : T(0);
+              col[e] = own ? gaw + (size_t)sSlot[k] * ldg : a.G + (size_t)sInd[k] * ldg;
             }
+            V g[UNR][NQ];
 #pragma unroll
-          for (int e = 0; e < UNR; ++e)
-            if (k0 + e < s) {
+            for (int e = 0; e < UNR; ++e)
+#pragma unroll
+              for (int q = 0; q < NQ; ++q)
+                g[e][q] = *reinterpret_cast<const V*>(col[e] + offset(q));
+#pragma unroll
+            for (int e = 0; e < UNR; ++e)
 #pragma unroll
               for (int q = 0; q < NQ; ++q) Areg[q] += uk[e] * g[e][q];
-            }
-        }
-        for (int k = 0; k < s; ++k) {
-          const int r = sInd[k];
-          const int rc = r / VEC;
-          if (rc % GT == tid) {
-            const int q = rc / GT;
-            const T add = diag * sU[k];
-#pragma unroll
-            for (int qq = 0; qq < NQ; ++qq)
-              if (qq == q) Areg[qq][r % VEC] += add;
           }
+        };
+        if (PT <= ldg) {
+          stream([&](int q) { return VEC * (tid + GT * q); });
+        } else {
+          stream([&](int q) {
+            const int j0 = VEC * (tid + GT * q);
+            return j0 < ldg ? j0 : 0;
+          });
         }
         if (wt) {
           // A ./= weights (decomp/decomp.h:2005)
@@ -679,7 +716,8 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
               const int j = VEC * (tid + GT * q) + e;
-              const bool cand = (j < p) & !((act >> (q * VEC + e)) & 1);
+              // rows past p carry d = +INF, hence keys +INF (or NaN)
+              const bool cand = !((act >> (q * VEC + e)) & 1);
               const T Aj = Areg[q][e];
               const T d = sDtR[j];
               const bool ok1 = cand & (Aj < INF) & (Aj > T(-1)) & !a.pos;
@@ -902,7 +940,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
     }
 
     // ---------- optional FP64 refit (PENALTY, at lambda) or least-squares fit
-    // ols: alpha_A = (G_AA + diag I)^-1 (D'x)_A, the coefficients the
+    // ols: alpha_A = G_AA^-1 (D'x)_A, the coefficients the
     // reference's ols option returns (decomp/decomp.h:1087-1091)
     if (!mk && ((a.refit && reached) || a.ols) && !overflow && s_active > 0) {
       __syncthreads();
@@ -918,10 +956,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) ? 4 : 1) void lars_ker
           const T* gcol = a.G + (size_t)ir * ldg;
           double acc = static_cast<double>(dtx[ir]);
           if (!a.ols) acc -= static_cast<double>(lam) * (sCoef[r] > 0 ? 1.0 : -1.0);
-          for (int q = 0; q < s; ++q) {
-            const double g = static_cast<double>(gcol[sInd[q]]) + (q == r ? static_cast<double>(diag) : 0.0);
-            acc -= g * sAlpha[q];
-          }
+          for (int q = 0; q < s; ++q) acc -= static_cast<double>(gcol[sInd[q]]) * sAlpha[q];
           sRes[r] = acc;
         }
         __syncthreads();

@@ -20,9 +20,11 @@ typedef ::spx_dict Dict;
 // Kernel arguments of the LARS solver (coreLARS2 semantics).
 template <typename T>
 struct LarsArgs {
-  const T* G;       // p x p Gram without diagonal shift, leading dim ldg
+  const T* G;       // p x p Gram with the ridge on its diagonal, leading dim ldg
   int ldg;
-  T diag;           // max(lambda2, 1e-10), added to G's diagonal on the fly
+  T diag;           // the ridge max(lambda2, 1e-10), which G carries on its
+                    // diagonal; added to the masked Gram columns formed over
+                    // the observed rows (0 when G is used as given)
   const T* DtX;     // p x n, leading dim ldd
   int ldd;
   const T* norms;   // ||x_i||^2, n
@@ -262,6 +264,10 @@ struct spx_dict {
   void* D = nullptr;    // m x p
   void* Dt = nullptr;   // p x m
   void* G = nullptr;    // ldg x p
+  // G + 1e-10 I: the Gram the LARS solver uses for lambda2 <= 1e-10 (the
+  // ridge max(lambda2, 1e-10) of decomp/decomp.h:1566); null for a Gram-only
+  // dictionary (used as given)
+  void* Gr = nullptr;
   double lipschitz = 0; // largest eigenvalue of G (power iteration), lazily
   int waves[2] = {0, 0};     // autotuned wavefronts per signal (LARS, OMP)
   int gram_only = 0;         // created from a Gram matrix (no D; no ridge)
