@@ -166,14 +166,19 @@ template <typename T>
 int fista_run(Ctx* ctx, const Dict* dict, const T* X, int n, const T* W0,
               T* W, const spx_fista_opts* o, T* info);
 
-// order[i] = indices 0..n-1 sorted by key ascending (stable); scratch slot 9.
+// order[i] = indices 0..n-1 sorted by key ascending (stable radix sort on
+// the 16 leading bits of the keys rounded to FP32, NaN last: a work order,
+// never a result); scratch slot 9.
 template <typename T>
 int sort_order(Ctx* ctx, const T* key, int n, int* order);
 
-// out[0:count) = list[0:count) stably sorted by key[0:count) ascending,
-// where count = *count_dev (device value <= n); scratch slots 9, 11.
+// out[0:count) = list[0:count) stably sorted by key[0:count) ascending (as
+// sort_order), where count = *count_dev (device value <= n); scratch slot 9.
 template <typename T>
 int sort_list(Ctx* ctx, const T* key, const int* list, const int* count_dev, int n, int* out);
+
+// Scratch bytes the sorts of n entries take (slot 9).
+size_t sorted_scratch_bytes(int n);
 
 // Masked-signal set-up (one wavefront per signal): Xm (m x n) = X with the
 // unobserved rows (B == 0) zeroed, obs[i] = signal i's count of observed
