@@ -463,6 +463,27 @@ int spx_lasso_dtx_dev_f64(spx_ctx* ctx, const spx_dict* dict,
                           int n, const double* lambda, int n_lambda,
                           double lambda2, const spx_lasso_opts* opts,
                           spx_lists_dev* out);
+/* Allocation-free spx_lasso_dtx_dev_f32.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as spx_lasso_dtx_dev_f32, bit for bit; the
+ * caller passes the inputs and outputs, and every piece of solver scratch
+ * (work order, overflow list, hand-over states, inverse slots, a ridge Gram
+ * for lambda2 > 1e-10) comes from a static 64 MiB device array, one per
+ * device, instead of the context's allocations.  A call whose scratch
+ * exceeds 64 MiB fails with SPX_ERR_NOMEM and a message naming the need
+ * (about 6.6 KB per signal at p = 1024 when L > 24, plus 512 KB per
+ * workgroup of the full-capacity pass at L = 256; the pass runs as many
+ * workgroups as fit).  No hipMalloc, hipFree, event creation or memset.
+ * Every call on a device shares that device's array, so calls that overlap
+ * on one device (from any context or thread) race; calls in sequence are
+ * safe, and each clears what must start at zero. */
+int spx_lasso_dtx_dev_noalloc_f32(spx_ctx* ctx, const spx_dict* dict,
+                                  const float* DtX, int ldd,
+                                  const float* norms, int n,
+                                  const float* lambda, int n_lambda,
+                                  float lambda2, const spx_lasso_opts* opts,
+                                  spx_lists_dev* out);
 /* ||x_i||^2 of the n columns of X (device, leading dimension ldx). */
 int spx_sqnorms_dev_f32(spx_ctx* ctx, const float* X, int m, int ldx, int n,
                         float* norms);

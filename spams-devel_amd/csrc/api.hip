@@ -198,7 +198,7 @@ int tuned_waves(const Dict* d, int solver) {
 // ------------------------------------------------------------------ timers
 
 ScopedTimer::ScopedTimer(Ctx* ctx, int id) : ctx_(ctx), id_(id) {
-  if (!ctx_->timing || id_ < 0) return;
+  if (!ctx_->timing || id_ < 0 || ctx_->arena) return;
   a_ = ctx_->event_get();
   if (a_ && hipEventRecord(a_, ctx_->stream) != hipSuccess) a_ = nullptr;
 }
@@ -414,7 +414,11 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   if (out->path && out->path_cols > 0) {
     a.path = static_cast<T*>(out->path);
     a.path_cols = out->path_cols;
-    SPX_HIP(hipMemsetAsync(out->path, 0, sizeof(T) * (size_t)d->p * out->path_cols, ctx->stream));
+    // cleared by a kernel (no memset node when a caller captures the call)
+    const size_t np = (size_t)d->p * out->path_cols;
+    hipLaunchKernelGGL(fill_kernel<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
+                       ctx->stream, static_cast<T*>(out->path), (int)np, T(0));
+    SPX_HIP(hipGetLastError());
   }
   if (W) {
     // lassoWeight: G + 1e-10 I (decomp/decomp.h:1337), tolerance 1e-10 and
@@ -1108,6 +1112,24 @@ static int fista_host(spx_ctx* ctx, const T* X, int m, int n, const T* D, int p,
 
 int spx_ctx::scratch_get(int id, size_t bytes, void** out) {
   std::lock_guard<std::mutex> lk(mu);
+  if (arena) {
+    if ((size_t)id >= arena_slots.size()) arena_slots.resize(id + 1);
+    Slot& a = arena_slots[id];
+    if (!a.ptr || a.bytes < bytes) {
+      const size_t want = (bytes + 255) & ~size_t(255);
+      if (want > arena_bytes - arena_used) {
+        spx::set_error("allocation-free call: scratch of %zu bytes exceeds the %zu bytes left "
+                       "of the %zu-byte static scratch (use the allocating entry point, or "
+                       "fewer signals per call)", want, arena_bytes - arena_used, arena_bytes);
+        return SPX_ERR_NOMEM;
+      }
+      a.ptr = arena + arena_used;
+      a.bytes = want;
+      arena_used += want;
+    }
+    *out = a.ptr;
+    return SPX_OK;
+  }
   if ((size_t)id >= slots.size()) slots.resize(id + 1);
   Slot& s = slots[id];
   if (s.bytes < bytes) {
@@ -1187,6 +1209,66 @@ extern "C" {
 const char* spx_last_error(void) { return spx::g_err; }
 const char* spx_version(void) { return "spx 0.1 (gfx950)"; }
 
+// ---------------------------------------------- allocation-free entry point
+// Static scratch of the allocation-free entry point: one copy per device,
+// held by the loaded code object until the process ends.  Host code uses
+// only the device address of each copy (taken once per device at context
+// creation), never the symbol itself.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of this device's copy (the device must be current)
+int scratch_address(int device, char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    spx::set_error("spx: device %d beyond the supported %d", device, kMaxDevices);
+    return SPX_ERR_ARG;
+  }
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      spx::set_error("spx: address of the static scratch on device %d: %s", device,
+                     hipGetErrorString(e));
+      return SPX_ERR_HIP;
+    }
+    g_scratch_ptr[device] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return SPX_OK;
+}
+}  // namespace
+
+int spx_lasso_dtx_dev_noalloc_f32(spx_ctx* ctx, const spx_dict* d, const float* DtX, int ldd,
+                                  const float* norms, int n, const float* lambda, int n_lambda,
+                                  float lambda2, const spx_lasso_opts* o, spx_lists_dev* out) {
+  if (!ctx) {
+    spx::set_error("lasso_dtx_dev_noalloc: no context");
+    return SPX_ERR_ARG;
+  }
+  spx::DeviceGuard g(ctx->device);
+  char* base = nullptr;
+  int rc = scratch_address(ctx->device, &base);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->arena = base;
+    ctx->arena_bytes = kNoAllocScratchBytes;
+    ctx->arena_used = 0;
+    for (auto& sl : ctx->arena_slots) sl = spx_ctx::Slot{};
+  }
+  rc = spx::lasso_dtx_dev<float>(ctx, d, DtX, ldd, norms, n, lambda, n_lambda, lambda2, o, out);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->arena = nullptr;
+  ctx->arena_used = 0;
+  return rc;
+}
+
 int spx_ctx_create(spx_ctx** out, int device) {
   if (!out) return SPX_ERR_ARG;
   int ndev = 0;
@@ -1195,6 +1277,13 @@ int spx_ctx_create(spx_ctx** out, int device) {
     return SPX_ERR_HIP;
   }
   spx::DeviceGuard g(device);
+  {
+    // once per device, with the device current: the address of its copy of
+    // the allocation-free entry point's static scratch
+    char* sp = nullptr;
+    const int rs = scratch_address(device, &sp);
+    if (rs) return rs;
+  }
   spx_ctx* c = new spx_ctx();
   c->device = device;
   if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
@@ -1204,6 +1293,9 @@ int spx_ctx_create(spx_ctx** out, int device) {
     return SPX_ERR_HIP;
   }
   c->own_stream = true;
+  // slot table of allocation-free calls sized once here, so that such a call
+  // does not grow it
+  c->arena_slots.resize(32);
   *out = c;
   return SPX_OK;
 }

@@ -1073,6 +1073,11 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
     if (blocks < 1) blocks = 1;
     if (CAP == 0) {
       const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
+      // an allocation-free call runs as many workgroups as inverse slots
+      // fit its static scratch (the results do not depend on the grid)
+      const size_t room = ctx->scratch_room();
+      if (slot > 0 && room / slot < (size_t)blocks) blocks = (long)(room / slot);
+      if (blocks < 1) blocks = 1;
       T* scratch = nullptr;
       int rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
       if (rc) return rc;
@@ -1098,8 +1103,11 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
   if (per_cu < 1) per_cu = 1;
   long blocks = (long)per_cu * ctx->num_cus;
   if (blocks > a.n) blocks = a.n;
-  if (blocks < 1) blocks = 1;
   const size_t slot = 2 * sizeof(T) * (size_t)a.L * a.L;
+  const size_t per_block = slot + (VAR == 2 ? sizeof(T) * (size_t)a.ldg * a.L : 0);
+  const size_t room = ctx->scratch_room();
+  if (per_block > 0 && room / per_block < (size_t)blocks) blocks = (long)(room / per_block);
+  if (blocks < 1) blocks = 1;
   T* scratch = nullptr;
   int rc = ctx->scratch_get(0, slot * blocks, reinterpret_cast<void**>(&scratch));
   if (rc) return rc;

@@ -232,6 +232,15 @@ struct spx_ctx {
   };
   std::vector<Slot> slots;
   std::mutex mu;
+  // Allocation-free calls: while `arena` is set, scratch_get carves every
+  // slot from it (bump order, 256-byte aligned, a slot re-requested with no
+  // more bytes keeps its region) and fails past arena_bytes instead of
+  // allocating; launch timers record nothing.
+  char* arena = nullptr;
+  size_t arena_bytes = 0, arena_used = 0;
+  std::vector<Slot> arena_slots;
+  // bytes of the arena still free (SIZE_MAX when no arena is set)
+  size_t scratch_room() const { return arena ? arena_bytes - arena_used : ~size_t(0); }
   // launch timing (spx_ctx_set_timing): pending event pairs per timer id,
   // folded into the totals when read
   bool timing = false;

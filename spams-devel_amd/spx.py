@@ -765,15 +765,20 @@ def lasso_mask_dev(ctx, d: Dictionary, X, B, lam, V, R, count, lambda2=0.0, L=-1
 
 def lasso_dtx_dev(ctx, d: Dictionary, DtX, norms, lam, V, R, count, lambda2=0.0,
                   L=-1, mode=PENALTY, pos=False, max_length_path=-1,
-                  form="batched", waves_per_signal=0, stats=None):
-    """Device LARS from precomputed D'X (torch (n, ldd)) and ||x||^2 (n,)."""
+                  form="batched", waves_per_signal=0, stats=None, noalloc=False):
+    """Device LARS from precomputed D'X (torch (n, ldd)) and ||x||^2 (n,).
+    ``noalloc`` (FP32 only) selects the allocation-free entry point, whose
+    scratch is a static 64 MiB device array (it raises when that is too
+    small for the call)."""
     n = DtX.shape[0]
     Lout = V.shape[1]
     sfx = "f32" if d.dtype == np.float32 else "f64"
+    if noalloc and sfx != "f32":
+        raise ValueError("lasso_dtx_dev: the allocation-free entry point is FP32")
     o = spx_lasso_opts(L, mode, int(pos), max_length_path, 0, FORM[form],
                        waves_per_signal)
     lists = spx_lists_dev(_tptr(V), _tptr(R), _tptr(count), Lout, _tptr(stats))
-    f = getattr(lib(), "spx_lasso_dtx_dev_" + sfx)
+    f = getattr(lib(), ("spx_lasso_dtx_dev_noalloc_" if noalloc else "spx_lasso_dtx_dev_") + sfx)
     f.argtypes = [_P, _P, _P, _I, _P, _I, _P, _I, _ct(d.dtype),
                   ctypes.POINTER(spx_lasso_opts), ctypes.POINTER(spx_lists_dev)]
     _check(f(ctx.h, d.h, _tptr(DtX), DtX.shape[1], _tptr(norms), n, _tptr(lam),
