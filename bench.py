@@ -233,6 +233,9 @@ def served_report(n_jobs=120, rate=150.0, max_n=20000, lanes=4):
         rep["saturated"] = {"throughput_signals_per_s": srep["throughput_signals_per_s"],
                             "makespan_s": srep["makespan_s"], "batches": srv.n_batches - nb0}
         rep["autotune"] = {f"dict{k[0]}/{k[1]}": v[0] for k, v in tuned.items()}
+        rep["hbm"] = {"budget_bytes": srv.budget, "peak_used_bytes": int(srv.peak_used),
+                      "what": "device memory in use beyond the resident dictionaries, "
+                              "read after every batch"}
     finally:
         srv.close()
     rep.update({"offered_jobs_per_s": rate, "lanes": lanes,

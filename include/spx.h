@@ -63,7 +63,8 @@ extern "C" {
 
 /* solver launch form */
 #define SPX_FORM_BATCHED 0 /* persistent kernel + device work queue (default) */
-#define SPX_FORM_PLAIN 1   /* one workgroup per signal, chunked grid launches  */
+#define SPX_FORM_PLAIN 1   /* a grid of resident workgroups, one signal each at
+                            * a time, assigned grid-stride (no queue, no order) */
 
 typedef struct spx_ctx spx_ctx;
 typedef struct spx_dict spx_dict;
@@ -145,6 +146,12 @@ void spx_ctx_destroy(spx_ctx* ctx);
 int spx_ctx_set_stream(spx_ctx* ctx, void* hip_stream);
 int spx_ctx_own_stream(spx_ctx* ctx);
 int spx_ctx_synchronize(spx_ctx* ctx);
+/* Device scratch the context holds (bytes), and its release (waits for the
+ * context's stream, then frees every scratch buffer; later calls allocate
+ * again).  Engine controls with no reference counterpart: a caller that
+ * budgets device memory (the served form) counts and trims them. */
+size_t spx_ctx_scratch_bytes(const spx_ctx* ctx);
+int spx_ctx_release_scratch(spx_ctx* ctx);
 
 /* Launch timing.  While on, the batched solvers, FISTA and the CSC assembly
  * record HIP events on the context's stream around their launches, under

@@ -1359,6 +1359,36 @@ int spx_ctx_read_timing(spx_ctx* ctx, double* ms, long long* launches, int count
   return rc;
 }
 
+size_t spx_ctx_scratch_bytes(const spx_ctx* ctx) {
+  if (!ctx) return 0;
+  size_t b = 0;
+  for (const auto& sl : ctx->slots) b += sl.bytes;
+  return b;
+}
+
+int spx_ctx_release_scratch(spx_ctx* ctx) {
+  if (!ctx) return SPX_ERR_ARG;
+  spx::DeviceGuard g(ctx->device);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (ctx->stream && hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    spx::set_error("release_scratch: stream synchronisation failed");
+    return SPX_ERR_HIP;
+  }
+  // captured FISTA loops address the scratch: they go with it
+  for (auto& gr : ctx->graphs)
+    if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+  ctx->graphs.clear();
+  for (auto& sl : ctx->slots) {
+    if (sl.ptr && hipFree(sl.ptr) != hipSuccess) {
+      spx::set_error("release_scratch: device free failed");
+      return SPX_ERR_HIP;
+    }
+    sl.ptr = nullptr;
+    sl.bytes = 0;
+  }
+  return SPX_OK;
+}
+
 int spx_ctx_reset_timing(spx_ctx* ctx) {
   if (!ctx) return SPX_ERR_ARG;
   spx::DeviceGuard g(ctx->device);

@@ -16,6 +16,18 @@ Gram matrix).  The scheduler
 A job whose footprint exceeds the whole budget is rejected with an error
 (admission failure) instead of being queued forever.  Latency = completion
 time - arrival time.
+
+HBM accounting (the budget covers everything the served jobs hold beyond the
+resident dictionaries):
+  * each lane's engine scratch is counted while the lane holds it (it is
+    kept between batches: the context's slots only grow), read back from the
+    context after every batch, and released when it exceeds the lane's share
+    of the budget;
+  * a batch claims its device tensors plus the engine scratch it may add to
+    its lane beyond what the lane holds (an upper bound of the solver's
+    scratch slots, with their allocation slack);
+  * device memory torch keeps cached but unused is counted too, and returned
+    to the device when it exceeds a quarter of the budget.
 """
 from __future__ import annotations
 
@@ -58,6 +70,7 @@ class _Lane:
     stream: object
     busy: bool = False
     thread: threading.Thread | None = None
+    retained: int = 0           # engine scratch the lane's context holds (bytes)
 
 
 class SparseCodingServer:
@@ -78,7 +91,13 @@ class SparseCodingServer:
             c.set_stream(s.cuda_stream)
             self.lanes.append(_Lane(c, s))
         self.budget = int(hbm_budget)
-        self.in_use = 0
+        self.in_use = 0             # claims of the batches on the GPU
+        self.num_cus = torch.cuda.get_device_properties(self.dev).multi_processor_count
+        # device memory in use once the dictionaries are resident: the served
+        # jobs' usage is measured against it (peak_used, after every batch)
+        torch.cuda.synchronize(self.dev)
+        self.base_free = torch.cuda.mem_get_info(self.dev)[0]
+        self.peak_used = 0
         self.max_batch = int(max_batch_signals)
         self.fuse = fuse
         self.lock = threading.Condition()
@@ -111,22 +130,68 @@ class SparseCodingServer:
         return out
 
     # ---------------------------------------------------------- footprint
-    def footprint(self, dict_id: int, kind: str, n: int) -> int:
+    def _lout(self, dict_id: int, kind: str, L) -> int:
+        """List length of a batch: min(m, p) for LARS (lasso_L), the largest
+        job L (clamped) for ORMP."""
+        d = self.dicts[dict_id]
+        Lout = min(d.m, d.p)
+        return Lout if kind == "lasso" or L is None else max(1, min(Lout, int(L)))
+
+    def _tensor_bytes(self, dict_id: int, kind: str, n: int, L=None) -> int:
+        """Device tensors of a batch of n signals: X, lambda / L, the (value,
+        index) lists and counts, and the CSC (pB, r, v); 2 MiB of caching
+        allocator rounding per tensor."""
+        d = self.dicts[dict_id]
+        Lout = self._lout(dict_id, kind, L)
+        return n * (4 * d.m + 4 + 16 * Lout + 12) + 16 + 9 * (2 << 20)
+
+    def _scratch_bytes(self, dict_id: int, kind: str, n: int, L=None) -> int:
+        """Upper bound of the engine scratch a batch of n signals takes on its
+        lane's context: D'X and norms; for LARS the work order, overflow list
+        and keys, the sort buffers, the hand-over states (one per signal) and
+        the inverse slots of the full-capacity pass (one per resident
+        workgroup, at most n); for ORMP its inverse-factor slots when they do
+        not fit LDS; the CSC scan.  Each slot is allocated 1/8 larger, and
+        8 MiB cover allocation granularity."""
         d = self.dicts[dict_id]
         m, p = d.m, d.p
         ldg = (p + 15) // 16 * 16
-        L = min(m, p)
-        lists = n * L * 8 + n * 4
-        per = n * m * 4 + n * ldg * 4 + n * 4 + lists + n * 4
-        # solver scratch: one slot per resident workgroup, bounded by n
-        slot = (2 if kind == "lasso" else 1) * L * L * 4
-        wg = min(n, 256 * 16)
-        # LARS hand-over state between its two passes: one slot per signal
-        handover = n * (4 * (ldg + 24 + 24 * 24 + 2) + 4 * 28) if kind == "lasso" else 0
-        return per + wg * slot + handover + (1 << 20)
+        L = self._lout(dict_id, kind, L)
+        b = n * ldg * 4 + n * 4                      # D'X, ||x||^2
+        b += 8 * n + 8 * 1024 + 64                   # CSC count scan
+        if kind == "lasso":
+            state = (4 * (ldg + 24 + 24 * 24 + 2) + 4 * 28 + 15) // 16 * 16
+            b += 16 * n + 16 * n + 4 * 256 * (n // 2048 + 1) + 4096   # order, lists, sort
+            b += n * state                                           # hand-over states
+            b += min(n, 4 * self.num_cus) * 2 * L * L * 4            # full-pass inverse slots
+        else:
+            if L * L * 4 > 32 * 1024:
+                b += min(n, 16 * self.num_cus) * L * L * 4
+            b += 64
+        return b + b // 8 + (8 << 20)
+
+    def footprint(self, dict_id: int, kind: str, n: int, L=None) -> int:
+        """Device bytes a job of n signals (ORMP: at most L atoms) may need on
+        a lane that holds no scratch (the admission test for a job alone)."""
+        return (self._tensor_bytes(dict_id, kind, n, L) +
+                self._scratch_bytes(dict_id, kind, n, L))
+
+    def _claim(self, lane: _Lane, dict_id: int, kind: str, n: int, L=None) -> int:
+        """A batch's claim on lane `lane`: its tensors plus the scratch it may
+        add beyond what the lane already holds."""
+        grow = max(0, self._scratch_bytes(dict_id, kind, n, L) - lane.retained)
+        return self._tensor_bytes(dict_id, kind, n, L) + grow
+
+    def _cached_unused(self) -> int:
+        torch = self.torch
+        return max(0, torch.cuda.memory_reserved(self.dev) - torch.cuda.memory_allocated(self.dev))
+
+    def _held(self) -> int:
+        """Bytes counted against the budget now (lock held)."""
+        return self.in_use + sum(l.retained for l in self.lanes) + self._cached_unused()
 
     # --------------------------------------------------------------- run
-    def _run_batch(self, lane: _Lane, jobs):
+    def _run_batch(self, lane: _Lane, jobs, claim: int):
         torch = self.torch
         d = self.dicts[jobs[0].dict_id]
         kind = jobs[0].kind
@@ -175,8 +240,18 @@ class SparseCodingServer:
                 j.error = str(e)
                 j.t_done = t
         finally:
+            torch.cuda.synchronize(self.dev)
+            used = self.base_free - torch.cuda.mem_get_info(self.dev)[0]
+            held = lane.ctx.scratch_bytes()
+            if held > self.budget // max(len(self.lanes), 1):
+                lane.ctx.release_scratch()
+                held = 0
             with self.lock:
-                self.in_use -= sum(self.footprint(j.dict_id, j.kind, j.n) for j in jobs)
+                self.peak_used = max(self.peak_used, used)
+                self.in_use -= claim
+                lane.retained = held
+                if self._cached_unused() > self.budget // 4:
+                    torch.cuda.empty_cache()
                 lane.busy = False
                 self.lock.notify_all()
 
@@ -192,7 +267,7 @@ class SparseCodingServer:
             now = time.perf_counter() - self.t0
             while nxt < len(jobs) and (not realtime or jobs[nxt].arrival <= now):
                 j = jobs[nxt]
-                if self.footprint(j.dict_id, j.kind, j.n) > self.budget:
+                if self.footprint(j.dict_id, j.kind, j.n, j.param) > self.budget:
                     j.error = "admission: job footprint exceeds the HBM budget"
                     j.t_done = now
                     done += 1
@@ -201,34 +276,49 @@ class SparseCodingServer:
                 nxt += 1
             launched = False
             with self.lock:
-                lane = next((l for l in self.lanes if not l.busy), None)
+                # the free lane holding the most scratch (its claim grows least)
+                free = [l for l in self.lanes if not l.busy]
+                lane = max(free, key=lambda l: l.retained) if free else None
                 if lane is not None and pending:
                     head = pending[0]
-                    fp = self.footprint(head.dict_id, head.kind, head.n)
-                    if self.in_use + fp <= self.budget:
+                    held = self._held()
+                    claim = self._claim(lane, head.dict_id, head.kind, head.n, head.param)
+                    if held + claim > self.budget and not any(l.busy for l in self.lanes):
+                        # an idle server whose retained scratch or cached
+                        # tensors block the head job gives them back first
+                        for l in self.lanes:
+                            l.ctx.release_scratch()
+                            l.retained = 0
+                        self.torch.cuda.empty_cache()
+                        held = self._held()
+                        claim = self._claim(lane, head.dict_id, head.kind, head.n, head.param)
+                    if held + claim <= self.budget:
                         batch = [pending.popleft()]
-                        self.in_use += fp
+                        tot = head.n
+                        Lb = head.param
                         if self.fuse:
                             rest = deque()
-                            tot = head.n
                             while pending:
                                 j = pending.popleft()
-                                f = self.footprint(j.dict_id, j.kind, j.n)
+                                L2 = max(Lb, j.param)
+                                c2 = self._claim(lane, head.dict_id, head.kind, tot + j.n, L2)
                                 if (j.dict_id == head.dict_id and j.kind == head.kind
                                         and tot + j.n <= self.max_batch
-                                        and self.in_use + f <= self.budget):
+                                        and held + c2 <= self.budget):
                                     batch.append(j)
                                     tot += j.n
-                                    self.in_use += f
+                                    Lb = L2
+                                    claim = c2
                                 else:
                                     rest.append(j)
                             pending = rest
+                        self.in_use += claim
                         for j in batch:
                             j.t_admit = now
                             j.batch_id = self.n_batches
                         self.n_batches += 1
                         lane.busy = True
-                        th = threading.Thread(target=self._run_batch, args=(lane, batch))
+                        th = threading.Thread(target=self._run_batch, args=(lane, batch, claim))
                         th.start()
                         threads.append((th, batch))
                         launched = True

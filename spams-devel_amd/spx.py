@@ -163,6 +163,18 @@ class Context:
         _check(f(self.h, ms, cnt, len(TIMERS)))
         return {name: (ms[i], int(cnt[i])) for i, name in enumerate(TIMERS)}
 
+    def scratch_bytes(self) -> int:
+        """Device scratch held by the context (bytes)."""
+        f = lib().spx_ctx_scratch_bytes
+        f.restype = ctypes.c_size_t
+        f.argtypes = [_P]
+        return int(f(self.h))
+
+    def release_scratch(self) -> None:
+        """Waits for the context's stream and frees its device scratch."""
+        lib().spx_ctx_release_scratch.argtypes = [_P]
+        _check(lib().spx_ctx_release_scratch(self.h))
+
     def reset_timing(self) -> None:
         lib().spx_ctx_reset_timing.argtypes = [_P]
         _check(lib().spx_ctx_reset_timing(self.h))

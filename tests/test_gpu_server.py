@@ -79,3 +79,37 @@ def test_admission_rejects_oversized_job():
         assert out.error is not None and "budget" in out.error
     finally:
         srv.close()
+
+
+def test_hbm_budget_holds_with_large_jobs_on_several_lanes():
+    """Large LARS and OMP jobs on three lanes under a 1.25 GiB budget: every
+    job completes, and the device memory in use beyond the resident
+    dictionaries (read after every batch, and at the end) stays within the
+    budget, lane scratch and torch's cache included."""
+    import torch
+
+    import server
+    from server import Job
+    print("case server hbm budget", flush=True)
+    dicts, _ = _jobs(14, 2, 10)
+    rng = np.random.default_rng(15)
+    D = dicts[2]                                   # (256, 1024)
+    jobs = []
+    for k in range(6):
+        X = W.sparse_signals(rng, D, 12000, 5, 0.05, True, np.float32)
+        kind = "lasso" if k % 2 == 0 else "omp"
+        jobs.append(Job(0.0, 2, kind, X, 0.05 if kind == "lasso" else 24))
+    budget = 5 << 28
+    srv = server.SparseCodingServer(dicts, lanes=3, hbm_budget=budget)
+    try:
+        assert all(srv.footprint(2, j.kind, j.n, j.param) <= budget for j in jobs)
+        done = srv.run_trace(jobs, realtime=False)
+        assert all(j.error is None for j in done), [j.error for j in done]
+        torch.cuda.synchronize()
+        end_used = srv.base_free - torch.cuda.mem_get_info()[0]
+        print("  peak %.0f MB, end %.0f MB, budget %.0f MB"
+              % (srv.peak_used / 1e6, end_used / 1e6, budget / 1e6), flush=True)
+        assert srv.peak_used <= budget
+        assert end_used <= budget
+    finally:
+        srv.close()
