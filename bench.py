@@ -119,17 +119,19 @@ def measured_traffic(names):
     return float(sum(d[k] for k in names))
 
 
-def measured_traffic_ormp():
-    """HBM bytes per ormp_kernel launch of the OMP workload from the
-    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary; null when absent."""
-    path = os.path.join(ROOT, "profiles", "ormp_counters.json")
+def profiled_bytes(fname, kernel, signals):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary under profiles/, when that summary was
+    taken on the same kernel at the same batch size; null otherwise."""
+    path = os.path.join(ROOT, "profiles", fname)
     try:
         with open(path) as f:
-            d = json.load(f)["bytes_per_dispatch"]
-    except (OSError, ValueError, KeyError):
+            d = json.load(f)
+        if d.get("signals") != signals:
+            return None
+        return float(d["bytes_per_dispatch"][kernel])
+    except (OSError, ValueError, KeyError, TypeError):
         return None
-    k = [v for n_, v in d.items() if n_.startswith("ormp_kernel")]
-    return float(k[0]) if k else None
 
 
 def valu_issue():
@@ -145,6 +147,13 @@ def valu_issue():
     return {k: round(v["valu_issue_frac"], 3) for k, v in d.items()}
 
 
+def ormp_kernel_name(p, w):
+    """Template name of the FP32 ORMP launch at w waves per signal with the
+    triangular factor in LDS (L <= 64)."""
+    nq = next(q for q in (1, 2, 4, 8, 16) if q >= -(-p // (4 * 64 * w)))
+    return f"ormp_kernel<float, {w}, {nq}, true, false>"
+
+
 def lars_kernel_names(p, w, elem=4):
     """Template names of the two LARS launches at group width w: the LDS pass
     (capacity 24 in FP32, 16 in FP64) and the full-capacity pass (width >= 4);
@@ -157,6 +166,17 @@ def lars_kernel_names(p, w, elem=4):
         return next(q for q in (1, 2, 4, 8, 16) if q >= c)
     w2 = max(4, w)
     return [f"lars_kernel<{t}, {w}, {nq(w)}, {cap}, 0>", f"lars_kernel<{t}, {w2}, {nq(w2)}, 0, 0>"]
+
+
+def calib_copy(E):
+    """One 256 MiB read + 256 MiB write elementwise kernel: under a profiler
+    its FETCH_SIZE / WRITE_SIZE per byte calibrate the other kernels'."""
+    torch = E.torch
+    a = torch.ones((64 << 20,), dtype=torch.float32, device=E.dev)
+    b = torch.empty_like(a)
+    torch.add(a, 0.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
 
 
 class Engine:
@@ -292,11 +312,7 @@ def run_lars(args):
         tune = {"waves_per_signal": w, "trial_ms": trial, "sample_signals": ns}
 
     if args.calib_copy:
-        a = torch.ones((64 << 20,), dtype=torch.float32, device=E.dev)
-        b = torch.empty_like(a)
-        torch.add(a, 0.0, out=b)
-        torch.cuda.synchronize()
-        del a, b
+        calib_copy(E)
     log("bench: warmup %d" % args.warmup)
     for _ in range(args.warmup):
         step("batched")
@@ -598,7 +614,7 @@ def c2_report(E, args, n=100000):
                         "achieved": solver_bytes / (ormp_ms * 1e-3) / 1e9,
                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": solver_bytes / (ormp_ms * 1e-3) / HBM_PEAK,
-                        "traffic": measured_traffic_ormp(),
+                        "traffic": profiled_bytes("ormp_counters.json", ormp_kernel_name(p, w), n),
                         "algorithmic_bytes": solver_bytes, "launch_ms": ormp_ms,
                         "gram_restream_TB_per_s":
                             float(4.0 * p * np.sum(counts * (counts + 1) / 2))
@@ -698,7 +714,9 @@ def c4_report(E, args, n=8192):
                                   "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                                   "frac": prox_bytes / (prox_ms * 1e-3) / HBM_PEAK,
                                   "launch_ms": prox_ms / its,
-                                  "algorithmic_bytes_per_launch": 5.0 * 4 * p * n},
+                                  "algorithmic_bytes_per_launch": 5.0 * 4 * p * n,
+                                  "traffic": profiled_bytes("fista_prox_counters.json",
+                                                            "prox_step_kernel<float>", n)},
                 "eager_ms": tm["fista_products"][0] + tm["fista_prox"][0],
                 "objective_rel_gap_vs_fp64_max": float(np.max(np.abs(gap))),
                 "lipschitz": L0}
@@ -761,6 +779,8 @@ def run_omp(args):
         ns = min(n, 16384)
         w, trial = d.autotune_omp(Xt[:ns], L=L_t, eps=e_t)
         tune = {"waves_per_signal": w, "trial_ms": trial, "sample_signals": ns}
+    if args.calib_copy:
+        calib_copy(E)
     for _ in range(args.warmup):
         step("batched")
     torch.cuda.synchronize()
@@ -780,6 +800,7 @@ def run_omp(args):
     solve_ms = tot_ms - gemm_ms
     counts = cnt.cpu().numpy().astype(np.int64)
     solver_bytes = float(np.sum(4 * p * (1 + counts) + 8 * counts))
+    ormp_name = ormp_kernel_name(p, args.waves if args.waves > 0 else tune["waves_per_signal"])
     plain = None
     if not args.no_plain:
         step("plain")
@@ -807,7 +828,9 @@ def run_omp(args):
             "roofline": {"bound": "hbm", "kernel": "ormp_kernel",
                          "achieved": solver_bytes / (solve_ms * 1e-3) / 1e9,
                          "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": solver_bytes / (solve_ms * 1e-3) / HBM_PEAK, "traffic": None},
+                         "frac": solver_bytes / (solve_ms * 1e-3) / HBM_PEAK,
+                         "traffic": profiled_bytes("ormp_counters.json", ormp_name, n),
+                         "algorithmic_bytes_per_launch": solver_bytes},
             "roofline_gemm": {"bound": "mfma",
                               "achieved": 2.0 * m * p * n / (gemm_ms * 1e-3) / 1e12,
                               "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s"},
@@ -997,15 +1020,20 @@ def run_fista(args):
            "variants": {}}
     ns = 32
     Xs = np.asfortranarray(X[:, :ns])
+    its = args.max_it
+    if args.calib_copy:
+        calib_copy(E)
     for prec in ("fp32", "bf16x3"):
         d = spx.Dictionary(E.ctx, D, precision=prec)
         L0 = d.lipschitz(200)
         for regul in ("l1", "elastic-net"):
+            if args.fista_variant and args.fista_variant != f"{prec}/{regul}":
+                continue
             W0 = torch.zeros((n, p), dtype=torch.float32, device=E.dev)
             Wt = torch.zeros((n, p), dtype=torch.float32, device=E.dev)
             info = torch.zeros((n, 4), dtype=torch.float32, device=E.dev)
-            kw = dict(regul=regul, lambda2=0.01, max_it=200, L0=L0, fixed_step=True,
-                      tol=0.0, it0=1000, precision=prec)
+            kw = dict(regul=regul, lambda2=0.01, max_it=its, L0=L0, fixed_step=True,
+                      tol=0.0, it0=1000, precision=prec, graph=not args.eager)
             spx.fista_flat_dev(E.ctx, d, Xt, W0, Wt, info, 0.05, **kw)
             torch.cuda.synchronize()
             ev = E.events(2)
@@ -1014,11 +1042,11 @@ def run_fista(args):
             ev[1].record(E.stream)
             torch.cuda.synchronize()
             ms = ev[0].elapsed_time(ev[1])
-            flops = 200 * 4.0 * m * p * n
+            flops = its * 4.0 * m * p * n
             obj = info.cpu().numpy()[:ns, 0].astype(np.float64)
             _, io = oracle.fista_flat(Xs.astype(np.float64), D.astype(np.float64),
                                       np.zeros((p, ns)), 0.05, regul=regul, lambda2=0.01,
-                                      max_it=200, L0=L0, fixed_step=True, tol=0.0, it0=1000,
+                                      max_it=its, L0=L0, fixed_step=True, tol=0.0, it0=1000,
                                       num_threads=cpu_threads())
             gap = (obj - io[0]) / np.abs(io[0])
             peak = FP32_MFMA_PEAK if prec == "fp32" else BF16_MFMA_PEAK / 3
@@ -1029,6 +1057,9 @@ def run_fista(args):
                 "objective_rel_gap_vs_fp64_max": float(np.max(np.abs(gap))),
                 "lipschitz": L0}
         d.close()
+    out["value"] = next(iter(out["variants"].values()))["signals_per_s"]
+    if args.fista_variant:
+        return out
     # launch-bound regime: a small batch, the 200-iteration loop as eager
     # launches vs one replayed hipGraph (captured on the first call)
     d = spx.Dictionary(E.ctx, D)
@@ -1072,6 +1103,11 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the sub-objects of the other configurations (c1, c2, c4)")
     ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--max-it", type=int, default=200, help="FISTA iterations (--workload fista)")
+    ap.add_argument("--eager", action="store_true",
+                    help="FISTA as eager launches instead of a replayed graph")
+    ap.add_argument("--fista-variant", default="",
+                    help="run one FISTA variant only (fp32/l1, fp32/elastic-net, bf16x3/...)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB read + 256 MiB write copy kernel first "
                          "(calibrates HBM byte counters under a profiler)")
