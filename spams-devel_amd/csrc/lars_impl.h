@@ -98,6 +98,7 @@ struct LarsLds {
     size_t b = sizeof(T) * (PT + GT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC;
     b = (b + 15) & ~size_t(15);
     b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
+    b += sizeof(unsigned long long) * 2 * W + sizeof(T) * 4 * W + 16;  // parity buffers
     b = (b + 15) & ~size_t(15);
     if (CAP > 0) b += 2 * sizeof(T) * CAP * CAP + 16;
     b = (b + 15) & ~size_t(15);
@@ -112,66 +113,85 @@ struct LarsLds {
 };
 
 // y(0:n) = M(0:n, 0:n) x for a symmetric M (leading dimension ldm), with
-// all GT threads: row r is split into P = GT / n parts whose partial sums
-// are combined in part order (deterministic).  x and y in LDS, part a
-// GT-entry LDS buffer.  Ends with a barrier.
+// all GT threads.  One thread per row, the sum over q in order, eight loads
+// in flight; when M is in HBM and has fewer rows than threads (SPLIT), row r
+// is split into P = GT / n parts whose partial sums are combined in part
+// order (deterministic).  x and y in LDS, part a GT-entry LDS buffer.  Ends
+// with a barrier.
 template <typename T, int GT, bool SPLIT>
 __device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
                                           T* part, int n) {
   const int tid = threadIdx.x;
-  // rows are split only when M is in HBM (several loads in flight per row);
-  // with M in LDS one thread per row is faster
   const int P = (!SPLIT || n >= GT) ? 1 : GT / n;
-  const int seg = (n + P - 1) / P;
-  for (int t = tid; t < n * P; t += GT) {
-    const int r = t % n, k = t / n;
-    const int q0 = k * seg, q1 = q0 + seg < n ? q0 + seg : n;
-    T acc = 0;
-    int q = q0;
-    if (SPLIT) {
-      // M in HBM: eight loads in flight, then the sums in order
-      for (; q + 8 <= q1; q += 8) {
-        T mv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) mv[u] = M[r + (q + u) * ldm];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc += mv[u] * x[q + u];
-      }
-    }
-    for (; q < q1; ++q) acc += M[r + q * ldm] * x[q];
-    if (P == 1) y[r] = acc;
-    else part[k * n + r] = acc;
-  }
-  __syncthreads();
-  if (P > 1) {
+  if (P == 1) {
     for (int r = tid; r < n; r += GT) {
-      T acc = part[r];
-      for (int k = 1; k < P; ++k) acc += part[k * n + r];
+      T acc = 0;
+      int q = 0;
+      for (; q + 8 <= n; q += 8) {
+        T mv[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          mv[u] = M[r + (q + u) * ldm];
+          xv[u] = x[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += mv[u] * xv[u];
+      }
+      for (; q < n; ++q) acc += M[r + q * ldm] * x[q];
       y[r] = acc;
     }
     __syncthreads();
+    return;
   }
+  // n * P <= GT: one (row, part) per thread
+  const int seg = (n + P - 1) / P;
+  if (tid < n * P) {
+    const int k = tid / n, r = tid - k * n;
+    const int q0 = k * seg, q1 = q0 + seg < n ? q0 + seg : n;
+    T acc = 0;
+    int q = q0;
+    for (; q + 8 <= q1; q += 8) {
+      T mv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) mv[u] = M[r + (q + u) * ldm];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += mv[u] * x[q + u];
+    }
+    for (; q < q1; ++q) acc += M[r + q * ldm] * x[q];
+    part[k * n + r] = acc;
+  }
+  __syncthreads();
+  for (int r = tid; r < n; r += GT) {
+    T acc = part[r];
+    for (int k = 1; k < P; ++k) acc += part[k * n + r];
+    y[r] = acc;
+  }
+  __syncthreads();
 }
 
 // Admits the i-th active atom (i >= 1) into the explicit inverse M of the
 // active Gram block, i x i -> (i+1) x (i+1), by its Schur complement
 // (decomp/decomp.h:1756-1770); sGs(0:i+1) = G(active, new atom).
 // v = M gs lands in sV.  An FP32 pivot below 1e-3 of the diagonal (or not
-// positive) is recomputed with FP64 accumulation.  ELEM: the rank-one update
-// runs one element per thread with eight loads in flight (M in HBM, or a
-// wide group); else one row per thread.  Elements (r,q) and (q,r) get the
-// identical update v[min] * (schur * v[max]), so M stays exactly symmetric.
-// Ends with a barrier.
+// positive) is recomputed with FP64 accumulation.  Elements (r,q) and (q,r)
+// get the identical update v[min] * (schur * v[max]), so M stays exactly
+// symmetric.  ELEM: the rank-one update runs one element per thread with
+// eight loads in flight (M in HBM, or a wide group), and the caller forms the
+// direction afterwards.  Else one row per thread, which also forms the
+// direction u(0:i+1) = M_new w (sum over q in order, as symv_dist) from the
+// updated row it holds, so M is read once more instead of twice.  Returns
+// whether u was formed.  Ends with a barrier.
 template <typename T, int W, bool SPLIT, bool ELEM>
-__device__ __forceinline__ void inverse_admit(T* M, int ldm, int i, const T* sGs, T* sV,
-                                              T* sPart, T* sRed3, double* sRes) {
+__device__ __forceinline__ bool inverse_admit(T* M, int ldm, int i, const T* sGs, T* sV,
+                                              T* sPart, T* sRedS, int& par, double* sRes,
+                                              const T* w, T* u) {
   constexpr int GT = 64 * W;
   const int tid = threadIdx.x;
   // v = invGs(i x i) * gs(0:i)
   symv_dist<T, GT, SPLIT>(M, ldm, sGs, sV, sPart, i);
-  T dv = 0, z1 = 0, z2 = 0;
+  T dv = 0;
   for (int r = tid; r < i; r += GT) dv += sV[r] * sGs[r];
-  group_sum3<T, W>(dv, z1, z2, sRed3);
+  dv = group_sum1<T, W>(dv, sRedS, par);
   T schur = T(1) / (sGs[i] - dv);
   if (sizeof(T) == 4 && !(sGs[i] - dv > T(1e-3) * sGs[i])) {
     __syncthreads();
@@ -224,23 +244,62 @@ __device__ __forceinline__ void inverse_admit(T* M, int ldm, int i, const T* sGs
         }
       }
     }
-  } else {
     for (int r = tid; r < i; r += GT) {
-      const T vr = sV[r];
-      for (int q = 0; q < i; ++q) {
-        const T vq = sV[q];
-        const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-        M[r + q * ldm] += upd;
+      const T off = -schur * sV[r];
+      M[r + i * ldm] = off;
+      M[i + r * ldm] = off;
+    }
+    if (tid == 0) M[i + i * ldm] = schur;
+    __syncthreads();
+    return false;
+  }
+  // one row per thread: row r (r < i) of the block, then its border
+  // element, accumulating u[r]; thread i % GT forms u[i] from the border row
+  for (int r = tid; r < i; r += GT) {
+    const T vr = sV[r];
+    T acc = 0;
+    int q = 0;
+    for (; q + 8 <= i; q += 8) {
+      T mv[8], vq[8], wq[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        mv[e] = M[r + (q + e) * ldm];
+        vq[e] = sV[q + e];
+        wq[e] = w[q + e];
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const T upd = r <= q + e ? vr * (schur * vq[e]) : vq[e] * (schur * vr);
+        const T mn = mv[e] + upd;
+        M[r + (q + e) * ldm] = mn;
+        acc += mn * wq[e];
       }
     }
-  }
-  for (int r = tid; r < i; r += GT) {
-    const T off = -schur * sV[r];
+    for (; q < i; ++q) {
+      const T vq = sV[q];
+      const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
+      const T mn = M[r + q * ldm] + upd;
+      M[r + q * ldm] = mn;
+      acc += mn * w[q];
+    }
+    const T off = -schur * vr;
     M[r + i * ldm] = off;
     M[i + r * ldm] = off;
+    acc += off * w[i];
+    u[r] = acc;
   }
-  if (tid == 0) M[i + i * ldm] = schur;
+  if (tid == i % GT) {
+    M[i + i * ldm] = schur;
+    T acc = 0;
+    for (int q = 0; q < i; ++q) {
+      const T off = -schur * sV[q];
+      acc += off * w[q];
+    }
+    acc += schur * w[i];
+    u[i] = acc;
+  }
   __syncthreads();
+  return true;
 }
 
 // Md (i x i) = Ms ((i+1) x (i+1)) without row and column fz, downdated by
@@ -363,33 +422,31 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
   T* sV = sGs + LCL;                      // LC
   int* sInd = reinterpret_cast<int*>(sV + LCL);  // LC
   int* sTmpI = sInd + LCL;                       // LC
-  KeyIdx<T>* sRed = reinterpret_cast<KeyIdx<T>*>(
-      (reinterpret_cast<uintptr_t>(sTmpI + LCL) + 15) & ~uintptr_t(15));
+  KeyIdx<T>* sRed = lds_next16<KeyIdx<T>>(smem, sTmpI + LCL);
   T* sRed3 = reinterpret_cast<T*>(sRed + W);  // 3W
-  int* sSig = reinterpret_cast<int*>(sRed3 + 3 * W);
-  T* sM0 = reinterpret_cast<T*>(
-      (reinterpret_cast<uintptr_t>(sSig + 1) + 15) & ~uintptr_t(15));  // CAP x CAP
+  // single-barrier reductions: 2 buffers of W packed keys, 2 of 2W sums
+  unsigned long long* sRedU = lds_next16<unsigned long long>(smem, sRed3 + 3 * W);
+  T* sRedS = reinterpret_cast<T*>(sRedU + 2 * W);  // 4W
+  int* sSig = reinterpret_cast<int*>(sRedS + 4 * W);
+  T* sM0 = lds_next16<T>(smem, sSig + 1);  // CAP x CAP
   T* sM1 = sM0 + (CAP > 0 ? CAP * CAP : 0);
-  double* sAlpha = reinterpret_cast<double*>(
-      (reinterpret_cast<uintptr_t>(sM1 + (CAP > 0 ? CAP * CAP : 0)) + 15) & ~uintptr_t(15));
+  double* sAlpha = lds_next16<double>(smem, sM1 + (CAP > 0 ? CAP * CAP : 0));
   double* sRes = sAlpha + LCL;  // FP64 refit vectors; sRes has max(LC, 64)
   // full-capacity pass: LDS copy of the inverse while the active set has at
   // most CL atoms (CL x CL, leading dimension CL)
   const int CL = CAP == 0 ? a.lds_cap : 0;
-  T* sMl = reinterpret_cast<T*>(
-      (reinterpret_cast<uintptr_t>(sRes + (LCL > 64 ? LCL : 64)) + 15) & ~uintptr_t(15));
+  T* sMl = lds_next16<T>(smem, sRes + (LCL > 64 ? LCL : 64));
   // weighted penalty: the signal's weights of the group's rows (1 past p)
-  T* sWt = reinterpret_cast<T*>(
-      (reinterpret_cast<uintptr_t>(sMl + CL * CL) + 15) & ~uintptr_t(15));
+  T* sWt = lds_next16<T>(smem, sMl + CL * CL);
   constexpr bool wt = VAR == 1;  // weighted penalty
   constexpr bool mk = VAR == 2;  // masked signals
   // masked: active position -> Gram-column slot, and its shift buffer
-  int* sSlot = reinterpret_cast<int*>(
-      (reinterpret_cast<uintptr_t>(sMl + CL * CL) + 15) & ~uintptr_t(15));
+  int* sSlot = lds_next16<int>(smem, sMl + CL * CL);
   int* sSlotT = sSlot + LCL;
   T* const gaw = mk ? a.ga + (size_t)blockIdx.x * a.ga_stride : nullptr;
 
   const int tid = threadIdx.x;
+  int par = 0;  // buffer parity of the single-barrier reductions
   T* const M0 = CAP > 0 ? sM0 : a.scratch + (size_t)blockIdx.x * 2 * L * L;
   T* const M1 = CAP > 0 ? sM1 : M0 + (size_t)L * L;
   const int ldm = CAP > 0 ? CAP : L;
@@ -559,6 +616,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
       T thrs = thrs0;
       for (int i = i0; i < Lm; ++i) {
         ++iter;
+        bool have_u = false;  // the admission formed the direction
         if (newAtom) {
           if (CAP > 0 && i >= LC) {
             // the active set outgrows the fast pass: hand the signal over
@@ -581,7 +639,14 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
                                        mdiag);
             gcol = slot;
           }
-          for (int j = tid; j <= i; j += GT) sGs[j] = gcol[sInd[j]];
+          // the new Gram column's active entries, and the signed weights of
+          // the direction (coreLARS2W: the weights, decomp/decomp.h:1986-1987)
+          for (int j = tid; j <= i; j += GT) {
+            const int ij = sInd[j];
+            sGs[j] = gcol[ij];
+            const T one = wt ? sWt[ij] : T(1);
+            sW[j] = sDtR[ij] > 0 ? one : -one;
+          }
           __syncthreads();
           if (i == 0) {
             if (tid == 0) {
@@ -598,8 +663,12 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
               __syncthreads();
               inl = false;
             }
-            if (inl) inverse_admit<T, W, true, true>(sMl, CL, i, sGs, sV, sPart, sRed3, sRes);
-            else inverse_admit<T, W, CAP == 0, CAP == 0>(Mc, ldm, i, sGs, sV, sPart, sRed3, sRes);
+            if (inl)
+              have_u = inverse_admit<T, W, true, true>(sMl, CL, i, sGs, sV, sPart, sRedS, par,
+                                                       sRes, sW, sU);
+            else
+              have_u = inverse_admit<T, W, CAP == 0, CAP == 0>(Mc, ldm, i, sGs, sV, sPart,
+                                                               sRedS, par, sRes, sW, sU);
           }
           // ols: normX is the least-squares residual of the active set
           // (decomp/decomp.h:1044-1050)
@@ -610,28 +679,41 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
 
         const int s = i + 1;
         // ------------------------------------------ direction u = invGs*w
-        // (coreLARS2W: the signed weights, decomp/decomp.h:1986-1987)
-        for (int j = tid; j < s; j += GT) {
-          const T one = wt ? sWt[sInd[j]] : T(1);
-          sW[j] = sDtR[sInd[j]] > 0 ? one : -one;
+        // (coreLARS2W: the signed weights, decomp/decomp.h:1986-1987); an
+        // admission has set the weights, and formed u with the row-wise update
+        if (!newAtom) {
+          for (int j = tid; j < s; j += GT) {
+            const T one = wt ? sWt[sInd[j]] : T(1);
+            sW[j] = sDtR[sInd[j]] > 0 ? one : -one;
+          }
+          __syncthreads();
         }
-        __syncthreads();
-        if (inl) symv_dist<T, GT, true>(sMl, CL, sW, sU, sPart, s);
-        else symv_dist<T, GT, CAP == 0>(Mc, ldm, sW, sU, sPart, s);
+        if (!have_u) {
+          if (inl) symv_dist<T, GT, true>(sMl, CL, sW, sU, sPart, s);
+          else symv_dist<T, GT, CAP == 0>(Mc, ldm, sW, sU, sPart, s);
+        }
 
         // ------------------ zero-crossing step, polynomial coefficients
         T step_max;
         int first_zero;
         T coeff1 = 0, coeff2 = 0;
         {
-          T key = INF, pay = 0, z = 0;
+          T key = INF, pay = 0;
           int idx = INT_MAX;
+          unsigned long long kz = pack_key(__builtin_huge_valf(), INT_MAX);
           for (int j = tid; j < s; j += GT) {
             const T uj = sU[j];
             const T ratio = -sCoef[j] / uj;
             // the sequential scan keeps the LAST j on ties: index L - j
-            take_if_better(key, idx, pay, ratio > 0 ? ratio : INF,
-                           ratio > 0 ? L - j : INT_MAX, T(0));
+            if constexpr (sizeof(T) == 4) {
+              const unsigned long long k =
+                  ratio > 0 ? pack_key(static_cast<float>(ratio), L - j)
+                            : pack_key(__builtin_huge_valf(), INT_MAX);
+              kz = k < kz ? k : kz;
+            } else {
+              take_if_better(key, idx, pay, ratio > 0 ? ratio : INF,
+                             ratio > 0 ? L - j : INT_MAX, T(0));
+            }
             const T dj = sDtR[sInd[j]];
             if (wt) {
               // decomp/decomp.h:2032-2037
@@ -643,8 +725,14 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
               coeff2 += dj * uj;
             }
           }
-          group_argmin<T, W>(key, idx, pay, sRed);
-          group_sum3<T, W>(coeff1, coeff2, z, sRed3);
+          if constexpr (sizeof(T) == 4) {
+            kz = group_min_u64<W>(kz, sRedU, par);
+            key = packed_key(kz);
+            idx = static_cast<int>(static_cast<unsigned>(kz));
+          } else {
+            group_argmin<T, W>(key, idx, pay, sRed);
+          }
+          group_sum2<T, W>(coeff1, coeff2, sRedS, par);
           step_max = key;
           first_zero = idx == INT_MAX ? -1 : L - idx;
         }
@@ -667,9 +755,12 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
 #pragma unroll
             for (int e = 0; e < UNR; ++e) {
               const int k = k0 + e < s ? k0 + e : s - 1;
-              const T ur = sU[k];
+              // uniform across the group: held in scalar registers, so the
+              // column addresses and multipliers cost no vector instructions
+              const T ur = uniform_v(sU[k]);
               uk[e] = k0 + e < s ? ur : T(0);
-              col[e] = own ? gaw + (size_t)sSlot[k] * ldg : a.G + (size_t)sInd[k] * ldg;
+              const int ck = uniform_v(own ? sSlot[k] : sInd[k]);
+              col[e] = (own ? gaw : a.G) + (size_t)ck * ldg;
             }
             V g[UNR][NQ];
 #pragma unroll
@@ -705,90 +796,9 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
         // index of the minimum, IEEE quotients.
         T bestk = INF, bestv = INF;
         int besti = INT_MAX;
-        if constexpr (sizeof(T) == 4) {
-          // pass 1: keys from the hardware reciprocal, within 3 ulp of the
-          // IEEE quotient; each thread keeps its smallest key (m1, slot i1)
-          // and its second smallest (m2)
-          T m1 = INF, m2 = INF;
-          int i1 = 0;
-#pragma unroll
-          for (int q = 0; q < NQ; ++q)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-              const int j = VEC * (tid + GT * q) + e;
-              // rows past p carry d = +INF, hence keys +INF (or NaN)
-              const bool cand = !((act >> (q * VEC + e)) & 1);
-              const T Aj = Areg[q][e];
-              const T d = sDtR[j];
-              const bool ok1 = cand & (Aj < INF) & (Aj > T(-1)) & !a.pos;
-              const bool ok2 = cand & (Aj < INF) & (Aj < T(1));
-              const T k1 = ok1 ? T(fabs((d + cc) * __builtin_amdgcn_rcpf(T(1) + Aj))) : INF;
-              const T k2 = ok2 ? T(fabs((cc - d) * __builtin_amdgcn_rcpf(T(1) - Aj))) : INF;
-              const int sl = 2 * (q * VEC + e);
-              bool lt = k1 < m1;
-              m2 = lt ? m1 : fminf(m2, k1);
-              i1 = lt ? sl : i1;
-              m1 = lt ? k1 : m1;
-              lt = k2 < m1;
-              m2 = lt ? m1 : fminf(m2, k2);
-              i1 = lt ? sl + 1 : i1;
-              m1 = lt ? k2 : m1;
-            }
-          // pass 2 recomputes what it needs: no pass-1 temporaries stay live
-          // (register pressure sets the signals in flight per CU)
-#pragma unroll
-          for (int q = 0; q < NQ; ++q)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-              T x = Areg[q][e];
-              asm volatile("" : "+v"(x));
-              Areg[q][e] = x;
-            }
-          asm volatile("" ::: "memory");
-          // every key within the margin of the group minimum may be the
-          // minimum of the IEEE keys: those get the exact quotient
-          T amin = m1, zp = 0;
-          int zi = 0;
-          group_argmin<T, W>(amin, zi, zp, sRed);
-          const T thr = amin * T(1.0f + 0x1.0p-18f) + T(1e-37f);
-          if (m2 <= thr) {
-            // two of this thread's keys are near the minimum (or no key is
-            // finite): all of its keys exactly (a superset of the keys
-            // within the margin; the group arg-min is unchanged).  Rare, so
-            // one row at a time (few registers live).
-#pragma unroll 1
-            for (int sl = 0; sl < NQ * VEC; ++sl) {
-              T Aj = 0;
-#pragma unroll
-              for (int q = 0; q < NQ; ++q)
-#pragma unroll
-                for (int e = 0; e < VEC; ++e) Aj = sl == q * VEC + e ? Areg[q][e] : Aj;
-              const int j = VEC * (tid + GT * (sl / VEC)) + sl % VEC;
-              const bool cand = (j < p) & !((act >> sl) & 1);
-              const T d = sDtR[j < PT ? j : 0];
-              T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
-              c1 = a.pos ? INF : c1;
-              const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
-              take_if_better(bestk, besti, bestv, cand ? T(fabs(c1)) : INF,
-                             cand ? j : INT_MAX, c1);
-              take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
-                             cand ? p + j : INT_MAX, c2);
-            }
-          } else if (m1 <= thr) {
-            // one key of this thread near the minimum (finite, so valid)
-            const int sl = i1 >> 1;
-            T Aj = 0;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q)
-#pragma unroll
-              for (int e = 0; e < VEC; ++e) Aj = sl == q * VEC + e ? Areg[q][e] : Aj;
-            const int j = VEC * (tid + GT * (sl / VEC)) + sl % VEC;
-            const T d = sDtR[j];
-            const bool second = i1 & 1;
-            const T c = second ? (cc - d) / (T(1) - Aj) : (d + cc) / (T(1) + Aj);
-            take_if_better(bestk, besti, bestv, T(fabs(c)), second ? p + j : j, c);
-          }
-        } else {
+        // exact keys of every row of this thread (IEEE quotients and the
+        // reference's validity rules), folded into (bestk, besti, bestv)
+        auto exact_rows = [&]() {
 #pragma unroll
           for (int q = 0; q < NQ; ++q)
 #pragma unroll
@@ -805,8 +815,104 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
               take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
                              cand ? p + j : INT_MAX, c2);
             }
+        };
+        if constexpr (sizeof(T) == 4) {
+          // pass 1: one key per row, min(|c1|, |c2|), from hardware
+          // reciprocals (within 4 ulp of the IEEE quotients) of denominators
+          // clamped at zero: an invalid candidate (denominator <= 0 or NaN)
+          // gets +INF or NaN, which never wins; under pos c1 gets a NaN
+          // numerator.  The key's low SB bits are replaced by the row's slot
+          // (a truncation below 2^-17 relative) and an active row's key by
+          // all ones.  Each thread keeps its two smallest packed keys.
+          constexpr int NR = NQ * VEC;  // rows per thread
+          constexpr int SB = NR <= 16 ? 4 : (NR <= 32 ? 5 : 6);
+          constexpr unsigned SMASK = (1u << SB) - 1u;
+          const T cc1 = a.pos ? __builtin_nanf("") : cc;
+          unsigned m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              const int sl = q * VEC + e;
+              const T Aj = Areg[q][e];
+              const T d = sDtR[VEC * (tid + GT * q) + e];
+              const T r1 = __builtin_amdgcn_rcpf(fmaxf(T(1) + Aj, T(0)));
+              const T r2 = __builtin_amdgcn_rcpf(fmaxf(T(1) - Aj, T(0)));
+              const T k = fminf(fabsf((d + cc1) * r1), fabsf((cc - d) * r2));
+              const unsigned ex = 0u - static_cast<unsigned>((act >> sl) & 1);
+              const unsigned kt = ((__float_as_uint(k) & ~SMASK) | static_cast<unsigned>(sl)) | ex;
+              const unsigned hi = m1 > kt ? m1 : kt;
+              m2 = m2 < hi ? m2 : hi;
+              m1 = m1 < kt ? m1 : kt;
+            }
+          // pass 2 recomputes what it needs: no pass-1 temporaries stay live
+          // (register pressure sets the signals in flight per CU)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              T x = Areg[q][e];
+              asm volatile("" : "+v"(x));
+              Areg[q][e] = x;
+            }
+          asm volatile("" ::: "memory");
+          // every row whose exact key may be the minimum has its truncated
+          // key within 2^-16 of the group minimum (4 ulp of reciprocal error
+          // and the 2^-17 truncation, both ways): those rows get the exact
+          // quotients.  No finite key anywhere (NaN minimum): every row.
+          const T amin = group_min_f32<W>(__uint_as_float(m1 & ~SMASK), sRedS, par);
+          const bool anyk = amin == amin;
+          const T thr = anyk ? amin * T(1.0f + 0x1.0p-16f) + T(1e-37f) : INF;
+          const unsigned thb = anyk ? __float_as_uint(thr) : 0xFFFFFFFFu;
+          if ((m2 & ~SMASK) <= thb) {
+            // two rows of this thread near the minimum: all its rows
+            exact_rows();
+          } else if ((m1 & ~SMASK) <= thb) {
+            // one row of this thread near the minimum
+            const int sl = static_cast<int>(m1 & SMASK);
+            T Aj = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+              for (int e = 0; e < VEC; ++e) Aj = sl == q * VEC + e ? Areg[q][e] : Aj;
+            const int j = VEC * (tid + GT * (sl / VEC)) + sl % VEC;
+            const bool cand = j < p;
+            const T d = sDtR[j];
+            T c1 = (Aj < INF && Aj > T(-1)) ? (d + cc) / (T(1) + Aj) : INF;
+            c1 = a.pos ? INF : c1;
+            const T c2 = (Aj < INF && Aj < T(1)) ? (cc - d) / (T(1) - Aj) : INF;
+            take_if_better(bestk, besti, bestv, cand ? T(fabs(c1)) : INF, cand ? j : INT_MAX,
+                           c1);
+            take_if_better(bestk, besti, bestv, cand ? T(fabs(c2)) : INF,
+                           cand ? p + j : INT_MAX, c2);
+          }
+          // (|c|, index, sign of c) packed: the sign rides below the index
+          auto reduce = [&]() {
+            unsigned long long kb = pack_key(
+                bestk, besti == INT_MAX
+                           ? 0xFFFFFFFFu
+                           : 2u * static_cast<unsigned>(besti) +
+                                 (__float_as_uint(static_cast<float>(bestv)) >> 31));
+            kb = group_min_u64<W>(kb, sRedU, par);
+            const unsigned lo = static_cast<unsigned>(kb);
+            bestk = packed_key(kb);
+            besti = lo == 0xFFFFFFFFu ? INT_MAX : static_cast<int>(lo >> 1);
+            bestv = (lo & 1u) ? -bestk : bestk;
+          };
+          reduce();
+          if (!(bestk <= thr)) {
+            // a pass-1 key below its exact value (a correlation change of
+            // +INF, whose clamped reciprocal is 0): every row exactly
+            bestk = INF;
+            bestv = INF;
+            besti = INT_MAX;
+            exact_rows();
+            reduce();
+          }
+        } else {
+          exact_rows();
+          group_argmin<T, W>(bestk, besti, bestv, sRed);
         }
-        group_argmin<T, W>(bestk, besti, bestv, sRed);
         T step = besti == INT_MAX ? INF : bestv;
         const int nextInd = besti == INT_MAX ? 0 : besti % p;
 

@@ -67,10 +67,9 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
   T* sGc = sX + L;                        // L  (G(c, active))
   T* sScal = sGc + L;                     // 4  (Rdn[c], norm[c])
   int* sInd = reinterpret_cast<int*>(sScal + 4);  // L
-  KeyIdx<T>* sRed = reinterpret_cast<KeyIdx<T>*>(
-      (reinterpret_cast<uintptr_t>(sInd + L) + 15) & ~uintptr_t(15));
+  KeyIdx<T>* sRed = lds_next16<KeyIdx<T>>(smem, sInd + L);
   int* sSig = reinterpret_cast<int*>(sRed + W);
-  T* sUn = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(sSig + 1) + 15) & ~uintptr_t(15));
+  T* sUn = lds_next16<T>(smem, sSig + 1);
 
   const int tid = threadIdx.x;
   T* const Un = ULDS ? sUn : a.scratch + (size_t)blockIdx.x * L * L;  // column-major L x L

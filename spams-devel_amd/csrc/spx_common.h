@@ -187,6 +187,160 @@ __device__ __forceinline__ void group_sum3(T& a, T& b, T& c, T* red) {
   }
 }
 
+// A value the caller knows to be the same in every lane (an LDS word read at
+// a uniform address), moved to a scalar register: arithmetic on it (column
+// addresses, multipliers) then costs no vector instructions.
+__device__ __forceinline__ int uniform_v(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform_v(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform_v(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+// ---- reductions with fewer instructions and barriers ----------------------
+//
+// A non-negative FP32 key (sign bit clear; +INF; NaN orders after +INF) and a
+// 32-bit index pack into one unsigned 64-bit word whose unsigned order is the
+// order of the arg-min searches: the smaller key first, the lower index on
+// ties, a NaN key after every number.  One 64-bit minimum then replaces the
+// compare-and-select of key, index and payload.
+__device__ __forceinline__ unsigned long long pack_key(float key, unsigned idx) {
+  return (static_cast<unsigned long long>(__float_as_uint(key)) << 32) | idx;
+}
+__device__ __forceinline__ float packed_key(unsigned long long v) {
+  return __uint_as_float(static_cast<unsigned>(v >> 32));
+}
+
+template <int CTRL>
+__device__ __forceinline__ void min_step_u64(unsigned long long& v) {
+  const unsigned lo = static_cast<unsigned>(dpp_i<CTRL>(static_cast<int>(v)));
+  const unsigned hi = static_cast<unsigned>(dpp_i<CTRL>(static_cast<int>(v >> 32)));
+  const unsigned long long o = (static_cast<unsigned long long>(hi) << 32) | lo;
+  v = o < v ? o : v;
+}
+__device__ __forceinline__ unsigned long long lane_u64(unsigned long long v, int lane) {
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+  const unsigned hi =
+      static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Wave-wide unsigned 64-bit minimum; every lane returns it.
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+  min_step_u64<0xB1>(v);
+  min_step_u64<0x4E>(v);
+  min_step_u64<0x124>(v);
+  min_step_u64<0x128>(v);
+  unsigned long long a = lane_u64(v, 0), b = lane_u64(v, 16);
+  unsigned long long c = lane_u64(v, 32), d = lane_u64(v, 48);
+  a = b < a ? b : a;
+  c = d < c ? d : c;
+  return c < a ? c : a;
+}
+
+// Wave-wide minimum of FP32 values (a NaN never wins over a number); every
+// lane returns it.
+__device__ __forceinline__ float wave_min_f32(float v) {
+  v = fminf(v, dpp_v<0xB1>(v));
+  v = fminf(v, dpp_v<0x4E>(v));
+  v = fminf(v, dpp_v<0x124>(v));
+  v = fminf(v, dpp_v<0x128>(v));
+  return fminf(fminf(lane_v(v, 0), lane_v(v, 16)), fminf(lane_v(v, 32), lane_v(v, 48)));
+}
+
+// Group-wide reductions for a group of W wavefronts that is exactly one
+// workgroup, with ONE barrier each: the per-wave results go to one of two LDS
+// buffers, alternately (`parity`, the same in every thread).  A buffer is
+// written again only two reductions later, and every thread has passed the
+// barrier of the reduction in between, which it reached after reading the
+// buffer.  All threads return the result.
+template <int W>
+__device__ __forceinline__ unsigned long long group_min_u64(unsigned long long v,
+                                                            unsigned long long* red,
+                                                            int& parity) {
+  v = wave_min_u64(v);
+  if constexpr (W > 1) {
+    unsigned long long* r = red + parity * W;
+    parity ^= 1;
+    if ((threadIdx.x & (kWave - 1)) == 0) r[threadIdx.x / kWave] = v;
+    __syncthreads();
+    v = r[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) v = r[w] < v ? r[w] : v;
+  }
+  return v;
+}
+
+template <int W>
+__device__ __forceinline__ float group_min_f32(float v, float* red, int& parity) {
+  v = wave_min_f32(v);
+  if constexpr (W > 1) {
+    float* r = red + parity * 2 * W;
+    parity ^= 1;
+    if ((threadIdx.x & (kWave - 1)) == 0) r[threadIdx.x / kWave] = v;
+    __syncthreads();
+    v = r[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) v = fminf(v, r[w]);
+  }
+  return v;
+}
+
+// Sums of one or two values, in the order of group_sum3 (wave sums, then the
+// waves in order from zero), so the results are those of group_sum3.
+template <typename T, int W>
+__device__ __forceinline__ void group_sum2(T& a, T& b, T* red, int& parity) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if constexpr (W > 1) {
+    T* r = red + parity * 2 * W;
+    parity ^= 1;
+    const int wid = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      r[2 * wid] = a;
+      r[2 * wid + 1] = b;
+    }
+    __syncthreads();
+    a = b = T(0);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      a += r[2 * w];
+      b += r[2 * w + 1];
+    }
+  }
+}
+template <typename T, int W>
+__device__ __forceinline__ T group_sum1(T a, T* red, int& parity) {
+  a = wave_sum(a);
+  if constexpr (W > 1) {
+    T* r = red + parity * 2 * W;
+    parity ^= 1;
+    if ((threadIdx.x & (kWave - 1)) == 0) r[threadIdx.x / kWave] = a;
+    __syncthreads();
+    a = T(0);
+#pragma unroll
+    for (int w = 0; w < W; ++w) a += r[w];
+  }
+  return a;
+}
+
+// Sub-array of the dynamic LDS block `base` starting at the first 16-byte
+// boundary at or after `end` (a pointer into the same block).  The address is
+// formed by pointer arithmetic on the block, so the compiler keeps it in the
+// LDS address space and emits ds_* instructions; a round trip through an
+// integer would make it a generic pointer, accessed with flat_* instructions
+// (longer latency, and each access waits on the vector-memory counter too).
+template <typename U, typename P>
+__device__ __forceinline__ U* lds_next16(char* base, P* end) {
+  const int off = static_cast<int>(reinterpret_cast<char*>(end) - base);
+  return reinterpret_cast<U*>(base + ((off + 15) & ~15));
+}
+
 template <typename T>
 __device__ __forceinline__ T dev_inf() {
   return __builtin_huge_val();
