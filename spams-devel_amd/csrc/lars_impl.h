@@ -95,7 +95,8 @@ struct LarsLds {
   // byte size for active-set capacity LC; variant 1 (weighted) adds the
   // per-row weights, variant 2 (masked) the Gram-column slot map
   static size_t bytes(int LC, int CL = 0, int var = 0) {
-    size_t b = sizeof(T) * (PT + GT + 5 * (size_t)LC) + sizeof(int) * 2 * (size_t)LC;
+    const size_t LCA = ((size_t)LC + 3) & ~size_t(3);  // array stride (16-byte multiple)
+    size_t b = sizeof(T) * (PT + GT + 5 * LCA) + sizeof(int) * 2 * LCA;
     b = (b + 15) & ~size_t(15);
     b += sizeof(KeyIdx<T>) * W + sizeof(T) * 3 * W + 16;
     b += sizeof(unsigned long long) * 2 * W + sizeof(T) * 4 * W + 16;  // parity buffers
@@ -192,8 +193,9 @@ __device__ __forceinline__ bool inverse_admit(T* M, int ldm, int i, const T* sGs
   T dv = 0;
   for (int r = tid; r < i; r += GT) dv += sV[r] * sGs[r];
   dv = group_sum1<T, W>(dv, sRedS, par);
-  T schur = T(1) / (sGs[i] - dv);
-  if (sizeof(T) == 4 && !(sGs[i] - dv > T(1e-3) * sGs[i])) {
+  const T gii = uniform_v(sGs[i]);
+  T schur = T(1) / (gii - dv);
+  if (sizeof(T) == 4 && !(gii - dv > T(1e-3) * gii)) {
     __syncthreads();
     double pd = 0;
     for (int r = tid; r < i; r += GT) {
@@ -205,7 +207,7 @@ __device__ __forceinline__ bool inverse_admit(T* M, int ldm, int i, const T* sGs
     }
     double z1d = 0, z2d = 0;
     group_sum3<double, W>(pd, z1d, z2d, sRes);
-    schur = static_cast<T>(1.0 / (static_cast<double>(sGs[i]) - pd));
+    schur = static_cast<T>(1.0 / (static_cast<double>(gii) - pd));
     __syncthreads();
   }
   if (ELEM && i > 0) {
@@ -416,13 +418,17 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
   T* sDtR = reinterpret_cast<T*>(smem);  // PT
   T* sPart = sDtR + PT;                   // GT partial sums
   T* sCoef = sPart + GT;                  // LC
-  T* sU = sCoef + LCL;                    // LC
-  T* sW = sU + LCL;                       // LC
-  T* sGs = sW + LCL;                      // LC
-  T* sV = sGs + LCL;                      // LC
-  int* sInd = reinterpret_cast<int*>(sV + LCL);  // LC
-  int* sTmpI = sInd + LCL;                       // LC
-  KeyIdx<T>* sRed = lds_next16<KeyIdx<T>>(smem, sTmpI + LCL);
+  // the per-position arrays are LCA apart (a multiple of 4 entries), so
+  // runs of 2 or 4 entries starting at an even / multiple-of-4 position are
+  // read with one 8- / 16-byte LDS load
+  const int LCA = (LCL + 3) & ~3;
+  T* sU = sCoef + LCA;                    // LC
+  T* sW = sU + LCA;                       // LC
+  T* sGs = sW + LCA;                      // LC
+  T* sV = sGs + LCA;                      // LC
+  int* sInd = reinterpret_cast<int*>(sV + LCA);  // LC
+  int* sTmpI = sInd + LCA;                       // LC
+  KeyIdx<T>* sRed = lds_next16<KeyIdx<T>>(smem, sTmpI + LCA);
   T* sRed3 = reinterpret_cast<T*>(sRed + W);  // 3W
   // single-barrier reductions: 2 buffers of W packed keys, 2 of 2W sums
   unsigned long long* sRedU = lds_next16<unsigned long long>(smem, sRed3 + 3 * W);
@@ -442,7 +448,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
   constexpr bool mk = VAR == 2;  // masked signals
   // masked: active position -> Gram-column slot, and its shift buffer
   int* sSlot = lds_next16<int>(smem, sMl + CL * CL);
-  int* sSlotT = sSlot + LCL;
+  int* sSlotT = sSlot + LCA;
   T* const gaw = mk ? a.ga + (size_t)blockIdx.x * a.ga_stride : nullptr;
 
   const int tid = threadIdx.x;
@@ -470,7 +476,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
         __syncthreads();
         if (tid == 0) *sSig = PREFETCH ? pre : atomicAdd(a.queue, chunk);
         __syncthreads();
-        w = *sSig;
+        w = uniform_v(*sSig);
         wend = w + chunk;
         if (PREFETCH && tid == 0 && w < nwork) pre = atomicAdd(a.queue, chunk);
       }
@@ -482,7 +488,11 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
     }
     if (w >= nwork) break;
     const int wcur = w++;
-    const int sig = a.list ? a.list[wcur] : (a.order ? a.order[wcur] : wcur);
+    // Values the whole group shares (signal, lambda, norms, counters, the
+    // step quantities) are moved to scalar registers where they come from
+    // memory: the compiler then keeps the step loop's control flow uniform
+    // (scalar branches and loop counters instead of exec-mask bookkeeping).
+    const int sig = uniform_v(a.list ? a.list[wcur] : (a.order ? a.order[wcur] : wcur));
 
     // ---------------------------------------------------------- set-up
     __syncthreads();  // the previous signal's LDS state is no longer read
@@ -525,7 +535,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
     }
     // active flags of this thread's rows (bit q * VEC + e)
     ActMask act = 0;
-    const T lam = a.lambda[a.nlam > 1 ? sig : 0];
+    const T lam = uniform_v(a.lambda[a.nlam > 1 ? sig : 0]);
     T normX;
     T thrs0 = 0;
     int currentInd;
@@ -535,7 +545,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
     bool empty = false, overflow = false, reached = false;
     // masked: partly observed signals use Gram columns over their observed
     // rows; a signal with no observed row has an empty column
-    const int nobs = mk ? a.mobs[sig] : 0;
+    const int nobs = mk ? uniform_v(a.mobs[sig]) : 0;
     const bool own = mk && nobs > 0 && nobs < a.m;
     const int* mrl = own ? a.mrows + (size_t)sig * a.m : nullptr;
     const bool mcomp = own && masked_gram_comp(nobs, a.m);
@@ -546,7 +556,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
     if (mk)
       for (int j = tid; j < LC; j += GT) sSlot[j] = j;
     if (resume) {
-      const int s0 = st.isc[2];
+      const int s0 = uniform_v(st.isc[2]);
       if (own) {
         // the Gram columns of the handed-over active set, slots 0..s0-1
         for (int k = 0; k < s0; ++k)
@@ -566,20 +576,20 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
         const int c = st.ind[k];
         if (tid == (c / VEC) % GT) act |= ActMask(1) << ((c / VEC / GT) * VEC + c % VEC);
       }
-      normX = st.sc[0];
-      thrs0 = st.sc[1];
-      iter = st.isc[0];
-      admissions = st.isc[1];
+      normX = uniform_v(st.sc[0]);
+      thrs0 = uniform_v(st.sc[1]);
+      iter = uniform_v(st.isc[0]);
+      admissions = uniform_v(st.isc[1]);
       i0 = s0;
       s_active = s0;
-      currentInd = st.isc[3];
+      currentInd = uniform_v(st.isc[3]);
       __syncthreads();
     } else {
       for (int j = tid; j < LC; j += GT) {
         sCoef[j] = 0;
         sInd[j] = -1;
       }
-      normX = a.norms[sig];
+      normX = uniform_v(a.norms[sig]);
       if (mk && nobs == 0) empty = true;
       __syncthreads();
 
@@ -597,9 +607,9 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
             take_if_better(key, idx, pay, k, j < p ? j : INT_MAX, T(0));
           }
         group_argmin<T, W>(key, idx, pay, sRed);
-        currentInd = idx == INT_MAX ? 0 : idx;
+        currentInd = uniform_v(idx == INT_MAX ? 0 : idx);
       }
-      const T dc = sDtR[currentInd];
+      const T dc = uniform_v(sDtR[currentInd]);
       if (a.mode == SPX_PENALTY && fabs(dc) < lam) empty = true;
       if (a.mode == SPX_L2ERROR && normX < lam) empty = true;
     }
@@ -614,6 +624,8 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
     if (!empty) {
       bool newAtom = true;
       T thrs = thrs0;
+      T gpre = 0;          // prefetched entry of the next admission's column
+      bool pre_ok = false;
       for (int i = i0; i < Lm; ++i) {
         ++iter;
         bool have_u = false;  // the admission formed the direction
@@ -643,7 +655,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
           // the direction (coreLARS2W: the weights, decomp/decomp.h:1986-1987)
           for (int j = tid; j <= i; j += GT) {
             const int ij = sInd[j];
-            sGs[j] = gcol[ij];
+            sGs[j] = pre_ok ? gpre : gcol[ij];
             const T one = wt ? sWt[ij] : T(1);
             sW[j] = sDtR[ij] > 0 ? one : -one;
           }
@@ -736,7 +748,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
           step_max = key;
           first_zero = idx == INT_MAX ? -1 : L - idx;
         }
-        const T cc = fabs(sDtR[sInd[0]]);
+        const T cc = uniform_v(T(fabs(sDtR[sInd[0]])));
 
         // ------------------------------------- A = G(:, active) * u
         V Areg[NQ];
@@ -752,15 +764,34 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
           for (int k0 = 0; k0 < s; k0 += UNR) {
             T uk[UNR];
             const T* col[UNR];
+            // the round's multipliers and column numbers: one vector LDS read
+            // of consecutive entries each (k0 is a multiple of UNR), then
+            // scalar registers, so the column addresses and multipliers cost
+            // no vector instructions; entries past s repeat column k0 with
+            // multiplier 0
+            constexpr int AL = UNR * 4 < 16 ? UNR * 4 : 16;
+            const int* cp = static_cast<const int*>(
+                __builtin_assume_aligned((own ? sSlot : sInd) + k0, AL));
+            const T* up = static_cast<const T*>(__builtin_assume_aligned(sU + k0, AL));
+            int cv[UNR];
+            T uv[UNR];
 #pragma unroll
             for (int e = 0; e < UNR; ++e) {
-              const int k = k0 + e < s ? k0 + e : s - 1;
-              // uniform across the group: held in scalar registers, so the
-              // column addresses and multipliers cost no vector instructions
-              const T ur = uniform_v(sU[k]);
-              uk[e] = k0 + e < s ? ur : T(0);
-              const int ck = uniform_v(own ? sSlot[k] : sInd[k]);
-              col[e] = (own ? gaw : a.G) + (size_t)ck * ldg;
+              cv[e] = cp[e];
+              uv[e] = up[e];
+            }
+            int cs[UNR];
+            T us[UNR];
+#pragma unroll
+            for (int e = 0; e < UNR; ++e) {
+              cs[e] = uniform_v(cv[e]);
+              us[e] = uniform_v(uv[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < UNR; ++e) {
+              const bool live = k0 + e < s;
+              uk[e] = live ? us[e] : T(0);
+              col[e] = (own ? gaw : a.G) + (size_t)(live ? cs[e] : cs[0]) * ldg;
             }
             V g[UNR][NQ];
 #pragma unroll
@@ -915,6 +946,10 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
         }
         T step = besti == INT_MAX ? INF : bestv;
         const int nextInd = besti == INT_MAX ? 0 : besti % p;
+        // the next admission's Gram entries G(active, nextInd), loaded now so
+        // that their latency overlaps the updates (unused if the step drops)
+        pre_ok = !own && s < GT;
+        if (pre_ok && tid <= s) gpre = a.G[(size_t)nextInd * ldg + (tid < s ? sInd[tid] : nextInd)];
 
         const T coeff3 = normX - lam;
         T step_max2;
@@ -970,7 +1005,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
         if (step == step_max) {
           // ------------------- drop atom first_zero (decomp/decomp.h:1875-1903)
           const int fz = first_zero;
-          const int dropped = sInd[fz];
+          const int dropped = uniform_v(sInd[fz]);
           if (inl) {
             // LDS copy: stage the block in the HBM slot, downdate back
             for (int e = tid; e < (i + 1) * (i + 1); e += GT) {

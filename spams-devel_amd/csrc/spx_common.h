@@ -69,6 +69,21 @@ __device__ __forceinline__ double lane_v(double v, int lane) {
                               static_cast<unsigned int>(lo));
 }
 
+// A value the caller knows to be the same in every lane (an LDS word read at
+// a uniform address), moved to a scalar register: arithmetic on it (column
+// addresses, multipliers) then costs no vector instructions.
+__device__ __forceinline__ int uniform_v(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform_v(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform_v(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
 template <typename T, int CTRL>
 __device__ __forceinline__ T sum_step(T v) {
   return v + dpp_v<CTRL>(v);
@@ -154,9 +169,9 @@ __device__ __forceinline__ void group_argmin(T& key, int& idx, T& payload,
       const KeyIdx<T> c = red[w];
       take_if_better(b.key, b.idx, b.payload, c.key, c.idx, c.payload);
     }
-    key = b.key;
-    idx = b.idx;
-    payload = b.payload;
+    key = uniform_v(b.key);
+    idx = uniform_v(b.idx);
+    payload = uniform_v(b.payload);
     __syncthreads();
   }
 }
@@ -183,23 +198,11 @@ __device__ __forceinline__ void group_sum3(T& a, T& b, T& c, T* red) {
       b += red[3 * w + 1];
       c += red[3 * w + 2];
     }
+    a = uniform_v(a);
+    b = uniform_v(b);
+    c = uniform_v(c);
     __syncthreads();
   }
-}
-
-// A value the caller knows to be the same in every lane (an LDS word read at
-// a uniform address), moved to a scalar register: arithmetic on it (column
-// addresses, multipliers) then costs no vector instructions.
-__device__ __forceinline__ int uniform_v(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ float uniform_v(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-__device__ __forceinline__ double uniform_v(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffffLL));
-  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
-  return __longlong_as_double((static_cast<long long>(hi) << 32) |
-                              static_cast<unsigned int>(lo));
 }
 
 // ---- reductions with fewer instructions and barriers ----------------------
@@ -272,6 +275,9 @@ __device__ __forceinline__ unsigned long long group_min_u64(unsigned long long v
     v = r[0];
 #pragma unroll
     for (int w = 1; w < W; ++w) v = r[w] < v ? r[w] : v;
+    v = (static_cast<unsigned long long>(static_cast<unsigned>(uniform_v(static_cast<int>(v >> 32))))
+         << 32) |
+        static_cast<unsigned>(uniform_v(static_cast<int>(v)));
   }
   return v;
 }
@@ -287,6 +293,7 @@ __device__ __forceinline__ float group_min_f32(float v, float* red, int& parity)
     v = r[0];
 #pragma unroll
     for (int w = 1; w < W; ++w) v = fminf(v, r[w]);
+    v = uniform_v(v);
   }
   return v;
 }
@@ -312,6 +319,8 @@ __device__ __forceinline__ void group_sum2(T& a, T& b, T* red, int& parity) {
       a += r[2 * w];
       b += r[2 * w + 1];
     }
+    a = uniform_v(a);
+    b = uniform_v(b);
   }
 }
 template <typename T, int W>
@@ -325,6 +334,7 @@ __device__ __forceinline__ T group_sum1(T a, T* red, int& parity) {
     a = T(0);
 #pragma unroll
     for (int w = 0; w < W; ++w) a += r[w];
+    a = uniform_v(a);
   }
   return a;
 }
