@@ -176,13 +176,10 @@ __device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
 // v = M gs lands in sV.  An FP32 pivot below 1e-3 of the diagonal (or not
 // positive) is recomputed with FP64 accumulation.  Elements (r,q) and (q,r)
 // get the identical update v[min] * (schur * v[max]), so M stays exactly
-// symmetric.  ELEM: the rank-one update runs one element per thread with
-// eight loads in flight (M in HBM, or a wide group), and the caller forms the
-// direction afterwards.  Else one row per thread, which also forms the
-// direction u(0:i+1) = M_new w (sum over q in order, as symv_dist) from the
-// updated row it holds, so M is read once more instead of twice.  Returns
-// whether u was formed.  Ends with a barrier.
-template <typename T, int W, bool SPLIT, bool ELEM>
+// symmetric.  The same pass forms the direction u(0:i+1) = M_new w from the
+// rows it updates (SPLIT: the row segments of symv_dist), so M is read once
+// more instead of twice.  Returns true (u formed).  Ends with a barrier.
+template <typename T, int W, bool SPLIT>
 __device__ __forceinline__ bool inverse_admit(T* M, int ldm, int i, const T* sGs, T* sV,
                                               T* sPart, T* sRedS, int& par, double* sRes,
                                               const T* w, T* u) {
@@ -210,97 +207,77 @@ __device__ __forceinline__ bool inverse_admit(T* M, int ldm, int i, const T* sGs
     schur = static_cast<T>(1.0 / (static_cast<double>(gii) - pd));
     __syncthreads();
   }
-  if (ELEM && i > 0) {
-    // element e = r + q i of the i x i block; (r, q) advance by the uniform
-    // (GT % i, GT / i) per GT elements, so no per-element division
-    const int nn = i * i;
-    const int dqG = GT / i, drG = GT - dqG * i;
-    int r0 = tid % i, q0 = tid / i;
-    for (int e0 = tid; e0 < nn; e0 += 8 * GT) {
-      T mv[8];
-      int rr[8], qq[8];
-      int r = r0, q = q0;
+  // Rank-one update, border and direction in one pass: thread (r, k) takes
+  // row r of the new (i+1) x (i+1) block over the columns of segment k, the
+  // segments symv_dist uses for that size, and sums M_new(r, q) w(q) over
+  // them in order; the part sums are combined in part order, so u comes out
+  // exactly as symv_dist would form it from M_new.  Each element is written
+  // once, by the thread of its row and segment.
+  const int n = i + 1;
+  const int P = (!SPLIT || n >= GT) ? 1 : GT / n;
+  const int seg = (n + P - 1) / P;
+  for (int t = tid; t < n * P; t += GT) {
+    const int k = P == 1 ? 0 : t / n, r = t - k * n;
+    const int q0 = k * seg, q1 = q0 + seg < n ? q0 + seg : n;
+    const int qe = q1 < i ? q1 : i;      // columns of the old block
+    const bool hasi = q0 <= i && i < q1;  // the segment holds column i
+    T acc = 0;
+    if (r < i) {
+      const T vr = sV[r];
+      int q = q0;
+      for (; q + 8 <= qe; q += 8) {
+        T mv[8], vq[8], wq[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        rr[u] = r;
-        qq[u] = q;
-        const int e = e0 + u * GT;
-        mv[u] = e < nn ? M[r + q * ldm] : T(0);
-        r += drG;
-        q += dqG;
-        if (r >= i) {
-          r -= i;
-          ++q;
+        for (int e = 0; e < 8; ++e) {
+          mv[e] = M[r + (q + e) * ldm];
+          vq[e] = sV[q + e];
+          wq[e] = w[q + e];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const T upd = r <= q + e ? vr * (schur * vq[e]) : vq[e] * (schur * vr);
+          const T mn = mv[e] + upd;
+          M[r + (q + e) * ldm] = mn;
+          acc += mn * wq[e];
         }
       }
-      r0 = r;
-      q0 = q;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = e0 + u * GT;
-        if (e < nn) {
-          const int ru = rr[u], qu = qq[u];
-          const T vr = sV[ru], vq = sV[qu];
-          const T upd = ru <= qu ? vr * (schur * vq) : vq * (schur * vr);
-          M[ru + qu * ldm] = mv[u] + upd;
-        }
+      for (; q < qe; ++q) {
+        const T vq = sV[q];
+        const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
+        const T mn = M[r + q * ldm] + upd;
+        M[r + q * ldm] = mn;
+        acc += mn * w[q];
+      }
+      if (hasi) {
+        // the border column
+        const T off = -schur * vr;
+        M[r + i * ldm] = off;
+        acc += off * w[i];
+      }
+    } else {
+      // the border row
+      for (int q = q0; q < qe; ++q) {
+        const T off = -schur * sV[q];
+        M[i + q * ldm] = off;
+        acc += off * w[q];
+      }
+      if (hasi) {
+        M[i + i * ldm] = schur;
+        acc += schur * w[i];
       }
     }
-    for (int r = tid; r < i; r += GT) {
-      const T off = -schur * sV[r];
-      M[r + i * ldm] = off;
-      M[i + r * ldm] = off;
-    }
-    if (tid == 0) M[i + i * ldm] = schur;
-    __syncthreads();
-    return false;
-  }
-  // one row per thread: row r (r < i) of the block, then its border
-  // element, accumulating u[r]; thread i % GT forms u[i] from the border row
-  for (int r = tid; r < i; r += GT) {
-    const T vr = sV[r];
-    T acc = 0;
-    int q = 0;
-    for (; q + 8 <= i; q += 8) {
-      T mv[8], vq[8], wq[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        mv[e] = M[r + (q + e) * ldm];
-        vq[e] = sV[q + e];
-        wq[e] = w[q + e];
-      }
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const T upd = r <= q + e ? vr * (schur * vq[e]) : vq[e] * (schur * vr);
-        const T mn = mv[e] + upd;
-        M[r + (q + e) * ldm] = mn;
-        acc += mn * wq[e];
-      }
-    }
-    for (; q < i; ++q) {
-      const T vq = sV[q];
-      const T upd = r <= q ? vr * (schur * vq) : vq * (schur * vr);
-      const T mn = M[r + q * ldm] + upd;
-      M[r + q * ldm] = mn;
-      acc += mn * w[q];
-    }
-    const T off = -schur * vr;
-    M[r + i * ldm] = off;
-    M[i + r * ldm] = off;
-    acc += off * w[i];
-    u[r] = acc;
-  }
-  if (tid == i % GT) {
-    M[i + i * ldm] = schur;
-    T acc = 0;
-    for (int q = 0; q < i; ++q) {
-      const T off = -schur * sV[q];
-      acc += off * w[q];
-    }
-    acc += schur * w[i];
-    u[i] = acc;
+    if (P == 1) u[r] = acc;
+    else sPart[k * n + r] = acc;
   }
   __syncthreads();
+  if (P > 1) {
+    for (int r = tid; r < n; r += GT) {
+      T acc = sPart[r];
+      for (int k = 1; k < P; ++k) acc += sPart[k * n + r];
+      u[r] = acc;
+    }
+    __syncthreads();
+  }
   return true;
 }
 
@@ -676,10 +653,10 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
               inl = false;
             }
             if (inl)
-              have_u = inverse_admit<T, W, true, true>(sMl, CL, i, sGs, sV, sPart, sRedS, par,
+              have_u = inverse_admit<T, W, true>(sMl, CL, i, sGs, sV, sPart, sRedS, par,
                                                        sRes, sW, sU);
             else
-              have_u = inverse_admit<T, W, CAP == 0, CAP == 0>(Mc, ldm, i, sGs, sV, sPart,
+              have_u = inverse_admit<T, W, CAP == 0>(Mc, ldm, i, sGs, sV, sPart,
                                                                sRedS, par, sRes, sW, sU);
           }
           // ols: normX is the least-squares residual of the active set
