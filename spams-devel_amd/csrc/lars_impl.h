@@ -114,16 +114,18 @@ struct LarsLds {
 };
 
 // y(0:n) = M(0:n, 0:n) x for a symmetric M (leading dimension ldm), with
-// all GT threads.  One thread per row, the sum over q in order, eight loads
+// all GT threads; DOT: also returns this thread's part of y'x (its rows, in
+// order), and a group of several waves leaves the closing barrier to the
+// caller's reduction of that sum.  One thread per row, the sum over q in order, eight loads
 // in flight; when M is in HBM and has fewer rows than threads (SPLIT), row r
 // is split into P = GT / n parts whose partial sums are combined in part
 // order (deterministic).  x and y in LDS, part a GT-entry LDS buffer.  Ends
 // with a barrier.
-template <typename T, int GT, bool SPLIT>
-__device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
-                                          T* part, int n) {
+template <typename T, int GT, bool SPLIT, bool DOT = false>
+__device__ __forceinline__ T symv_dist(const T* M, int ldm, const T* x, T* y, T* part, int n) {
   const int tid = threadIdx.x;
   const int P = (!SPLIT || n >= GT) ? 1 : GT / n;
+  T dot = 0;
   if (P == 1) {
     for (int r = tid; r < n; r += GT) {
       T acc = 0;
@@ -140,9 +142,10 @@ __device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
       }
       for (; q < n; ++q) acc += M[r + q * ldm] * x[q];
       y[r] = acc;
+      if (DOT) dot += acc * x[r];
     }
-    __syncthreads();
-    return;
+    if (!DOT || GT == 64) __syncthreads();
+    return dot;
   }
   // n * P <= GT: one (row, part) per thread
   const int seg = (n + P - 1) / P;
@@ -166,8 +169,10 @@ __device__ __forceinline__ void symv_dist(const T* M, int ldm, const T* x, T* y,
     T acc = part[r];
     for (int k = 1; k < P; ++k) acc += part[k * n + r];
     y[r] = acc;
+    if (DOT) dot += acc * x[r];
   }
-  __syncthreads();
+  if (!DOT || GT == 64) __syncthreads();
+  return dot;
 }
 
 // Admits the i-th active atom (i >= 1) into the explicit inverse M of the
@@ -185,10 +190,8 @@ __device__ __forceinline__ bool inverse_admit(T* M, int ldm, int i, const T* sGs
                                               const T* w, T* u) {
   constexpr int GT = 64 * W;
   const int tid = threadIdx.x;
-  // v = invGs(i x i) * gs(0:i)
-  symv_dist<T, GT, SPLIT>(M, ldm, sGs, sV, sPart, i);
-  T dv = 0;
-  for (int r = tid; r < i; r += GT) dv += sV[r] * sGs[r];
+  // v = invGs(i x i) * gs(0:i) and gs'v; the reduction's barrier publishes v
+  T dv = symv_dist<T, GT, SPLIT, true>(M, ldm, sGs, sV, sPart, i);
   dv = group_sum1<T, W>(dv, sRedS, par);
   const T gii = uniform_v(sGs[i]);
   T schur = T(1) / (gii - dv);
@@ -620,7 +623,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
           ++s_active;
           if (tid == 0) sInd[i] = c;
           if (tid == (c / VEC) % GT) act |= ActMask(1) << ((c / VEC / GT) * VEC + c % VEC);
-          __syncthreads();
+          if (own) __syncthreads();
           const T* gcol = a.G + (size_t)c * ldg;
           if (own) {
             T* slot = gaw + (size_t)sSlot[i] * ldg;
@@ -631,7 +634,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
           // the new Gram column's active entries, and the signed weights of
           // the direction (coreLARS2W: the weights, decomp/decomp.h:1986-1987)
           for (int j = tid; j <= i; j += GT) {
-            const int ij = sInd[j];
+            const int ij = j < i ? sInd[j] : c;  // sInd[i] is published below
             sGs[j] = pre_ok ? gpre : gcol[ij];
             const T one = wt ? sWt[ij] : T(1);
             sW[j] = sDtR[ij] > 0 ? one : -one;
