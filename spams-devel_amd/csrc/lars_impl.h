@@ -103,7 +103,8 @@ struct LarsLds {
     b = (b + 15) & ~size_t(15);
     if (CAP > 0) b += 2 * sizeof(T) * CAP * CAP + 16;
     b = (b + 15) & ~size_t(15);
-    b += sizeof(double) * (size_t)(LC + (LC > 64 ? LC : 64)) + 16;  // FP64 vectors
+    // FP64 refit vectors (the LDS pass keeps them in its idle inverse buffer)
+    if (CAP == 0) b += sizeof(double) * (size_t)(LC + (LC > 64 ? LC : 64)) + 16;
     b = (b + 15) & ~size_t(15);
     b += sizeof(T) * (size_t)CL * CL + 16;  // LDS copy of the inverse (full pass)
     b = (b + 15) & ~size_t(15);
@@ -416,12 +417,16 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
   int* sSig = reinterpret_cast<int*>(sRedS + 4 * W);
   T* sM0 = lds_next16<T>(smem, sSig + 1);  // CAP x CAP
   T* sM1 = sM0 + (CAP > 0 ? CAP * CAP : 0);
-  double* sAlpha = lds_next16<double>(smem, sM1 + (CAP > 0 ? CAP * CAP : 0));
-  double* sRes = sAlpha + LCL;  // FP64 refit vectors; sRes has max(LC, 64)
+  // FP64 refit vectors (alpha LC, residual / reduction scratch max(LC, 64)):
+  // the full-capacity pass has them here; the LDS pass keeps them in its
+  // idle inverse buffer, which no live data occupies outside a drop
+  double* const sAlphaF =
+      lds_next16<double>(smem, sM1 + (CAP > 0 ? CAP * CAP : 0));  // used when CAP == 0
+  double* const sResF = sAlphaF + LCL;
   // full-capacity pass: LDS copy of the inverse while the active set has at
   // most CL atoms (CL x CL, leading dimension CL)
   const int CL = CAP == 0 ? a.lds_cap : 0;
-  T* sMl = lds_next16<T>(smem, sRes + (LCL > 64 ? LCL : 64));
+  T* sMl = CAP > 0 ? reinterpret_cast<T*>(sAlphaF) : lds_next16<T>(smem, sResF + (LCL > 64 ? LCL : 64));
   // weighted penalty: the signal's weights of the group's rows (1 past p)
   T* sWt = lds_next16<T>(smem, sMl + CL * CL);
   constexpr bool wt = VAR == 1;  // weighted penalty
@@ -598,6 +603,8 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
 
     T* Mc = M0;
     T* Mn = M1;
+    auto sAlpha_ = [&]() { return CAP > 0 ? reinterpret_cast<double*>(Mn) : sAlphaF; };
+    auto sRes_ = [&]() { return CAP > 0 ? reinterpret_cast<double*>(Mn) + LCL : sResF; };
     // full-capacity pass: the inverse starts in LDS (capacity CL) and moves
     // to the HBM slot when the active set outgrows it
     bool inl = CAP == 0 && CL > 0 && (!resume || CL >= i0);
@@ -657,16 +664,16 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
             }
             if (inl)
               have_u = inverse_admit<T, W, true>(sMl, CL, i, sGs, sV, sPart, sRedS, par,
-                                                       sRes, sW, sU);
+                                                       sRes_(), sW, sU);
             else
               have_u = inverse_admit<T, W, CAP == 0>(Mc, ldm, i, sGs, sV, sPart,
-                                                               sRedS, par, sRes, sW, sU);
+                                                               sRedS, par, sRes_(), sW, sU);
           }
           // ols: normX is the least-squares residual of the active set
           // (decomp/decomp.h:1044-1050)
           if (a.ols)
             normX = ls_residual<T, W>(inl ? sMl : Mc, inl ? CL : ldm, i + 1, dtx, sInd,
-                                      a.norms[sig], sRes);
+                                      a.norms[sig], sRes_());
         }
 
         const int s = i + 1;
@@ -1035,7 +1042,7 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
           // (downDateLasso, decomp/decomp.h:1191-1201)
           if (a.ols && s_active > 0)
             normX = ls_residual<T, W>(inl ? sMl : Mc, inl ? CL : ldm, s_active, dtx, sInd,
-                                      a.norms[sig], sRes);
+                                      a.norms[sig], sRes_());
           i = i - 2;
           // an emptied active set has no direction: end the path (the
           // reference would read ind[-1] here)
@@ -1068,6 +1075,8 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
       const int s = s_active;
       const T* Mf = inl ? sMl : Mc;  // the inverse of the final active block
       const int ldf = inl ? CL : ldm;
+      double* const sAlpha = sAlpha_();
+      double* const sRes = sRes_();
       for (int r = tid; r < s; r += GT) sAlpha[r] = a.ols ? 0.0 : static_cast<double>(sCoef[r]);
       __syncthreads();
       for (int it = 0; it < (a.ols ? 3 : 2); ++it) {
