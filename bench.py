@@ -307,7 +307,7 @@ def run_lars(args):
     tune = None
     if args.waves <= 0:
         log("bench: autotune")
-        ns = min(n, 16384)
+        ns = min(n, 65536)
         w, trial = d.autotune_lasso(Xt[:ns], lam_t[:ns])
         tune = {"waves_per_signal": w, "trial_ms": trial, "sample_signals": ns}
 
@@ -569,7 +569,7 @@ def c2_report(E, args, n=100000):
     cv = torch.empty((n * Lout,), dtype=torch.float32, device=E.dev)
     ldg = (p + 15) // 16 * 16
     DtX = torch.empty((n, ldg), dtype=torch.float32, device=E.dev)
-    ns_t = min(n, 16384)
+    ns_t = min(n, 65536)
     w, trial = d.autotune_omp(Xt[:ns_t], L=L_t, eps=e_t)
 
     def step(form="batched"):
@@ -776,7 +776,7 @@ def run_omp(args):
 
     tune = None
     if args.waves <= 0:
-        ns = min(n, 16384)
+        ns = min(n, 65536)
         w, trial = d.autotune_omp(Xt[:ns], L=L_t, eps=e_t)
         tune = {"waves_per_signal": w, "trial_ms": trial, "sample_signals": ns}
     if args.calib_copy:

@@ -204,7 +204,7 @@ size_t spx_dict_bytes(const spx_dict* dict);
 
 /* Autotuner (no counterpart in the reference, whose OpenMP loop has one
  * thread per signal): times the batched solver at 1, 2, 4 and 16 wavefronts
- * per signal on the first min(n, 16384) signals of a sample batch and stores
+ * per signal on the first min(n, 65536) signals of a sample batch and stores
  * the fastest width on the dictionary; later batched calls on the dictionary
  * with waves_per_signal <= 0 use it (results depend on the width only through
  * FP32 summation order).  `ms` (optional, 4 entries) receives each width's

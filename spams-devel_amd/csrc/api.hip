@@ -576,7 +576,7 @@ static int omp_solve_dtx(spx_ctx* ctx, const spx_dict* d, const T* DtX, const T*
 
 // ------------------------------------------------------------- autotuner
 
-constexpr int kTuneSignals = 16384;
+constexpr int kTuneSignals = 65536;
 constexpr int kTuneWidths[4] = {1, 2, 4, 16};
 
 // Device lists for the autotuner's trial runs (freed on scope exit).
