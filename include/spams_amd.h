@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "spx.h"
 
@@ -67,8 +68,22 @@ class Vector {
   bool _extern = false;
 };
 
+// AbstractMatrixB (linalg/linalg.h:138): the common base of the dense and
+// sparse matrices that _fistaFlat takes as its dictionary.  Here it gives
+// the sizes and a dense column-major copy (the engine's products are dense
+// MFMA GEMMs, so a sparse dictionary is expanded once per call).
 template <typename T>
-class Matrix {
+class AbstractMatrixB {
+ public:
+  virtual ~AbstractMatrixB() {}
+  virtual int m() const = 0;
+  virtual int n() const = 0;
+  // writes the m() x n() matrix, column-major, to out
+  virtual void copyDense(T* out) const = 0;
+};
+
+template <typename T>
+class Matrix : public AbstractMatrixB<T> {
  public:
   Matrix() {}
   Matrix(int m, int n) { resize(m, n); }
@@ -92,8 +107,11 @@ class Matrix {
   void setZeros() {
     if (_X) std::memset(_X, 0, sizeof(T) * (size_t)_m * _n);
   }
-  int m() const { return _m; }
-  int n() const { return _n; }
+  int m() const override { return _m; }
+  int n() const override { return _n; }
+  void copyDense(T* out) const override {
+    if (_X) std::memcpy(out, _X, sizeof(T) * (size_t)_m * _n);
+  }
   T* rawX() const { return _X; }
   T& operator()(int i, int j) { return _X[i + (size_t)j * _m]; }
   T operator()(int i, int j) const { return _X[i + (size_t)j * _m]; }
@@ -104,10 +122,6 @@ class Matrix {
   bool _extern = false;
 };
 
-// AbstractMatrixB stand-in for _fistaFlat's dictionary argument: a dense
-// column-major matrix (the SpMatrix dictionary form is not provided).
-template <typename T>
-using AbstractMatrixB = Matrix<T>;
 // Data<T> stand-in for trainDL's signal argument: dense column-major signals.
 template <typename T>
 using Data = Matrix<T>;
@@ -116,31 +130,44 @@ template <typename T>
 using AbstractMatrix = Matrix<T>;
 
 template <typename T>
-class SpMatrix {
+class SpMatrix : public AbstractMatrixB<T> {
  public:
   SpMatrix() {}
+  // wraps caller-owned CSC arrays (linalg/linalg.h:4119-4122): column j has
+  // rows r[pB[j] .. pE[j]-1] with values v[...]
+  SpMatrix(T* v, int* r, int* pB, int* pE, int m, int n, int nzmax)
+      : _pB(pB), _pEx(pE), _r(r), _v(v), _m(m), _n(n), _nzmax(nzmax), _extern(true) {}
   ~SpMatrix() { clear(); }
   SpMatrix(const SpMatrix&) = delete;
   SpMatrix& operator=(const SpMatrix&) = delete;
   void clear() {
-    std::free(_pB);
-    std::free(_r);
-    std::free(_v);
-    _pB = _r = nullptr;
+    if (!_extern) {
+      std::free(_pB);
+      std::free(_r);
+      std::free(_v);
+    }
+    _pB = _pEx = _r = nullptr;
     _v = nullptr;
     _m = _n = 0;
     _nzmax = 0;
+    _extern = false;
   }
-  int m() const { return _m; }
-  int n() const { return _n; }
+  int m() const override { return _m; }
+  int n() const override { return _n; }
+  void copyDense(T* out) const override {
+    std::memset(out, 0, sizeof(T) * (size_t)_m * _n);
+    for (int j = 0; j < _n; ++j)
+      for (int k = _pB[j]; k < pE()[j]; ++k) out[_r[k] + (size_t)j * _m] += _v[k];
+  }
   int nzmax() const { return (int)_nzmax; }
   int* pB() const { return _pB; }
-  int* pE() const { return _pB ? _pB + 1 : nullptr; }
+  int* pE() const { return _pEx ? _pEx : (_pB ? _pB + 1 : nullptr); }
   int* r() const { return _r; }
   T* v() const { return _v; }
   // takes ownership of a library CSC result
   void adopt(spx_csc& c) {
     clear();
+    _extern = false;
     _m = c.m;
     _n = c.n;
     _nzmax = c.nnz;
@@ -153,10 +180,12 @@ class SpMatrix {
 
  private:
   int* _pB = nullptr;
+  int* _pEx = nullptr;  // column ends of wrapped arrays (null: pB + 1)
   int* _r = nullptr;
   T* _v = nullptr;
   int _m = 0, _n = 0;
   long long _nzmax = 0;
+  bool _extern = false;
 };
 
 namespace spams_amd_detail {
@@ -548,7 +577,7 @@ Matrix<T>* _fistaFlat(Matrix<T>* X, AbstractMatrixB<T>* D, Matrix<T>* alpha0, Ma
                       Vector<T>* inner_weights, int size_group, bool sqrt_step,
                       bool transpose, int linesearch_mode) {
   (void)num_threads; (void)delta; (void)lambda3; (void)a; (void)b; (void)c;
-  (void)compute_gram; (void)resetflow; (void)verbose; (void)clever; (void)log;
+  (void)resetflow; (void)verbose; (void)clever; (void)log;
   (void)logName; (void)inner_weights; (void)size_group; (void)sqrt_step;
   const std::string loss = name_loss ? name_loss : "";
   const std::string reg = name_regul ? name_regul : "";
@@ -566,10 +595,17 @@ Matrix<T>* _fistaFlat(Matrix<T>* X, AbstractMatrixB<T>* D, Matrix<T>* alpha0, Ma
   if (alpha->m() != p || alpha->n() != n) alpha->resize(p, n);
   spx_fista_opts o{max_it, (double)L0, fixed_step ? 1 : 0, (double)gamma, (double)_lambda,
                    (double)lambda2, (double)tol, it0, max_iter_backtracking, regul,
-                   pos ? 1 : 0, ista ? 1 : 0, SPX_PREC_FP32};
+                   pos ? 1 : 0, ista ? 1 : 0, SPX_PREC_FP32, 0, compute_gram ? 1 : 0};
+  // a dense dictionary is passed as it is; any other (SpMatrix) is expanded
+  const Matrix<T>* Dm = dynamic_cast<const Matrix<T>*>(D);
+  std::vector<T> Dd;
+  if (!Dm) {
+    Dd.resize((size_t)m * p);
+    D->copyDense(Dd.data());
+  }
   Matrix<T>* info = new Matrix<T>(4, n);
-  const int rc = spams_amd_detail::fista_call(X->rawX(), m, n, D->rawX(), p, alpha0->rawX(),
-                                              alpha->rawX(), &o, info->rawX());
+  const int rc = spams_amd_detail::fista_call(X->rawX(), m, n, Dm ? Dm->rawX() : Dd.data(), p,
+                                              alpha0->rawX(), alpha->rawX(), &o, info->rawX());
   if (rc != SPX_OK) {
     delete info;
     throw spx_last_error();

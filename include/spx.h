@@ -132,6 +132,9 @@ typedef struct {
   int graph;           /* fixed step: replay the whole iteration loop as one
                         * hipGraph, captured once per (buffers, sizes,
                         * options) and cached on the context */
+  int compute_gram;    /* gradient G alpha - D'x with G = D'D and D'X formed
+                        * once, backtracking test (y-z)'G(y-z)
+                        * (prox/fista.h:364-413); 0 = D'(D alpha - x) */
 } spx_fista_opts;
 
 /* ---------------------------------------------------------------- context */

@@ -126,6 +126,7 @@ __global__ __launch_bounds__(256) void prox_trial_kernel(
 // Accept/reject a backtracking trial: passes when ||D diff||^2 <= L ||diff||^2.
 template <typename T>
 __global__ void backtrack_check_kernel(int n, int m, const T* __restrict__ Ud,
+                                       const T* __restrict__ Vd,
                                        const T* __restrict__ dn, T gamma,
                                        int last, T* __restrict__ Lsig,
                                        int* __restrict__ pending,
@@ -133,9 +134,12 @@ __global__ void backtrack_check_kernel(int n, int m, const T* __restrict__ Ud,
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (j >= n || !pending[j]) return;
+  // ||D (y - z)||^2, or with the Gram matrix (y - z)' G (y - z) = V'U
+  // with U = G (y - z), V = y - z (prox/fista.h:404-413)
   const T* u = Ud + (size_t)j * m;
+  const T* v = Vd ? Vd + (size_t)j * m : u;
   T s = 0;
-  for (int k = lane; k < m; k += 64) s += u[k] * u[k];
+  for (int k = lane; k < m; k += 64) s += u[k] * v[k];
   s = wave_sum(s);
   if (lane == 0) {
     // the reference's while loop multiplies L on every failed trial, the
@@ -308,12 +312,18 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
   // two products when a small batch leaves most of the chip idle (the inner
   // dimension is then split, gemm_tn_product)
   const bool bt = !o->fixed_step;
+  // compute_gram: the gradient as G alpha - D'x with G = D'D and D'X formed
+  // once, and the backtracking test as (y-z)' G (y-z) (prox/fista.h:364-413)
+  const bool gram = o->compute_gram != 0;
   const size_t pn = (size_t)p * n, mn = (size_t)m * n;
   const int sR = gemm_small_split(m, n, p), sG = gemm_small_split(p, n, m);
-  const size_t npart = std::max(sR > 1 ? (size_t)sR * mn : 0, sG > 1 ? (size_t)sG * pn : 0);
+  const int sQ = gram ? gemm_small_split(p, n, p) : 1;
+  const size_t npart = std::max({sR > 1 ? (size_t)sR * mn : 0, sG > 1 ? (size_t)sG * pn : 0,
+                                 sQ > 1 ? (size_t)sQ * pn : 0});
   // every piece starts 256-byte aligned (take below)
   auto padded = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
-  const size_t need = padded(sizeof(T) * pn) * (bt ? 4 : 2) + padded(sizeof(T) * mn) +
+  const size_t need = padded(sizeof(T) * pn) * ((bt ? 4 : 2) + (gram ? (bt ? 2 : 1) : 0)) +
+                      padded(sizeof(T) * mn) +
                       padded(sizeof(T) * n) * 2 + padded(sizeof(T) * 2 * n) +
                       padded(sizeof(int) * n) * 2 + padded(sizeof(int) * 4) +
                       padded(sizeof(T) * npart);
@@ -338,6 +348,10 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
   int* pending = reinterpret_cast<int*>(take(sizeof(int) * n));
   int* npend = reinterpret_cast<int*>(take(sizeof(int) * 4));
   T* part = npart ? reinterpret_cast<T*>(take(sizeof(T) * npart)) : nullptr;
+  T* DtX = gram ? reinterpret_cast<T*>(take(sizeof(T) * pn)) : nullptr;
+  T* Gq = gram && bt ? reinterpret_cast<T*>(take(sizeof(T) * pn)) : nullptr;
+  const T* G = static_cast<const T*>(d->G);
+  const int ldg = d->ldg;
   (void)cur;
 
   ProxParams<T> pp;
@@ -361,9 +375,20 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
     // launch timers only for eager launches on the context's stream (a
     // captured loop records no events)
     const bool timed = qs == st;
-    // gradient of the square loss at `at`: R = D*at - X, grad = D'R
+    if (gram) {
+      // D'X, once (SqLoss::init)
+      ScopedTimer tm(ctx, timed ? SPX_TIMER_FISTA_PRODUCTS : -1);
+      int r = gemm_tn_product<T>(qs, p, n, m, D, m, X, m, DtX, p, T(1), T(0), nullptr, 0, part,
+                                 sG, prec);
+      if (r) return r;
+    }
+    // gradient of the square loss at `at`: R = D*at - X, grad = D'R; with
+    // the Gram matrix grad = G at - D'X
     auto gradient = [&](const T* at) -> int {
       ScopedTimer tm(ctx, timed ? SPX_TIMER_FISTA_PRODUCTS : -1);
+      if (gram)
+        return gemm_tn_product<T>(qs, p, n, p, G, ldg, at, p, grad, p, T(1), T(-1), DtX, p, part,
+                                  sQ, prec);
       int r = gemm_tn_product<T>(qs, m, n, p, Dt, p, at, p, R, m, T(1), T(-1), X, m, part, sR,
                                  prec);
       if (r) return r;
@@ -414,13 +439,17 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
           hipLaunchKernelGGL(prox_trial_kernel<T>, cols4(n), dim3(256), 0, qs, p,
                              n, base, grad, Lsig, pending, pp, zb, diff, stopsum);
           SPX_HIP(hipGetLastError());
-          rc2 = gemm_tn_product<T>(qs, m, n, p, Dt, p, diff, p, R, m, T(1), T(0), nullptr, 0,
-                                   part, sR, prec);
+          if (gram)
+            rc2 = gemm_tn_product<T>(qs, p, n, p, G, ldg, diff, p, Gq, p, T(1), T(0), nullptr,
+                                     0, part, sQ, prec);
+          else
+            rc2 = gemm_tn_product<T>(qs, m, n, p, Dt, p, diff, p, R, m, T(1), T(0), nullptr, 0,
+                                     part, sR, prec);
           if (rc2) return rc2;
           hipLaunchKernelGGL(zero_one_kernel, dim3(1), dim3(1), 0, qs, npend);
           hipLaunchKernelGGL(backtrack_check_kernel<T>, cols4(n), dim3(256), 0, qs,
-                             n, m, R, stopsum, static_cast<T>(o->gamma), last,
-                             Lsig, pending, npend);
+                             n, gram ? p : m, gram ? Gq : R, gram ? diff : nullptr, stopsum,
+                             static_cast<T>(o->gamma), last, Lsig, pending, npend);
           SPX_HIP(hipGetLastError());
           int h = 0;
           SPX_HIP(hipMemcpyAsync(&h, npend, sizeof(int), hipMemcpyDeviceToHost, qs));
@@ -451,7 +480,7 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
 
   // ---- fixed step: the whole loop as one graph, captured once per key
   struct Key {
-    int elem, m, p, n, max_it, regul, pos, ista, it0, prec;
+    int elem, m, p, n, max_it, regul, pos, ista, it0, prec, gram;
     const void *X, *W0, *W, *info, *D, *Dt, *ws;
     double L0, lambda1, lambda2, tol;
   } key;
@@ -466,6 +495,7 @@ int fista_run(Ctx* ctx, const Dict* d, const T* X, int n, const T* W0, T* W,
   key.ista = ista;
   key.it0 = it0;
   key.prec = prec;
+  key.gram = gram;
   key.X = X;
   key.W0 = W0;
   key.W = W;

@@ -57,7 +57,8 @@ class spx_fista_opts(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("lambda1", ctypes.c_double),
                 ("lambda2", ctypes.c_double), ("tol", ctypes.c_double),
                 ("it0", _I), ("max_iter_backtracking", _I), ("regul", _I),
-                ("pos", _I), ("ista", _I), ("precision", _I), ("graph", _I)]
+                ("pos", _I), ("ista", _I), ("precision", _I), ("graph", _I),
+                ("compute_gram", _I)]
 
 
 class spx_train_dl_opts(ctypes.Structure):
@@ -565,7 +566,7 @@ def fistaFlat(Y, X, W0, return_optim_info=False, numThreads=-1, max_it=1000,
     info = np.zeros((4, n), dtype=dtype, order="F")
     o = spx_fista_opts(max_it, L0, int(fixed_step), gamma, lambda1, lambda2,
                        tol, it0, max_iter_backtracking, regs[regul], int(pos),
-                       int(ista), PREC[precision])
+                       int(ista), PREC[precision], 0, int(bool(compute_gram)))
     ctx = ctx or default_context()
     f = getattr(lib(), "spx_fista_flat_" + sfx)
     f.argtypes = [_P, _P, _I, _I, _P, _I, _P, _P,
@@ -844,17 +845,17 @@ def omp_mask_dev(ctx, d: Dictionary, X, B, V, R, count, L=None, eps=None, lambda
 def fista_flat_dev(ctx, d: Dictionary, X, W0, W, info, lambda1, regul="l1",
                    lambda2=0.0, max_it=1000, L0=1.0, fixed_step=False,
                    gamma=1.5, tol=1e-6, it0=100, max_iter_backtracking=1000,
-                   pos=False, ista=False, precision="fp32", graph=True):
+                   pos=False, ista=False, precision="fp32", graph=True, compute_gram=False):
     """Device FISTA: X torch (n, m), W0/W torch (n, p), info torch (n, 4).
     With a fixed step and graph=True the iteration loop is captured once per
     (buffers, sizes, options) as a hipGraph cached on the context and
-    replayed."""
+    replayed.  compute_gram: gradient G alpha - D'x (prox/fista.h:364-413)."""
     n = X.shape[0]
     sfx = "f32" if d.dtype == np.float32 else "f64"
     o = spx_fista_opts(max_it, L0, int(fixed_step), gamma, lambda1, lambda2,
                        tol, it0, max_iter_backtracking,
                        {"l1": 0, "elastic-net": 1}[regul], int(pos), int(ista),
-                       PREC[precision], int(bool(graph)))
+                       PREC[precision], int(bool(graph)), int(bool(compute_gram)))
     f = getattr(lib(), "spx_fista_flat_dev_" + sfx)
     f.argtypes = [_P, _P, _P, _I, _P, _P, ctypes.POINTER(spx_fista_opts), _P]
     _check(f(ctx.h, d.h, _tptr(X), n, _tptr(W0), _tptr(W), ctypes.byref(o),

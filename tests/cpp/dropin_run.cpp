@@ -150,6 +150,35 @@ int main(int argc, char** argv) {
         false, false, lname, false, nullptr, 1, true, false, 0);
     write_raw("fista_W", Wf.rawX(), sizeof(double) * (size_t)p * n);
     delete info;
+    // the same call with the dictionary as an SpMatrix (AbstractMatrixB),
+    // and with compute_gram and backtracking (prox/fista.h:364-413)
+    {
+      std::vector<int> pB(p + 1), rr;
+      std::vector<double> vv;
+      for (int j = 0; j < p; ++j) {
+        pB[j] = (int)rr.size();
+        for (int i = 0; i < m; ++i)
+          if (D(i, j) != 0.0) {
+            rr.push_back(i);
+            vv.push_back(D(i, j));
+          }
+      }
+      pB[p] = (int)rr.size();
+      SpMatrix<double> Ds(vv.data(), rr.data(), pB.data(), pB.data() + 1, m, p, (int)rr.size());
+      Matrix<double> Ws(p, n), Wg(p, n);
+      info = _fistaFlat<double>(
+          &X, &Ds, &W0, &Ws, &groups, 1, 60, L0, true, 1.5, 0.05, 1.0, 0.0, 0.0, 1.0, 0.0, 1.0,
+          0.0, 100, 1000, false, false, false, false, false, rl1, lsq, false, false, false,
+          false, false, false, lname, false, nullptr, 1, true, false, 0);
+      write_raw("fista_sparseD_W", Ws.rawX(), sizeof(double) * (size_t)p * n);
+      delete info;
+      info = _fistaFlat<double>(
+          &X, &D, &W0, &Wg, &groups, 1, 60, 0.1, false, 1.5, 0.05, 1.0, 0.0, 0.0, 1.0, 0.0, 1.0,
+          0.0, 100, 1000, true, false, false, false, false, rl1, lsq, false, false, false,
+          false, false, false, lname, false, nullptr, 1, true, false, 0);
+      write_raw("fista_gram_W", Wg.rawX(), sizeof(double) * (size_t)p * n);
+      delete info;
+    }
     // cppLasso / cppOMP (cpp_library/cppspams.h:45-51, :106-113)
     Matrix<double>* cpath = nullptr;
     SpMatrix<double>* cl = cppLasso(X, D, &cpath, false, 10, 0.15);

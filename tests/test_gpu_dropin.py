@@ -113,6 +113,15 @@ def test_cpp_dropin_runs_and_matches_oracle(tmp_path):
     Wo, _ = oracle.fista_flat(X, D, np.zeros((p, n)), 0.05, regul="l1", max_it=60, L0=L0,
                               fixed_step=True, tol=0.0, it0=100)
     assert np.abs(Wf - Wo).max() < 1e-9
+    # the dictionary given as an SpMatrix: the same result
+    Ws = np.fromfile(str(tmp_path / "fista_sparseD_W.bin"), np.float64).reshape(n, p).T
+    assert np.abs(Ws - Wo).max() < 1e-9
+    # compute_gram with backtracking (L0 = 0.1): the Gram-form gradient and
+    # backtracking test of the reference (prox/fista.h:364-413)
+    Wg = np.fromfile(str(tmp_path / "fista_gram_W.bin"), np.float64).reshape(n, p).T
+    Wgo, _ = oracle.fista_flat(X, D, np.zeros((p, n)), 0.05, regul="l1", max_it=60, L0=0.1,
+                               fixed_step=False, tol=0.0, it0=100, compute_gram=True)
+    assert np.abs(Wg - Wgo).max() < 1e-9
     # cppLasso (L = 10), cppOMP (L = 8), lasso2 with D (L = 10)
     refl = oracle.lasso(X, D, 0.15, L=10).toarray()
     assert np.abs(read_csc(str(tmp_path / "cpp_lasso.bin"), p) - refl).max() < 1e-9

@@ -387,6 +387,30 @@ def test_fista_small_batch_split_products_match_oracle(spx, orc, fixed, shape):
         np.testing.assert_allclose(ig[0], io[0], rtol=1e-9)
 
 
+@pytest.mark.parametrize("fixed", [True, False])
+@pytest.mark.parametrize("shape", [(64, 256, 40), (512, 2048, 64)])
+def test_fista_compute_gram_matches_oracle(spx, orc, fixed, shape):
+    """compute_gram=True (prox/fista.h:364-413): the gradient as G alpha - D'x
+    with G = D'D and D'X formed once, and the backtracking test as
+    (y-z)' G (y-z).  FP64 against the oracle's Gram form (bit-exact to the
+    reference build) within 1e-9, l1 and elastic-net, fixed step and
+    backtracking; the second shape splits the p x p product over its inner
+    dimension (a small batch)."""
+    print("case fista compute_gram fixed=%d shape=%s" % (fixed, shape), flush=True)
+    m, p, n = shape
+    X, D = rand_problem(11, m, p, n, np.float64, k=8, noise=0.02, norm=False)
+    L0 = float(np.linalg.eigvalsh(D.T @ D)[-1]) if fixed else 0.1
+    W0 = np.zeros((p, n), np.float64, order="F")
+    for regul in ("l1", "elastic-net"):
+        kw = dict(regul=regul, lambda2=0.01, max_it=60, L0=L0, fixed_step=fixed, tol=0.0,
+                  it0=1000, compute_gram=True)
+        Wg, ig = spx.fistaFlat(X, D, W0, return_optim_info=True, loss="square",
+                               lambda1=0.05, **kw)
+        Wo, io = orc.fista_flat(X, D, W0, 0.05, **kw)
+        np.testing.assert_allclose(Wg, Wo, atol=1e-9)
+        np.testing.assert_allclose(ig[0], io[0], rtol=1e-9)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
 def test_fista_c4_shape_against_fp64_oracle(spx, orc, precision):
     """The FISTA-flat benchmark recipe (D 512x2048 unit-norm, X from 20-sparse
