@@ -556,6 +556,10 @@ def fistaFlat(Y, X, W0, return_optim_info=False, numThreads=-1, max_it=1000,
     dtype = np.asarray(Y).dtype
     sfx = _sfx(dtype)
     Yf = np.asfortranarray(Y, dtype=dtype)
+    # the binding takes the dictionary dense or sparse (its AbstractMatrixB
+    # typemap, swig/spams.i:187); the engine's products are dense
+    if hasattr(X, "toarray"):
+        X = X.toarray()
     Xf = np.asfortranarray(X, dtype=dtype)
     W0f = np.asfortranarray(W0, dtype=dtype)
     m, n = Yf.shape

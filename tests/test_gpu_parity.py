@@ -409,6 +409,15 @@ def test_fista_compute_gram_matches_oracle(spx, orc, fixed, shape):
         Wo, io = orc.fista_flat(X, D, W0, 0.05, **kw)
         np.testing.assert_allclose(Wg, Wo, atol=1e-9)
         np.testing.assert_allclose(ig[0], io[0], rtol=1e-9)
+    # the dictionary given as a scipy sparse matrix (the binding's
+    # AbstractMatrixB typemap): the same iterates
+    import scipy.sparse as sp
+    Ws = spx.fistaFlat(X, sp.csc_matrix(D), W0, loss="square", lambda1=0.05, regul="l1",
+                       max_it=60, L0=L0, fixed_step=fixed, tol=0.0, it0=1000,
+                       compute_gram=True)
+    Wo, _ = orc.fista_flat(X, D, W0, 0.05, regul="l1", max_it=60, L0=L0, fixed_step=fixed,
+                           tol=0.0, it0=1000, compute_gram=True)
+    np.testing.assert_allclose(Ws, Wo, atol=1e-9)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
