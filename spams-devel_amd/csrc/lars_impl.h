@@ -1221,6 +1221,12 @@ int lars_launch(Ctx* ctx, const LarsArgs<T>& a, bool persistent, int* counter,
       if (rc) return rc;
     }
     b.queue = counter;
+    // the LDS pass takes 4 signals per grab of the shared counter when the
+    // batch gives every workgroup 32 or more: one device-scope atomic per
+    // signal on one address queues up at the L2 and its latency reaches the
+    // set-up of every signal; the batch's last signals (its shortest, in
+    // the longest-first order) bound the extra tail
+    if (CAP > 0 && b.chunk <= 1 && a.n >= 32 * blocks) b.chunk = 4;
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), lds, st, b);
     SPX_HIP(hipGetLastError());
     return SPX_OK;
