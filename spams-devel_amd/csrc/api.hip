@@ -123,6 +123,17 @@ __global__ void add_diag_kernel(T* G, int ldg, int p, T v) {
   if (j < p) G[(size_t)j * ldg + j] += v;
 }
 
+// G(i, j) = G(j, i) for i > j: the Gram matrix exactly symmetric whatever
+// the product's rounding (the BF16x3 split rounds G(i,j) and G(j,i) apart),
+// so a solver may read G(c, k) from column k, which it has just streamed,
+// instead of from column c.
+template <typename T>
+__global__ void sym_lower_kernel(T* G, int ldg, int p) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long j = e / p, i = e - j * p;
+  if (j < p && i > j) G[(size_t)j * ldg + i] = G[(size_t)i * ldg + j];
+}
+
 // dst = src + v I (p x p, leading dimension ldg, padding rows included)
 template <typename T>
 static int gram_with_ridge(hipStream_t st, const T* src, T* dst, int ldg, int p, T v) {
@@ -248,6 +259,11 @@ static int dict_create(spx_ctx* ctx, const T* D, int m, int p, int on_device,
   int rc = gemm_tn<T>(st, p, p, m, static_cast<const T*>(d->D), m,
                       static_cast<const T*>(d->D), m, static_cast<T*>(d->G),
                       d->ldg, T(1), T(0), nullptr, 0, precision);
+  if (!rc) {
+    hipLaunchKernelGGL(sym_lower_kernel<T>, dim3((unsigned)(((long long)p * p + 255) / 256)),
+                       dim3(256), 0, st, static_cast<T*>(d->G), d->ldg, p);
+    SPX_HIP(hipGetLastError());
+  }
   if (!rc)
     rc = gram_with_ridge<T>(st, static_cast<const T*>(d->G), static_cast<T*>(d->Gr), d->ldg, p,
                             static_cast<T>(1e-10));
@@ -367,6 +383,7 @@ static int lasso_dtx_dev(spx_ctx* ctx, const spx_dict* d, const T* DtX, int ldd,
   LarsArgs<T> a{};
   a.tol = T(1e-15);
   a.ldg = d->ldg;
+  a.gsym = !d->gram_only;  // the engine's Gram matrices are exactly symmetric
   // G + max(lambda2, 1e-10) I (decomp/decomp.h:1566, ProdMatrix::addDiag);
   // a Gram given as such is used unchanged (decomp/decomp.h:1574).  The
   // dictionary holds G + 1e-10 I; another ridge is formed in scratch.
@@ -547,6 +564,7 @@ static int omp_solve_dtx(spx_ctx* ctx, const spx_dict* d, const T* DtX, const T*
     a.m = d->m;
   }
   a.G = static_cast<const T*>(d->G);
+  a.gsym = !d->gram_only;
   a.ldg = d->ldg;
   a.DtX = DtX;
   a.ldd = d->ldg;

@@ -642,7 +642,9 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
           // the direction (coreLARS2W: the weights, decomp/decomp.h:1986-1987)
           for (int j = tid; j <= i; j += GT) {
             const int ij = j < i ? sInd[j] : c;  // sInd[i] is published below
-            sGs[j] = pre_ok ? gpre : gcol[ij];
+            // G(ij, c): from column ij (just streamed, cached) when G is
+            // exactly symmetric
+            sGs[j] = pre_ok ? gpre : (a.gsym && !own ? a.G[(size_t)ij * ldg + c] : gcol[ij]);
             const T one = wt ? sWt[ij] : T(1);
             sW[j] = sDtR[ij] > 0 ? one : -one;
           }
@@ -936,7 +938,10 @@ __global__ __launch_bounds__(64 * W, (CAP == 0 && W == 4) || (CAP > 0 && W <= 2 
         // the next admission's Gram entries G(active, nextInd), loaded now so
         // that their latency overlaps the updates (unused if the step drops)
         pre_ok = !own && s < GT;
-        if (pre_ok && tid <= s) gpre = a.G[(size_t)nextInd * ldg + (tid < s ? sInd[tid] : nextInd)];
+        if (pre_ok && tid <= s) {
+          const int ij = tid < s ? sInd[tid] : nextInd;
+          gpre = a.gsym ? a.G[(size_t)ij * ldg + nextInd] : a.G[(size_t)nextInd * ldg + ij];
+        }
 
         const T coeff3 = normX - lam;
         T step_max2;

@@ -247,7 +247,10 @@ __global__ __launch_bounds__(64 * W, NQ <= 4 ? 4 : 1) void ormp_kernel(OmpArgs<T
           masked_gram_col<T, GT, NQ>(a.Dt, a.Dd, a.G, a.m, p, ldg, mrl, mnsel, mcomp, c, slot, T(1e-10));
           gcol = slot;
         }
-        for (int k = tid; k < j; k += GT) sGc[k] = gcol[sInd[k]];
+        // G(c, active): from the active columns (streamed in the last step,
+        // cached) when G is exactly symmetric
+        for (int k = tid; k < j; k += GT)
+          sGc[k] = a.gsym && !own ? a.G[(size_t)sInd[k] * ldg + c] : gcol[sInd[k]];
         __syncthreads();
         // Undn(c, k) = sum_{k' <= k} G(c, ind k') Un(k', k), k < j
         for (int k = tid; k < j; k += GT) {

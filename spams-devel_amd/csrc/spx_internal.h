@@ -22,6 +22,7 @@ template <typename T>
 struct LarsArgs {
   const T* G;       // p x p Gram with the ridge on its diagonal, leading dim ldg
   int ldg;
+  int gsym;         // G is exactly symmetric: G(c, k) is read from column k
   T diag;           // the ridge max(lambda2, 1e-10), which G carries on its
                     // diagonal; added to the masked Gram columns formed over
                     // the observed rows (0 when G is used as given)
@@ -87,6 +88,7 @@ template <typename T>
 struct OmpArgs {
   const T* G;       // p x p Gram (no diagonal shift), leading dim ldg
   int ldg;
+  int gsym;         // G is exactly symmetric: G(c, k) is read from column k
   const T* DtX;     // p x n
   int ldd;
   const T* norms;   // n
