@@ -134,17 +134,40 @@ def profiled_bytes(fname, kernel, signals):
         return None
 
 
-def valu_issue():
-    """VALU issue fraction of the two LARS passes (VALU instructions x 4
-    cycles / (SIMDs x launch cycles)) from the committed rocprofv3 SQ counter
-    summary of the headline workload; null when the summary is absent."""
+def valu_counters():
+    """Per-dispatch SQ counters of the headline workload's kernels from the
+    committed rocprofv3 summary; null when the summary is absent."""
     path = os.path.join(ROOT, "profiles", "lars_valu_counters.json")
     try:
         with open(path) as f:
-            d = json.load(f)["kernels"]
+            return json.load(f)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
+
+
+def valu_issue():
+    """VALU issue fraction of the two LARS passes (VALU instructions x 4
+    cycles / (SIMDs x launch cycles)); null when the summary is absent."""
+    d = valu_counters()
+    if d is None:
+        return None
     return {k: round(v["valu_issue_frac"], 3) for k, v in d.items()}
+
+
+def valu_per_step(names, steps):
+    """VALU wave instructions per homotopy step of each LARS pass: the
+    pass's SQ_INSTS_VALU per dispatch (all waves of the signal's group) over
+    the steps that pass takes in this run; null without the summary."""
+    d = valu_counters()
+    if d is None:
+        return None
+    out = {}
+    for key, kname in names.items():
+        if kname in d and steps.get(key):
+            out[kname] = {"valu_instructions": d[kname]["SQ_INSTS_VALU"],
+                          "homotopy_steps": steps[key],
+                          "per_step": round(d[kname]["SQ_INSTS_VALU"] / steps[key], 1)}
+    return out or None
 
 
 def ormp_kernel_name(p, w):
@@ -343,6 +366,20 @@ def run_lars(args):
     # atom admission, (value, index) per final coefficient out
     solver_bytes = float(np.sum(4 * p * (1 + admissions) + 8 * counts))
     gemm_flops = 2.0 * m * p * n
+    # homotopy steps per pass: the LDS pass (active sets up to 24 atoms) takes
+    # every signal as far as a solve with lists of 24 entries (L = 24) goes;
+    # the full-capacity pass the rest of each longer path
+    st24 = torch.empty_like(stats)
+    spx.lasso_dtx_dev(E.ctx, d, DtX, norms, lam_t,
+                      torch.empty((n, 24), dtype=torch.float32, device=E.dev),
+                      torch.empty((n, 24), dtype=torch.int32, device=E.dev),
+                      torch.empty_like(cnt), form="batched", waves_per_signal=waves_used,
+                      stats=st24)
+    torch.cuda.synchronize()
+    steps_all = st[:, 0].astype(np.int64)
+    steps_lds = np.minimum(st24.cpu().numpy()[:, 0].astype(np.int64), steps_all)
+    step_counts = {"lars_fast": int(steps_lds.sum()),
+                   "lars_full": int(steps_all.sum() - steps_lds.sum())}
 
     # the host boundary: spx_lasso_f32 with host X in and a host CSC out
     # (H2D of X, dictionary set-up with G = D'D, D'X, solver, CSC, D2H)
@@ -466,7 +503,10 @@ def run_lars(args):
                          "TB_per_s": restream_bytes / (kern_ms * 1e-3) / 1e12},
                      # diagnostic: vector-ALU issue of the two passes from the committed
                      # rocprofv3 SQ counters of this workload (the binding limit)
-                     "valu_issue": valu_issue()},
+                     "valu_issue": valu_issue(),
+                     "valu_per_step": valu_per_step(
+                         dict(zip(("lars_fast", "lars_full"), lars_kernel_names(p, waves_used))),
+                         step_counts)},
         "roofline_gemm": {"bound": "mfma", "kernel": "gemm_tn_kernel<float>",
                           "achieved": gemm_flops / (gemm_ms * 1e-3) / 1e12,
                           "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
