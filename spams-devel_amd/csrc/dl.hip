@@ -495,6 +495,95 @@ __global__ __launch_bounds__(kDlBlockThreads) void dl_block_kernel(
   for (int e = tid; e < nm; e += kDlBlockThreads) Dg[e] = Db[e];
 }
 
+// Atoms per block of the register form below: its per-thread row of the
+// block's P, B and D columns (3 x NK values) stays in registers.
+template <typename T>
+constexpr int dl_push_atoms() {
+  return sizeof(T) == 4 ? 64 : 32;
+}
+
+// dl_block_kernel for m <= kDlBlockThreads (one row per thread), with the
+// block's columns in registers and the corrections pushed forward: once
+// atom c is updated by dd = d_c^new - d_c^old, every later atom j of the
+// block takes t_j += dd(r) A(k0 + c, k0 + j) at once, so that atom kk finds
+// t_kk = P(:, kk) + sum_{c < kk} dd_c A(k0 + c, k0 + kk) ready (the sum in
+// atom order) and its step is one element per thread, a group-wide norm and
+// the pushes, whose A row comes from LDS.  The atom loop is unrolled, so
+// the register arrays are indexed by constants.
+template <typename T>
+__global__ __launch_bounds__(kDlBlockThreads) void dl_block_push_kernel(
+    T* D, int m, int K, const T* __restrict__ A, const T* __restrict__ B, int k0, int nk,
+    const T* __restrict__ P, int posD, int zero_unused) {
+  constexpr int NK = dl_push_atoms<T>();
+  __shared__ __attribute__((aligned(16))) T Ar[NK * NK];  // Ar[c NK + j] = A(k0 + c, k0 + j)
+  __shared__ T red[2 * 4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = tid;
+  const bool live = r < m;
+  T t[NK], bb[NK], d[NK];
+  // unconditional loads (row and atom clamped into range), then zeros
+  // outside: no branch between the loads, so they are all in flight
+  const int rr = live ? r : m - 1;
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    const bool in = live && j < nk;
+    const int jj = j < nk ? j : nk - 1;
+    const T pv = P[(size_t)jj * m + rr];
+    const T bv = B[(size_t)(k0 + jj) * m + rr];
+    const T dv = D[(size_t)(k0 + jj) * m + rr];
+    t[j] = in ? pv : T(0);
+    bb[j] = in ? bv : T(0);
+    d[j] = in ? dv : T(0);
+  }
+  // entries past nk are zero, so the pushes need no guard
+  for (int e = tid; e < NK * NK; e += kDlBlockThreads) {
+    const int c = e / NK, j = e - c * NK;
+    Ar[e] = (c < nk && j < nk) ? A[(size_t)(k0 + j) * K + k0 + c] : T(0);
+  }
+  __syncthreads();
+  // 1 / A_kk of the next atom is formed during the step before it, off the
+  // step's dependency chain
+  T akk = Ar[0];
+  T inv = T(1) / akk;
+#pragma unroll
+  for (int kk = 0; kk < NK; ++kk) {
+    if (kk < nk) {
+      // the pushes' row of A, read before the step's barrier
+      const T* ar = Ar + kk * NK;
+      T arow[NK];
+#pragma unroll
+      for (int j = kk + 1; j < NK; ++j) arow[j] = ar[j];
+      const T akn = kk + 1 < NK ? Ar[(kk + 1) * NK + kk + 1] : T(1);
+      const T invn = T(1) / akn;
+      T dd;
+      if (akk > T(1e-6)) {
+        T v = live ? (bb[kk] - t[kk]) * inv + d[kk] : T(0);
+        if (posD && v < T(0)) v = 0;
+        const T sw = wave_sum(v * v);
+        T* slot = red + (kk & 1) * 4;
+        if (lane == 0) slot[w] = sw;
+        __syncthreads();
+        const T nrm = sqrt((slot[0] + slot[1]) + (slot[2] + slot[3]));
+        if (nrm > T(1)) v = v * (T(1) / nrm);
+        dd = v - d[kk];
+        d[kk] = v;
+      } else {
+        dd = zero_unused ? -d[kk] : T(0);
+        if (zero_unused) d[kk] = 0;
+      }
+#pragma unroll
+      for (int j = kk + 1; j < NK; ++j) t[j] += dd * arow[j];
+      akk = akn;
+      inv = invn;
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int j = 0; j < NK; ++j)
+      if (j < nk) D[(size_t)(k0 + j) * m + r] = d[j];
+  }
+}
+
 // ------------------------------------------------------------------ host
 
 inline unsigned blocks_for(long long n, int threads) {
@@ -511,6 +600,7 @@ size_t dl_block_lds(int nk, int m, size_t elem) {
 // m x b) and its A block (b x b) in kDlBlockLds of LDS, at most 64 atoms.
 template <typename T>
 int dl_block_atoms(int m) {
+  if (m <= kDlBlockThreads) return dl_push_atoms<T>();
   int b = 64;
   while (b > 1 && dl_block_lds(b, m, sizeof(T)) > kDlBlockLds) --b;
   return b;
@@ -541,9 +631,14 @@ int dl_update_dict(hipStream_t st, T* D, int m, int K, const T* A, const T* B, T
     hipLaunchKernelGGL(dl_panel_sum_kernel<T>, dim3((np + 255) / 256), dim3(256), 0, st,
                        static_cast<const T*>(Pp), S, np, P);
     SPX_HIP(hipGetLastError());
-    const size_t lds = dl_block_lds(nk, m, sizeof(T));
-    hipLaunchKernelGGL(dl_block_kernel<T>, dim3(1), dim3(kDlBlockThreads), lds, st, D, m, K,
-                       A, B, k0, nk, static_cast<const T*>(P), posD, zero_unused);
+    if (m <= kDlBlockThreads) {
+      hipLaunchKernelGGL(dl_block_push_kernel<T>, dim3(1), dim3(kDlBlockThreads), 0, st, D, m,
+                         K, A, B, k0, nk, static_cast<const T*>(P), posD, zero_unused);
+    } else {
+      const size_t lds = dl_block_lds(nk, m, sizeof(T));
+      hipLaunchKernelGGL(dl_block_kernel<T>, dim3(1), dim3(kDlBlockThreads), lds, st, D, m, K,
+                         A, B, k0, nk, static_cast<const T*>(P), posD, zero_unused);
+    }
     SPX_HIP(hipGetLastError());
   }
   return SPX_OK;
