@@ -545,6 +545,10 @@ __global__ __launch_bounds__(kDlBlockThreads) void dl_block_push_kernel(
   // step's dependency chain
   T akk = Ar[0];
   T inv = T(1) / akk;
+  // reduction slot parity, flipped on every step that has a barrier (an
+  // atom with A_kk <= 1e-6 has none: a parity by atom would let a wave
+  // overwrite a slot another wave has still to read)
+  int phase = 0;
 #pragma unroll
   for (int kk = 0; kk < NK; ++kk) {
     if (kk < nk) {
@@ -560,7 +564,8 @@ __global__ __launch_bounds__(kDlBlockThreads) void dl_block_push_kernel(
         T v = live ? (bb[kk] - t[kk]) * inv + d[kk] : T(0);
         if (posD && v < T(0)) v = 0;
         const T sw = wave_sum(v * v);
-        T* slot = red + (kk & 1) * 4;
+        T* slot = red + phase * 4;
+        phase ^= 1;
         if (lane == 0) slot[w] = sw;
         __syncthreads();
         const T nrm = sqrt((slot[0] + slot[1]) + (slot[2] + slot[3]));

@@ -269,3 +269,38 @@ def test_train_dl_ormp_modes_mid_size_fp64(spx, mode, lam):
     assert mdl["iter"] == ito == 3
     assert max(np.abs(D - Do).max(), np.abs(mdl["A"] - Ao).max(),
                np.abs(mdl["B"] - Bo).max()) < 1e-9
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_train_dl_unused_atoms_between_used_ones(spx, dt):
+    """Batch mode with a dictionary whose odd atoms span the orthogonal
+    complement of the signals (never selected: A_kk = 0, so the update skips
+    them and their step has no norm reduction), interleaved with used even
+    atoms: the update's block walk alternates steps with and without a
+    barrier.  FP64 within 1e-10 of the oracle, FP32 at the module's bar."""
+    import oracle
+    rng = np.random.default_rng(71)
+    m, K, n = 64, 64, 600
+    Q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    E = Q[:, :24] @ rng.standard_normal((24, K // 2))
+    O = Q[:, 24:48] @ rng.standard_normal((24, K // 2))
+    D0 = np.empty((m, K))
+    D0[:, 0::2] = E / np.linalg.norm(E, axis=0)
+    D0[:, 1::2] = O / np.linalg.norm(O, axis=0)
+    codes = np.zeros((K // 2, n))
+    for j in range(n):
+        codes[rng.choice(K // 2, 3, replace=False), j] = rng.standard_normal(3)
+    X = D0[:, 0::2] @ codes
+    X /= np.linalg.norm(X, axis=0)
+    X = np.asfortranarray(X.astype(dt))
+    D0 = np.asfortranarray(D0.astype(dt))
+    # batch mode: the binding's default of 5 update sweeps, given to both
+    kw = dict(lambda1=0.05, batchsize=n, iter=3, batch=True, clean=False, iter_updateD=5)
+    D, mdl = spx.trainDL(X, D=D0, return_model=True, **kw)
+    Do, Ao, Bo, ito = oracle.train_dl(X, D=D0, **kw)
+    err = [np.abs(D - Do).max(), np.abs(mdl["A"] - Ao).max(), np.abs(mdl["B"] - Bo).max()]
+    print("max |diff| D, A, B:", err, flush=True)
+    assert mdl["iter"] == ito
+    assert np.all(np.diag(Ao)[1::2] == 0)
+    td, tab = _tol(dt)
+    assert err[0] < td and max(err[1:]) < tab, err
