@@ -767,6 +767,74 @@ def test_fp32_objective_on_the_headline_batch(spx, orc):
         assert np.abs(gap).max() <= 1e-5, (form, int(np.argmax(np.abs(gap))), float(np.abs(gap).max()))
 
 
+def test_lasso_headline_full_batch_optimality(spx):
+    """The whole benchmark batch (m=256, p=1024, n=200000, per-signal
+    lambda), FP32 through the device path (D'X and norms on MFMA, batched
+    LARS, CSC assembly): a property check at full size, where the oracle
+    would take minutes.  Every signal's coefficients satisfy the lasso's
+    optimality conditions, evaluated in FP64 on the GPU with
+    r = x - D alpha: |d_j'r| <= lambda (1 + 2e-4) for every atom, and
+    d_j'r = lambda sign(alpha_j) within 2e-4 lambda on the support (the
+    1e-10 Gram ridge is far below that).  The CSC holds the counts (pB the
+    prefix sums) with the rows of each column strictly ascending."""
+    import torch
+    print("case lasso full headline batch optimality", flush=True)
+    dev = torch.device("cuda:0")
+    X, D, lam = W.lars_headline(n=200000)
+    m, n = X.shape
+    p = D.shape[1]
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    lt = torch.from_numpy(np.ascontiguousarray(lam)).to(dev)
+    Lout = spx.lasso_L(m, p, -1, spx.PENALTY, float(lam.max()))
+    ldg = (p + 15) // 16 * 16
+    DtX = torch.empty((n, ldg), dtype=torch.float32, device=dev)
+    norms = torch.empty((n,), dtype=torch.float32, device=dev)
+    V = torch.empty((n, Lout), dtype=torch.float32, device=dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+    spx.dtx_dev(ctx, d, Xt, DtX)
+    spx.sqnorms_dev(ctx, Xt, norms)
+    spx.lasso_dtx_dev(ctx, d, DtX, norms, lt, V, R, cnt)
+    pB = torch.empty((n + 1,), dtype=torch.int32, device=dev)
+    cr = torch.empty((n * Lout,), dtype=torch.int32, device=dev)
+    cv = torch.empty((n * Lout,), dtype=torch.float32, device=dev)
+    nnz = spx.lists_to_csc_dev(ctx, V, R, cnt, pB, cr, cv)
+    torch.cuda.synchronize()
+    counts = cnt.long()
+    assert nnz == int(counts.sum())
+    assert int(pB[0]) == 0
+    assert torch.equal(pB[1:].long(), torch.cumsum(counts, 0))
+    col = torch.repeat_interleave(torch.arange(n, device=dev), counts)
+    same = col[1:] == col[:-1]
+    rows = cr[:nnz].long()
+    assert bool(torch.all(rows[1:][same] > rows[:-1][same]))
+    Dd = torch.from_numpy(D.astype(np.float64)).to(dev)
+    worst_in = worst_act = 0.0
+    for c0 in range(0, n, 20000):
+        c1 = min(n, c0 + 20000)
+        k = c1 - c0
+        cc = counts[c0:c1]
+        lmax = max(int(cc.max()), 1)
+        live = torch.arange(lmax, device=dev)[None, :] < cc[:, None]
+        A = torch.zeros((k, p), dtype=torch.float64, device=dev)
+        A.scatter_add_(1, torch.where(live, R[c0:c1, :lmax].long(), 0),
+                       torch.where(live, V[c0:c1, :lmax].double(), 0.0))
+        res = Xt[c0:c1].double() - A @ Dd.T
+        C = res @ Dd
+        lc = lt[c0:c1].double()[:, None]
+        worst_in = max(worst_in, float(((C.abs() - lc) / lc).max()))
+        act = A != 0
+        if bool(act.any()):
+            worst_act = max(worst_act,
+                            float(((C - lc * torch.sign(A)).abs() / lc)[act].max()))
+    print("  max (|d'r| - lambda) / lambda %.3g, max |d'r - lambda sign| / lambda on the "
+          "support %.3g" % (worst_in, worst_act), flush=True)
+    assert worst_in <= 2e-4
+    assert worst_act <= 2e-4
+
+
 # -------------------------------------------------------- weighted penalty
 
 def _golden_dense(g):
