@@ -835,6 +835,72 @@ def test_lasso_headline_full_batch_optimality(spx):
     assert worst_act <= 2e-4
 
 
+def test_omp_full_batch_properties(spx):
+    """The whole OMP benchmark batch (D 256x1024, 100000 signals of 1 to 32
+    atoms + 0.01 noise, L = 32, eps = 1e-3), FP32 through the device path:
+    properties that fix coreORMP's output at full size, evaluated in FP64 on
+    the GPU.  At most L distinct atoms per signal; the coefficients are the
+    least-squares fit on the support (|d_j'r| <= 1e-4 ||x|| there, r = x -
+    D alpha); a signal that stops short of L atoms (none, when ||x||^2 <= eps)
+    has ||r||^2 <= eps (the stopping rule of decomp/decomp.h:566, bar 1e-3
+    relative).  With the
+    benchmark's eps every signal takes L atoms (its noise alone exceeds eps),
+    so a second run with eps = 0.05 checks the rule on signals that stop
+    early (most of them)."""
+    import torch
+    print("case omp full batch properties", flush=True)
+    dev = torch.device("cuda:0")
+    X, D, L, eps0 = W.omp_batch()
+    m, n = X.shape
+    p = D.shape[1]
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    Lout = spx.omp_L(m, p, L)
+    V = torch.empty((n, Lout), dtype=torch.float32, device=dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+    Dd = torch.from_numpy(D.astype(np.float64)).to(dev)
+    for eps in (eps0, 0.05):
+        spx.omp_dev(ctx, d, Xt, V, R, cnt, L=torch.tensor([L], dtype=torch.int32, device=dev),
+                    eps=torch.tensor([eps], dtype=torch.float32, device=dev))
+        torch.cuda.synchronize()
+        counts = cnt.long()
+        assert int(counts.max()) <= L and int(counts.min()) >= 0
+        nshort = int((counts < L).sum())
+        worst_ls = worst_eps = -1.0
+        for c0 in range(0, n, 20000):
+            c1 = min(n, c0 + 20000)
+            k = c1 - c0
+            cc = counts[c0:c1]
+            lmax = int(cc.max())
+            live = torch.arange(lmax, device=dev)[None, :] < cc[:, None]
+            idx = torch.where(live, R[c0:c1, :lmax].long(), -1)
+            srt = torch.sort(idx, dim=1).values
+            assert not bool(((srt[:, 1:] == srt[:, :-1]) & (srt[:, 1:] >= 0)).any())
+            A = torch.zeros((k, p), dtype=torch.float64, device=dev)
+            A.scatter_add_(1, idx.clamp(min=0), torch.where(live, V[c0:c1, :lmax].double(), 0.0))
+            hits = torch.zeros((k, p), dtype=torch.int32, device=dev)
+            hits.scatter_add_(1, idx.clamp(min=0), live.int())
+            xc = Xt[c0:c1].double()
+            res = xc - A @ Dd.T
+            C = res @ Dd
+            nx = xc.norm(dim=1, keepdim=True)
+            if bool((hits > 0).any()):
+                worst_ls = max(worst_ls, float((C.abs() / nx)[hits > 0].max()))
+            short = cc < L
+            if bool(short.any()):
+                r2 = (res * res).sum(dim=1)
+                worst_eps = max(worst_eps, float(((r2 - eps) / eps)[short].max()))
+        print("  eps %g: %d of %d signals short of L; max |d'r| / ||x|| on the support %.3g, "
+              "max (||r||^2 - eps) / eps when short %.3g" % (eps, nshort, n, worst_ls, worst_eps),
+              flush=True)
+        assert worst_ls <= 1e-4
+        assert worst_eps <= 1e-3
+        if eps > 0.01:
+            assert nshort >= n // 2
+
+
 # -------------------------------------------------------- weighted penalty
 
 def _golden_dense(g):
