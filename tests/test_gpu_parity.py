@@ -936,6 +936,81 @@ def test_fista_full_c4_batch_fp32_bf16x3_against_fp64_engine(spx):
         assert gap.max() <= tol, (prec, gap.max())
 
 
+def _kkt_chunks(torch, Xt, Dd, V, R, cnt, lam, w=None, mask=None, chunk=20000):
+    """Worst relative violations, over all signals, of the (weighted,
+    masked) lasso optimality conditions in FP64: max (|c_j| - t_j) / t_j and,
+    on the support, max |c_j - t_j sign(a_j)| / t_j, with c = D'(r o mask),
+    r = x - D a and t = lam * w (lam: (n,) tensor)."""
+    n, p = Xt.shape[0], Dd.shape[1]
+    dev = Xt.device
+    worst_in = worst_act = 0.0
+    counts = cnt.long()
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        k = c1 - c0
+        cc = counts[c0:c1]
+        lmax = max(int(cc.max()), 1)
+        live = torch.arange(lmax, device=dev)[None, :] < cc[:, None]
+        A = torch.zeros((k, p), dtype=torch.float64, device=dev)
+        A.scatter_add_(1, torch.where(live, R[c0:c1, :lmax].long(), 0),
+                       torch.where(live, V[c0:c1, :lmax].double(), 0.0))
+        res = Xt[c0:c1].double() - A @ Dd.T
+        if mask is not None:
+            res = res * mask[c0:c1].double()
+        C = res @ Dd
+        t = lam[c0:c1].double()[:, None]
+        if w is not None:
+            t = t * w[c0:c1].double()
+        worst_in = max(worst_in, float(((C.abs() - t) / t).max()))
+        act = A != 0
+        if bool(act.any()):
+            worst_act = max(worst_act, float(((C - t * torch.sign(A)).abs() / t)[act].max()))
+    return worst_in, worst_act
+
+
+def test_weighted_and_masked_lasso_full_batch_optimality(spx):
+    """The weighted and masked benchmark batches (the headline signals,
+    100000 of them, D 256x1024, lambda 0.1; weights U[0.5, 2] per atom and
+    signal; 85 % of the rows observed), FP32 through the device path: every
+    signal meets its problem's optimality conditions in FP64 on the GPU.
+    Weighted (coreLARS2W): |d_j'r| <= lambda w_j, equality with sign on the
+    support.  Masked (lasso_mask, decomp/decomp.h:1466-1561): the lasso on
+    the observed rows with lambda |M| / m.  Bar 2e-4 of the threshold."""
+    import torch
+    print("case weighted / masked lasso full batch optimality", flush=True)
+    dev = torch.device("cuda:0")
+    n = 100000
+    rng = np.random.default_rng(2024)
+    X, D, _ = W.lars_headline(n=n)
+    m, p = D.shape
+    lam = 0.1
+    Wt = rng.uniform(0.5, 2.0, (p, n)).astype(np.float32)
+    B = (rng.uniform(size=(m, n)) < 0.85).astype(np.uint8)
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    Wd = torch.from_numpy(np.ascontiguousarray(Wt.T)).to(dev)
+    Bd = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev)
+    lam_t = torch.tensor([lam], dtype=torch.float32, device=dev)
+    Lout = spx.lasso_L(m, p, lambda1=lam)
+    V = torch.empty((n, Lout), dtype=torch.float32, device=dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+    Dd = torch.from_numpy(D.astype(np.float64)).to(dev)
+    lam_n = torch.full((n,), lam, dtype=torch.float64, device=dev)
+    spx.lasso_weighted_dev(ctx, d, Xt, Wd, lam_t, V, R, cnt)
+    torch.cuda.synchronize()
+    wi, wa = _kkt_chunks(torch, Xt, Dd, V, R, cnt, lam_n, w=Wd)
+    print("  weighted: max violation %.3g off the support, %.3g on it" % (wi, wa), flush=True)
+    assert wi <= 2e-4 and wa <= 2e-4
+    spx.lasso_mask_dev(ctx, d, Xt, Bd, lam_t, V, R, cnt)
+    torch.cuda.synchronize()
+    lam_m = lam * Bd.double().sum(dim=1) / m
+    mi, ma = _kkt_chunks(torch, Xt, Dd, V, R, cnt, lam_m, mask=Bd)
+    print("  masked: max violation %.3g off the support, %.3g on it" % (mi, ma), flush=True)
+    assert mi <= 2e-4 and ma <= 2e-4
+
+
 # -------------------------------------------------------- weighted penalty
 
 def _golden_dense(g):
