@@ -159,6 +159,7 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_kernel(
   // 16 in FP64 (same bytes)
   constexpr int BK = sizeof(T) == 4 ? 32 : 16;
   constexpr int LD = BK + 4;  // LDS row stride (elements)
+  auto pos = [](int r, int e) { return r * LD + e; };  // element e of row r
   constexpr int NV = BK / 16;  // float4 (double4) loads per row half and operand
   typedef typename Vec4<T>::type V4;
   __shared__ __attribute__((aligned(16))) T As[2][BM * LD];
@@ -215,8 +216,8 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_kernel(
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        *reinterpret_cast<V4*>(&As[sb][(lr + 64 * h) * LD + lq + 16 * v]) = ra[h][v];
-        *reinterpret_cast<V4*>(&Bs[sb][(lr + 64 * h) * LD + lq + 16 * v]) = rb[h][v];
+        *reinterpret_cast<V4*>(&As[sb][pos(lr + 64 * h, lq + 16 * v)]) = ra[h][v];
+        *reinterpret_cast<V4*>(&Bs[sb][pos(lr + 64 * h, lq + 16 * v)]) = rb[h][v];
       }
   };
   const int kq = 4 * (lane >> 4);
@@ -226,8 +227,8 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_kernel(
       V4 a4[4], b4[4];
 #pragma unroll
       for (int x = 0; x < 4; ++x) {
-        a4[x] = *reinterpret_cast<const V4*>(&As[sb][(wm + 16 * x + (lane & 15)) * LD + kq + 16 * v]);
-        b4[x] = *reinterpret_cast<const V4*>(&Bs[sb][(wn + 16 * x + (lane & 15)) * LD + kq + 16 * v]);
+        a4[x] = *reinterpret_cast<const V4*>(&As[sb][pos(wm + 16 * x + (lane & 15), kq + 16 * v)]);
+        b4[x] = *reinterpret_cast<const V4*>(&Bs[sb][pos(wn + 16 * x + (lane & 15), kq + 16 * v)]);
       }
       __builtin_amdgcn_s_setprio(1);
 #pragma unroll
@@ -284,10 +285,16 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_bf16x3_kernel(
     C += (size_t)blockIdx.z * zstride;
   }
   constexpr int BK = 32;
-  constexpr int LD = BK + 8;
+  // LDS rows of BK = 32 BF16 (64 bytes, no padding): row r's 16-byte slot s
+  // (k 8 s .. 8 s + 7) sits at slot s ^ ((r >> 1) & 3).  The 8 lanes an LDS
+  // access serves together then hit distinct banks both when the MFMA
+  // operands are read (8 consecutive rows, one slot) and when the loaded
+  // chunk is stored (4 rows, two slots 2 apart).
+  constexpr int LD = BK;
   typedef float f4 __attribute__((ext_vector_type(4)));
   __shared__ __attribute__((aligned(16))) __bf16 Ah[2][BM * LD], Al[2][BM * LD];
   __shared__ __attribute__((aligned(16))) __bf16 Bh[2][BN * LD], Bl[2][BN * LD];
+  auto swz = [](int r, int slot) { return r * LD + ((slot ^ ((r >> 1) & 3)) << 3); };
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wid = tid >> 6;
@@ -302,6 +309,7 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_bf16x3_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
   const int lr = tid >> 1, lk = (tid & 1) * 16;
+  const int o0 = swz(lr, lk >> 3), o1 = swz(lr, (lk >> 3) + 1);  // LDS offsets of its 2 slots
   struct Regs { f4 a[4], b[4]; };
   // this thread's rows of A and B (rows past M / N read row M-1 / N-1,
   // whose products land in C rows / columns that are never stored)
@@ -346,25 +354,27 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_bf16x3_kernel(
   auto store = [&](int sb, const Regs& R) {
     bf16x8 h, l;
     split8(R.a[0], R.a[1], h, l);
-    *reinterpret_cast<bf16x8*>(&Ah[sb][lr * LD + lk]) = h;
-    *reinterpret_cast<bf16x8*>(&Al[sb][lr * LD + lk]) = l;
+    *reinterpret_cast<bf16x8*>(&Ah[sb][o0]) = h;
+    *reinterpret_cast<bf16x8*>(&Al[sb][o0]) = l;
     split8(R.a[2], R.a[3], h, l);
-    *reinterpret_cast<bf16x8*>(&Ah[sb][lr * LD + lk + 8]) = h;
-    *reinterpret_cast<bf16x8*>(&Al[sb][lr * LD + lk + 8]) = l;
+    *reinterpret_cast<bf16x8*>(&Ah[sb][o1]) = h;
+    *reinterpret_cast<bf16x8*>(&Al[sb][o1]) = l;
     split8(R.b[0], R.b[1], h, l);
-    *reinterpret_cast<bf16x8*>(&Bh[sb][lr * LD + lk]) = h;
-    *reinterpret_cast<bf16x8*>(&Bl[sb][lr * LD + lk]) = l;
+    *reinterpret_cast<bf16x8*>(&Bh[sb][o0]) = h;
+    *reinterpret_cast<bf16x8*>(&Bl[sb][o0]) = l;
     split8(R.b[2], R.b[3], h, l);
-    *reinterpret_cast<bf16x8*>(&Bh[sb][lr * LD + lk + 8]) = h;
-    *reinterpret_cast<bf16x8*>(&Bl[sb][lr * LD + lk + 8]) = l;
+    *reinterpret_cast<bf16x8*>(&Bh[sb][o1]) = h;
+    *reinterpret_cast<bf16x8*>(&Bl[sb][o1]) = l;
   };
-  const int ko = 8 * (lane >> 4);
+  // this lane's operand offsets: row (lane & 15) of each 16-row block, slot
+  // lane >> 4 (rows 16 x apart share the swizzle)
+  const int ro = swz(lane & 15, lane >> 4);
   auto mma = [&](int cur) {
     bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
-      const int ra_ = (wm + 16 * x + (lane & 15)) * LD + ko;
-      const int rb_ = (wn + 16 * x + (lane & 15)) * LD + ko;
+      const int ra_ = (wm + 16 * x) * LD + ro;
+      const int rb_ = (wn + 16 * x) * LD + ro;
       ah[x] = *reinterpret_cast<const bf16x8*>(&Ah[cur][ra_]);
       al[x] = *reinterpret_cast<const bf16x8*>(&Al[cur][ra_]);
       bh[x] = *reinterpret_cast<const bf16x8*>(&Bh[cur][rb_]);
