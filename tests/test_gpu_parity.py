@@ -840,35 +840,39 @@ def test_lasso_headline_full_batch_optimality(spx, dt):
     assert worst_act <= tol
 
 
-def test_omp_full_batch_properties(spx):
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_omp_full_batch_properties(spx, dt):
     """The whole OMP benchmark batch (D 256x1024, 100000 signals of 1 to 32
-    atoms + 0.01 noise, L = 32, eps = 1e-3), FP32 through the device path:
+    atoms + 0.01 noise, L = 32, eps = 1e-3), FP32 and FP64 through the
+    device path:
     properties that fix coreORMP's output at full size, evaluated in FP64 on
     the GPU.  At most L distinct atoms per signal; the coefficients are the
-    least-squares fit on the support (|d_j'r| <= 1e-4 ||x|| there, r = x -
-    D alpha); a signal that stops short of L atoms (none, when ||x||^2 <= eps)
+    least-squares fit on the support (|d_j'r| <= 1e-4 ||x|| there in FP32,
+    1e-10 in FP64, r = x - D alpha); a signal that stops short of L atoms (none, when ||x||^2 <= eps)
     has ||r||^2 <= eps (the stopping rule of decomp/decomp.h:566, bar 1e-3
     relative).  With the
     benchmark's eps every signal takes L atoms (its noise alone exceeds eps),
     so a second run with eps = 0.05 checks the rule on signals that stop
     early (most of them)."""
     import torch
-    print("case omp full batch properties", flush=True)
+    print("case omp full batch properties", dt.__name__, flush=True)
     dev = torch.device("cuda:0")
-    X, D, L, eps0 = W.omp_batch()
+    tdt = torch.float64 if dt == np.float64 else torch.float32
+    ls_tol = 1e-10 if dt == np.float64 else 1e-4
+    X, D, L, eps0 = W.omp_batch(dtype=dt)
     m, n = X.shape
     p = D.shape[1]
     ctx = gpu_ctx(spx)
     d = spx.Dictionary(ctx, D)
     Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
     Lout = spx.omp_L(m, p, L)
-    V = torch.empty((n, Lout), dtype=torch.float32, device=dev)
+    V = torch.empty((n, Lout), dtype=tdt, device=dev)
     R = torch.empty((n, Lout), dtype=torch.int32, device=dev)
     cnt = torch.empty((n,), dtype=torch.int32, device=dev)
     Dd = torch.from_numpy(D.astype(np.float64)).to(dev)
     for eps in (eps0, 0.05):
         spx.omp_dev(ctx, d, Xt, V, R, cnt, L=torch.tensor([L], dtype=torch.int32, device=dev),
-                    eps=torch.tensor([eps], dtype=torch.float32, device=dev))
+                    eps=torch.tensor([eps], dtype=tdt, device=dev))
         torch.cuda.synchronize()
         counts = cnt.long()
         assert int(counts.max()) <= L and int(counts.min()) >= 0
@@ -900,7 +904,7 @@ def test_omp_full_batch_properties(spx):
         print("  eps %g: %d of %d signals short of L; max |d'r| / ||x|| on the support %.3g, "
               "max (||r||^2 - eps) / eps when short %.3g" % (eps, nshort, n, worst_ls, worst_eps),
               flush=True)
-        assert worst_ls <= 1e-4
+        assert worst_ls <= ls_tol
         assert worst_eps <= 1e-3
         if eps > 0.01:
             assert nshort >= n // 2
