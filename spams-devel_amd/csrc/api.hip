@@ -129,9 +129,10 @@ __global__ void add_diag_kernel(T* G, int ldg, int p, T v) {
 // instead of from column c.
 template <typename T>
 __global__ void sym_lower_kernel(T* G, int ldg, int p) {
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  const long long j = e / p, i = e - j * p;
-  if (j < p && i > j) G[(size_t)j * ldg + i] = G[(size_t)i * ldg + j];
+  // column j = blockIdx.x, rows i of the 256-row tile blockIdx.y
+  const int j = blockIdx.x;
+  const int i = blockIdx.y * 256 + threadIdx.x;
+  if (i < p && i > j) G[(size_t)j * ldg + i] = G[(size_t)i * ldg + j];
 }
 
 // dst = src + v I (p x p, leading dimension ldg, padding rows included)
@@ -260,7 +261,9 @@ static int dict_create(spx_ctx* ctx, const T* D, int m, int p, int on_device,
                       static_cast<const T*>(d->D), m, static_cast<T*>(d->G),
                       d->ldg, T(1), T(0), nullptr, 0, precision);
   if (!rc) {
-    hipLaunchKernelGGL(sym_lower_kernel<T>, dim3((unsigned)(((long long)p * p + 255) / 256)),
+    // a 2-D grid: p columns x row tiles (p * p threads in one dimension
+    // would pass 2^32 at p = 65536)
+    hipLaunchKernelGGL(sym_lower_kernel<T>, dim3((unsigned)p, (unsigned)((p + 255) / 256)),
                        dim3(256), 0, st, static_cast<T*>(d->G), d->ldg, p);
     SPX_HIP(hipGetLastError());
   }

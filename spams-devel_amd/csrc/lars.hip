@@ -100,14 +100,7 @@ int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves) {
     if (w < waves) continue;
     int rc;
     if (form == SPX_FORM_BATCHED) {
-      int nq;
-      switch (w) {
-        case 1: nq = lars_nq<T, 1>(a.p); break;
-        case 2: nq = lars_nq<T, 2>(a.p); break;
-        case 4: nq = lars_nq<T, 4>(a.p); break;
-        default: nq = lars_nq<T, 16>(a.p); break;
-      }
-      if (nq < 0) continue;
+      if (!lars_fits<T>(w, a.p)) continue;
       rc = lars_batched<T>(ctx, a, w);
     } else {
       rc = lars_pass_w<T>(ctx, w, false, a, false, nullptr, 0);
@@ -118,13 +111,34 @@ int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves) {
   return SPX_ERR_UNSUP;
 }
 
+namespace {
+
+// Whether group width W covers p: its row chunks reach p and the LDS pass's
+// layout (the group's correlation rows plus the capacity-FastCap inverse)
+// fits the 160 KiB of LDS; past that no active-set size does.
+template <typename T, int W>
+bool lars_fits_w(int p) {
+  constexpr int C = FastCap<T>::C;
+  constexpr size_t kLds = 160 * 1024;
+  switch (lars_nq<T, W>(p)) {
+    case 1: return LarsLds<T, W, 1, C>::bytes(C) <= kLds;
+    case 2: return LarsLds<T, W, 2, C>::bytes(C) <= kLds;
+    case 4: return LarsLds<T, W, 4, C>::bytes(C) <= kLds;
+    case 8: return LarsLds<T, W, 8, C>::bytes(C) <= kLds;
+    case 16: return LarsLds<T, W, 16, C>::bytes(C) <= kLds;
+    default: return false;
+  }
+}
+
+}  // namespace
+
 template <typename T>
 bool lars_fits(int w, int p) {
   switch (w) {
-    case 1: return lars_nq<T, 1>(p) > 0;
-    case 2: return lars_nq<T, 2>(p) > 0;
-    case 4: return lars_nq<T, 4>(p) > 0;
-    case 16: return lars_nq<T, 16>(p) > 0;
+    case 1: return lars_fits_w<T, 1>(p);
+    case 2: return lars_fits_w<T, 2>(p);
+    case 4: return lars_fits_w<T, 4>(p);
+    case 16: return lars_fits_w<T, 16>(p);
     default: return false;
   }
 }

@@ -1214,3 +1214,169 @@ def test_omp_mask_path_and_device_entry(spx, orc):
         np.testing.assert_allclose(A2, ref, atol=1e-9)
     d.close()
     ctx.close()
+
+
+# ------------------------------------------------ dictionaries past 16384 atoms
+
+@pytest.mark.parametrize("dt,p", [(np.float32, 32768), (np.float32, 20000),
+                                  (np.float64, 16384)])
+def test_lasso_large_dictionary_optimality(spx, dt, p):
+    """p = 32768 (FP32) and 16384 (FP64), the largest the LARS kernels
+    take: the widest group (16 wavefronts) with 8 row chunks per lane and
+    the LDS-resident correlation vector at its largest; p = 20000 leaves
+    the last chunk ragged.  Both LARS passes run (L = m = 64 > the LDS pass's
+    capacity, so the large active sets resume from the hand-over slot).
+    The oracle would take minutes to form these Grams; the lasso's
+    optimality conditions fix the solution instead, evaluated in FP64 on
+    the GPU (bar 2e-4 of lambda in FP32, 1e-8 in FP64) over the signals
+    whose path ended below the L cap."""
+    import torch
+    print("case lasso large dictionary", dt.__name__, p, flush=True)
+    dev = torch.device("cuda:0")
+    tdt = torch.float64 if dt == np.float64 else torch.float32
+    tol = 1e-8 if dt == np.float64 else 2e-4
+    rng = np.random.default_rng(4100 + p)
+    m, n = 64, 512
+    D = W.unit_dict(rng, m, p, dt)
+    X = W.sparse_signals(rng, D, n, 5, 0.05, True, dt)
+    lam = rng.uniform(0.08, 0.3, size=n).astype(dt)
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    lt = torch.from_numpy(lam).to(dev)
+    Lout = spx.lasso_L(m, p, -1, spx.PENALTY, float(lam.max()))
+    assert Lout == m
+    ldg = (p + 15) // 16 * 16
+    DtX = torch.empty((n, ldg), dtype=tdt, device=dev)
+    norms = torch.empty((n,), dtype=tdt, device=dev)
+    V = torch.empty((n, Lout), dtype=tdt, device=dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+    spx.dtx_dev(ctx, d, Xt, DtX)
+    spx.sqnorms_dev(ctx, Xt, norms)
+    spx.lasso_dtx_dev(ctx, d, DtX, norms, lt, V, R, cnt)
+    torch.cuda.synchronize()
+    counts = cnt.long()
+    done = counts < Lout
+    print("  active set sizes %d..%d, %d of %d below the cap, %d past the LDS pass"
+          % (int(counts.min()), int(counts.max()), int(done.sum()), n,
+             int((counts > 24).sum())), flush=True)
+    assert int(done.sum()) >= 0.9 * n
+    assert int((counts > 24).sum()) > 0
+    Dd = torch.from_numpy(D.astype(np.float64)).to(dev)
+    worst_in, worst_act = _kkt_chunks(torch, Xt[done], Dd, V[done], R[done], cnt[done],
+                                      lt[done], chunk=128)
+    print("  max (|d'r| - lambda) / lambda %.3g, max |d'r - lambda sign(a)| / lambda on the "
+          "support %.3g" % (worst_in, worst_act), flush=True)
+    assert worst_in <= tol and worst_act <= tol
+    del DtX
+    d.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("dt,p", [(np.float32, 65536), (np.float32, 20000),
+                                  (np.float64, 32768)])
+def test_omp_large_dictionary_greedy_rule(spx, dt, p):
+    """p = 65536 (FP32) and 32768 (FP64), the largest the ORMP kernels take
+    (16 wavefronts x 16 register chunks per lane; p = 20000 ragged), with
+    L = 12 and no eps: every step
+    picks the atom coreORMP picks (decomp/decomp.h:538-640), the maximiser
+    of (d_j'r)^2 / (1 - ||Q'd_j||^2) over the atoms not yet chosen (|d_j'x|
+    at the first step), with r the residual of the chosen atoms' least
+    squares and Q an orthonormal basis of them, evaluated in FP64 on the
+    GPU; a signal whose best and second-best scores are within 1e-3 (FP32)
+    or 1e-9 (FP64) of each other may take either.  The coefficients are the
+    least-squares fit on the support (|d_j'r| <= 1e-4 ||x|| there in FP32,
+    1e-10 in FP64)."""
+    import torch
+    print("case omp large dictionary", dt.__name__, p, flush=True)
+    dev = torch.device("cuda:0")
+    tdt = torch.float64 if dt == np.float64 else torch.float32
+    tie = 1e-9 if dt == np.float64 else 1e-3
+    ls_tol = 1e-10 if dt == np.float64 else 1e-4
+    rng = np.random.default_rng(4200 + p)
+    m, n, L = 64, 128, 12
+    D = W.unit_dict(rng, m, p, dt)
+    X = W.sparse_signals(rng, D, n, 8, 0.01, True, dt)
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    V = torch.empty((n, L), dtype=tdt, device=dev)
+    R = torch.empty((n, L), dtype=torch.int32, device=dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+    spx.omp_dev(ctx, d, Xt, V, R, cnt, L=torch.tensor([L], dtype=torch.int32, device=dev))
+    torch.cuda.synchronize()
+    assert bool((cnt == L).all())
+    Ri = R.long()
+    assert bool((Ri >= 0).all()) and bool((Ri < p).all())
+    srt = torch.sort(Ri, dim=1).values
+    assert not bool((srt[:, 1:] == srt[:, :-1]).any())
+    Dd = torch.from_numpy(D.astype(np.float64)).to(dev)
+    xd = Xt.double()
+    DT = Dd.T.contiguous()
+    rows = torch.arange(n, device=dev)
+    ties = 0
+    for k in range(L):
+        if k == 0:
+            score = (xd @ Dd) ** 2
+        else:
+            # orthonormal bases of the chosen atoms (n, m, k), factored on the CPU
+            Q = torch.linalg.qr(DT[Ri[:, :k]].transpose(1, 2).contiguous().cpu()).Q.to(dev)
+            Qt = Q.transpose(1, 2).contiguous()
+            res = xd - torch.matmul(Q, torch.matmul(Qt, xd[:, :, None]))[:, :, 0]
+            QD = torch.matmul(Qt, Dd)  # (n, k, p)
+            nrm = 1.0 - (QD * QD).sum(dim=1)
+            score = (res @ Dd) ** 2 / nrm
+            score[rows[:, None], Ri[:, :k]] = 0.0
+        top = torch.topk(score, 2, dim=1).values
+        chosen = score[rows, Ri[:, k]]
+        sep = (top[:, 0] - top[:, 1]) > tie * top[:, 0]
+        ties += int((~sep).sum())
+        assert bool((score.argmax(dim=1) == Ri[:, k])[sep].all()), "step %d" % k
+        assert bool((chosen >= top[:, 0] * (1 - tie)).all()), "step %d" % k
+    A = torch.zeros((n, p), dtype=torch.float64, device=dev)
+    A.scatter_add_(1, Ri, V.double())
+    res = xd - A @ DT
+    C = res @ Dd
+    ls = float((C.abs() / xd.norm(dim=1, keepdim=True)).gather(1, Ri).max())
+    print("  %d near-tied steps of %d; max |d'r| / ||x|| on the support %.3g"
+          % (ties, n * L, ls), flush=True)
+    assert ls <= ls_tol
+    d.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("solver,dt,p", [("lasso", np.float32, 32769), ("lasso", np.float64, 16385),
+                                         ("omp", np.float32, 65537)])
+def test_dictionary_past_the_largest_group_is_refused(spx, solver, dt, p):
+    """One atom past what the widest group holds (the LARS correlation rows
+    in LDS, the ORMP ones in registers) is refused with an error naming p,
+    before any solver kernel runs."""
+    import torch
+    print("case refusal", solver, dt.__name__, p, flush=True)
+    dev = torch.device("cuda:0")
+    tdt = torch.float64 if dt == np.float64 else torch.float32
+    rng = np.random.default_rng(4300)
+    m, n = 16, 4
+    D = W.unit_dict(rng, m, p, dt)
+    X = np.asfortranarray(rng.standard_normal((m, n)).astype(dt))
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    V = torch.zeros((n, m), dtype=tdt, device=dev)
+    R = torch.zeros((n, m), dtype=torch.int32, device=dev)
+    cnt = torch.zeros((n,), dtype=torch.int32, device=dev)
+    with pytest.raises(RuntimeError, match="p=%d exceeds" % p):
+        if solver == "lasso":
+            ldg = (p + 15) // 16 * 16
+            DtX = torch.zeros((n, ldg), dtype=tdt, device=dev)
+            norms = torch.ones((n,), dtype=tdt, device=dev)
+            spx.dtx_dev(ctx, d, Xt, DtX)
+            spx.sqnorms_dev(ctx, Xt, norms)
+            spx.lasso_dtx_dev(ctx, d, DtX, norms, torch.full((1,), 0.1, dtype=tdt, device=dev),
+                              V, R, cnt)
+        else:
+            spx.omp_dev(ctx, d, Xt, V, R, cnt, L=torch.tensor([m], dtype=torch.int32, device=dev))
+    torch.cuda.synchronize()
+    d.close()
+    ctx.close()
