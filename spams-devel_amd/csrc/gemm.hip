@@ -271,8 +271,10 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_kernel(
 // block.  The global loads run two 32-deep chunks ahead (two register sets,
 // the chunk loop unrolled by two), through one row pointer per thread (rows
 // past M / N read the last row: their products land in C elements that are
-// never stored).  The VALU split is the kernel's limiter: with the same
-// LDS reads and MFMAs but no loads or splits it runs 1.8x faster.
+// never stored).  With the same LDS reads and MFMAs but no global loads,
+// LDS stores or splits it runs 1.8x faster; operands split beforehand (no
+// split in the kernel) run no faster, so the loads' latency and the loop's
+// issue, not the split's VALU work, hold it back.
 template <bool VECLOAD>
 __global__ __launch_bounds__(256, 2) void gemm_tn_bf16x3_kernel(
     int M, int N, int K, const float* __restrict__ A, int lda,
