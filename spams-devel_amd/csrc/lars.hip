@@ -87,6 +87,35 @@ int lars_batched(Ctx* ctx, const LarsArgs<T>& a0, int w) {
   return rc;
 }
 
+// Whether group width W covers p for variant var (0 plain, 1 weighted, 2
+// masked): its row chunks reach p and the LDS pass's layout (the group's
+// correlation rows, the weighted variant's weights, the capacity-FastCap
+// inverse) fits the 160 KiB of LDS; past that no active-set size does.
+template <typename T, int W>
+bool lars_fits_w(int p, int var) {
+  constexpr int C = FastCap<T>::C;
+  constexpr size_t kLds = 160 * 1024;
+  switch (lars_nq<T, W>(p)) {
+    case 1: return LarsLds<T, W, 1, C>::bytes(C, 0, var) <= kLds;
+    case 2: return LarsLds<T, W, 2, C>::bytes(C, 0, var) <= kLds;
+    case 4: return LarsLds<T, W, 4, C>::bytes(C, 0, var) <= kLds;
+    case 8: return LarsLds<T, W, 8, C>::bytes(C, 0, var) <= kLds;
+    case 16: return LarsLds<T, W, 16, C>::bytes(C, 0, var) <= kLds;
+    default: return false;
+  }
+}
+
+template <typename T>
+bool lars_fits_var(int w, int p, int var) {
+  switch (w) {
+    case 1: return lars_fits_w<T, 1>(p, var);
+    case 2: return lars_fits_w<T, 2>(p, var);
+    case 4: return lars_fits_w<T, 4>(p, var);
+    case 16: return lars_fits_w<T, 16>(p, var);
+    default: return false;
+  }
+}
+
 }  // namespace
 
 // Group width `waves` as resolved by the caller (explicit, the dictionary's
@@ -100,7 +129,7 @@ int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves) {
     if (w < waves) continue;
     int rc;
     if (form == SPX_FORM_BATCHED) {
-      if (!lars_fits<T>(w, a.p)) continue;
+      if (!lars_fits_var<T>(w, a.p, a.weights ? 1 : a.mobs ? 2 : 0)) continue;
       rc = lars_batched<T>(ctx, a, w);
     } else {
       rc = lars_pass_w<T>(ctx, w, false, a, false, nullptr, 0);
@@ -111,36 +140,9 @@ int lars_solve(Ctx* ctx, const LarsArgs<T>& a, int form, int waves) {
   return SPX_ERR_UNSUP;
 }
 
-namespace {
-
-// Whether group width W covers p: its row chunks reach p and the LDS pass's
-// layout (the group's correlation rows plus the capacity-FastCap inverse)
-// fits the 160 KiB of LDS; past that no active-set size does.
-template <typename T, int W>
-bool lars_fits_w(int p) {
-  constexpr int C = FastCap<T>::C;
-  constexpr size_t kLds = 160 * 1024;
-  switch (lars_nq<T, W>(p)) {
-    case 1: return LarsLds<T, W, 1, C>::bytes(C) <= kLds;
-    case 2: return LarsLds<T, W, 2, C>::bytes(C) <= kLds;
-    case 4: return LarsLds<T, W, 4, C>::bytes(C) <= kLds;
-    case 8: return LarsLds<T, W, 8, C>::bytes(C) <= kLds;
-    case 16: return LarsLds<T, W, 16, C>::bytes(C) <= kLds;
-    default: return false;
-  }
-}
-
-}  // namespace
-
 template <typename T>
 bool lars_fits(int w, int p) {
-  switch (w) {
-    case 1: return lars_fits_w<T, 1>(p);
-    case 2: return lars_fits_w<T, 2>(p);
-    case 4: return lars_fits_w<T, 4>(p);
-    case 16: return lars_fits_w<T, 16>(p);
-    default: return false;
-  }
+  return lars_fits_var<T>(w, p, 0);
 }
 
 template bool lars_fits<float>(int, int);

@@ -1380,3 +1380,60 @@ def test_dictionary_past_the_largest_group_is_refused(spx, solver, dt, p):
     torch.cuda.synchronize()
     d.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("variant,p", [("weighted", 16384), ("masked", 32768)])
+def test_weighted_and_masked_lasso_large_dictionary_optimality(spx, variant, p):
+    """The weighted solver keeps the group's weights in LDS beside its
+    correlation rows, so it takes half the plain solver's p (16384 in
+    FP32); the masked solver takes the plain one's (32768).  At those sizes
+    (m = 64, 256 signals, FP32) every signal meets its problem's optimality
+    conditions in FP64 on the GPU (bar 2e-4 of the threshold); one atom
+    more is refused for the weighted solver."""
+    import torch
+    print("case lasso", variant, "large dictionary", p, flush=True)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(4400 + p)
+    m, n, lam = 64, 256, 0.15
+    D = W.unit_dict(rng, m, p, np.float32)
+    X = W.sparse_signals(rng, D, n, 5, 0.05, True, np.float32)
+    ctx = gpu_ctx(spx)
+    d = spx.Dictionary(ctx, D)
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    lam_t = torch.tensor([lam], dtype=torch.float32, device=dev)
+    Lout = spx.lasso_L(m, p, lambda1=lam)
+    V = torch.empty((n, Lout), dtype=torch.float32, device=dev)
+    R = torch.empty((n, Lout), dtype=torch.int32, device=dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+    Dd = torch.from_numpy(D.astype(np.float64)).to(dev)
+    if variant == "weighted":
+        Wd = torch.from_numpy(rng.uniform(0.5, 2.0, (n, p)).astype(np.float32)).to(dev)
+        spx.lasso_weighted_dev(ctx, d, Xt, Wd, lam_t, V, R, cnt)
+        torch.cuda.synchronize()
+        done = cnt.long() < Lout
+        wi, wa = _kkt_chunks(torch, Xt[done], Dd, V[done], R[done], cnt[done],
+                             torch.full((n,), lam, dtype=torch.float64, device=dev)[done],
+                             w=Wd[done], chunk=128)
+    else:
+        Bd = torch.from_numpy((rng.uniform(size=(n, m)) < 0.85).astype(np.uint8)).to(dev)
+        spx.lasso_mask_dev(ctx, d, Xt, Bd, lam_t, V, R, cnt)
+        torch.cuda.synchronize()
+        done = cnt.long() < Lout
+        lam_m = lam * Bd.double().sum(dim=1) / m
+        wi, wa = _kkt_chunks(torch, Xt[done], Dd, V[done], R[done], cnt[done], lam_m[done],
+                             mask=Bd[done], chunk=128)
+    print("  %d of %d below the cap (largest %d); max violation %.3g off the support, %.3g "
+          "on it" % (int(done.sum()), n, int(cnt.max()), wi, wa), flush=True)
+    assert int(done.sum()) >= 0.9 * n
+    assert wi <= 2e-4 and wa <= 2e-4
+    d.close()
+    if variant == "weighted":
+        p2 = p + 1
+        D2 = W.unit_dict(rng, m, p2, np.float32)
+        d2 = spx.Dictionary(ctx, D2)
+        W2 = torch.ones((n, p2), dtype=torch.float32, device=dev)
+        with pytest.raises(RuntimeError, match="p=%d exceeds" % p2):
+            spx.lasso_weighted_dev(ctx, d2, Xt, W2, lam_t, V, R, cnt)
+        torch.cuda.synchronize()
+        d2.close()
+    ctx.close()
