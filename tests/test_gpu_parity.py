@@ -1229,7 +1229,8 @@ def test_lasso_large_dictionary_optimality(spx, dt, p):
     The oracle would take minutes to form these Grams; the lasso's
     optimality conditions fix the solution instead, evaluated in FP64 on
     the GPU (bar 2e-4 of lambda in FP32, 1e-8 in FP64) over the signals
-    whose path ended below the L cap."""
+    whose path ended below the L cap, in the batched form and in the plain
+    one (one full-capacity pass, no work queue)."""
     import torch
     print("case lasso large dictionary", dt.__name__, p, flush=True)
     dev = torch.device("cuda:0")
@@ -1254,21 +1255,22 @@ def test_lasso_large_dictionary_optimality(spx, dt, p):
     cnt = torch.empty((n,), dtype=torch.int32, device=dev)
     spx.dtx_dev(ctx, d, Xt, DtX)
     spx.sqnorms_dev(ctx, Xt, norms)
-    spx.lasso_dtx_dev(ctx, d, DtX, norms, lt, V, R, cnt)
-    torch.cuda.synchronize()
-    counts = cnt.long()
-    done = counts < Lout
-    print("  active set sizes %d..%d, %d of %d below the cap, %d past the LDS pass"
-          % (int(counts.min()), int(counts.max()), int(done.sum()), n,
-             int((counts > 24).sum())), flush=True)
-    assert int(done.sum()) >= 0.9 * n
-    assert int((counts > 24).sum()) > 0
     Dd = torch.from_numpy(D.astype(np.float64)).to(dev)
-    worst_in, worst_act = _kkt_chunks(torch, Xt[done], Dd, V[done], R[done], cnt[done],
-                                      lt[done], chunk=128)
-    print("  max (|d'r| - lambda) / lambda %.3g, max |d'r - lambda sign(a)| / lambda on the "
-          "support %.3g" % (worst_in, worst_act), flush=True)
-    assert worst_in <= tol and worst_act <= tol
+    for form in ("batched", "plain"):
+        spx.lasso_dtx_dev(ctx, d, DtX, norms, lt, V, R, cnt, form=form)
+        torch.cuda.synchronize()
+        counts = cnt.long()
+        done = counts < Lout
+        print("  %s: active set sizes %d..%d, %d of %d below the cap, %d past the LDS pass"
+              % (form, int(counts.min()), int(counts.max()), int(done.sum()), n,
+                 int((counts > 24).sum())), flush=True)
+        assert int(done.sum()) >= 0.9 * n
+        assert int((counts > 24).sum()) > 0
+        worst_in, worst_act = _kkt_chunks(torch, Xt[done], Dd, V[done], R[done], cnt[done],
+                                          lt[done], chunk=128)
+        print("  max (|d'r| - lambda) / lambda %.3g, max |d'r - lambda sign(a)| / lambda on "
+              "the support %.3g" % (worst_in, worst_act), flush=True)
+        assert worst_in <= tol and worst_act <= tol
     del DtX
     d.close()
     ctx.close()
